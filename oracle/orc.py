@@ -1,0 +1,323 @@
+"""ctypes binding of the CPU oracle (oracle/liborc.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as the
+checker and by bench.py's cpu_baseline leg.  The product (din-mol-li_amd) never
+imports this module.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liborc.so")
+
+ORC_SITE_BROWN, ORC_SITE_ERMAK, ORC_SITE_DEPO, ORC_SITE_CONTACT, ORC_SITE_GCMC_U, ORC_SITE_GCMC_G = range(6)
+FL_ALIVE, FL_LIMBO, FL_REF, FL_SKIP, FL_GCMC = 1, 2, 4, 8, 16
+
+
+def build(force=False):
+    """Compile liborc.so with gcc (no fast-math, no FMA contraction)."""
+    srcs = [os.path.join(HERE, "orc.c")]
+    if not force and os.path.exists(LIB) and all(os.path.getmtime(LIB) >= os.path.getmtime(s) for s in srcs + [os.path.join(HERE, "orc.h")]):
+        return LIB
+    cmd = ["gcc", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math",
+           "-o", LIB + ".tmp"] + srcs + ["-lm"]
+    subprocess.check_call(cmd)
+    os.replace(LIB + ".tmp", LIB)
+    return LIB
+
+
+class Params(C.Structure):
+    _fields_ = [("box", C.c_double * 3), ("rcut", C.c_double), ("skin", C.c_double),
+                ("eps", C.c_double * 9), ("r0", C.c_double * 9), ("mass", C.c_double * 3),
+                ("h", C.c_double), ("gama", C.c_double), ("temp", C.c_double),
+                ("dif_sc", C.c_double), ("dif_sei", C.c_double), ("z_switch", C.c_double),
+                ("prob", C.c_double), ("z0", C.c_double), ("zmax", C.c_double),
+                ("ermak", C.c_int), ("piston", C.c_int)]
+
+
+class Counters(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in ("try_", "depo", "choques", "choques2", "choques3",
+                                         "nupd", "ins", "del_", "passes")]
+
+
+class Rng(C.Structure):
+    pass
+
+
+Rng._fields_ = [("unif", C.c_void_p), ("gauss", C.c_void_p), ("user", C.c_void_p),
+                ("idum", C.c_int32), ("ix", C.c_int32), ("iy", C.c_int32),
+                ("gaus_stored", C.c_int), ("gsave", C.c_double), ("ncalls", C.c_int64),
+                ("arr", C.c_void_p * 6), ("arr_n", C.c_int64 * 6), ("arr_stride", C.c_int * 6),
+                ("contact_next", C.c_int64), ("err", C.c_int)]
+
+
+class RunCfg(C.Structure):
+    _fields_ = [("idum", C.c_int32), ("prob", C.c_double), ("h", C.c_double),
+                ("nst", C.c_int32), ("nwr", C.c_int32),
+                ("xi", C.c_double), ("yi", C.c_double), ("dist", C.c_double), ("z0", C.c_double),
+                ("zmax", C.c_double), ("dif_sc", C.c_double), ("dif_sei", C.c_double),
+                ("nb_dcut", C.c_double), ("ermak", C.c_int32), ("reservoir", C.c_int32),
+                ("act", C.c_double), ("nadj", C.c_int32), ("nchunk", C.c_int32),
+                ("chunk_pos", C.c_void_p), ("ninit", C.c_int32), ("init_pos", C.c_void_p),
+                ("init_z", C.c_void_p)]
+
+
+class RunOut(C.Structure):
+    _fields_ = [("nat", C.c_int32), ("zmax", C.c_double), ("pos", C.c_void_p), ("z", C.c_void_p),
+                ("cap", C.c_int32), ("c", Counters), ("loop_seconds", C.c_double),
+                ("particle_steps", C.c_int64)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        vp = C.c_void_p
+        L.orc_sys_new.restype = vp
+        L.orc_sys_new.argtypes = [C.POINTER(Params), C.c_int]
+        L.orc_sys_free.argtypes = [vp]
+        L.orc_attach_initial.argtypes = [vp, C.c_int, vp]
+        for f in ("orc_nslots", "orc_nref"):
+            getattr(L, f).argtypes = [vp]
+        L.orc_list_total.restype = C.c_int64
+        L.orc_list_total.argtypes = [vp]
+        L.orc_test_update.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(Counters)]
+        L.orc_update.argtypes = [vp, C.POINTER(Counters)]
+        L.orc_brownian.argtypes = [vp, C.POINTER(Rng), C.POINTER(Counters)]
+        L.orc_ermak_a.argtypes = [vp, C.POINTER(Rng), C.POINTER(Counters)]
+        L.orc_overlap.argtypes = [vp, C.POINTER(Rng), C.POINTER(Counters), C.c_int]
+        L.orc_overlap_rounds.argtypes = [vp, vp, C.c_int, C.POINTER(Counters)]
+        L.orc_gcmc.argtypes = [vp, C.POINTER(Rng), C.c_double, C.c_int, C.POINTER(Counters)]
+        L.orc_dana_run.argtypes = [C.POINTER(RunCfg), C.POINTER(RunOut)]
+        L.orc_nr_ran.restype = C.c_double
+        L.orc_nr_ran.argtypes = [C.POINTER(Rng)]
+        L.orc_nr_gasdev.restype = C.c_double
+        L.orc_nr_gasdev.argtypes = [C.POINTER(Rng)]
+        L.orc_rng_init_nr.argtypes = [C.POINTER(Rng), C.c_int32]
+        L.orc_rng_init_arrays.argtypes = [C.POINTER(Rng)]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def dana_params(box, skin, h=1e-2, prob=1.0, z0=100.0, zmax=200.0, dif_sc=250.0, dif_sei=250.0,
+                ermak=True, piston=True, rcut=3.2):
+    """Parameters of dana (dana.F90:12-17, 86-100, 113) for the given run."""
+    p = Params()
+    p.box[:] = list(box)
+    p.rcut = rcut
+    p.skin = skin
+    eps = np.zeros((3, 3)); r0 = np.zeros((3, 3))
+    r0[1, 0] = r0[0, 1] = 3.5
+    eps[0, 0] = 2313.6; r0[0, 0] = 3.2
+    eps[2, 2] = 121.0; r0[2, 2] = 3.61
+    eps[0, 2] = eps[2, 0] = 529.1
+    r0[0, 2] = r0[2, 0] = 1.564
+    p.eps[:] = list(eps.ravel()); p.r0[:] = list(r0.ravel())
+    p.mass[:] = [6.94, 6.94, 6.94]
+    p.h, p.gama, p.temp = h, 1.0, 300.0
+    p.dif_sc, p.dif_sei, p.z_switch = dif_sc, dif_sei, 80.0
+    p.prob, p.z0, p.zmax = prob, z0, zmax
+    p.ermak, p.piston = int(ermak), int(piston)
+    return p
+
+
+class System:
+    """An oracle system built like config_inic from positions and species."""
+
+    def __init__(self, params, pos, species):
+        L = lib()
+        self.L = L
+        self.params = params
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        self.s = L.orc_sys_new(C.byref(params), max(256, len(pos) + 16))
+        for i in range(len(pos)):
+            p3 = np.ascontiguousarray(pos[i])
+            r = L.orc_attach_initial(self.s, int(species[i]), _p(p3))
+            assert r >= 0, r
+        self.c = Counters()
+        if params.ermak:
+            L.orc_set_ermak(C.c_void_p(self.s))
+
+    def __del__(self):
+        try:
+            self.L.orc_sys_free(self.s)
+        except Exception:
+            pass
+
+    @property
+    def n(self):
+        return self.L.orc_nslots(self.s)
+
+    @property
+    def nref(self):
+        return self.L.orc_nref(self.s)
+
+    def ermak_consts(self):
+        out = np.zeros(8)
+        self.L.orc_get_ermak(C.c_void_p(self.s), _p(out))
+        return out
+
+    def state(self):
+        n = self.n
+        d = {k: np.zeros(3 * n) for k in ("pos", "vel", "acel", "force", "pos_old", "old_cg")}
+        d["epot"] = np.zeros(n)
+        d["species"] = np.zeros(n, np.int8)
+        d["flags"] = np.zeros(n, np.uint8)
+        self.L.orc_get_state(C.c_void_p(self.s), _p(d["pos"]), _p(d["vel"]), _p(d["acel"]),
+                             _p(d["force"]), _p(d["epot"]), _p(d["pos_old"]), _p(d["old_cg"]),
+                             _p(d["species"]), _p(d["flags"]))
+        for k in ("pos", "vel", "acel", "force", "pos_old", "old_cg"):
+            d[k] = d[k].reshape(3, n).T.copy()
+        return d
+
+    def set_state(self, pos=None, vel=None, acel=None, pos_old=None, old_cg=None, species=None, flags=None):
+        n = self.n
+
+        def soa(a):
+            return None if a is None else np.ascontiguousarray(np.asarray(a, np.float64).T.reshape(-1))
+        arrs = [soa(pos), soa(vel), soa(acel), soa(pos_old), soa(old_cg),
+                None if species is None else np.ascontiguousarray(species, np.int8),
+                None if flags is None else np.ascontiguousarray(flags, np.uint8)]
+        self.L.orc_set_state(C.c_void_p(self.s), *[_p(a) for a in arrs])
+
+    def ref_order(self):
+        out = np.zeros(self.nref, np.int32)
+        self.L.orc_get_ref_order(C.c_void_p(self.s), _p(out))
+        return out
+
+    def set_zmax(self, z):
+        self.L.orc_set_zmax.argtypes = [C.c_void_p, C.c_double]
+        self.L.orc_set_zmax(self.s, z)
+
+    def test_update(self):
+        reb = C.c_int(0)
+        st = self.L.orc_test_update(self.s, C.byref(reb), C.byref(self.c))
+        return st, reb.value
+
+    def update(self):
+        return self.L.orc_update(self.s, C.byref(self.c))
+
+    def cells(self):
+        nc = np.zeros(3, np.int32); cell = np.zeros(3); uc = C.c_int()
+        self.L.orc_cells(C.c_void_p(self.s), _p(nc), _p(cell), C.byref(uc))
+        ci = np.zeros(3 * self.n, np.int32)
+        self.L.orc_cell_of(C.c_void_p(self.s), _p(ci))
+        return nc, cell, bool(uc.value), ci.reshape(-1, 3)
+
+    def lists(self):
+        n = self.n
+        tot = self.L.orc_list_total(self.s)
+        nn = np.zeros(n, np.int32); rp = np.zeros(n + 1, np.int64); idx = np.zeros(max(tot, 1), np.int32)
+        self.L.orc_list_export(C.c_void_p(self.s), _p(nn), _p(rp), _p(idx))
+        return nn, rp, idx[:tot]
+
+    def fuerza(self):
+        self.L.orc_fuerza(C.c_void_p(self.s))
+
+    def _arr_rng(self, **arrays):
+        r = Rng()
+        self.L.orc_rng_init_arrays(C.byref(r))
+        keep = []
+        for site, a in arrays.items():
+            a = np.ascontiguousarray(a, np.float64)
+            keep.append(a)
+            k = {"brown": 0, "ermak": 1, "depo": 2, "contact": 3, "gcmc_u": 4, "gcmc_g": 5}[site]
+            r.arr[k] = a.ctypes.data
+            r.arr_n[k] = a.size
+        return r, keep
+
+    def brownian(self, noise, u_dep):
+        r, keep = self._arr_rng(brown=noise, depo=u_dep)
+        return self.L.orc_brownian(self.s, C.byref(r), C.byref(self.c))
+
+    def ermak_a(self, noise, u_dep):
+        r, keep = self._arr_rng(ermak=noise, depo=u_dep)
+        return self.L.orc_ermak_a(self.s, C.byref(r), C.byref(self.c))
+
+    def ermak_b(self):
+        self.L.orc_ermak_b(C.c_void_p(self.s))
+
+    def ranv(self):
+        out = np.zeros(3 * self.nref)
+        self.L.orc_get_ranv(C.c_void_p(self.s), _p(out))
+        return out.reshape(3, -1).T.copy()
+
+    def overlap(self, u_contact, max_passes=100000):
+        r, keep = self._arr_rng(contact=u_contact)
+        st = self.L.orc_overlap(self.s, C.byref(r), C.byref(self.c), max_passes)
+        return st, r.contact_next
+
+    def overlap_rounds(self, u_round, max_rounds):
+        u = np.ascontiguousarray(u_round, np.float64)
+        return self.L.orc_overlap_rounds(self.s, _p(u), max_rounds, C.byref(self.c))
+
+    def f_to_cg(self, gcmc=False):
+        self.L.orc_f_to_cg(C.c_void_p(self.s), int(gcmc))
+
+    def gcmc(self, act, nadj, u, g):
+        r, keep = self._arr_rng(gcmc_u=u, gcmc_g=g)
+        return self.L.orc_gcmc(self.s, C.byref(r), act, nadj, C.byref(self.c))
+
+    def gcmc_trial_energy(self, pos3):
+        pos3 = np.ascontiguousarray(pos3, np.float64)
+        t = len(pos3)
+        du = np.zeros(t); hard = np.zeros(t, np.int32)
+        self.L.orc_gcmc_trial_energy(C.c_void_p(self.s), t, _p(pos3), _p(du), _p(hard))
+        return du, hard
+
+
+def nr_stream(idum, n, kind="ran"):
+    """The first n draws of ran() / gasdev() from seed idum."""
+    L = lib()
+    r = Rng()
+    L.orc_rng_init_nr(C.byref(r), idum)
+    f = L.orc_nr_ran if kind == "ran" else L.orc_nr_gasdev
+    return np.array([f(C.byref(r)) for _ in range(n)])
+
+
+def dana_run(idum, prob, h, nst, nwr, xi, yi, dist, z0, zmax, dif_sc, dif_sei, nb_dcut,
+             ermak, reservoir, act=0.0, nadj=0, chunk=None, init_pos=None, init_z=None, cap=1 << 20):
+    """Whole dana run; returns (nat, zmax, pos[nat,3], z[nat], counters, loop_s, psteps)."""
+    L = lib()
+    c = RunCfg()
+    c.idum, c.prob, c.h, c.nst, c.nwr = idum, prob, h, nst, nwr
+    c.xi, c.yi, c.dist, c.z0, c.zmax = xi, yi, dist, z0, zmax
+    c.dif_sc, c.dif_sei, c.nb_dcut = dif_sc, dif_sei, nb_dcut
+    c.ermak = int(ermak)
+    c.reservoir = {"piston": 0, "chunks": 1, "gcmc": 2}[reservoir] if isinstance(reservoir, str) else reservoir
+    c.act, c.nadj = act, nadj
+    keep = []
+    if chunk is not None:
+        ch = np.ascontiguousarray(chunk, np.float64)
+        keep.append(ch)
+        c.nchunk = len(ch); c.chunk_pos = ch.ctypes.data
+    if init_pos is not None:
+        ip = np.ascontiguousarray(init_pos, np.float64)
+        keep.append(ip)
+        c.ninit = len(ip); c.init_pos = ip.ctypes.data
+        if init_z is not None:
+            iz = np.ascontiguousarray(init_z, np.int8)
+            keep.append(iz)
+            c.init_z = iz.ctypes.data
+    o = RunOut()
+    pos = np.zeros(3 * cap); z = np.zeros(cap, np.int8)
+    o.pos = pos.ctypes.data; o.z = z.ctypes.data; o.cap = cap
+    st = L.orc_dana_run(C.byref(c), C.byref(o))
+    if st:
+        raise RuntimeError("orc_dana_run failed: %d" % st)
+    n = o.nat
+    return dict(nat=n, zmax=o.zmax, pos=pos[:3 * n].reshape(n, 3).copy(), z=z[:n].copy(),
+                counters={k: getattr(o.c, k) for k, _ in Counters._fields_},
+                loop_seconds=o.loop_seconds, particle_steps=o.particle_steps)
