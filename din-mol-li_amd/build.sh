@@ -1,0 +1,15 @@
+#!/bin/bash
+# Build libdml.so (HIP, gfx950) in-tree.  Used by __graft_entry__.build().
+set -euo pipefail
+here="$(cd "$(dirname "$0")" && pwd)"
+src="$here/csrc"
+out="$here/dml/libdml.so"
+hipcc=${HIPCC:-/opt/rocm/bin/hipcc}
+"$hipcc" -O3 --offload-arch=gfx950 -std=c++17 -fPIC -shared \
+  -ffp-contract=off -Xclang -target-feature -Xclang -packed-fp32-ops \
+  -Wall -Wno-unused-function -Wno-unused-value -Wno-unused-result \
+  "$src/dml_kernels.hip" "$src/dml_api.hip" -o "$out.tmp" 2>&1 \
+  | grep -v "packed-fp32-ops" || true
+[[ -f "$out.tmp" ]] || { echo "build failed" >&2; exit 1; }
+mv "$out.tmp" "$out"
+echo "built $out"

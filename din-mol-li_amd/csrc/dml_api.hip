@@ -1,0 +1,1033 @@
+// dml_api.hip -- host side of the C ABI declared in include/dml.h.
+//
+// The host keeps the per-replica geometry (tessellation, buffer offsets) and
+// sequences the kernels of dml_kernels.hip on the context's stream.  All
+// per-step state lives in HBM; the host reads device memory only where an
+// entry point returns data to the caller.
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "dml_kernels.h"
+
+#define HIPCHECK(x)                                                   \
+  do {                                                                \
+    hipError_t e_ = (x);                                              \
+    if (e_ != hipSuccess) {                                           \
+      fprintf(stderr, "libdml: %s failed: %s\n", #x, hipGetErrorString(e_)); \
+      return DML_E_HIP;                                               \
+    }                                                                 \
+  } while (0)
+
+namespace {
+
+constexpr int kMaxRounds = 32;
+
+struct HostRep {
+  dml_config cfg;
+  RepParams p;  // host mirror of the static parameters
+};
+
+}  // namespace
+
+struct dml_ctx {
+  int nrep = 0;
+  std::vector<HostRep> reps;
+  DevState d{};
+  int64_t P = 0;  // slots
+  int64_t C = 0;  // cells
+  int64_t B = 0;  // bucket entries
+  hipStream_t st = nullptr;
+  int32_t *again = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int graph_rounds = 0;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  double last_ms = 0.0;
+  int lpa = 8;
+  int rounds = 8;
+  int any_ermak = 0, any_brown = 0, any_rounds = 0, any_exact = 0, any_piston = 0;
+};
+
+namespace {
+
+int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+int next_pow2(int x) {
+  int p = 1;
+  while (p < x) p <<= 1;
+  return p;
+}
+
+// cgroup_tessellate (Cells.F90:180-265) on the host
+int tessellate(RepParams &p, int first) {
+  double brcut = p.rcut + p.skin;
+  if (!first && p.use_cells) {
+    int ok = 1, ok2 = 1;
+    for (int k = 0; k < 3; k++)
+      if (!((double)(p.ncells[k] + 1) >= p.box[k] / brcut)) ok = 0;
+    for (int k = 0; k < 3; k++)
+      if (!(brcut < p.box[k] / (double)p.ncells[k])) ok2 = 0;
+    if (ok && ok2) {
+      for (int k = 0; k < 3; k++) p.cell[k] = p.box[k] / (double)p.ncells[k];
+      return DML_OK;
+    }
+  }
+  int nc[3];
+  for (int k = 0; k < 3; k++) nc[k] = (int)(p.box[k] / brcut);
+  if (nc[0] < 4 && nc[1] < 4 && nc[2] < 4) {
+    if (!first && p.use_cells) return DML_E_UNSUPPORTED;  // stale-cell quirk
+    for (int k = 0; k < 3; k++) {
+      p.ncells[k] = nc[k];
+      p.cell[k] = p.box[k];
+    }
+    p.use_cells = 0;
+    return DML_OK;
+  }
+  for (int k = 0; k < 3; k++) {
+    p.ncells[k] = nc[k];
+    p.cell[k] = p.box[k] / (double)nc[k];
+  }
+  p.use_cells = 1;
+  return DML_OK;
+}
+
+int ncells_total(const RepParams &p) {
+  return p.use_cells ? (p.ncells[0] + 2) * (p.ncells[1] + 2) * (p.ncells[2] + 2) : 1;
+}
+
+int bucket_cap(const RepParams &p) {
+  if (!p.use_cells) return (int)p.pcap;
+  double v = p.cell[0] * p.cell[1] * p.cell[2];
+  int est = (int)ceil(0.06 * v) + 8;
+  return std::max(8, next_pow2(est));
+}
+
+void set_ermak_consts(RepParams &p) {
+  // set_ermak (dana.F90:947-971) with dana's kB (dana.F90:22-24)
+  const double kB_eVK = 8.617330350e-5, kJm_ui = 100.0, eV_kJm = 96.485;
+  const double eV_ui = eV_kJm * kJm_ui;
+  const double kB_ui = kB_eVK * eV_ui;
+  double h = p.h, gama = 1.0;
+  p.cc0 = exp(-(h * gama));
+  p.cc1 = (1.0 - p.cc0) / gama;
+  p.cc2 = (1.0 - p.cc1 / h) / gama;
+  p.sdr = sqrt(h / gama * (2.0 - ((3.0 - 4.0 * p.cc0) + p.cc0 * p.cc0) / (h * gama)));
+  p.sdv = sqrt(1.0 - p.cc0 * p.cc0);
+  p.crv1 = (1.0 - p.cc0) * (1.0 - p.cc0) / (gama * p.sdr * p.sdv);
+  p.crv2 = sqrt(1.0 - (p.crv1 * p.crv1));
+  p.skt = sqrt(kB_ui * 300.0);
+}
+
+int status_from_err(int e) {
+  if (e & ERR_TESS) return DML_E_OUT_OF_TESS;
+  if (e & ERR_DEPZ0) return DML_E_DEPOSIT_Z0;
+  if (e & ERR_NOPART) return DML_E_NO_PARTICLE;
+  if (e & ERR_OVERLAP) return DML_E_OVERLAP;
+  if (e & ERR_LIST) return DML_E_LIST_FULL;
+  if (e & ERR_BUCKET) return DML_E_NOMEM;
+  return DML_OK;
+}
+
+template <typename T>
+int dalloc(T **p, int64_t n) {
+  if (n <= 0) n = 1;
+  if (hipMalloc((void **)p, sizeof(T) * (size_t)n) != hipSuccess) return DML_E_NOMEM;
+  return DML_OK;
+}
+
+int read_err(dml_ctx *c, int reset) {
+  int e = 0;
+  if (hipMemcpyAsync(&e, c->d.err, sizeof(int), hipMemcpyDeviceToHost, c->st) != hipSuccess)
+    return -1;
+  if (hipStreamSynchronize(c->st) != hipSuccess) return -1;
+  if (reset && e) {
+    int z = 0;
+    hipMemcpyAsync(c->d.err, &z, sizeof(int), hipMemcpyHostToDevice, c->st);
+    hipStreamSynchronize(c->st);
+  }
+  return e;
+}
+
+int upload_params(dml_ctx *c) {
+  std::vector<RepParams> v(c->nrep);
+  for (int r = 0; r < c->nrep; r++) v[r] = c->reps[r].p;
+  HIPCHECK(hipMemcpyAsync(c->d.rp, v.data(), sizeof(RepParams) * c->nrep, hipMemcpyHostToDevice,
+                          c->st));
+  return DML_OK;
+}
+
+// the mutable device fields are authoritative: fetch them into the mirror
+int fetch_params(dml_ctx *c) {
+  std::vector<RepParams> v(c->nrep);
+  HIPCHECK(hipMemcpyAsync(v.data(), c->d.rp, sizeof(RepParams) * c->nrep, hipMemcpyDeviceToHost,
+                          c->st));
+  HIPCHECK(hipStreamSynchronize(c->st));
+  for (int r = 0; r < c->nrep; r++) c->reps[r].p = v[r];
+  return DML_OK;
+}
+
+int alloc_cells(dml_ctx *c) {
+  int64_t C = 0, B = 0;
+  for (auto &h : c->reps) {
+    h.p.coff = (int32_t)C;
+    h.p.ccap = ncells_total(h.p);
+    h.p.boff = B;
+    C += h.p.ccap;
+    B += (int64_t)h.p.ccap * h.p.bcap;
+  }
+  hipFree(c->d.cell_cnt);
+  hipFree(c->d.bucket);
+  hipFree(c->d.bx);
+  hipFree(c->d.by);
+  hipFree(c->d.bz);
+  c->d.cell_cnt = nullptr;
+  c->d.bucket = nullptr;
+  c->d.bx = c->d.by = c->d.bz = nullptr;
+  int e = 0;
+  e |= dalloc(&c->d.cell_cnt, C);
+  e |= dalloc(&c->d.bucket, B);
+  e |= dalloc(&c->d.bx, B);
+  e |= dalloc(&c->d.by, B);
+  e |= dalloc(&c->d.bz, B);
+  if (e) return DML_E_NOMEM;
+  HIPCHECK(hipMemsetAsync(c->d.cell_cnt, 0, sizeof(int32_t) * C, c->st));
+  c->C = C;
+  c->B = B;
+  return DML_OK;
+}
+
+int grid_cells(const dml_ctx *c) {
+  int64_t mx = 1;
+  for (auto &h : c->reps) mx = std::max<int64_t>(mx, h.p.ccap);
+  return (int)std::min<int64_t>((mx + DML_BLOCK - 1) / DML_BLOCK, 4096);
+}
+
+// ---- phases -------------------------------------------------------------
+void enq_cells_lists(dml_ctx *c) {
+  dim3 gc(grid_cells(c), c->nrep);
+  int nb = c->d.nblocks;
+  hipLaunchKernelGGL(k_zero_cells, gc, dim3(DML_BLOCK), 0, c->st, c->d);
+  hipLaunchKernelGGL(k_cell_key, dim3(nb), dim3(DML_BLOCK), 0, c->st, c->d);
+  hipLaunchKernelGGL(k_cell_sort, gc, dim3(DML_BLOCK), 0, c->st, c->d);
+  hipLaunchKernelGGL(k_listbuild<4>, dim3((unsigned)(c->P / 4)), dim3(4 * 64), 0, c->st, c->d);
+}
+
+void enq_test_update(dml_ctx *c) {
+  hipLaunchKernelGGL(k_pbc_dispmax, dim3(c->d.nblocks), dim3(DML_BLOCK), 0, c->st, c->d);
+  enq_cells_lists(c);
+}
+
+template <int FUSEB>
+void launch_force_lpa(dml_ctx *c, int lpa, int only_ermak) {
+  dim3 g((unsigned)(c->P * lpa / DML_BLOCK));
+  switch (lpa) {
+    case 1: hipLaunchKernelGGL((k_force<1, FUSEB>), g, dim3(DML_BLOCK), 0, c->st, c->d, only_ermak); break;
+    case 2: hipLaunchKernelGGL((k_force<2, FUSEB>), g, dim3(DML_BLOCK), 0, c->st, c->d, only_ermak); break;
+    case 4: hipLaunchKernelGGL((k_force<4, FUSEB>), g, dim3(DML_BLOCK), 0, c->st, c->d, only_ermak); break;
+    case 16: hipLaunchKernelGGL((k_force<16, FUSEB>), g, dim3(DML_BLOCK), 0, c->st, c->d, only_ermak); break;
+    case 32: hipLaunchKernelGGL((k_force<32, FUSEB>), g, dim3(DML_BLOCK), 0, c->st, c->d, only_ermak); break;
+    case 64: hipLaunchKernelGGL((k_force<64, FUSEB>), g, dim3(DML_BLOCK), 0, c->st, c->d, only_ermak); break;
+    default: hipLaunchKernelGGL((k_force<8, FUSEB>), g, dim3(DML_BLOCK), 0, c->st, c->d, only_ermak); break;
+  }
+}
+
+void enq_forces(dml_ctx *c, int fuse_b, int only_ermak) {
+  if (fuse_b)
+    launch_force_lpa<1>(c, c->lpa, only_ermak);
+  else
+    launch_force_lpa<0>(c, c->lpa, only_ermak);
+  hipLaunchKernelGGL(k_ghost_scatter, dim3(c->d.nblocks), dim3(DML_BLOCK), 0, c->st, c->d,
+                     only_ermak);
+  if (fuse_b)
+    hipLaunchKernelGGL(k_ermak_b, dim3(c->d.nblocks), dim3(DML_BLOCK), 0, c->st, c->d, 1);
+}
+
+void enq_overlap_rounds(dml_ctx *c, const double *u_round, int64_t stride, int rounds,
+                        int err_if_pending) {
+  int nr = c->nrep;
+  hipLaunchKernelGGL(k_zero_i32, dim3((unsigned)((kMaxRounds * nr + 255) / 256)), dim3(256), 0,
+                     c->st, c->again, (int64_t)kMaxRounds * nr);
+  for (int q = 0; q < rounds; q++) {
+    hipLaunchKernelGGL(k_ov_scan, dim3(c->d.nblocks), dim3(DML_BLOCK), 0, c->st, c->d,
+                       u_round, stride, q, c->again);
+    hipLaunchKernelGGL(k_ov_apply, dim3(c->d.nblocks), dim3(DML_BLOCK), 0, c->st, c->d, q,
+                       (const int32_t *)c->again);
+  }
+  hipLaunchKernelGGL(k_ov_end, dim3((nr + 63) / 64), dim3(64), 0, c->st, c->d, rounds,
+                     (const int32_t *)c->again, err_if_pending);
+}
+
+void enq_overlap(dml_ctx *c) {
+  if (c->any_exact)
+    hipLaunchKernelGGL(k_ov_exact, dim3(c->nrep), dim3(64), 0, c->st, c->d, (const double *)nullptr,
+                       (int64_t)0, 100000);
+  if (c->any_rounds) enq_overlap_rounds(c, nullptr, 0, c->rounds, 0);
+}
+
+void enq_f2cg(dml_ctx *c) {
+  int nb = c->d.nblocks;
+  hipLaunchKernelGGL(k_f2cg_mark, dim3(nb), dim3(DML_BLOCK), 0, c->st, c->d);
+  hipLaunchKernelGGL(k_f2cg_scan, dim3(c->nrep), dim3(64), 0, c->st, c->d);
+  hipLaunchKernelGGL(k_f2cg_scatter, dim3(nb), dim3(DML_BLOCK), 0, c->st, c->d);
+  hipLaunchKernelGGL(k_f2cg_copy, dim3(nb), dim3(DML_BLOCK), 0, c->st, c->d);
+  hipLaunchKernelGGL(k_f2cg_end, dim3((c->nrep + 63) / 64), dim3(64), 0, c->st, c->d);
+}
+
+void enq_step(dml_ctx *c) {
+  int nb = c->d.nblocks;
+  if (c->any_ermak) {
+    hipLaunchKernelGGL(k_ermak_a, dim3(nb), dim3(DML_BLOCK), 0, c->st, c->d,
+                       (const double *)nullptr, (const double *)nullptr);
+    enq_forces(c, 1, 1);
+  }
+  if (c->any_brown)
+    hipLaunchKernelGGL(k_brownian, dim3(nb), dim3(DML_BLOCK), 0, c->st, c->d,
+                       (const double *)nullptr, (const double *)nullptr);
+  enq_test_update(c);
+  enq_overlap(c);
+  enq_test_update(c);
+  enq_f2cg(c);
+  if (c->any_piston) {
+    hipLaunchKernelGGL(k_calc_rho, dim3(nb), dim3(DML_BLOCK), 0, c->st, c->d);
+    hipLaunchKernelGGL(k_maxz, dim3(nb), dim3(DML_BLOCK), 0, c->st, c->d);
+  }
+  hipLaunchKernelGGL(k_step_end, dim3((c->nrep + 63) / 64), dim3(64), 0, c->st, c->d);
+}
+
+// grow neighbour rows or buckets after an overflow and redo the rebuild of
+// the replicas whose rebuild flag is still set
+int handle_overflow(dml_ctx *c, int e) {
+  for (int attempt = 0; attempt < 8 && (e & (ERR_LIST | ERR_BUCKET)); attempt++) {
+    int rc = fetch_params(c);
+    if (rc) return rc;
+    if (e & ERR_LIST) {
+      int need = 0;
+      for (auto &h : c->reps) need = std::max(need, h.p.max_nn_seen);
+      int nm = std::max(next_pow2(need + need / 4 + 1), c->d.maxnn * 2);
+      int32_t *nbr2 = nullptr;
+      if (dalloc(&nbr2, c->P * (int64_t)nm)) return DML_E_NOMEM;
+      // keep the rows of replicas that are not rebuilt
+      HIPCHECK(hipMemcpy2DAsync(nbr2, sizeof(int32_t) * nm, c->d.nbr, sizeof(int32_t) * c->d.maxnn,
+                                sizeof(int32_t) * c->d.maxnn, (size_t)c->P, hipMemcpyDeviceToDevice,
+                                c->st));
+      HIPCHECK(hipStreamSynchronize(c->st));
+      hipFree(c->d.nbr);
+      c->d.nbr = nbr2;
+      c->d.maxnn = nm;
+    }
+    if (e & ERR_BUCKET) {
+      for (auto &h : c->reps) {
+        if (!h.p.rebuild && h.p.reservoir != DML_GCMC) continue;
+        if (h.p.use_cells) h.p.bcap *= 2;
+      }
+      int rc2 = alloc_cells(c);
+      if (rc2) return rc2;
+      // only the layout fields changed: patch them into the device table
+      std::vector<RepParams> v(c->nrep);
+      HIPCHECK(hipMemcpy(v.data(), c->d.rp, sizeof(RepParams) * c->nrep, hipMemcpyDeviceToHost));
+      for (int r = 0; r < c->nrep; r++) {
+        v[r].coff = c->reps[r].p.coff;
+        v[r].ccap = c->reps[r].p.ccap;
+        v[r].boff = c->reps[r].p.boff;
+        v[r].bcap = c->reps[r].p.bcap;
+      }
+      HIPCHECK(hipMemcpy(c->d.rp, v.data(), sizeof(RepParams) * c->nrep, hipMemcpyHostToDevice));
+    }
+    int z = 0;
+    HIPCHECK(hipMemcpyAsync(c->d.err, &z, sizeof(int), hipMemcpyHostToDevice, c->st));
+    enq_cells_lists(c);
+    e = read_err(c, 0);
+    if (e < 0) return DML_E_HIP;
+  }
+  return DML_OK;
+}
+
+int finish(dml_ctx *c) {
+  int e = read_err(c, 0);
+  if (e < 0) return DML_E_HIP;
+  if (e & (ERR_LIST | ERR_BUCKET)) {
+    int rc = handle_overflow(c, e);
+    if (rc) return rc;
+    e = read_err(c, 0);
+  }
+  if (e) {
+    int z = 0;
+    hipMemcpy(c->d.err, &z, sizeof(int), hipMemcpyHostToDevice);
+    return status_from_err(e);
+  }
+  return DML_OK;
+}
+
+double *upload_tmp(dml_ctx *c, const double *h, int64_t n) {
+  if (!h) return nullptr;
+  double *d = nullptr;
+  if (hipMalloc((void **)&d, sizeof(double) * (size_t)std::max<int64_t>(n, 1)) != hipSuccess)
+    return nullptr;
+  hipMemcpyAsync(d, h, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, c->st);
+  return d;
+}
+
+void refresh_modes(dml_ctx *c) {
+  c->any_ermak = c->any_brown = c->any_rounds = c->any_exact = c->any_piston = 0;
+  for (auto &h : c->reps) {
+    if (h.p.integrator == DML_ERMAK) c->any_ermak = 1; else c->any_brown = 1;
+    if (h.p.overlap_mode == DML_OVERLAP_ROUNDS) c->any_rounds = 1; else c->any_exact = 1;
+    if (h.p.piston) c->any_piston = 1;
+  }
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char *dml_strerror(int s) {
+  switch (s) {
+    case DML_OK: return "ok";
+    case DML_E_ARG: return "bad argument";
+    case DML_E_HIP: return "HIP runtime error";
+    case DML_E_NOMEM: return "out of device memory or capacity";
+    case DML_E_OUT_OF_TESS: return "particle out of tessellation";
+    case DML_E_LIST_FULL: return "neighbour row capacity exceeded";
+    case DML_E_DEPOSIT_Z0: return "deposit above z0";
+    case DML_E_NO_PARTICLE: return "chosen particle does not exist";
+    case DML_E_OVERLAP: return "overlap resolution did not finish";
+    case DML_E_STATE: return "call not valid in this state";
+    case DML_E_UNSUPPORTED: return "unsupported configuration";
+  }
+  return "unknown status";
+}
+
+int32_t dml_num_replicas(const dml_ctx *c) { return c ? c->nrep : 0; }
+
+int64_t dml_total_slots(const dml_ctx *c) { return c ? c->P : 0; }
+
+int64_t dml_ref_offset(const dml_ctx *c, int32_t rep) {
+  if (!c || rep < 0 || rep >= c->nrep) return -1;
+  return c->reps[rep].p.poff;
+}
+
+int dml_create(const dml_config *cfg, dml_ctx **out) { return dml_create_ensemble(1, cfg, out); }
+
+int dml_create_ensemble(int32_t nrep, const dml_config *cfgs, dml_ctx **out) {
+  if (!out || !cfgs || nrep <= 0) return DML_E_ARG;
+  *out = nullptr;
+  dml_ctx *c = new dml_ctx();
+  c->nrep = nrep;
+  c->reps.resize(nrep);
+  int64_t P = 0;
+  int maxnn = 0;
+  int lpa = 0;
+  for (int r = 0; r < nrep; r++) {
+    const dml_config &g = cfgs[r];
+    if (g.capacity <= 0 || g.box[0] <= 0 || g.box[1] <= 0 || g.box[2] <= 0 || g.rcut <= 0) {
+      delete c;
+      return DML_E_ARG;
+    }
+    HostRep &h = c->reps[r];
+    h.cfg = g;
+    RepParams &p = h.p;
+    memset(&p, 0, sizeof(p));
+    for (int k = 0; k < 3; k++) {
+      p.box[k] = g.box[k];
+      p.one_box[k] = 1.0 / g.box[k];
+    }
+    p.rcut = g.rcut;
+    p.rcut2 = g.rcut * g.rcut;
+    p.skin = g.skin;
+    double rl = g.rcut + g.skin;
+    p.rl2 = rl * rl;
+    p.zmax = g.zmax;
+    p.z0 = g.z0;
+    p.prob = g.prob;
+    p.h = g.h;
+    p.z_switch = g.z_switch;
+    p.dif_sc = g.dif_sc;
+    p.dif_sei = g.dif_sei;
+    for (int q = 0; q < 9; q++) {
+      p.eps[q] = g.eps[q];
+      p.r0[q] = g.r0[q];
+    }
+    for (int q = 0; q < 3; q++) p.mass[q] = g.mass[q];
+    set_ermak_consts(p);
+    p.act = g.act;
+    p.nadj = g.nadj;
+    p.integrator = g.integrator;
+    p.reservoir = g.reservoir;
+    p.piston = g.reservoir == DML_PISTON;
+    p.precision = g.precision;
+    p.overlap_mode = g.overlap_mode;
+    p.seed = g.seed;
+    p.pcap = (int32_t)round_up(g.capacity, DML_BLOCK);
+    p.poff = (int32_t)P;
+    P += p.pcap;
+    if (tessellate(p, 1)) {
+      delete c;
+      return DML_E_UNSUPPORTED;
+    }
+    p.bcap = bucket_cap(p);
+    int mnn = g.max_neighbors;
+    if (mnn <= 0) {
+      double vol = 4.0 / 3.0 * M_PI * rl * rl * rl;
+      mnn = next_pow2((int)(0.06 * vol * 1.3) + 32);
+    }
+    maxnn = std::max(maxnn, mnn);
+    if (g.lanes_per_atom > 0) lpa = g.lanes_per_atom;
+  }
+  if (P > INT32_MAX / 2) {
+    delete c;
+    return DML_E_ARG;
+  }
+  c->P = P;
+  if (lpa) c->lpa = lpa;
+  if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return DML_E_HIP;
+  }
+  DevState &d = c->d;
+  int e = 0;
+  e |= dalloc(&d.px, P); e |= dalloc(&d.py, P); e |= dalloc(&d.pz, P);
+  e |= dalloc(&d.vx, P); e |= dalloc(&d.vy, P); e |= dalloc(&d.vz, P);
+  e |= dalloc(&d.ax, P); e |= dalloc(&d.ay, P); e |= dalloc(&d.az, P);
+  e |= dalloc(&d.fx, P); e |= dalloc(&d.fy, P); e |= dalloc(&d.fz, P);
+  e |= dalloc(&d.epot, P);
+  e |= dalloc(&d.ox, P); e |= dalloc(&d.oy, P); e |= dalloc(&d.oz, P);
+  e |= dalloc(&d.cx, P); e |= dalloc(&d.cy, P); e |= dalloc(&d.cz, P);
+  e |= dalloc(&d.rvx, P); e |= dalloc(&d.rvy, P); e |= dalloc(&d.rvz, P);
+  e |= dalloc(&d.species, P);
+  e |= dalloc(&d.flags, P);
+  e |= dalloc(&d.rowflag, P);
+  e |= dalloc(&d.bkey, P);
+  e |= dalloc(&d.pid, P);
+  e |= dalloc(&d.ref_order, P);
+  e |= dalloc(&d.ref_tmp, P);
+  e |= dalloc(&d.ref_pos, P);
+  e |= dalloc(&d.nn, P);
+  e |= dalloc(&d.nbr, P * (int64_t)maxnn);
+  e |= dalloc(&d.ov_rev, P);
+  e |= dalloc(&d.ov_st, P);
+  e |= dalloc(&d.part_d, 2 * (P / DML_BLOCK));
+  e |= dalloc(&d.part_i, P / DML_BLOCK);
+  e |= dalloc(&d.blk2rep, P / DML_BLOCK);
+  e |= dalloc(&d.rp, nrep);
+  e |= dalloc(&d.cnt, nrep);
+  e |= dalloc(&d.err, 1);
+  e |= dalloc(&c->again, (int64_t)kMaxRounds * nrep);
+  if (e) {
+    dml_destroy(c);
+    return DML_E_NOMEM;
+  }
+  d.maxnn = maxnn;
+  d.nblocks = (int32_t)(P / DML_BLOCK);
+  d.nrep = nrep;
+  // zero state buffers once
+  double *dbl[] = {d.px, d.py, d.pz, d.vx, d.vy, d.vz, d.ax, d.ay, d.az, d.fx, d.fy, d.fz,
+                   d.epot, d.ox, d.oy, d.oz, d.cx, d.cy, d.cz, d.rvx, d.rvy, d.rvz};
+  for (double *q : dbl) hipMemsetAsync(q, 0, sizeof(double) * P, c->st);
+  hipMemsetAsync(d.species, 0, P, c->st);
+  hipMemsetAsync(d.flags, 0, P, c->st);
+  hipMemsetAsync(d.rowflag, 0, P, c->st);
+  hipMemsetAsync(d.nn, 0, sizeof(int32_t) * P, c->st);
+  hipMemsetAsync(d.ov_rev, 0, sizeof(int32_t) * P, c->st);
+  hipMemsetAsync(d.ov_st, 0, P, c->st);
+  hipMemsetAsync(d.ref_pos, 0xff, sizeof(int32_t) * P, c->st);
+  hipMemsetAsync(d.cnt, 0, sizeof(DevCounters) * nrep, c->st);
+  hipMemsetAsync(d.err, 0, sizeof(int32_t), c->st);
+  std::vector<int32_t> b2r(P / DML_BLOCK);
+  for (int r = 0; r < nrep; r++)
+    for (int q = 0; q < c->reps[r].p.pcap / DML_BLOCK; q++)
+      b2r[c->reps[r].p.poff / DML_BLOCK + q] = r;
+  hipMemcpyAsync(d.blk2rep, b2r.data(), sizeof(int32_t) * b2r.size(), hipMemcpyHostToDevice, c->st);
+  if (alloc_cells(c) || upload_params(c)) {
+    dml_destroy(c);
+    return DML_E_NOMEM;
+  }
+  hipEventCreate(&c->e0);
+  hipEventCreate(&c->e1);
+  refresh_modes(c);
+  if (hipStreamSynchronize(c->st) != hipSuccess) {
+    dml_destroy(c);
+    return DML_E_HIP;
+  }
+  *out = c;
+  return DML_OK;
+}
+
+int dml_destroy(dml_ctx *c) {
+  if (!c) return DML_OK;
+  if (c->st) hipStreamSynchronize(c->st);
+  if (c->gexec) hipGraphExecDestroy(c->gexec);
+  if (c->graph) hipGraphDestroy(c->graph);
+  DevState &d = c->d;
+  void *ptrs[] = {d.px, d.py, d.pz, d.vx, d.vy, d.vz, d.ax, d.ay, d.az, d.fx, d.fy, d.fz,
+                  d.epot, d.ox, d.oy, d.oz, d.cx, d.cy, d.cz, d.rvx, d.rvy, d.rvz,
+                  d.species, d.flags, d.rowflag, d.bkey, d.pid, d.ref_order, d.ref_tmp,
+                  d.ref_pos, d.nn, d.nbr, d.cell_cnt, d.bucket, d.bx, d.by, d.bz, d.ov_rev,
+                  d.ov_st, d.part_d, d.part_i, d.blk2rep, d.rp, d.cnt, d.err, c->again};
+  for (void *q : ptrs)
+    if (q) hipFree(q);
+  if (c->e0) hipEventDestroy(c->e0);
+  if (c->e1) hipEventDestroy(c->e1);
+  if (c->st) hipStreamDestroy(c->st);
+  delete c;
+  return DML_OK;
+}
+
+int dml_sync(dml_ctx *c) {
+  if (!c) return DML_E_ARG;
+  return finish(c);
+}
+
+int dml_set_state(dml_ctx *c, int32_t rep, int32_t n, const double *pos, const double *vel,
+                  const double *acel, const double *pos_old, const double *old_cg,
+                  const int8_t *species, const uint8_t *flags) {
+  if (!c || rep < 0 || rep >= c->nrep || n < 0 || !pos) return DML_E_ARG;
+  int rc = fetch_params(c);
+  if (rc) return rc;
+  RepParams &p = c->reps[rep].p;
+  if (n > p.pcap) return DML_E_NOMEM;
+  int64_t o = p.poff;
+  std::vector<double> tmp(std::max(n, 1));
+  auto put = [&](double *dst, const double *src, int k, double dflt) -> int {
+    for (int i = 0; i < n; i++) tmp[i] = src ? src[(int64_t)k * n + i] : dflt;
+    HIPCHECK(hipMemcpy(dst + o, tmp.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    return DML_OK;
+  };
+  double *P3[3] = {c->d.px, c->d.py, c->d.pz};
+  double *V3[3] = {c->d.vx, c->d.vy, c->d.vz};
+  double *A3[3] = {c->d.ax, c->d.ay, c->d.az};
+  double *O3[3] = {c->d.ox, c->d.oy, c->d.oz};
+  double *G3[3] = {c->d.cx, c->d.cy, c->d.cz};
+  for (int k = 0; k < 3; k++) {
+    if ((rc = put(P3[k], pos, k, 0.0))) return rc;
+    if ((rc = put(V3[k], vel, k, 0.0))) return rc;
+    if ((rc = put(A3[k], acel, k, 0.0))) return rc;
+    if ((rc = put(O3[k], pos_old ? pos_old : pos, k, 0.0))) return rc;
+    if ((rc = put(G3[k], old_cg, k, 1e8))) return rc;
+  }
+  std::vector<int8_t> sp(std::max(n, 1));
+  std::vector<uint8_t> fl(std::max(n, 1));
+  std::vector<int32_t> bk(std::max(n, 1)), rord, rpos(std::max(n, 1), -1);
+  std::vector<uint32_t> pid(std::max(n, 1));
+  for (int i = 0; i < n; i++) {
+    sp[i] = species ? species[i] : (int8_t)SP_LI;
+    fl[i] = flags ? flags[i]
+                  : (uint8_t)(DML_FL_ALIVE | (sp[i] != SP_CG ? DML_FL_REF : 0) |
+                              ((p.reservoir == DML_GCMC && sp[i] == SP_LI) ? DML_FL_GCMC : 0));
+    bk[i] = i;
+    pid[i] = (uint32_t)i;
+    if (fl[i] & DML_FL_REF) {
+      rpos[i] = (int32_t)rord.size();
+      rord.push_back((int32_t)(o + i));
+    }
+  }
+  HIPCHECK(hipMemcpy(c->d.species + o, sp.data(), n, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(c->d.flags + o, fl.data(), n, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(c->d.bkey + o, bk.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(c->d.pid + o, pid.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(c->d.ref_pos + o, rpos.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+  if (!rord.empty())
+    HIPCHECK(hipMemcpy(c->d.ref_order + o, rord.data(), sizeof(int32_t) * rord.size(),
+                       hipMemcpyHostToDevice));
+  HIPCHECK(hipMemset(c->d.nn + o, 0, sizeof(int32_t) * p.pcap));
+  p.nslots = n;
+  p.nref = (int32_t)rord.size();
+  p.listed = 0;
+  p.rebuild = 0;
+  p.any_f = 0;
+  p.step = 0;
+  if (p.piston) {
+    // initial calc_rho (dana.F90:145-146)
+    int g = 0;
+    for (int i = 0; i < n; i++) {
+      double z = pos[2 * (int64_t)n + i];
+      if (z > p.z0 && z < p.zmax) g++;
+    }
+    p.rho0 = (double)g / (p.box[0] * p.box[1] * (p.zmax - p.z0));
+  }
+  rc = upload_params(c);
+  if (rc) return rc;
+  HIPCHECK(hipStreamSynchronize(c->st));
+  return DML_OK;
+}
+
+int dml_set_order(dml_ctx *c, int32_t rep, int32_t nref, const int32_t *ref_order,
+                  const int32_t *bkey) {
+  if (!c || rep < 0 || rep >= c->nrep) return DML_E_ARG;
+  int rc = fetch_params(c);
+  if (rc) return rc;
+  RepParams &p = c->reps[rep].p;
+  int64_t o = p.poff;
+  int n = p.nslots;
+  if (ref_order) {
+    if (nref < 0 || nref > n) return DML_E_ARG;
+    std::vector<int32_t> g(std::max(nref, 1)), rpos(std::max(n, 1), -1);
+    for (int t = 0; t < nref; t++) {
+      if (ref_order[t] < 0 || ref_order[t] >= n) return DML_E_ARG;
+      g[t] = (int32_t)(o + ref_order[t]);
+      rpos[ref_order[t]] = t;
+    }
+    HIPCHECK(hipMemcpy(c->d.ref_order + o, g.data(), sizeof(int32_t) * nref, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(c->d.ref_pos + o, rpos.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    p.nref = nref;
+  }
+  if (bkey) HIPCHECK(hipMemcpy(c->d.bkey + o, bkey, sizeof(int32_t) * n, hipMemcpyHostToDevice));
+  return upload_params(c);
+}
+
+int dml_get_state(dml_ctx *c, int32_t rep, int32_t n, double *pos, double *vel, double *acel,
+                  double *force, double *epot, double *pos_old, double *old_cg, int8_t *species,
+                  uint8_t *flags) {
+  if (!c || rep < 0 || rep >= c->nrep || n < 0) return DML_E_ARG;
+  HIPCHECK(hipStreamSynchronize(c->st));
+  const RepParams &p = c->reps[rep].p;
+  if (n > p.pcap) return DML_E_ARG;
+  int64_t o = p.poff;
+  auto get3 = [&](double *dst, double *const *src) -> int {
+    if (!dst) return DML_OK;
+    for (int k = 0; k < 3; k++)
+      HIPCHECK(hipMemcpy(dst + (int64_t)k * n, src[k] + o, sizeof(double) * n, hipMemcpyDeviceToHost));
+    return DML_OK;
+  };
+  double *P3[3] = {c->d.px, c->d.py, c->d.pz};
+  double *V3[3] = {c->d.vx, c->d.vy, c->d.vz};
+  double *A3[3] = {c->d.ax, c->d.ay, c->d.az};
+  double *F3[3] = {c->d.fx, c->d.fy, c->d.fz};
+  double *O3[3] = {c->d.ox, c->d.oy, c->d.oz};
+  double *G3[3] = {c->d.cx, c->d.cy, c->d.cz};
+  int rc;
+  if ((rc = get3(pos, P3)) || (rc = get3(vel, V3)) || (rc = get3(acel, A3)) ||
+      (rc = get3(force, F3)) || (rc = get3(pos_old, O3)) || (rc = get3(old_cg, G3)))
+    return rc;
+  if (epot) HIPCHECK(hipMemcpy(epot, c->d.epot + o, sizeof(double) * n, hipMemcpyDeviceToHost));
+  if (species) HIPCHECK(hipMemcpy(species, c->d.species + o, n, hipMemcpyDeviceToHost));
+  if (flags) HIPCHECK(hipMemcpy(flags, c->d.flags + o, n, hipMemcpyDeviceToHost));
+  return DML_OK;
+}
+
+int dml_get_ref_order(dml_ctx *c, int32_t rep, int32_t *nref, int32_t *ref_order) {
+  if (!c || rep < 0 || rep >= c->nrep || !nref) return DML_E_ARG;
+  int rc = fetch_params(c);
+  if (rc) return rc;
+  const RepParams &p = c->reps[rep].p;
+  *nref = p.nref;
+  if (ref_order && p.nref > 0) {
+    std::vector<int32_t> g(p.nref);
+    HIPCHECK(hipMemcpy(g.data(), c->d.ref_order + p.poff, sizeof(int32_t) * p.nref,
+                       hipMemcpyDeviceToHost));
+    for (int t = 0; t < p.nref; t++) ref_order[t] = g[t] - p.poff;
+  }
+  return DML_OK;
+}
+
+int dml_set_zmax(dml_ctx *c, int32_t rep, double zmax, double z0) {
+  if (!c || rep < 0 || rep >= c->nrep) return DML_E_ARG;
+  int rc = fetch_params(c);
+  if (rc) return rc;
+  c->reps[rep].p.zmax = zmax;
+  c->reps[rep].p.z0 = z0;
+  return upload_params(c);
+}
+
+int dml_get_zmax(dml_ctx *c, int32_t rep, double *zmax) {
+  if (!c || rep < 0 || rep >= c->nrep || !zmax) return DML_E_ARG;
+  int rc = fetch_params(c);
+  if (rc) return rc;
+  *zmax = c->reps[rep].p.zmax;
+  return DML_OK;
+}
+
+int dml_set_box(dml_ctx *c, int32_t rep, const double box[3]) {
+  if (!c || rep < 0 || rep >= c->nrep || !box) return DML_E_ARG;
+  int rc = fetch_params(c);
+  if (rc) return rc;
+  RepParams &p = c->reps[rep].p;
+  for (int k = 0; k < 3; k++) {
+    p.box[k] = box[k];
+    p.one_box[k] = 1.0 / box[k];
+  }
+  int oldcap = p.ccap;
+  if ((rc = tessellate(p, 0))) return rc;
+  int nb = bucket_cap(p);
+  if (ncells_total(p) > oldcap || nb > p.bcap) {
+    p.bcap = std::max(p.bcap, nb);
+    if ((rc = alloc_cells(c))) return rc;
+  } else {
+    p.ccap = oldcap;
+  }
+  if (c->gexec) {
+    hipGraphExecDestroy(c->gexec);
+    hipGraphDestroy(c->graph);
+    c->gexec = nullptr;
+    c->graph = nullptr;
+  }
+  return upload_params(c);
+}
+
+int dml_test_update(dml_ctx *c, int32_t *rebuilt) {
+  if (!c) return DML_E_ARG;
+  enq_test_update(c);
+  int rc = finish(c);
+  if (rc) return rc;
+  if (rebuilt) {
+    if ((rc = fetch_params(c))) return rc;
+    for (int r = 0; r < c->nrep; r++) rebuilt[r] = c->reps[r].p.rebuild;
+  }
+  return DML_OK;
+}
+
+int dml_update(dml_ctx *c) {
+  if (!c) return DML_E_ARG;
+  hipLaunchKernelGGL(k_force_rebuild, dim3((c->nrep + 63) / 64), dim3(64), 0, c->st, c->d);
+  enq_cells_lists(c);
+  return finish(c);
+}
+
+int dml_cells(dml_ctx *c, int32_t rep, int32_t ncells[3], double cell[3], int32_t *use_cells,
+              int32_t *cell_of) {
+  if (!c || rep < 0 || rep >= c->nrep) return DML_E_ARG;
+  int rc = fetch_params(c);
+  if (rc) return rc;
+  const RepParams &p = c->reps[rep].p;
+  for (int k = 0; k < 3; k++) {
+    if (ncells) ncells[k] = p.ncells[k];
+    if (cell) cell[k] = p.cell[k];
+  }
+  if (use_cells) *use_cells = p.use_cells;
+  if (cell_of) {
+    int n = p.nslots;
+    for (int i = 0; i < 3 * n; i++) cell_of[i] = -1;
+    if (!p.use_cells) return DML_OK;
+    int nc = ncells_total(p);
+    std::vector<int32_t> cnt(nc);
+    std::vector<int32_t> bk((size_t)nc * p.bcap);
+    HIPCHECK(hipMemcpy(cnt.data(), c->d.cell_cnt + p.coff, sizeof(int32_t) * nc, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(bk.data(), c->d.bucket + p.boff, sizeof(int32_t) * bk.size(),
+                       hipMemcpyDeviceToHost));
+    int nx = p.ncells[0] + 2, ny = p.ncells[1] + 2;
+    for (int q = 0; q < nc; q++) {
+      int ci = q % nx, cj = (q / nx) % ny, ck = q / (nx * ny);
+      for (int e = 0; e < std::min(cnt[q], p.bcap); e++) {
+        int s = bk[(size_t)q * p.bcap + e] - p.poff;
+        if (s < 0 || s >= n) continue;
+        cell_of[s] = ci;
+        cell_of[n + s] = cj;
+        cell_of[2 * n + s] = ck;
+      }
+    }
+  }
+  return DML_OK;
+}
+
+int dml_neighbors(dml_ctx *c, int32_t rep, int32_t *nn, int64_t *rowptr, int32_t *idx,
+                  int64_t cap, int64_t *total) {
+  if (!c || rep < 0 || rep >= c->nrep) return DML_E_ARG;
+  int rc = fetch_params(c);
+  if (rc) return rc;
+  const RepParams &p = c->reps[rep].p;
+  int n = p.nslots;
+  std::vector<int32_t> hn(std::max(n, 1));
+  std::vector<uint8_t> fl(std::max(n, 1));
+  HIPCHECK(hipMemcpy(hn.data(), c->d.nn + p.poff, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(fl.data(), c->d.flags + p.poff, n, hipMemcpyDeviceToHost));
+  int64_t tot = 0;
+  for (int i = 0; i < n; i++) {
+    if (!(fl[i] & DML_FL_REF)) hn[i] = 0;
+    tot += hn[i];
+  }
+  if (total) *total = tot;
+  if (nn) memcpy(nn, hn.data(), sizeof(int32_t) * n);
+  if (!idx && !rowptr) return DML_OK;
+  int mx = c->d.maxnn;
+  std::vector<int32_t> rows((size_t)n * mx);
+  HIPCHECK(hipMemcpy(rows.data(), c->d.nbr + (int64_t)p.poff * mx, sizeof(int32_t) * rows.size(),
+                     hipMemcpyDeviceToHost));
+  int64_t q = 0;
+  for (int i = 0; i < n; i++) {
+    if (rowptr) rowptr[i] = q;
+    for (int e = 0; e < hn[i]; e++, q++)
+      if (idx && q < cap) idx[q] = rows[(size_t)i * mx + e] - p.poff;
+  }
+  if (rowptr) rowptr[n] = q;
+  return q > cap && idx ? DML_E_ARG : DML_OK;
+}
+
+int dml_forces(dml_ctx *c) {
+  if (!c) return DML_E_ARG;
+  enq_forces(c, 0, 0);
+  return finish(c);
+}
+
+int dml_brownian(dml_ctx *c, const double *noise, const double *u_dep) {
+  if (!c) return DML_E_ARG;
+  double *dn = upload_tmp(c, noise, 3 * c->P), *du = upload_tmp(c, u_dep, c->P);
+  hipLaunchKernelGGL(k_brownian, dim3(c->d.nblocks), dim3(DML_BLOCK), 0, c->st, c->d,
+                     (const double *)dn, (const double *)du);
+  int rc = finish(c);
+  hipFree(dn);
+  hipFree(du);
+  return rc;
+}
+
+int dml_ermak_a(dml_ctx *c, const double *noise, const double *u_dep) {
+  if (!c) return DML_E_ARG;
+  double *dn = upload_tmp(c, noise, 6 * c->P), *du = upload_tmp(c, u_dep, c->P);
+  hipLaunchKernelGGL(k_ermak_a, dim3(c->d.nblocks), dim3(DML_BLOCK), 0, c->st, c->d,
+                     (const double *)dn, (const double *)du);
+  int rc = finish(c);
+  hipFree(dn);
+  hipFree(du);
+  return rc;
+}
+
+int dml_ermak_b(dml_ctx *c) {
+  if (!c) return DML_E_ARG;
+  hipLaunchKernelGGL(k_ermak_b, dim3(c->d.nblocks), dim3(DML_BLOCK), 0, c->st, c->d, 0);
+  return finish(c);
+}
+
+int dml_overlap_moveback(dml_ctx *c, const double *u_contact, int64_t n_u, int32_t max_passes) {
+  if (!c) return DML_E_ARG;
+  // n_u uniforms per replica, replica r at [r*n_u, (r+1)*n_u)
+  if (u_contact && n_u <= 0) return DML_E_ARG;
+  double *du = u_contact ? upload_tmp(c, u_contact, n_u * c->nrep) : nullptr;
+  hipLaunchKernelGGL(k_ov_exact, dim3(c->nrep), dim3(64), 0, c->st, c->d, (const double *)du, n_u,
+                     max_passes > 0 ? max_passes : 100000);
+  int rc = finish(c);
+  hipFree(du);
+  return rc;
+}
+
+int dml_overlap_rounds(dml_ctx *c, const double *u_round, int32_t max_rounds) {
+  if (!c || max_rounds <= 0 || max_rounds > kMaxRounds) return DML_E_ARG;
+  double *du = u_round ? upload_tmp(c, u_round, (int64_t)max_rounds * c->P) : nullptr;
+  enq_overlap_rounds(c, du, c->P, max_rounds, 1);
+  int rc = finish(c);
+  hipFree(du);
+  return rc;
+}
+
+int dml_f_to_cg(dml_ctx *c) {
+  if (!c) return DML_E_ARG;
+  enq_f2cg(c);
+  return finish(c);
+}
+
+int dml_gcmc(dml_ctx *c, int32_t rep, double act, int32_t nadj, const double *u, const double *g) {
+  (void)c; (void)rep; (void)act; (void)nadj; (void)u; (void)g;
+  return DML_E_UNSUPPORTED;
+}
+
+int dml_gcmc_trial_energy(dml_ctx *c, int32_t rep, int32_t ntrial, const double *pos3, double *du,
+                          int32_t *hard) {
+  if (!c || rep < 0 || rep >= c->nrep || ntrial < 0 || !pos3 || !du || !hard) return DML_E_ARG;
+  if (ntrial == 0) return DML_OK;
+  {
+    const RepParams &p = c->reps[rep].p;
+    for (int t = 0; t < ntrial; t++) {
+      const double *q = pos3 + 3 * (int64_t)t;
+      if (!(q[0] >= 0 && q[0] < p.box[0] && q[1] >= 0 && q[1] < p.box[1] && q[2] >= 0 &&
+            q[2] <= p.box[2]))
+        return DML_E_ARG;
+    }
+  }
+  double *dp = upload_tmp(c, pos3, 3 * (int64_t)ntrial);
+  double *de = nullptr;
+  int32_t *dh = nullptr;
+  if (dalloc(&de, ntrial) || dalloc(&dh, ntrial)) return DML_E_NOMEM;
+  int64_t threads = (int64_t)ntrial * 64;
+  hipLaunchKernelGGL(k_gcmc_energy, dim3((unsigned)((threads + DML_BLOCK - 1) / DML_BLOCK)),
+                     dim3(DML_BLOCK), 0, c->st, c->d, rep, ntrial, (const double *)dp, de, dh);
+  int rc = finish(c);
+  if (!rc) {
+    HIPCHECK(hipMemcpy(du, de, sizeof(double) * ntrial, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(hard, dh, sizeof(int32_t) * ntrial, hipMemcpyDeviceToHost));
+  }
+  hipFree(dp);
+  hipFree(de);
+  hipFree(dh);
+  return rc;
+}
+
+int dml_run(dml_ctx *c, int32_t nsteps, int32_t use_graph) {
+  if (!c || nsteps < 0) return DML_E_ARG;
+  int rc = fetch_params(c);
+  if (rc) return rc;
+  refresh_modes(c);
+  int need_init = 0;
+  for (auto &h : c->reps) need_init |= !h.p.listed;
+  if (need_init) {
+    // dana.F90:140-142: test_update, then fuerza for Ermak runs
+    enq_test_update(c);
+    if ((rc = finish(c))) return rc;
+    if (c->any_ermak) {
+      enq_forces(c, 0, 1);
+      if ((rc = finish(c))) return rc;
+    }
+  }
+  if (use_graph && (!c->gexec || c->graph_rounds != c->rounds)) {
+    if (c->gexec) {
+      hipGraphExecDestroy(c->gexec);
+      hipGraphDestroy(c->graph);
+      c->gexec = nullptr;
+    }
+    HIPCHECK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+    enq_step(c);
+    HIPCHECK(hipStreamEndCapture(c->st, &c->graph));
+    HIPCHECK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+    c->graph_rounds = c->rounds;
+  }
+  HIPCHECK(hipEventRecord(c->e0, c->st));
+  for (int q = 0; q < nsteps; q++) {
+    if (use_graph)
+      HIPCHECK(hipGraphLaunch(c->gexec, c->st));
+    else
+      enq_step(c);
+  }
+  HIPCHECK(hipEventRecord(c->e1, c->st));
+  HIPCHECK(hipEventSynchronize(c->e1));
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, c->e0, c->e1);
+  c->last_ms = ms;
+  return finish(c);
+}
+
+int dml_last_run_ms(dml_ctx *c, double *ms) {
+  if (!c || !ms) return DML_E_ARG;
+  *ms = c->last_ms;
+  return DML_OK;
+}
+
+int dml_get_counters(dml_ctx *c, int32_t rep, dml_counters *out) {
+  if (!c || rep < 0 || rep >= c->nrep || !out) return DML_E_ARG;
+  DevCounters h;
+  HIPCHECK(hipMemcpy(&h, c->d.cnt + rep, sizeof(h), hipMemcpyDeviceToHost));
+  out->try_ = (int64_t)h.try_;
+  out->depo = (int64_t)h.depo;
+  out->choques = (int64_t)h.choques;
+  out->choques3 = (int64_t)h.choques3;
+  out->nupd = (int64_t)h.nupd;
+  out->ins = (int64_t)h.ins;
+  out->del = (int64_t)h.del;
+  out->passes = (int64_t)h.passes;
+  return DML_OK;
+}
+
+int dml_list_stats(dml_ctx *c, int32_t rep, double *mean_nn, int64_t *rebuilds, int64_t *steps) {
+  if (!c || rep < 0 || rep >= c->nrep) return DML_E_ARG;
+  DevCounters h;
+  HIPCHECK(hipMemcpy(&h, c->d.cnt + rep, sizeof(h), hipMemcpyDeviceToHost));
+  if (mean_nn) *mean_nn = h.nn_rows ? (double)h.nn_sum / (double)h.nn_rows : 0.0;
+  if (rebuilds) *rebuilds = (int64_t)h.nupd;
+  if (steps) *steps = (int64_t)h.steps;
+  return DML_OK;
+}
+
+}  // extern "C"

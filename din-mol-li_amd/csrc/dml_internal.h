@@ -1,0 +1,207 @@
+// dml_internal.h -- device data layout and shared helpers of libdml.
+//
+// Layout (HBM):  every replica r owns a slot range [poff, poff + pcap) of the
+// global structure-of-arrays particle buffers, pcap a multiple of the
+// 256-thread block, so one block never straddles two replicas and a
+// block -> replica table (blk2rep) resolves the replica with one scalar load.
+// Cells are buckets: replica r owns cells [coff, coff + ccap) (ghost layers
+// included) and every cell a fixed run of `bcap` bucket entries holding the
+// slots of its atoms in cell-chain order.  Neighbour rows are
+// ELL-style, one fixed-capacity row of `maxnn` int32 slots per particle.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/dml.h"
+
+#define DML_BLOCK 256
+
+// species codes (dana.F90:81-84)
+#define SP_LI 1
+#define SP_CG 2
+#define SP_F 3
+
+// error bits collected on the device
+#define ERR_TESS 1
+#define ERR_LIST 2
+#define ERR_DEPZ0 4
+#define ERR_NOPART 8
+#define ERR_OVERLAP 16
+#define ERR_BUCKET 32
+
+struct RepParams {
+  double box[3], one_box[3];
+  double rcut, rcut2, rl2, skin;
+  double zmax, z0, prob, h, z_switch, dif_sc, dif_sei;
+  double cc0, cc1, cc2, sdr, sdv, crv1, crv2, skt;
+  double eps[9], r0[9], mass[3];
+  double cell[3];
+  double rho0, lohi;
+  double act;
+  int32_t ncells[3];
+  int32_t use_cells;
+  int32_t poff, pcap;
+  int32_t coff, ccap;
+  int64_t boff;   // first bucket entry of this replica
+  int32_t bcap;   // bucket entries per cell
+  int32_t nslots; // slots in use (amax)
+  int32_t nref;
+  int32_t listed;
+  int32_t integrator, reservoir, piston, precision, overlap_mode;
+  int32_t step;
+  int32_t nadj;
+  int32_t rebuild;      // set by the skin test of the current test_update
+  int32_t any_ghost;    // some ref row was built from a ghost-layer cell
+  int32_t ov_pending;   // overlap rounds not finished
+  int32_t ov_round;     // rounds done in the current overlap call
+  int32_t any_f;        // F atoms to convert / ref list to compact
+  int32_t dispmax_done; // blocks finished in the dispmax reduction
+  int32_t rho_done;     // blocks finished in calc_rho
+  int32_t rho_count;
+  int32_t max_nn_seen;
+  int32_t nref_new;
+  uint64_t seed;
+  double d1, d2;        // top-2 squared displacements of the last skin test
+};
+
+struct DevCounters {
+  unsigned long long try_, depo, choques, choques3, nupd, ins, del, passes;
+  unsigned long long nn_sum, nn_rows, steps, ov_unfinished;
+};
+
+// ------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw: "Parallel random numbers: as
+// easy as 1, 2, 3", SC'11), counter-based.
+// ------------------------------------------------------------------------
+struct u32x4 {
+  uint32_t x, y, z, w;
+};
+
+__host__ __device__ inline void philox_mulhilo(uint32_t a, uint32_t b,
+                                               uint32_t *hi, uint32_t *lo) {
+  uint64_t p = (uint64_t)a * (uint64_t)b;
+  *hi = (uint32_t)(p >> 32);
+  *lo = (uint32_t)p;
+}
+
+__host__ __device__ inline u32x4 philox4x32_10(u32x4 c, uint32_t k0,
+                                               uint32_t k1) {
+  const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+  const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+  for (int r = 0; r < 10; r++) {
+    uint32_t hi0, lo0, hi1, lo1;
+    philox_mulhilo(M0, c.x, &hi0, &lo0);
+    philox_mulhilo(M1, c.z, &hi1, &lo1);
+    u32x4 n;
+    n.x = hi1 ^ c.y ^ k0;
+    n.y = lo1;
+    n.z = hi0 ^ c.w ^ k1;
+    n.w = lo0;
+    c = n;
+    k0 += W0;
+    k1 += W1;
+  }
+  return c;
+}
+
+// uniform double in (0,1) from two words: 53 bits, centred in their ulp
+__host__ __device__ inline double u01_from(uint32_t a, uint32_t b) {
+  uint64_t m = ((((uint64_t)a) << 32) | b) >> 11;
+  return ((double)m + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+// counter layout: (particle id, step, site << 24 | draw, replica seed hi)
+enum { RS_BROWN = 1, RS_ERMAK = 2, RS_DEPO = 3, RS_CONTACT = 4, RS_GCMC = 5 };
+
+__host__ __device__ inline u32x4 dml_philox(uint64_t seed, uint32_t pid,
+                                            uint32_t step, uint32_t site,
+                                            uint32_t draw) {
+  u32x4 c;
+  c.x = pid;
+  c.y = step;
+  c.z = (site << 24) | (draw & 0xFFFFFFu);
+  c.w = 0x6d6c6921u;  // fixed word: keeps streams of this library distinct
+  return philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+}
+
+// Box-Muller pair from four Philox words
+__device__ inline void gauss2(u32x4 w, double *g0, double *g1) {
+  double u1 = u01_from(w.x, w.y);
+  double u2 = u01_from(w.z, w.w);
+  double r = sqrt(-2.0 * log(u1));
+  double s, c;
+  sincospi(2.0 * u2, &s, &c);
+  *g0 = r * c;
+  *g1 = r * s;
+}
+
+// ------------------------------------------------------------------------
+// geometry exactly as the reference evaluates it
+// ------------------------------------------------------------------------
+__host__ __device__ inline double dot3(double a, double b, double c) {
+  return (a * a + b * b) + c * c;
+}
+
+// vdistance with mic, pbc = (T,T,F) (Groups.F90:995-1016): round half away
+// from zero like idnint
+__device__ inline void vdist_mic(const RepParams &p, double ax, double ay,
+                                 double az, double bx, double by, double bz,
+                                 double *dx, double *dy, double *dz) {
+  double vx = ax - bx, vy = ay - by, vz = az - bz;
+  vx = vx - p.box[0] * round(vx * p.one_box[0]);
+  vy = vy - p.box[1] * round(vy * p.one_box[1]);
+  *dx = vx;
+  *dy = vy;
+  *dz = vz;
+}
+
+// the 27-cell stencil in the reference's order (Cells.F90:28-36)
+__constant__ static const int8_t kMap[27][3] = {
+    {0, 0, 0},   {1, 0, 0},   {1, 1, 0},   {0, 1, 0},  {-1, 1, 0},
+    {1, 0, -1},  {1, 1, -1},  {0, 1, -1},  {-1, 1, -1}, {1, 0, 1},
+    {1, 1, 1},   {0, 1, 1},   {-1, 1, 1},  {0, 0, 1},  {-1, 0, 0},
+    {-1, -1, 0}, {0, -1, 0},  {1, -1, 0},  {-1, 0, 1}, {-1, -1, 1},
+    {0, -1, 1},  {1, -1, 1},  {-1, 0, -1}, {-1, -1, -1}, {0, -1, -1},
+    {1, -1, -1}, {0, 0, -1}};
+
+// cell_pbc with mic (Cells.F90:387-391), C '%' == Fortran mod here
+__device__ inline int cell_wrap(int r, int n) { return ((r - 1 + n) % n) + 1; }
+
+__device__ inline int cell_lin(const RepParams &p, int i, int j, int k) {
+  return i + (p.ncells[0] + 2) * (j + (p.ncells[1] + 2) * k);
+}
+
+struct DevState {
+  // particle SoA
+  double *px, *py, *pz, *vx, *vy, *vz, *ax, *ay, *az, *fx, *fy, *fz, *epot;
+  double *ox, *oy, *oz;   // pos_old
+  double *cx, *cy, *cz;   // old_cg
+  double *rvx, *rvy, *rvz; // ranv of ermak_a
+  int8_t *species;
+  uint8_t *flags;
+  uint8_t *rowflag;       // bit0: row built from a ghost-layer cell
+  int32_t *bkey;          // hs%b slot: cell-chain order key
+  uint32_t *pid;          // stable particle id (Philox counter)
+  int32_t *ref_order, *ref_tmp, *ref_pos;
+  int32_t *nn, *nbr;
+  int32_t maxnn;
+  // cells
+  int32_t *cell_cnt;      // per cell
+  int32_t *bucket;        // slots, bucket-ordered
+  double *bx, *by, *bz;   // positions, bucket-ordered
+  // overlap rounds
+  int32_t *ov_rev;        // round stamp of a revert
+  uint8_t *ov_st;         // bit0 active bit1 dominated bit2 deposit bit3 self-revert
+  // reductions
+  double *part_d;         // 2 per particle block
+  int32_t *part_i;        // per particle block
+  // tables
+  int32_t *blk2rep;
+  RepParams *rp;
+  DevCounters *cnt;
+  int32_t *err;
+  int32_t nblocks;        // particle blocks
+  int32_t nrep;
+};

@@ -1,0 +1,34 @@
+// dml_kernels.h -- kernels of libdml (defined in dml_kernels.hip)
+#pragma once
+#include "dml_internal.h"
+
+__global__ void k_pbc_dispmax(DevState s);
+__global__ void k_force_rebuild(DevState s);
+__global__ void k_zero_cells(DevState s);
+__global__ void k_cell_key(DevState s);
+__global__ void k_cell_sort(DevState s);
+template <int WPB>
+__global__ void k_listbuild(DevState s);
+template <int LPA, int FUSEB>
+__global__ void k_force(DevState s, int only_ermak);
+__global__ void k_ghost_scatter(DevState s, int only_ermak);
+__global__ void k_ermak_b(DevState s, int only_if_ghost);
+__global__ void k_brownian(DevState s, const double *noise, const double *udep);
+__global__ void k_ermak_a(DevState s, const double *noise, const double *udep);
+__global__ void k_ov_exact(DevState s, const double *u, int64_t n_u, int max_passes);
+__global__ void k_ov_begin(DevState s);
+__global__ void k_ov_scan(DevState s, const double *u_round, int64_t u_stride, int round,
+                          int32_t *again);
+__global__ void k_ov_apply(DevState s, int round, const int32_t *again);
+__global__ void k_ov_end(DevState s, int rounds, const int32_t *again, int err_if_pending);
+__global__ void k_zero_i32(int32_t *a, int64_t n);
+__global__ void k_f2cg_mark(DevState s);
+__global__ void k_f2cg_scan(DevState s);
+__global__ void k_f2cg_scatter(DevState s);
+__global__ void k_f2cg_copy(DevState s);
+__global__ void k_f2cg_end(DevState s);
+__global__ void k_calc_rho(DevState s);
+__global__ void k_maxz(DevState s);
+__global__ void k_step_end(DevState s);
+__global__ void k_gcmc_energy(DevState s, int r, int ntrial, const double *pos3, double *du,
+                              int32_t *hard);

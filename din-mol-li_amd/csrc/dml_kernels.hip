@@ -1,0 +1,1063 @@
+// dml_kernels.hip -- HIP kernels (gfx950, wave64) of the din-mol-Li hot path.
+//
+// Each kernel cites the reference routine whose semantics it carries.  All
+// kernels cover every replica of a context in one launch: particle kernels
+// run one thread per slot (or per ref position) and read the replica from
+// the block -> replica table; cell kernels run a 2-D grid (cells x replica).
+#include "dml_kernels.h"
+
+#define WAVE 64
+
+__device__ inline const RepParams &rep_of_block(const DevState &s, int blk,
+                                                int *r) {
+  *r = s.blk2rep[blk];
+  return s.rp[*r];
+}
+
+__device__ inline double atomic_load_d(const double *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline void top2_merge(double &a1, double &a2, double b1, double b2) {
+  double n1 = fmax(a1, b1);
+  double n2 = fmax(fmin(a1, b1), fmax(a2, b2));
+  a1 = n1;
+  a2 = n2;
+}
+
+__device__ inline void block_top2(double &a1, double &a2, double *sh) {
+  for (int off = WAVE / 2; off > 0; off >>= 1) {
+    double b1 = __shfl_xor(a1, off), b2 = __shfl_xor(a2, off);
+    top2_merge(a1, a2, b1, b2);
+  }
+  int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+  if (lane == 0) {
+    sh[2 * w] = a1;
+    sh[2 * w + 1] = a2;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 1; q < (int)(blockDim.x / WAVE); q++) top2_merge(a1, a2, sh[2 * q], sh[2 * q + 1]);
+  }
+  __syncthreads();
+}
+
+// ------------------------------------------------------------------------
+// test_update part 1: do_pbc (Groups.F90:1440-1467) and inq_dispmax
+// (Neighbor.F90:635-666) with the rebuild decision of Neighbor.F90:696-710.
+// The last block of a replica merges the per-block top-2 values.
+// ------------------------------------------------------------------------
+__global__ void __launch_bounds__(DML_BLOCK) k_pbc_dispmax(DevState s) {
+  __shared__ double sh[2 * DML_BLOCK / WAVE];
+  __shared__ int last;
+  int r;
+  const RepParams &P = rep_of_block(s, blockIdx.x, &r);
+  RepParams *Pw = s.rp + r;
+  int slot = blockIdx.x * DML_BLOCK + threadIdx.x;
+  int local = slot - P.poff;
+  double d = -1.0;
+  if (local < P.nslots && (s.flags[slot] & DML_FL_ALIVE)) {
+    double x = s.px[slot], y = s.py[slot];
+    double bx = P.box[0], by = P.box[1];
+    if (x >= bx) {
+      x -= bx; s.px[slot] = x; s.ox[slot] -= bx;
+    } else if (x < 0.0) {
+      x += bx; s.px[slot] = x; s.ox[slot] += bx;
+    }
+    if (y >= by) {
+      y -= by; s.py[slot] = y; s.oy[slot] -= by;
+    } else if (y < 0.0) {
+      y += by; s.py[slot] = y; s.oy[slot] += by;
+    }
+    d = dot3(x - s.ox[slot], y - s.oy[slot], s.pz[slot] - s.oz[slot]);
+  }
+  double a1 = d, a2 = -1.0;
+  block_top2(a1, a2, sh);
+  if (threadIdx.x == 0) {
+    s.part_d[2 * blockIdx.x] = a1;
+    s.part_d[2 * blockIdx.x + 1] = a2;
+    __threadfence();
+    int nb = P.pcap / DML_BLOCK;
+    int ticket = atomicAdd(&Pw->dispmax_done, 1);
+    last = (ticket == nb - 1);
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  int b0 = P.poff / DML_BLOCK, nb = P.pcap / DML_BLOCK;
+  a1 = -1.0;
+  a2 = -1.0;
+  for (int q = threadIdx.x; q < nb; q += blockDim.x)
+    top2_merge(a1, a2, atomic_load_d(&s.part_d[2 * (b0 + q)]),
+               atomic_load_d(&s.part_d[2 * (b0 + q) + 1]));
+  block_top2(a1, a2, sh);
+  if (threadIdx.x == 0) {
+    top2_merge(a1, a2, 1e-16, 1e-16);  // initial values of inq_dispmax
+    Pw->d1 = a1;
+    Pw->d2 = a2;
+    int reb = (!P.listed) || (sqrt(a1) + sqrt(a2) > P.skin);
+    Pw->rebuild = reb;
+    if (reb) {
+      Pw->listed = 1;
+      Pw->any_ghost = 0;
+      atomicAdd(&s.cnt[r].nupd, 1ull);
+    }
+    Pw->dispmax_done = 0;
+  }
+}
+
+__global__ void k_force_rebuild(DevState s) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= s.nrep) return;
+  RepParams *Pw = s.rp + r;
+  Pw->rebuild = 1;
+  Pw->listed = 1;
+  Pw->any_ghost = 0;
+  atomicAdd(&s.cnt[r].nupd, 1ull);
+}
+
+// ------------------------------------------------------------------------
+// cells: cgroup_tessellate/sort (Cells.F90:180-302) as buckets
+// ------------------------------------------------------------------------
+__device__ inline int ncell_total(const RepParams &P) {
+  return P.use_cells ? (P.ncells[0] + 2) * (P.ncells[1] + 2) * (P.ncells[2] + 2) : 1;
+}
+
+// resort: 1 = cells of this replica are rebuilt in this test_update
+__device__ inline int want_cells(const RepParams &P) {
+  return P.rebuild || P.reservoir == DML_GCMC;
+}
+
+__global__ void k_zero_cells(DevState s) {
+  int r = blockIdx.y;
+  const RepParams &P = s.rp[r];
+  if (!want_cells(P)) return;
+  int nc = ncell_total(P);
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x)
+    s.cell_cnt[P.coff + c] = 0;
+}
+
+__global__ void __launch_bounds__(DML_BLOCK) k_cell_key(DevState s) {
+  int r;
+  const RepParams &P = rep_of_block(s, blockIdx.x, &r);
+  if (!want_cells(P)) return;
+  int slot = blockIdx.x * DML_BLOCK + threadIdx.x;
+  int local = slot - P.poff;
+  if (local >= P.nslots) return;
+  uint8_t f = s.flags[slot];
+  if (P.rebuild && (f & DML_FL_LIMBO)) {  // igroup_clean (Groups.F90:1036-1053)
+    s.flags[slot] = f & (uint8_t)~DML_FL_LIMBO;
+    return;
+  }
+  if (!(f & DML_FL_ALIVE)) return;
+  double x = s.px[slot], y = s.py[slot], z = s.pz[slot];
+  if (P.rebuild) {  // update(): pos_old = pos (Neighbor.F90:620-624)
+    s.ox[slot] = x;
+    s.oy[slot] = y;
+    s.oz[slot] = z;
+  }
+  int cell = 0;
+  if (P.use_cells) {
+    int c0 = (int)(x / P.cell[0]) + 1;
+    int c1 = (int)(y / P.cell[1]) + 1;
+    int c2 = (int)(z / P.cell[2]) + 1;
+    if (c0 < 0 || c1 < 0 || c2 < 0 || c0 > P.ncells[0] + 1 || c1 > P.ncells[1] + 1 ||
+        c2 > P.ncells[2] + 1) {
+      atomicOr(s.err, ERR_TESS);
+      return;
+    }
+    cell = cell_lin(P, c0, c1, c2);
+  }
+  int rank = atomicAdd(&s.cell_cnt[P.coff + cell], 1);
+  if (rank >= P.bcap) {
+    atomicOr(s.err, ERR_BUCKET);
+    return;
+  }
+  s.bucket[P.boff + (int64_t)cell * P.bcap + rank] = slot;
+}
+
+// order each bucket like the head/next chain: descending hs%b slot for
+// cells (push-front, Cells.F90:299-300), ascending for the Verlet scan
+// (Neighbor.F90:397); then copy positions into bucket order.
+__global__ void k_cell_sort(DevState s) {
+  int r = blockIdx.y;
+  const RepParams &P = s.rp[r];
+  if (!want_cells(P)) return;
+  int nc = ncell_total(P);
+  int desc = P.use_cells;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
+    int cnt = s.cell_cnt[P.coff + c];
+    if (cnt > P.bcap) cnt = P.bcap;
+    if (cnt == 0) continue;
+    int64_t base = P.boff + (int64_t)c * P.bcap;
+    int32_t *b = s.bucket + base;
+    for (int q = 1; q < cnt; q++) {
+      int v = b[q], kv = s.bkey[v];
+      int w = q - 1;
+      while (w >= 0) {
+        int kw = s.bkey[b[w]];
+        if (desc ? (kw >= kv) : (kw <= kv)) break;
+        b[w + 1] = b[w];
+        w--;
+      }
+      b[w + 1] = v;
+    }
+    for (int q = 0; q < cnt; q++) {
+      int v = b[q];
+      s.bx[base + q] = s.px[v];
+      s.by[base + q] = s.py[v];
+      s.bz[base + q] = s.pz[v];
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
+// neighbour rows: ngroup_cells (Neighbor.F90:465-548) -- one wave per ref
+// atom; candidates of the 27 stencil cells in stencil order, then chain
+// order; strict '<' (':530'); compaction by ballot keeps that order.
+// ngroup_verlet (358-424) when the box has fewer than 4 cells per side.
+// ------------------------------------------------------------------------
+template <int WPB>
+__global__ void __launch_bounds__(WPB *WAVE) k_listbuild(DevState s) {
+  int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+  int t = blockIdx.x * WPB + w;
+  int r = s.blk2rep[t / DML_BLOCK];
+  const RepParams &P = s.rp[r];
+  if (!P.rebuild) return;
+  int local = t - P.poff;
+  if (local >= P.nref) return;
+  int i = s.ref_order[t];
+  double xi = s.px[i], yi = s.py[i], zi = s.pz[i];
+  int ghost = 0;
+  int cnt = 0;
+  int64_t base = 0;
+  if (P.use_cells) {
+    int c0 = (int)(xi / P.cell[0]) + 1;
+    int c1 = (int)(yi / P.cell[1]) + 1;
+    int c2 = (int)(zi / P.cell[2]) + 1;
+    if (c0 < 0 || c1 < 0 || c2 < 0 || c0 > P.ncells[0] + 1 || c1 > P.ncells[1] + 1 ||
+        c2 > P.ncells[2] + 1) {
+      // out of tessellation: k_cell_key has flagged it; leave the row empty
+      if (lane == 0) s.nn[i] = 0;
+      return;
+    }
+    ghost = (c0 < 1 || c1 < 1 || c2 < 1 || c0 > P.ncells[0] || c1 > P.ncells[1] ||
+             c2 > P.ncells[2]);
+    if (lane < 27) {
+      int n0 = cell_wrap(c0 + kMap[lane][0], P.ncells[0]);
+      int n1 = cell_wrap(c1 + kMap[lane][1], P.ncells[1]);
+      int n2 = cell_wrap(c2 + kMap[lane][2], P.ncells[2]);
+      int cell = cell_lin(P, n0, n1, n2);
+      cnt = s.cell_cnt[P.coff + cell];
+      if (cnt > P.bcap) cnt = P.bcap;
+      base = P.boff + (int64_t)cell * P.bcap;
+    }
+  } else if (lane == 0) {
+    cnt = s.cell_cnt[P.coff];
+    if (cnt > P.bcap) cnt = P.bcap;
+    base = P.boff;
+  }
+  // inclusive scan of the 27 counts held by lanes 0..26
+  int incl = cnt;
+  for (int off = 1; off < 32; off <<= 1) {
+    int v = __shfl_up(incl, off);
+    if (lane >= off) incl += v;
+  }
+  const int pre = incl - cnt;  // exclusive prefix of stencil cell `lane`
+  const int total = __shfl(incl, 26);
+  const double rl2 = P.rl2;
+  const int strict = P.use_cells;
+  const int maxnn = s.maxnn;
+  int32_t *row = s.nbr + (int64_t)i * maxnn;
+  int nn = 0;
+  for (int c0 = 0; c0 < total; c0 += WAVE) {
+    int c = c0 + lane;
+    int cc = c < total ? c : total - 1;
+    int lo = 0, hi = 26;  // largest k with pre(k) <= cc
+    for (int it = 0; it < 5; it++) {
+      int mid = (lo + hi + 1) >> 1;
+      int pm = __shfl(pre, mid);
+      if (lo < hi) {
+        if (pm <= cc) lo = mid; else hi = mid - 1;
+      }
+    }
+    int64_t bk = __shfl(base, lo);
+    int pk = __shfl(pre, lo);
+    int hit = 0, j = -1;
+    if (c < total) {
+      int64_t pos = bk + (cc - pk);
+      j = s.bucket[pos];
+      double dx, dy, dz;
+      vdist_mic(P, s.bx[pos], s.by[pos], s.bz[pos], xi, yi, zi, &dx, &dy, &dz);
+      double rd = dot3(dx, dy, dz);
+      hit = (strict ? (rd < rl2) : !(rd > rl2)) && (j != i);
+    }
+    unsigned long long m = __ballot(hit);
+    int pr = __popcll(m & ((1ull << lane) - 1ull));
+    if (hit && nn + pr < maxnn) row[nn + pr] = j;
+    nn += __popcll(m);
+  }
+  if (lane == 0) {
+    if (nn > maxnn) {
+      atomicOr(s.err, ERR_LIST);
+      atomicMax(&s.rp[r].max_nn_seen, nn);
+      nn = maxnn;
+    }
+    s.nn[i] = nn;
+    s.rowflag[i] = (uint8_t)ghost;
+    if (ghost) s.rp[r].any_ghost = 1;
+    atomicAdd(&s.cnt[r].nn_sum, (unsigned long long)nn);
+    atomicAdd(&s.cnt[r].nn_rows, 1ull);
+  }
+}
+
+template __global__ void k_listbuild<4>(DevState s);
+
+// ------------------------------------------------------------------------
+// fuerza (dana.F90:1055-1139) as a gather: the full list plus reaction of
+// the reference counts every ref-ref pair twice, so entry j of row i
+// weighs 2 when j is a ref atom (its row holds i) and 1 otherwise.  Rows
+// built from a ghost-layer cell are found by nobody; their reactions are
+// added by k_ghost_scatter.  Optionally fused with ermak_b (1031-1052).
+// ------------------------------------------------------------------------
+__device__ inline double pow6d(double x) {
+  double x2 = x * x, x4 = x2 * x2;
+  return x2 * x4;
+}
+__device__ inline double pow7d(double x) {
+  double x2 = x * x, x4 = x2 * x2;
+  return (x * x2) * x4;
+}
+__device__ inline float pow6f(float x) {
+  float x2 = x * x, x4 = x2 * x2;
+  return x2 * x4;
+}
+
+__device__ inline void ermak_b_one(const DevState &s, const RepParams &P, int i,
+                                   double fx, double fy, double fz) {
+  if (s.species[i] == SP_CG) return;
+  double m = P.mass[s.species[i] - 1];
+  double vx = s.vx[i], vy = s.vy[i], vz = s.vz[i];
+  double ax = s.ax[i], ay = s.ay[i], az = s.az[i];
+  s.vx[i] = P.cc0 * vx + (P.cc1 - P.cc2) * ax + P.cc2 * fx / m + s.rvx[i];
+  s.vy[i] = P.cc0 * vy + (P.cc1 - P.cc2) * ay + P.cc2 * fy / m + s.rvy[i];
+  s.vz[i] = P.cc0 * vz + (P.cc1 - P.cc2) * az + P.cc2 * fz / m + s.rvz[i];
+  s.ax[i] = fx / m;
+  s.ay[i] = fy / m;
+  s.az[i] = fz / m;
+}
+
+template <int LPA, int FUSEB>
+__global__ void __launch_bounds__(DML_BLOCK) k_force(DevState s, int only_ermak) {
+  int gid = blockIdx.x * DML_BLOCK + threadIdx.x;
+  int t = gid / LPA, sub = gid % LPA;
+  int r = s.blk2rep[t / DML_BLOCK];
+  const RepParams &P = s.rp[r];
+  if (only_ermak && P.integrator != DML_ERMAK) return;
+  const int FP32 = P.precision == DML_FP32_FORCE;
+  int local = t - P.poff;
+  int active = local < P.nref;
+  int i = active ? s.ref_order[t] : 0;
+  double fx = 0.0, fy = 0.0, fz = 0.0, ea = 0.0;
+  if (active) {
+    int k = s.species[i];
+    double xi = s.px[i], yi = s.py[i], zi = s.pz[i];
+    double hbx = P.box[0] * .5, hby = P.box[1] * .5;
+    int ghost = s.rowflag[i] & 1;
+    int nn = s.nn[i];
+    const int32_t *row = s.nbr + (int64_t)i * s.maxnn;
+    for (int e = sub; e < nn; e += LPA) {
+      int j = row[e];
+      uint8_t fj = s.flags[j];
+      if (fj & DML_FL_LIMBO) continue;
+      int m = s.species[j];
+      double vx = xi - s.px[j], vy = yi - s.py[j], vz = zi - s.pz[j];
+      if (vx > hbx) vx = vx - P.box[0];
+      else if (vx < -hbx) vx = vx + P.box[0];
+      if (vy > hby) vy = vy - P.box[1];
+      else if (vy < -hby) vy = vy + P.box[1];
+      if (m == SP_CG && k == SP_CG) continue;
+      double dr2 = dot3(vx, vy, vz);
+      int km = (k - 1) * 3 + (m - 1);
+      double r0 = P.r0[km];
+      if (dr2 > r0 * r0) continue;
+      double eps = P.eps[km];
+      double w = ((fj & DML_FL_REF) && !ghost) ? 2.0 : 1.0;
+      if (FP32) {
+        float drf = sqrtf((float)dr2);
+        float r0f = (float)r0, epsf = (float)eps;
+        float b6 = pow6f(r0f);
+        float inv = 1.0f / drf;
+        float i2 = inv * inv, i6 = i2 * i2 * i2;
+        float b = b6 * i6;
+        float c = epsf * 12.0f * b * inv;
+        float aux = c * (b - 1.0f) * inv;
+        fx += w * (double)(aux * (float)vx);
+        fy += w * (double)(aux * (float)vy);
+        fz += w * (double)(aux * (float)vz);
+        float u = epsf * b * (b - 2.0f) + epsf;
+        ea += w * (double)(u * 0.5f);
+      } else {
+        double dr = sqrt(dr2);
+        double b = pow6d(r0);
+        double c = eps * 12.0 * b;
+        c = c / pow7d(dr);
+        b = b / pow6d(dr);
+        double aux = c * (b - 1);
+        fx += w * (aux * vx / dr);
+        fy += w * (aux * vy / dr);
+        fz += w * (aux * vz / dr);
+        double u = eps * b * (b - 2);
+        u = u + eps;
+        ea += w * (u * .5);
+      }
+    }
+  }
+  for (int off = LPA / 2; off > 0; off >>= 1) {
+    fx += __shfl_xor(fx, off);
+    fy += __shfl_xor(fy, off);
+    fz += __shfl_xor(fz, off);
+    ea += __shfl_xor(ea, off);
+  }
+  if (active && sub == 0) {
+    s.fx[i] = fx;
+    s.fy[i] = fy;
+    s.fz[i] = fz;
+    s.epot[i] = ea;
+    if (FUSEB && !P.any_ghost) ermak_b_one(s, P, i, fx, fy, fz);
+  }
+}
+
+#define INST_FORCE(L)                                              \
+  template __global__ void k_force<L, 0>(DevState s, int only_ermak); \
+  template __global__ void k_force<L, 1>(DevState s, int only_ermak);
+INST_FORCE(1)
+INST_FORCE(2)
+INST_FORCE(4)
+INST_FORCE(8)
+INST_FORCE(16)
+INST_FORCE(32)
+INST_FORCE(64)
+
+// reactions of rows built from ghost-layer cells (nobody lists those atoms)
+__global__ void __launch_bounds__(DML_BLOCK) k_ghost_scatter(DevState s, int only_ermak) {
+  int t = blockIdx.x * DML_BLOCK + threadIdx.x;
+  int r;
+  const RepParams &P = rep_of_block(s, blockIdx.x, &r);
+  if (!P.any_ghost) return;
+  if (only_ermak && P.integrator != DML_ERMAK) return;
+  if (t - P.poff >= P.nref) return;
+  int i = s.ref_order[t];
+  if (!(s.rowflag[i] & 1)) return;
+  int k = s.species[i];
+  double xi = s.px[i], yi = s.py[i], zi = s.pz[i];
+  double hbx = P.box[0] * .5, hby = P.box[1] * .5;
+  const int32_t *row = s.nbr + (int64_t)i * s.maxnn;
+  for (int e = 0; e < s.nn[i]; e++) {
+    int j = row[e];
+    uint8_t fj = s.flags[j];
+    if ((fj & DML_FL_LIMBO) || !(fj & DML_FL_REF)) continue;
+    int m = s.species[j];
+    double vx = xi - s.px[j], vy = yi - s.py[j], vz = zi - s.pz[j];
+    if (vx > hbx) vx = vx - P.box[0];
+    else if (vx < -hbx) vx = vx + P.box[0];
+    if (vy > hby) vy = vy - P.box[1];
+    else if (vy < -hby) vy = vy + P.box[1];
+    if (m == SP_CG && k == SP_CG) continue;
+    double dr2 = dot3(vx, vy, vz);
+    int km = (k - 1) * 3 + (m - 1);
+    double r0 = P.r0[km];
+    if (dr2 > r0 * r0) continue;
+    double eps = P.eps[km];
+    double dr = sqrt(dr2);
+    double b = pow6d(r0);
+    double c = eps * 12.0 * b;
+    c = c / pow7d(dr);
+    b = b / pow6d(dr);
+    double aux = c * (b - 1);
+    atomicAdd(&s.fx[j], -(aux * vx / dr));
+    atomicAdd(&s.fy[j], -(aux * vy / dr));
+    atomicAdd(&s.fz[j], -(aux * vz / dr));
+    double u = eps * b * (b - 2);
+    u = u + eps;
+    atomicAdd(&s.epot[j], u * .5);
+  }
+}
+
+// ermak_b after the ghost reactions (only when some row is a ghost row or
+// when called on its own)
+__global__ void __launch_bounds__(DML_BLOCK) k_ermak_b(DevState s, int only_if_ghost) {
+  int t = blockIdx.x * DML_BLOCK + threadIdx.x;
+  int r;
+  const RepParams &P = rep_of_block(s, blockIdx.x, &r);
+  if (P.integrator != DML_ERMAK) return;
+  if (only_if_ghost && !P.any_ghost) return;
+  if (t - P.poff >= P.nref) return;
+  int i = s.ref_order[t];
+  ermak_b_one(s, P, i, s.fx[i], s.fy[i], s.fz[i]);
+}
+
+// ------------------------------------------------------------------------
+// integrators: cbrownian_hs (dana.F90:798-846), ermak_a (974-1028), both
+// with atom_pbc (1187-1250)
+// ------------------------------------------------------------------------
+__device__ inline int atom_pbc(const DevState &s, const RepParams &P, int r,
+                               int i, int64_t t, double &x, double &y, double &z,
+                               const double *udep) {
+  int depos = 0;
+  double bx = P.box[0], by = P.box[1];
+  if (x > bx) { x = x - bx; s.ox[i] = s.ox[i] - bx; }
+  if (x < 0.0) { x = x + bx; s.ox[i] = s.ox[i] + bx; }
+  if (y > by) { y = y - by; s.oy[i] = s.oy[i] - by; }
+  if (y < 0.0) { y = y + by; s.oy[i] = s.oy[i] + by; }
+  if (z > P.zmax) {
+    if (P.integrator == DML_ERMAK) {
+      z = z - 2.0 * (z - P.zmax);
+      s.vz[i] = -s.vz[i];
+    } else {
+      x = s.cx[i]; y = s.cy[i]; z = s.cz[i];
+    }
+  }
+  if (z <= 0.0) {
+    atomicAdd(&s.cnt[r].try_, 1ull);
+    double ne;
+    if (udep) {
+      ne = udep[t];
+    } else {
+      u32x4 w = dml_philox(P.seed, s.pid[i], (uint32_t)P.step, RS_DEPO, 0);
+      ne = u01_from(w.x, w.y);
+    }
+    if (ne < P.prob) {
+      atomicAdd(&s.cnt[r].depo, 1ull);
+      s.species[i] = SP_F;
+      s.rp[r].any_f = 1;
+      depos = 1;
+    }
+    x = s.cx[i]; y = s.cy[i]; z = s.cz[i];
+  }
+  return depos;
+}
+
+__global__ void __launch_bounds__(DML_BLOCK) k_brownian(DevState s, const double *noise,
+                                                         const double *udep) {
+  int t = blockIdx.x * DML_BLOCK + threadIdx.x;
+  int r;
+  const RepParams &P = rep_of_block(s, blockIdx.x, &r);
+  if (P.integrator != DML_BROWNIAN) return;
+  if (t - P.poff >= P.nref) return;
+  int i = s.ref_order[t];
+  double x = s.px[i], y = s.py[i], z = s.pz[i];
+  s.cx[i] = x; s.cy[i] = y; s.cz[i] = z;
+  double dif = z > P.z_switch ? P.dif_sc : P.dif_sei;
+  double fac1 = sqrt(2.0 * dif * P.h);
+  double g0, g1, g2;
+  if (noise) {
+    g0 = noise[3 * t]; g1 = noise[3 * t + 1]; g2 = noise[3 * t + 2];
+  } else {
+    double g3;
+    gauss2(dml_philox(P.seed, s.pid[i], (uint32_t)P.step, RS_BROWN, 0), &g0, &g1);
+    gauss2(dml_philox(P.seed, s.pid[i], (uint32_t)P.step, RS_BROWN, 1), &g2, &g3);
+  }
+  double nx = x + g0 * fac1, ny = y + g1 * fac1, nz = z + g2 * fac1;
+  s.vx[i] = (nx - x) / P.h;
+  s.vy[i] = (ny - y) / P.h;
+  s.vz[i] = (nz - z) / P.h;
+  int depos = atom_pbc(s, P, r, i, t, nx, ny, nz, udep);
+  s.px[i] = nx; s.py[i] = ny; s.pz[i] = nz;
+  if (!depos) s.flags[i] &= (uint8_t)~DML_FL_SKIP;
+}
+
+__global__ void __launch_bounds__(DML_BLOCK) k_ermak_a(DevState s, const double *noise,
+                                                        const double *udep) {
+  int t = blockIdx.x * DML_BLOCK + threadIdx.x;
+  int r;
+  const RepParams &P = rep_of_block(s, blockIdx.x, &r);
+  if (P.integrator != DML_ERMAK) return;
+  if (t - P.poff >= P.nref) return;
+  int i = s.ref_order[t];
+  double x = s.px[i], y = s.py[i], z = s.pz[i];
+  s.cx[i] = x; s.cy[i] = y; s.cz[i] = z;
+  double g[6];
+  if (noise) {
+    for (int q = 0; q < 6; q++) g[q] = noise[6 * t + q];
+  } else {
+    for (int q = 0; q < 3; q++)
+      gauss2(dml_philox(P.seed, s.pid[i], (uint32_t)P.step, RS_ERMAK, q), &g[2 * q], &g[2 * q + 1]);
+  }
+  double m = P.mass[s.species[i] - 1];
+  double sq = P.skt / sqrt(m);
+  double np[3] = {x, y, z};
+  double rv[3];
+  const double vel[3] = {s.vx[i], s.vy[i], s.vz[i]};
+  const double acc[3] = {s.ax[i], s.ay[i], s.az[i]};
+  for (int k = 0; k < 3; k++) {
+    double r1 = g[2 * k], r2 = g[2 * k + 1];
+    double ranr = sq * P.sdr * r1;
+    np[k] = np[k] + P.cc1 * vel[k] + P.cc2 * P.h * acc[k] + ranr;
+    rv[k] = sq * P.sdv * (P.crv1 * r1 + P.crv2 * r2);
+  }
+  s.rvx[i] = rv[0]; s.rvy[i] = rv[1]; s.rvz[i] = rv[2];
+  int depos = atom_pbc(s, P, r, i, t, np[0], np[1], np[2], udep);
+  s.px[i] = np[0]; s.py[i] = np[1]; s.pz[i] = np[2];
+  if (!depos) s.flags[i] &= (uint8_t)~DML_FL_SKIP;
+}
+
+// ------------------------------------------------------------------------
+// overlap_moveback (dana.F90:849-943)
+// ------------------------------------------------------------------------
+__device__ inline int same_pos(const DevState &s, int i) {
+  return s.px[i] == s.cx[i] && s.py[i] == s.cy[i] && s.pz[i] == s.cz[i];
+}
+
+// exact reference order: one wave per replica walks the ref list in order;
+// rows are scanned 64 entries at a time, hits acted on in list order and
+// re-checked against the current positions (an earlier revert may have
+// moved a repeated entry).
+__global__ void __launch_bounds__(WAVE) k_ov_exact(DevState s, const double *u,
+                                                    int64_t n_u, int max_passes) {
+  int r = blockIdx.x;
+  if (r >= s.nrep) return;
+  const RepParams &P = s.rp[r];
+  if (P.overlap_mode != DML_OVERLAP_EXACT) return;
+  int lane = threadIdx.x;
+  const double *ur = u ? u + (int64_t)r * n_u : nullptr;
+  int64_t ucount = 0;
+  unsigned long long n_try = 0, n_depo = 0, n_ch = 0, n_ch3 = 0, n_pass = 0;
+  int nref = P.nref;
+  int again = 1;
+  for (int pass = 0; again; pass++) {
+    if (pass >= max_passes) {
+      if (lane == 0) atomicOr(s.err, ERR_OVERLAP);
+      break;
+    }
+    again = 0;
+    n_pass++;
+    for (int t0 = 0; t0 < nref; t0 += WAVE) {
+      int tl = t0 + lane;
+      int myi = tl < nref ? s.ref_order[P.poff + tl] : -1;
+      unsigned long long mask =
+          __ballot(myi >= 0 && !(s.flags[myi >= 0 ? myi : 0] & DML_FL_SKIP));
+      while (mask) {
+        int q = __ffsll((long long)mask) - 1;
+        int ii = __shfl(myi, q);
+        int reverted = 0;
+        s.flags[ii] |= DML_FL_SKIP;
+        double xi = s.px[ii], yi = s.py[ii], zi = s.pz[ii];
+        int nn = s.nn[ii];
+        const int32_t *row = s.nbr + (int64_t)ii * s.maxnn;
+        int done = 0;
+        for (int e0 = 0; e0 < nn && !done; e0 += WAVE) {
+          int e = e0 + lane;
+          int j = e < nn ? row[e] : -1;
+          int hit = 0;
+          if (j >= 0 && !(s.flags[j] & DML_FL_LIMBO)) {
+            double dx, dy, dz;
+            vdist_mic(P, xi, yi, zi, s.px[j], s.py[j], s.pz[j], &dx, &dy, &dz);
+            hit = !(dot3(dx, dy, dz) > P.rcut2);
+          }
+          unsigned long long hm = __ballot(hit);
+          while (hm) {
+            int q2 = __ffsll((long long)hm) - 1;
+            hm &= hm - 1ull;
+            int jj = __shfl(j, q2);
+            double dx, dy, dz;
+            vdist_mic(P, xi, yi, zi, s.px[jj], s.py[jj], s.pz[jj], &dx, &dy, &dz);
+            if (dot3(dx, dy, dz) > P.rcut2) continue;
+            if (s.species[jj] == SP_CG) {
+              n_try++;
+              double ne;
+              if (ur) {
+                ne = ucount < n_u ? ur[ucount] : 0.5;
+                if (ucount >= n_u && lane == 0) atomicOr(s.err, ERR_OVERLAP);
+              } else {
+                u32x4 w = dml_philox(P.seed, s.pid[ii], (uint32_t)P.step, RS_CONTACT,
+                                     (uint32_t)ucount);
+                ne = u01_from(w.x, w.y);
+              }
+              ucount++;
+              if (ne < P.prob) {
+                n_depo++;
+                s.species[ii] = SP_F;
+                if (lane == 0) s.rp[r].any_f = 1;
+                if (zi > P.z0 && lane == 0) atomicOr(s.err, ERR_DEPZ0);
+              } else {
+                s.px[ii] = s.cx[ii]; s.py[ii] = s.cy[ii]; s.pz[ii] = s.cz[ii];
+                s.flags[ii] &= (uint8_t)~DML_FL_SKIP;
+                reverted = 1;
+              }
+              done = 1;
+              break;
+            }
+            if (P.piston && same_pos(s, jj) && same_pos(s, ii)) {
+              n_ch3++;
+              continue;
+            }
+            s.px[jj] = s.cx[jj]; s.py[jj] = s.cy[jj]; s.pz[jj] = s.cz[jj];
+            s.vx[jj] = 0.0; s.vy[jj] = 0.0; s.vz[jj] = 0.0;
+            s.ax[jj] = 0.0; s.ay[jj] = 0.0; s.az[jj] = 0.0;
+            s.flags[jj] &= (uint8_t)~DML_FL_SKIP;
+            n_ch++;
+            again = 1;
+            reverted = 1;
+          }
+        }
+        unsigned long long above = (q == 63) ? 0ull : (~0ull << (q + 1));
+        if (reverted) {
+          int fl = myi >= 0 ? s.flags[myi] : DML_FL_SKIP;
+          mask = __ballot(myi >= 0 && !(fl & DML_FL_SKIP)) & above;
+        } else {
+          mask &= above;
+        }
+      }
+    }
+  }
+  if (lane == 0) {
+    atomicAdd(&s.cnt[r].try_, n_try);
+    atomicAdd(&s.cnt[r].depo, n_depo);
+    atomicAdd(&s.cnt[r].choques, n_ch);
+    atomicAdd(&s.cnt[r].choques3, n_ch3);
+    atomicAdd(&s.cnt[r].passes, n_pass);
+  }
+}
+
+// deterministic parallel rounds (this build's schedule): see orc.c
+// orc_overlap_rounds for the specification the kernels follow.
+__global__ void k_ov_begin(DevState s) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= s.nrep) return;
+  RepParams *Pw = s.rp + r;
+  Pw->ov_pending = 1;
+  Pw->ov_round = 0;
+}
+
+__global__ void __launch_bounds__(DML_BLOCK) k_ov_scan(DevState s, const double *u_round,
+                                                        int64_t u_stride, int round,
+                                                        int32_t *again) {
+  int t = blockIdx.x * DML_BLOCK + threadIdx.x;
+  int r;
+  const RepParams &P = rep_of_block(s, blockIdx.x, &r);
+  if (P.overlap_mode != DML_OVERLAP_ROUNDS) return;
+  if (round > 0 && !again[(round - 1) * s.nrep + r]) return;
+  int local = t - P.poff;
+  if (local >= P.nref) return;
+  int i = s.ref_order[t];
+  uint8_t fi = s.flags[i];
+  if (fi & DML_FL_SKIP) {
+    s.ov_st[i] = 0;
+    return;
+  }
+  double xi = s.px[i], yi = s.py[i], zi = s.pz[i];
+  int nn = s.nn[i];
+  const int32_t *row = s.nbr + (int64_t)i * s.maxnn;
+  int dom = 0;
+  for (int e = 0; e < nn; e++) {
+    int j = row[e];
+    uint8_t fj = s.flags[j];
+    if ((fj & DML_FL_LIMBO) || !(fj & DML_FL_REF) || (fj & DML_FL_SKIP)) continue;
+    if (s.ref_pos[j] >= local) continue;
+    double dx, dy, dz;
+    vdist_mic(P, xi, yi, zi, s.px[j], s.py[j], s.pz[j], &dx, &dy, &dz);
+    if (dot3(dx, dy, dz) <= P.rcut2) {
+      dom = 1;
+      break;
+    }
+  }
+  uint8_t st = 1;
+  int any = dom;
+  unsigned long long n_ch = 0, n_ch3 = 0;
+  if (dom) {
+    st |= 2;
+  } else {
+    int stamp = P.ov_round + round + 1;
+    for (int e = 0; e < nn; e++) {
+      int j = row[e];
+      if (s.flags[j] & DML_FL_LIMBO) continue;
+      double dx, dy, dz;
+      vdist_mic(P, xi, yi, zi, s.px[j], s.py[j], s.pz[j], &dx, &dy, &dz);
+      if (dot3(dx, dy, dz) > P.rcut2) continue;
+      if (s.species[j] == SP_CG) {
+        atomicAdd(&s.cnt[r].try_, 1ull);
+        double ne;
+        if (u_round) {
+          ne = u_round[(int64_t)round * u_stride + i];
+        } else {
+          u32x4 w = dml_philox(P.seed, s.pid[i], (uint32_t)P.step, RS_CONTACT,
+                               (uint32_t)(1000 + round));
+          ne = u01_from(w.x, w.y);
+        }
+        st |= (ne < P.prob) ? 4 : 8;
+        break;
+      }
+      if (P.piston && same_pos(s, j) && same_pos(s, i)) {
+        n_ch3++;
+        continue;
+      }
+      s.ov_rev[j] = stamp;
+      n_ch++;
+      any = 1;
+    }
+  }
+  s.ov_st[i] = st;
+  if (n_ch) atomicAdd(&s.cnt[r].choques, n_ch);
+  if (n_ch3) atomicAdd(&s.cnt[r].choques3, n_ch3);
+  if (any) again[round * s.nrep + r] = 1;
+}
+
+__global__ void __launch_bounds__(DML_BLOCK) k_ov_apply(DevState s, int round,
+                                                         const int32_t *again) {
+  int t = blockIdx.x * DML_BLOCK + threadIdx.x;
+  int r;
+  const RepParams &P = rep_of_block(s, blockIdx.x, &r);
+  if (P.overlap_mode != DML_OVERLAP_ROUNDS) return;
+  if (round > 0 && !again[(round - 1) * s.nrep + r]) return;
+  if (t - P.poff >= P.nref) return;
+  int i = s.ref_order[t];
+  uint8_t st = s.ov_st[i];
+  uint8_t f = s.flags[i];
+  if ((st & 1) && !(st & 2)) f |= DML_FL_SKIP;
+  if (st & 4) {
+    atomicAdd(&s.cnt[r].depo, 1ull);
+    s.species[i] = SP_F;
+    s.rp[r].any_f = 1;
+    if (s.pz[i] > P.z0) atomicOr(s.err, ERR_DEPZ0);
+  }
+  if (st & 8) {
+    s.px[i] = s.cx[i]; s.py[i] = s.cy[i]; s.pz[i] = s.cz[i];
+    f &= (uint8_t)~DML_FL_SKIP;
+  }
+  if (s.ov_rev[i] == P.ov_round + round + 1) {
+    s.px[i] = s.cx[i]; s.py[i] = s.cy[i]; s.pz[i] = s.cz[i];
+    s.vx[i] = 0.0; s.vy[i] = 0.0; s.vz[i] = 0.0;
+    s.ax[i] = 0.0; s.ay[i] = 0.0; s.az[i] = 0.0;
+    f &= (uint8_t)~DML_FL_SKIP;
+  }
+  s.flags[i] = f;
+  s.ov_st[i] = 0;
+}
+
+// after the rounds: advance the stamp base, flag an unfinished replica
+__global__ void k_ov_end(DevState s, int rounds, const int32_t *again, int err_if_pending) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= s.nrep) return;
+  RepParams *Pw = s.rp + r;
+  if (Pw->overlap_mode != DML_OVERLAP_ROUNDS) return;
+  int pending = again[(rounds - 1) * s.nrep + r];
+  if (pending) {
+    if (err_if_pending) atomicOr(s.err, ERR_OVERLAP);
+    atomicAdd(&s.cnt[r].ov_unfinished, 1ull);
+  }
+  Pw->ov_pending = pending;
+  Pw->ov_round += rounds + 1;
+  atomicAdd(&s.cnt[r].passes, 1ull);
+}
+
+__global__ void k_zero_i32(int32_t *a, int64_t n) {
+  int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (q < n) a[q] = 0;
+}
+
+// ------------------------------------------------------------------------
+// F -> CG at step end and removal from hs%ref (dana.F90:228-236): stable
+// compaction of each replica's ref order
+// ------------------------------------------------------------------------
+__global__ void __launch_bounds__(DML_BLOCK) k_f2cg_mark(DevState s) {
+  __shared__ int cnt;
+  int t = blockIdx.x * DML_BLOCK + threadIdx.x;
+  int r;
+  const RepParams &P = rep_of_block(s, blockIdx.x, &r);
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  if (P.any_f && t - P.poff < P.nref) {
+    int i = s.ref_order[t];
+    int keep = 1;
+    if (s.species[i] == SP_F) {
+      s.species[i] = SP_CG;
+      s.flags[i] &= (uint8_t)~(DML_FL_REF | DML_FL_GCMC);
+      s.ref_pos[i] = -1;
+      keep = 0;
+    }
+    if (keep) atomicAdd(&cnt, 1);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) s.part_i[blockIdx.x] = cnt;
+}
+
+// one wave per replica: exclusive scan of the block counts
+__global__ void k_f2cg_scan(DevState s) {
+  int r = blockIdx.x;
+  RepParams *Pw = s.rp + r;
+  if (!Pw->any_f) return;
+  int lane = threadIdx.x;
+  int b0 = Pw->poff / DML_BLOCK, nb = Pw->pcap / DML_BLOCK;
+  int carry = 0;
+  for (int q0 = 0; q0 < nb; q0 += WAVE) {
+    int q = q0 + lane;
+    int v = q < nb ? s.part_i[b0 + q] : 0;
+    int incl = v;
+    for (int off = 1; off < WAVE; off <<= 1) {
+      int x = __shfl_up(incl, off);
+      if (lane >= off) incl += x;
+    }
+    if (q < nb) s.part_i[b0 + q] = carry + incl - v;
+    carry += __shfl(incl, WAVE - 1);
+  }
+  if (lane == 0) Pw->nref_new = carry;
+}
+
+__global__ void __launch_bounds__(DML_BLOCK) k_f2cg_scatter(DevState s) {
+  __shared__ int wsum[DML_BLOCK / WAVE];
+  int t = blockIdx.x * DML_BLOCK + threadIdx.x;
+  int r;
+  const RepParams &P = rep_of_block(s, blockIdx.x, &r);
+  if (!P.any_f) return;
+  int local = t - P.poff;
+  int i = local < P.nref ? s.ref_order[t] : -1;
+  int keep = i >= 0 && (s.flags[i] & DML_FL_REF);
+  unsigned long long m = __ballot(keep);
+  int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
+  if (lane == 0) wsum[w] = __popcll(m);
+  __syncthreads();
+  int off = s.part_i[blockIdx.x];
+  for (int q = 0; q < w; q++) off += wsum[q];
+  if (keep) s.ref_tmp[P.poff + off + __popcll(m & ((1ull << lane) - 1ull))] = i;
+}
+
+__global__ void __launch_bounds__(DML_BLOCK) k_f2cg_copy(DevState s) {
+  int t = blockIdx.x * DML_BLOCK + threadIdx.x;
+  int r;
+  const RepParams &P = rep_of_block(s, blockIdx.x, &r);
+  if (!P.any_f) return;
+  int nnew = P.nref_new;
+  int local = t - P.poff;
+  if (local < nnew) {
+    int i = s.ref_tmp[t];
+    s.ref_order[t] = i;
+    s.ref_pos[i] = local;
+  }
+}
+
+__global__ void k_f2cg_end(DevState s) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= s.nrep) return;
+  RepParams *Pw = s.rp + r;
+  if (!Pw->any_f) return;
+  Pw->nref = Pw->nref_new;
+  Pw->any_f = 0;
+}
+
+// ------------------------------------------------------------------------
+// piston: calc_rho (dana.F90:521-549) and maxz (776-794)
+// ------------------------------------------------------------------------
+__global__ void __launch_bounds__(DML_BLOCK) k_calc_rho(DevState s) {
+  __shared__ int cnt;
+  __shared__ int last;
+  int slot = blockIdx.x * DML_BLOCK + threadIdx.x;
+  int r;
+  const RepParams &P = rep_of_block(s, blockIdx.x, &r);
+  if (!P.piston) return;
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  int local = slot - P.poff;
+  if (local < P.nslots && (s.flags[slot] & DML_FL_ALIVE)) {
+    double z = s.pz[slot];
+    if (z > P.z0 && z < P.zmax) atomicAdd(&cnt, 1);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    RepParams *Pw = s.rp + r;
+    atomicAdd(&Pw->rho_count, cnt);
+    __threadfence();
+    int ticket = atomicAdd(&Pw->rho_done, 1);
+    last = ticket == P.pcap / DML_BLOCK - 1;
+    if (last) {
+      __threadfence();
+      int g = __hip_atomic_load(&Pw->rho_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      double vol = P.box[0] * P.box[1] * (P.zmax - P.z0);
+      double rho = (double)g / vol;
+      const double tau = 0.1;
+      Pw->lohi = ((P.h / tau) * ((P.rho0 - rho) / rho));
+      Pw->rho_count = 0;
+      Pw->rho_done = 0;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(DML_BLOCK) k_maxz(DevState s) {
+  int slot = blockIdx.x * DML_BLOCK + threadIdx.x;
+  int r;
+  const RepParams &P = rep_of_block(s, blockIdx.x, &r);
+  if (!P.piston) return;
+  int local = slot - P.poff;
+  if (local < P.nslots && (s.flags[slot] & DML_FL_ALIVE)) {
+    double z = s.pz[slot];
+    if (z > P.z0) s.pz[slot] = z - P.lohi * (z - P.z0);
+  }
+}
+
+// step end: advance the per-replica step counter (and the piston zmax,
+// which no kernel of k_maxz reads)
+__global__ void k_step_end(DevState s) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= s.nrep) return;
+  RepParams *Pw = s.rp + r;
+  if (Pw->piston) Pw->zmax = Pw->zmax - Pw->lohi * (Pw->zmax - Pw->z0);
+  Pw->step += 1;
+  s.cnt[r].steps += 1;
+}
+
+// ------------------------------------------------------------------------
+// GCMC (dana.F90:590-713)
+// ------------------------------------------------------------------------
+// batched insertion-trial energies: one wave per trial over the 27 cells
+// around the trial point (cells as sorted by the last test_update)
+__global__ void __launch_bounds__(DML_BLOCK) k_gcmc_energy(DevState s, int r, int ntrial,
+                                                            const double *pos3, double *du,
+                                                            int32_t *hard) {
+  int w = (blockIdx.x * DML_BLOCK + threadIdx.x) / WAVE, lane = threadIdx.x % WAVE;
+  if (w >= ntrial) return;
+  const RepParams &P = s.rp[r];
+  double x = pos3[3 * w], y = pos3[3 * w + 1], z = pos3[3 * w + 2];
+  double e = 0.0;
+  int h = 0;
+  int nstencil = P.use_cells ? 27 : 1;
+  int c0 = (int)(x / P.cell[0]) + 1, c1 = (int)(y / P.cell[1]) + 1, c2 = (int)(z / P.cell[2]) + 1;
+  if (P.use_cells && (c0 < 0 || c1 < 0 || c2 < 0 || c0 > P.ncells[0] + 1 ||
+                      c1 > P.ncells[1] + 1 || c2 > P.ncells[2] + 1)) {
+    if (lane == 0) {
+      du[w] = 0.0;
+      hard[w] = -1;
+    }
+    return;
+  }
+  for (int k = 0; k < nstencil; k++) {
+    int cell = 0;
+    if (P.use_cells) {
+      int n0 = cell_wrap(c0 + kMap[k][0], P.ncells[0]);
+      int n1 = cell_wrap(c1 + kMap[k][1], P.ncells[1]);
+      int n2 = cell_wrap(c2 + kMap[k][2], P.ncells[2]);
+      cell = cell_lin(P, n0, n1, n2);
+    }
+    int cnt = s.cell_cnt[P.coff + cell];
+    if (cnt > P.bcap) cnt = P.bcap;
+    int64_t base = P.boff + (int64_t)cell * P.bcap;
+    for (int q = lane; q < cnt; q += WAVE) {
+      int j = s.bucket[base + q];
+      double dx, dy, dz;
+      vdist_mic(P, x, y, z, s.px[j], s.py[j], s.pz[j], &dx, &dy, &dz);
+      double dr2 = dot3(dx, dy, dz);
+      if ((s.flags[j] & DML_FL_GCMC) && dr2 < P.rcut * P.rcut) h = 1;
+      int m = s.species[j];
+      double r0 = P.r0[m - 1], eps = P.eps[m - 1];
+      if (dr2 > r0 * r0) continue;
+      double dr = sqrt(dr2);
+      double b = pow6d(r0) / pow6d(dr);
+      e += eps * b * (b - 2) + eps;
+    }
+  }
+  for (int off = WAVE / 2; off > 0; off >>= 1) e += __shfl_xor(e, off);
+  unsigned long long hm = __ballot(h);
+  if (lane == 0) {
+    du[w] = e;
+    hard[w] = hm != 0;
+  }
+}

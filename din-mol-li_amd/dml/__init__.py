@@ -1,0 +1,316 @@
+"""Python binding of libdml (the HIP implementation of the din-mol-Li hot path).
+
+This module is a thin ctypes layer over the C ABI of include/dml.h, mirroring
+the reference's operator sequence (test_update, fuerza, cbrownian_hs,
+ermak_a/ermak_b, overlap_moveback, gcmc_run).  It loads the in-tree
+libdml.so and raises if that library is missing: there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdml.so")
+
+BROWNIAN, ERMAK = 0, 1
+PISTON, CHUNKS, GCMC, NO_RESERVOIR = 0, 1, 2, 3
+FP64, FP32_FORCE = 0, 1
+OVERLAP_EXACT, OVERLAP_ROUNDS = 0, 1
+FL_ALIVE, FL_LIMBO, FL_REF, FL_SKIP, FL_GCMC = 1, 2, 4, 8, 16
+
+STATUS = {0: "ok", -1: "bad argument", -2: "HIP runtime error", -3: "out of device memory",
+          -4: "particle out of tessellation", -5: "neighbour row capacity exceeded",
+          -6: "deposit above z0", -7: "chosen particle does not exist",
+          -8: "overlap resolution did not finish", -9: "bad state", -10: "unsupported"}
+
+# every symbol include/dml.h declares
+EXPORTS = ["dml_create", "dml_create_ensemble", "dml_destroy", "dml_sync", "dml_strerror",
+           "dml_num_replicas", "dml_total_slots", "dml_set_state", "dml_set_order",
+           "dml_get_state", "dml_get_ref_order", "dml_set_zmax", "dml_get_zmax", "dml_set_box",
+           "dml_test_update", "dml_update", "dml_cells", "dml_neighbors", "dml_forces",
+           "dml_brownian", "dml_ermak_a", "dml_ermak_b", "dml_ref_offset",
+           "dml_overlap_moveback", "dml_overlap_rounds", "dml_f_to_cg", "dml_gcmc",
+           "dml_gcmc_trial_energy", "dml_run", "dml_get_counters", "dml_list_stats",
+           "dml_last_run_ms"]
+
+
+class DmlError(RuntimeError):
+    def __init__(self, status, where):
+        self.status = status
+        super().__init__("%s: %s (%d)" % (where, STATUS.get(status, "?"), status))
+
+
+class Config(C.Structure):
+    _fields_ = [("box", C.c_double * 3), ("rcut", C.c_double), ("skin", C.c_double),
+                ("eps", C.c_double * 9), ("r0", C.c_double * 9), ("mass", C.c_double * 3),
+                ("h", C.c_double), ("gamma", C.c_double), ("temp", C.c_double),
+                ("dif_sc", C.c_double), ("dif_sei", C.c_double), ("z_switch", C.c_double),
+                ("prob", C.c_double), ("z0", C.c_double), ("zmax", C.c_double),
+                ("integrator", C.c_int32), ("reservoir", C.c_int32), ("precision", C.c_int32),
+                ("overlap_mode", C.c_int32), ("capacity", C.c_int32),
+                ("max_neighbors", C.c_int32), ("seed", C.c_uint64), ("act", C.c_double),
+                ("nadj", C.c_int32), ("lanes_per_atom", C.c_int32)]
+
+
+class Counters(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in ("try_", "depo", "choques", "choques3", "nupd",
+                                         "ins", "del_", "passes")]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libdml.so not built (%s); run __graft_entry__.build()" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.dml_create_ensemble.argtypes = [C.c_int32, C.POINTER(Config), C.POINTER(vp)]
+        L.dml_create.argtypes = [C.POINTER(Config), C.POINTER(vp)]
+        L.dml_destroy.argtypes = [vp]
+        L.dml_strerror.restype = C.c_char_p
+        L.dml_total_slots.restype = C.c_int64
+        L.dml_total_slots.argtypes = [vp]
+        L.dml_ref_offset.restype = C.c_int64
+        L.dml_ref_offset.argtypes = [vp, C.c_int32]
+        L.dml_set_state.argtypes = [vp, C.c_int32, C.c_int32] + [vp] * 7
+        L.dml_set_order.argtypes = [vp, C.c_int32, C.c_int32, vp, vp]
+        L.dml_get_state.argtypes = [vp, C.c_int32, C.c_int32] + [vp] * 9
+        L.dml_get_ref_order.argtypes = [vp, C.c_int32, vp, vp]
+        L.dml_set_zmax.argtypes = [vp, C.c_int32, C.c_double, C.c_double]
+        L.dml_get_zmax.argtypes = [vp, C.c_int32, vp]
+        L.dml_set_box.argtypes = [vp, C.c_int32, vp]
+        L.dml_test_update.argtypes = [vp, vp]
+        L.dml_update.argtypes = [vp]
+        L.dml_cells.argtypes = [vp, C.c_int32, vp, vp, vp, vp]
+        L.dml_neighbors.argtypes = [vp, C.c_int32, vp, vp, vp, C.c_int64, vp]
+        L.dml_forces.argtypes = [vp]
+        L.dml_brownian.argtypes = [vp, vp, vp]
+        L.dml_ermak_a.argtypes = [vp, vp, vp]
+        L.dml_ermak_b.argtypes = [vp]
+        L.dml_overlap_moveback.argtypes = [vp, vp, C.c_int64, C.c_int32]
+        L.dml_overlap_rounds.argtypes = [vp, vp, C.c_int32]
+        L.dml_f_to_cg.argtypes = [vp]
+        L.dml_gcmc.argtypes = [vp, C.c_int32, C.c_double, C.c_int32, vp, vp]
+        L.dml_gcmc_trial_energy.argtypes = [vp, C.c_int32, C.c_int32, vp, vp, vp]
+        L.dml_run.argtypes = [vp, C.c_int32, C.c_int32]
+        L.dml_get_counters.argtypes = [vp, C.c_int32, C.POINTER(Counters)]
+        L.dml_list_stats.argtypes = [vp, C.c_int32, vp, vp, vp]
+        L.dml_last_run_ms.argtypes = [vp, vp]
+        L.dml_sync.argtypes = [vp]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _chk(st, where):
+    if st != 0:
+        raise DmlError(st, where)
+
+
+def dana_config(box, skin, capacity, h=1e-2, prob=1.0, z0=100.0, zmax=200.0, dif_sc=250.0,
+                dif_sei=250.0, integrator=ERMAK, reservoir=PISTON, precision=FP64,
+                overlap_mode=OVERLAP_EXACT, seed=104012, rcut=3.2, max_neighbors=0, act=0.0,
+                nadj=0, lanes_per_atom=0):
+    """A Config with dana's constants (dana.F90:12-17, 86-100, 113)."""
+    c = Config()
+    c.box[:] = list(box)
+    c.rcut, c.skin = rcut, skin
+    eps = np.zeros((3, 3)); r0 = np.zeros((3, 3))
+    r0[1, 0] = r0[0, 1] = 3.5
+    eps[0, 0] = 2313.6; r0[0, 0] = 3.2
+    eps[2, 2] = 121.0; r0[2, 2] = 3.61
+    eps[0, 2] = eps[2, 0] = 529.1
+    r0[0, 2] = r0[2, 0] = 1.564
+    c.eps[:] = list(eps.ravel()); c.r0[:] = list(r0.ravel())
+    c.mass[:] = [6.94, 6.94, 6.94]
+    c.h, c.gamma, c.temp = h, 1.0, 300.0
+    c.dif_sc, c.dif_sei, c.z_switch = dif_sc, dif_sei, 80.0
+    c.prob, c.z0, c.zmax = prob, z0, zmax
+    c.integrator, c.reservoir, c.precision, c.overlap_mode = integrator, reservoir, precision, overlap_mode
+    c.capacity = capacity
+    c.max_neighbors = max_neighbors
+    c.seed = seed
+    c.act, c.nadj, c.lanes_per_atom = act, nadj, lanes_per_atom
+    return c
+
+
+class Context:
+    """One libdml context (one or more replicas on the current GPU)."""
+
+    def __init__(self, configs):
+        if isinstance(configs, Config):
+            configs = [configs]
+        self.L = lib()
+        arr = (Config * len(configs))(*configs)
+        h = C.c_void_p()
+        _chk(self.L.dml_create_ensemble(len(configs), arr, C.byref(h)), "dml_create_ensemble")
+        self.h = h
+        self.nrep = len(configs)
+        self.n = [0] * self.nrep
+        self.total = self.L.dml_total_slots(self.h)
+
+    def close(self):
+        if self.h:
+            self.L.dml_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def ref_offset(self, rep=0):
+        return self.L.dml_ref_offset(self.h, rep)
+
+    def set_state(self, pos, species=None, vel=None, acel=None, pos_old=None, old_cg=None,
+                  flags=None, rep=0):
+        pos = np.asarray(pos, np.float64)
+        n = len(pos)
+
+        def soa(a):
+            return None if a is None else np.ascontiguousarray(np.asarray(a, np.float64).T.reshape(-1))
+        arrs = [soa(pos), soa(vel), soa(acel), soa(pos_old), soa(old_cg),
+                None if species is None else np.ascontiguousarray(species, np.int8),
+                None if flags is None else np.ascontiguousarray(flags, np.uint8)]
+        _chk(self.L.dml_set_state(self.h, rep, n, *[_p(a) for a in arrs]), "dml_set_state")
+        self.n[rep] = n
+
+    def set_order(self, ref_order=None, bkey=None, rep=0):
+        ro = None if ref_order is None else np.ascontiguousarray(ref_order, np.int32)
+        bk = None if bkey is None else np.ascontiguousarray(bkey, np.int32)
+        _chk(self.L.dml_set_order(self.h, rep, 0 if ro is None else len(ro), _p(ro), _p(bk)),
+             "dml_set_order")
+
+    def state(self, rep=0):
+        n = self.n[rep]
+        d = {k: np.zeros(3 * n) for k in ("pos", "vel", "acel", "force", "pos_old", "old_cg")}
+        d["epot"] = np.zeros(n)
+        d["species"] = np.zeros(n, np.int8)
+        d["flags"] = np.zeros(n, np.uint8)
+        _chk(self.L.dml_get_state(self.h, rep, n, _p(d["pos"]), _p(d["vel"]), _p(d["acel"]),
+                                  _p(d["force"]), _p(d["epot"]), _p(d["pos_old"]),
+                                  _p(d["old_cg"]), _p(d["species"]), _p(d["flags"])),
+             "dml_get_state")
+        for k in ("pos", "vel", "acel", "force", "pos_old", "old_cg"):
+            d[k] = d[k].reshape(3, n).T.copy()
+        return d
+
+    def ref_order(self, rep=0):
+        nref = C.c_int32()
+        _chk(self.L.dml_get_ref_order(self.h, rep, C.byref(nref), None), "dml_get_ref_order")
+        out = np.zeros(max(nref.value, 1), np.int32)
+        _chk(self.L.dml_get_ref_order(self.h, rep, C.byref(nref), _p(out)), "dml_get_ref_order")
+        return out[:nref.value]
+
+    def set_zmax(self, zmax, z0, rep=0):
+        _chk(self.L.dml_set_zmax(self.h, rep, zmax, z0), "dml_set_zmax")
+
+    def zmax(self, rep=0):
+        z = C.c_double()
+        _chk(self.L.dml_get_zmax(self.h, rep, C.byref(z)), "dml_get_zmax")
+        return z.value
+
+    def set_box(self, box, rep=0):
+        b = np.ascontiguousarray(box, np.float64)
+        _chk(self.L.dml_set_box(self.h, rep, _p(b)), "dml_set_box")
+
+    def test_update(self):
+        reb = np.zeros(self.nrep, np.int32)
+        _chk(self.L.dml_test_update(self.h, _p(reb)), "dml_test_update")
+        return reb
+
+    def update(self):
+        _chk(self.L.dml_update(self.h), "dml_update")
+
+    def cells(self, rep=0):
+        nc = np.zeros(3, np.int32); cell = np.zeros(3); uc = C.c_int32()
+        n = self.n[rep]
+        ci = np.zeros(3 * max(n, 1), np.int32)
+        _chk(self.L.dml_cells(self.h, rep, _p(nc), _p(cell), C.byref(uc), _p(ci)), "dml_cells")
+        return nc, cell, bool(uc.value), ci[:3 * n].reshape(3, n).T.copy()
+
+    def lists(self, rep=0):
+        n = self.n[rep]
+        tot = C.c_int64()
+        nn = np.zeros(max(n, 1), np.int32)
+        _chk(self.L.dml_neighbors(self.h, rep, _p(nn), None, None, 0, C.byref(tot)), "dml_neighbors")
+        rp = np.zeros(n + 1, np.int64)
+        idx = np.zeros(max(tot.value, 1), np.int32)
+        _chk(self.L.dml_neighbors(self.h, rep, _p(nn), _p(rp), _p(idx), tot.value, C.byref(tot)),
+             "dml_neighbors")
+        return nn[:n], rp, idx[:tot.value]
+
+    def forces(self):
+        _chk(self.L.dml_forces(self.h), "dml_forces")
+
+    def _padded(self, a, per):
+        if a is None:
+            return None
+        a = np.asarray(a, np.float64).reshape(-1)
+        out = np.zeros(per * self.total)
+        out[:len(a)] = a
+        return out
+
+    def brownian(self, noise=None, u_dep=None):
+        """noise/u_dep are indexed by global ref position (replica 0 starts at 0)."""
+        nz, ud = self._padded(noise, 3), self._padded(u_dep, 1)
+        _chk(self.L.dml_brownian(self.h, _p(nz), _p(ud)), "dml_brownian")
+
+    def ermak_a(self, noise=None, u_dep=None):
+        nz, ud = self._padded(noise, 6), self._padded(u_dep, 1)
+        _chk(self.L.dml_ermak_a(self.h, _p(nz), _p(ud)), "dml_ermak_a")
+
+    def ermak_b(self):
+        _chk(self.L.dml_ermak_b(self.h), "dml_ermak_b")
+
+    def overlap(self, u_contact=None, n_u=0, max_passes=100000):
+        """u_contact: array (nrep, n_u) of uniforms consumed in contact order."""
+        u = None
+        if u_contact is not None:
+            u = np.ascontiguousarray(np.asarray(u_contact, np.float64).reshape(self.nrep, -1))
+            n_u = u.shape[1]
+        _chk(self.L.dml_overlap_moveback(self.h, _p(u), n_u, max_passes), "dml_overlap_moveback")
+
+    def overlap_rounds(self, u_round=None, max_rounds=8):
+        """u_round: array (max_rounds, total_slots) or None."""
+        u = None if u_round is None else np.ascontiguousarray(u_round, np.float64)
+        _chk(self.L.dml_overlap_rounds(self.h, _p(u), max_rounds), "dml_overlap_rounds")
+
+    def f_to_cg(self):
+        _chk(self.L.dml_f_to_cg(self.h), "dml_f_to_cg")
+
+    def gcmc_trial_energy(self, pos3, rep=0):
+        p = np.ascontiguousarray(pos3, np.float64)
+        t = len(p)
+        du = np.zeros(max(t, 1)); hard = np.zeros(max(t, 1), np.int32)
+        _chk(self.L.dml_gcmc_trial_energy(self.h, rep, t, _p(p), _p(du), _p(hard)),
+             "dml_gcmc_trial_energy")
+        return du[:t], hard[:t]
+
+    def run(self, nsteps, graph=True):
+        _chk(self.L.dml_run(self.h, nsteps, int(graph)), "dml_run")
+        ms = C.c_double()
+        self.L.dml_last_run_ms(self.h, C.byref(ms))
+        return ms.value
+
+    def counters(self, rep=0):
+        c = Counters()
+        _chk(self.L.dml_get_counters(self.h, rep, C.byref(c)), "dml_get_counters")
+        return {k: getattr(c, k) for k, _ in Counters._fields_}
+
+    def list_stats(self, rep=0):
+        m = C.c_double(); rb = C.c_int64(); st = C.c_int64()
+        _chk(self.L.dml_list_stats(self.h, rep, C.byref(m), C.byref(rb), C.byref(st)),
+             "dml_list_stats")
+        return m.value, rb.value, st.value
+
+    def sync(self):
+        _chk(self.L.dml_sync(self.h), "dml_sync")

@@ -1,0 +1,407 @@
+"""Parity of the HIP path (through the C ABI) with the oracle and with the
+reference-built cell/list fixtures, on identical inputs with injected noise.
+
+Bars (per north_star):
+* cells and neighbour lists: bit-exact, entry for entry (list order too);
+* FP64 forces, energies and one-step positions: relative error <= 1e-12
+  (norm with a floor of 1e-3 * rms so near-cutoff zeros do not divide by 0);
+* FP32-force mode: force error <= 1e-5 with the same norm;
+* integer/flag state (species, skip, ref order): exact.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+G = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def dml():
+    import dml as _dml
+    return _dml
+
+
+def rel_err(a, b):
+    """max_i |a_i - b_i| / max(|b_i|, 1e-3 * rms |b|) over rows"""
+    a = np.atleast_2d(a); b = np.atleast_2d(b)
+    na = np.linalg.norm(a - b, axis=-1)
+    nb = np.linalg.norm(b, axis=-1)
+    rms = np.sqrt(np.mean(nb ** 2)) if nb.size else 0.0
+    floor = max(1e-3 * rms, 1e-300)
+    return float(np.max(na / np.maximum(nb, floor))) if na.size else 0.0
+
+
+def rsa(rng, n, lo, hi, dmin, box):
+    pts = []
+    tries = 0
+    while len(pts) < n and tries < 100000:
+        tries += 1
+        p = lo + rng.random(3) * (hi - lo)
+        if pts:
+            d = np.array(pts) - p
+            d[:, :2] -= box[:2] * np.round(d[:, :2] / box[:2])
+            if np.min((d * d).sum(1)) < dmin * dmin:
+                continue
+        pts.append(p)
+    return np.array(pts)
+
+
+def electrode(a, nx, ny, nl, ztop):
+    pts = []
+    for l in range(nl):
+        off = 0.5 * (l % 2)
+        for i in range(nx):
+            for j in range(ny):
+                pts.append([(i + off) * a, (j + off) * a, ztop - l * a / 2])
+    return np.array(pts)
+
+
+def pair(orc, dml, box, skin, pos, z, **kw):
+    """an oracle system and a device context holding the same atoms"""
+    ermak = kw.get("ermak", True)
+    piston = kw.get("piston", False)
+    okw = dict(h=kw.get("h", 1e-2), prob=kw.get("prob", 1.0), z0=kw.get("z0", 100.0),
+               zmax=kw.get("zmax", box[2]), dif_sc=kw.get("dif_sc", 250.0),
+               dif_sei=kw.get("dif_sei", 250.0), ermak=ermak, piston=piston)
+    s = orc.System(orc.dana_params(box, skin, **okw), pos, z)
+    cfg = dml.dana_config(box, skin, len(pos) + 64, h=okw["h"], prob=okw["prob"], z0=okw["z0"],
+                          zmax=okw["zmax"], dif_sc=okw["dif_sc"], dif_sei=okw["dif_sei"],
+                          integrator=dml.ERMAK if ermak else dml.BROWNIAN,
+                          reservoir=dml.PISTON if piston else dml.NO_RESERVOIR,
+                          precision=kw.get("precision", dml.FP64),
+                          overlap_mode=kw.get("overlap_mode", dml.OVERLAP_EXACT),
+                          lanes_per_atom=kw.get("lpa", 0))
+    c = dml.Context(cfg)
+    c.set_state(pos, species=z)
+    return s, c
+
+
+def c1_system(seed=3, n_ion=600, skin=10.0):
+    rng = np.random.default_rng(seed)
+    box = np.array([50.0, 50.0, 100.0])
+    el = electrode(3.51, 14, 14, 2, 1.755)
+    ions = rsa(rng, n_ion, np.array([0, 0, 4.0]), np.array([50, 50, 99.0]), 2.6, box)
+    pos = np.vstack([el, ions])
+    z = np.array([2] * len(el) + [1] * len(ions), np.int8)
+    return box, skin, pos, z
+
+
+def assert_lists_equal(c, s):
+    nn_d, rp_d, idx_d = c.lists()
+    nn_o, rp_o, idx_o = s.lists()
+    np.testing.assert_array_equal(nn_d, nn_o)
+    np.testing.assert_array_equal(idx_d, idx_o)
+
+
+NB = np.load(os.path.join(G, "nb_cases.npz"))
+
+
+@pytest.mark.parametrize("name", list(NB["names"]))
+def test_cells_and_lists_equal_reference_modules(dml, name):
+    d = {k.split("/", 1)[1]: NB[k] for k in NB.files if k.startswith(name + "/")}
+    say("case", name, len(d["z"]))
+    rc, skin = d["rc_skin"]
+    cfg = dml.dana_config(d["box"], skin, len(d["z"]) + 64, rcut=rc, integrator=dml.BROWNIAN,
+                          reservoir=dml.NO_RESERVOIR)
+    c = dml.Context(cfg)
+    c.set_state(d["pos"], species=d["z"])
+    reb = c.test_update()
+    assert reb[0] == 1
+    ncells, cell, use_cells, ci = c.cells()
+    assert use_cells == bool(d["tess"])
+    np.testing.assert_array_equal(ncells, d["ncells"])
+    if use_cells:
+        np.testing.assert_array_equal(cell, d["cell"])
+        np.testing.assert_array_equal(ci, d["ci"])
+    nn, rp, idx = c.lists()
+    for r, row in enumerate(d["rows"]):
+        np.testing.assert_array_equal(idx[rp[row]:rp[row + 1]], d["idx"][d["rowptr"][r]:d["rowptr"][r + 1]])
+    assert int(nn.sum()) == len(d["idx"])
+    c.close()
+
+
+@pytest.mark.parametrize("lpa", [1, 8, 64])
+def test_forces_fp64_match_oracle(orc, dml, lpa):
+    box, skin, pos, z = c1_system()
+    # a few F atoms (deposited this step) interact with Li and F
+    z = z.copy()
+    ions = np.nonzero(z == 1)[0]
+    z[ions[::37]] = 3
+    say("forces fp64 lpa", lpa, len(pos))
+    s, c = pair(orc, dml, box, skin, pos, z, lpa=lpa)
+    s.test_update(); c.test_update()
+    assert_lists_equal(c, s)
+    s.fuerza(); c.forces()
+    so, sd = s.state(), c.state()
+    ref = so["species"] != 2
+    assert rel_err(sd["force"][ref], so["force"][ref]) <= 1e-12
+    assert rel_err(sd["epot"][ref, None], so["epot"][ref, None]) <= 1e-12
+    assert np.abs(so["force"][ref]).max() > 1.0  # the system has real forces
+    c.close()
+
+
+def test_forces_ghost_layer_rows(orc, dml):
+    """atoms above box(3): their rows are found by nobody; the reaction on
+    their neighbours comes from the ghost scatter."""
+    rng = np.random.default_rng(9)
+    box = np.array([60.0, 60.0, 60.0])
+    pos = rsa(rng, 500, np.zeros(3), np.array([60, 60, 70.0]), 2.6, box)
+    z = np.ones(len(pos), np.int8)
+    say("ghost forces", len(pos))
+    s, c = pair(orc, dml, box, 10.0, pos, z, zmax=80.0)
+    s.test_update(); c.test_update()
+    assert_lists_equal(c, s)
+    _, _, _, ci = s.cells()
+    assert (ci[:, 2] == 5).any() or (ci[:, 2] > 4).any()
+    s.fuerza(); c.forces()
+    so, sd = s.state(), c.state()
+    assert rel_err(sd["force"], so["force"]) <= 1e-12
+    assert rel_err(sd["epot"][:, None], so["epot"][:, None]) <= 1e-12
+    c.close()
+
+
+def test_forces_fp32_error_bound(orc, dml):
+    box, skin, pos, z = c1_system(seed=4)
+    say("forces fp32", len(pos))
+    s, c = pair(orc, dml, box, skin, pos, z, precision=dml.FP32_FORCE)
+    s.test_update(); c.test_update()
+    s.fuerza(); c.forces()
+    so, sd = s.state(), c.state()
+    ref = so["species"] != 2
+    err = rel_err(sd["force"][ref], so["force"][ref])
+    say("fp32 force rel err", err)
+    assert err <= 1e-5
+    assert rel_err(sd["epot"][ref, None], so["epot"][ref, None]) <= 1e-5
+    c.close()
+
+
+def _noise(rng, nref, k):
+    return rng.standard_normal(nref * k)
+
+
+def test_brownian_step_matches_oracle(orc, dml):
+    rng = np.random.default_rng(11)
+    box = np.array([50.0, 50.0, 100.0])
+    pos = rsa(rng, 400, np.array([0, 0, 0.5]), np.array([50, 50, 99.5]), 3.2, box)
+    z = np.ones(len(pos), np.int8)
+    say("brownian", len(pos))
+    s, c = pair(orc, dml, box, 2.0, pos, z, ermak=False, zmax=100.0, dif_sei=100.0, prob=0.5)
+    nz = _noise(rng, len(pos), 3)
+    ud = rng.random(len(pos))
+    assert s.brownian(nz, ud) == 0
+    c.brownian(nz, ud)
+    so, sd = s.state(), c.state()
+    assert rel_err(sd["pos"], so["pos"]) <= 1e-12
+    assert rel_err(sd["vel"], so["vel"]) <= 1e-12
+    np.testing.assert_array_equal(sd["old_cg"], so["old_cg"])
+    np.testing.assert_array_equal(sd["species"], so["species"])
+    np.testing.assert_array_equal(sd["flags"], so["flags"])
+    assert (so["species"] == 3).any()  # some electrode deposits happened
+    c.close()
+
+
+def test_ermak_step_matches_oracle(orc, dml):
+    box, skin, pos, z = c1_system(seed=5)
+    rng = np.random.default_rng(12)
+    say("ermak a/b", len(pos))
+    s, c = pair(orc, dml, box, skin, pos, z, zmax=100.0, prob=0.7)
+    vel = rng.standard_normal((len(pos), 3)) * 5
+    acel = rng.standard_normal((len(pos), 3)) * 50
+    s.set_state(vel=vel, acel=acel); c.set_state(pos, species=z, vel=vel, acel=acel)
+    s.test_update(); c.test_update()
+    s.fuerza(); c.forces()
+    nref = s.nref
+    nz = _noise(rng, nref, 6)
+    ud = rng.random(nref)
+    assert s.ermak_a(nz, ud) == 0
+    c.ermak_a(nz, ud)
+    so, sd = s.state(), c.state()
+    assert rel_err(sd["pos"], so["pos"]) <= 1e-12
+    np.testing.assert_array_equal(sd["species"], so["species"])
+    s.fuerza(); c.forces()
+    s.ermak_b(); c.ermak_b()
+    so, sd = s.state(), c.state()
+    ref = so["species"] != 2
+    assert rel_err(sd["vel"][ref], so["vel"][ref]) <= 1e-12
+    assert rel_err(sd["acel"][ref], so["acel"][ref]) <= 1e-12
+    c.close()
+
+
+def _step_pair(orc, dml, s, c, rng, mode, nsteps, brown=True):
+    """integrator + test_update + overlap + test_update + F->CG, on both"""
+    for k in range(nsteps):
+        nref = s.nref
+        if brown:
+            nz = _noise(rng, nref, 3)
+            ud = rng.random(nref)
+            assert s.brownian(nz, ud) == 0
+            c.brownian(nz, ud)
+        else:
+            nz = _noise(rng, nref, 6)
+            ud = rng.random(nref)
+            assert s.ermak_a(nz, ud) == 0
+            c.ermak_a(nz, ud)
+            s.fuerza(); c.forces()
+            s.ermak_b(); c.ermak_b()
+        st, r1 = s.test_update()
+        assert st == 0
+        r1d = c.test_update()[0]
+        assert r1 == r1d, k
+        if mode == "exact":
+            u = rng.random(4096)
+            st, used = s.overlap(u)
+            assert st == 0
+            c.overlap(u)
+        else:
+            u = rng.random((8, c.total))
+            assert s.overlap_rounds(u[:, :s.n].reshape(-1), 8) == 0
+            c.overlap_rounds(u, 8)
+        st, r2 = s.test_update()
+        assert st == 0
+        assert r2 == c.test_update()[0], k
+        s.f_to_cg(); c.f_to_cg()
+        so, sd = s.state(), c.state()
+        np.testing.assert_array_equal(sd["species"], so["species"], err_msg="step %d" % k)
+        np.testing.assert_array_equal(sd["flags"], so["flags"], err_msg="step %d" % k)
+        np.testing.assert_array_equal(c.ref_order(), s.ref_order())
+        assert rel_err(sd["pos"], so["pos"]) <= 1e-12, k
+    return so, sd
+
+
+@pytest.mark.parametrize("mode", ["exact", "rounds"])
+def test_brownian_steps_with_overlap_match_oracle(orc, dml, mode):
+    """several whole steps: the exact resolver follows the reference order
+    bit for bit; the round schedule follows its own oracle."""
+    rng = np.random.default_rng(21)
+    box = np.array([40.0, 40.0, 60.0])
+    el = electrode(3.51, 11, 11, 2, 1.755)
+    ions = rsa(rng, 700, np.array([0, 0, 5.0]), np.array([40, 40, 59.0]), 3.2, box)
+    pos = np.vstack([el, ions])
+    z = np.array([2] * len(el) + [1] * len(ions), np.int8)
+    say("steps", mode, len(pos))
+    s, c = pair(orc, dml, box, 2.0, pos, z, ermak=False, zmax=60.0, z0=30.0,
+                overlap_mode=dml.OVERLAP_EXACT if mode == "exact" else dml.OVERLAP_ROUNDS)
+    s.test_update(); c.test_update()
+    so, sd = _step_pair(orc, dml, s, c, rng, mode, 12)
+    cnt_o, cnt_d = s.c, c.counters()
+    assert cnt_o.choques == cnt_d["choques"]
+    assert cnt_o.depo == cnt_d["depo"]
+    assert cnt_o.choques > 0
+    c.close()
+
+
+def test_ermak_piston_steps_match_oracle(orc, dml):
+    box, skin, pos, z = c1_system(seed=7, n_ion=300)
+    rng = np.random.default_rng(22)
+    say("ermak piston steps", len(pos))
+    s, c = pair(orc, dml, box, skin, pos, z, zmax=100.0, z0=50.0, piston=True, prob=0.8)
+    s.test_update(); c.test_update()
+    s.fuerza(); c.forces()
+    so, sd = _step_pair(orc, dml, s, c, rng, "exact", 6, brown=False)
+    assert rel_err(sd["vel"][so["species"] != 2], so["vel"][so["species"] != 2]) <= 1e-12
+    c.close()
+
+
+def test_gcmc_trial_energy_matches_oracle(orc, dml):
+    rng = np.random.default_rng(31)
+    box = np.array([60.0, 60.0, 60.0])
+    pos = rsa(rng, 900, np.zeros(3), box, 2.8, box)
+    z = np.where(rng.random(len(pos)) < 0.2, 2, 1).astype(np.int8)
+    say("gcmc energy", len(pos))
+    s = orc.System(orc.dana_params(box, 2.0, ermak=False, piston=False), pos, z)
+    cfg = dml.dana_config(box, 2.0, len(pos) + 64, integrator=dml.BROWNIAN, reservoir=dml.GCMC)
+    c = dml.Context(cfg)
+    flags = (1 | np.where(z != 2, 4, 0) | np.where(z == 1, 16, 0)).astype(np.uint8)
+    c.set_state(pos, species=z, flags=flags)
+    s.test_update(); c.test_update()
+    # the oracle's gcmc membership: set by flags
+    fo = s.state()["flags"]
+    s.set_state(flags=flags)
+    trials = rng.random((2000, 3)) * box
+    du_d, hard_d = c.gcmc_trial_energy(trials)
+    # oracle over all atoms with gcmc membership from the flags
+    import ctypes
+    so = s.state()
+    d = trials[:, None, :] - pos[None, :, :]
+    d[..., :2] -= box[:2] * np.round(d[..., :2] / box[:2])
+    r2 = (d * d).sum(-1)
+    hard_o = ((r2 < 3.2 * 3.2) & (z == 1)[None, :]).any(1)
+    np.testing.assert_array_equal(hard_d.astype(bool), hard_o)
+    r0 = np.where(z == 1, 3.2, 3.5); eps = np.where(z == 1, 2313.6, 0.0)
+    inside = r2 <= r0 ** 2
+    rr = np.sqrt(np.where(inside, r2, 1.0))
+    b = (r0 / rr) ** 6
+    e = np.where(inside, eps * b * (b - 2) + eps, 0.0).sum(1)
+    np.testing.assert_allclose(du_d, e, rtol=1e-10, atol=1e-9)
+    c.close()
+
+
+def test_run_graph_equals_eager_and_conserves(dml):
+    """Philox steps: graph replays give the same bits as eager launches; atom
+    count and species bookkeeping hold; every step advances."""
+    rng = np.random.default_rng(41)
+    box = np.array([60.0, 60.0, 80.0])
+    el = electrode(3.51, 17, 17, 2, 1.755)
+    ions = rsa(rng, 1500, np.array([0, 0, 5.0]), np.array([60, 60, 79.0]), 3.2, box)
+    pos = np.vstack([el, ions])
+    z = np.array([2] * len(el) + [1] * len(ions), np.int8)
+    outs = []
+    for graph in (False, True):
+        for ermak in (False, True):
+            say("run graph", graph, "ermak", ermak)
+            cfg = dml.dana_config(box, 2.0, len(pos) + 64, integrator=dml.ERMAK if ermak else dml.BROWNIAN,
+                                  reservoir=dml.NO_RESERVOIR, overlap_mode=dml.OVERLAP_ROUNDS,
+                                  zmax=80.0, z0=40.0, seed=77)
+            c = dml.Context(cfg)
+            c.set_state(pos, species=z)
+            c.run(20, graph=graph)
+            st = c.state()
+            outs.append(st)
+            assert np.isfinite(st["pos"]).all()
+            assert (st["pos"][:, :2] >= 0).all() and (st["pos"][:, :2] < box[:2] + 1e-9).all()
+            assert (st["pos"][:, 2] > 0).all()
+            assert set(np.unique(st["species"])) <= {1, 2}
+            nm, reb, steps = c.list_stats()
+            assert steps == 20 and reb > 0
+            c.close()
+    for a, b in ((outs[0], outs[2]), (outs[1], outs[3])):
+        np.testing.assert_array_equal(a["pos"], b["pos"])
+        np.testing.assert_array_equal(a["species"], b["species"])
+
+
+def test_ensemble_replicas_match_single_runs(dml):
+    """replicas of different size and integrator in one context give the same
+    bits as each replica run alone (noise is keyed per replica seed)."""
+    rng = np.random.default_rng(51)
+    systems = []
+    for k, (n, L) in enumerate(((300, 40.0), (900, 60.0), (500, 50.0))):
+        box = np.array([L, L, 60.0])
+        pos = rsa(rng, n, np.array([0, 0, 3.0]), np.array([L, L, 59.0]), 3.2, box)
+        systems.append((box, pos))
+    cfgs = [dml.dana_config(b, 2.0, len(p) + 64, integrator=dml.ERMAK if k % 2 else dml.BROWNIAN,
+                            reservoir=dml.NO_RESERVOIR, overlap_mode=dml.OVERLAP_ROUNDS,
+                            zmax=60.0, z0=30.0, seed=1000 + k,
+                            precision=dml.FP32_FORCE if k == 1 else dml.FP64)
+            for k, (b, p) in enumerate(systems)]
+    say("ensemble of", len(cfgs))
+    ens = dml.Context(cfgs)
+    for k, (b, p) in enumerate(systems):
+        ens.set_state(p, rep=k)
+    ens.run(15, graph=True)
+    for k, (b, p) in enumerate(systems):
+        one = dml.Context(cfgs[k])
+        one.set_state(p)
+        one.run(15, graph=True)
+        a, o = ens.state(rep=k), one.state()
+        np.testing.assert_array_equal(a["pos"], o["pos"])
+        np.testing.assert_array_equal(a["species"], o["species"])
+        one.close()
+    ens.close()
