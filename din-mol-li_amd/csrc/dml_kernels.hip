@@ -384,19 +384,20 @@ __global__ void __launch_bounds__(DML_BLOCK) k_force(DevState s, int only_ermak)
       double eps = P.eps[km];
       double w = ((fj & DML_FL_REF) && !ghost) ? 2.0 : 1.0;
       if (FP32) {
-        float drf = sqrtf((float)dr2);
-        float r0f = (float)r0, epsf = (float)eps;
-        float b6 = pow6f(r0f);
-        float inv = 1.0f / drf;
-        float i2 = inv * inv, i6 = i2 * i2 * i2;
-        float b = b6 * i6;
-        float c = epsf * 12.0f * b * inv;
-        float aux = c * (b - 1.0f) * inv;
+        // FP32 pair arithmetic without cancellation: r0^2 - r^2 comes from
+        // the FP64 cutoff test, then b - 1 = (r0^6 - r^6)/r^6 and
+        // u = eps (b - 1)^2 are formed in FP32; sums stay FP64.
+        float q = (float)(r0 * r0 - dr2);
+        float s2 = (float)dr2, a2 = (float)(r0 * r0);
+        float s6 = s2 * s2 * s2;
+        float bm1 = q * (a2 * a2 + a2 * s2 + s2 * s2) / s6;
+        float b = 1.0f + bm1;
+        float epsf = (float)eps;
+        float aux = 12.0f * epsf * b * bm1 / s2;  // |F| / r
         fx += w * (double)(aux * (float)vx);
         fy += w * (double)(aux * (float)vy);
         fz += w * (double)(aux * (float)vz);
-        float u = epsf * b * (b - 2.0f) + epsf;
-        ea += w * (double)(u * 0.5f);
+        ea += w * (double)(0.5f * epsf * bm1 * bm1);
       } else {
         double dr = sqrt(dr2);
         double b = pow6d(r0);

@@ -190,7 +190,8 @@ def _noise(rng, nref, k):
 def test_brownian_step_matches_oracle(orc, dml):
     rng = np.random.default_rng(11)
     box = np.array([50.0, 50.0, 100.0])
-    pos = rsa(rng, 400, np.array([0, 0, 0.5]), np.array([50, 50, 99.5]), 3.2, box)
+    low = rsa(rng, 60, np.array([0, 0, 0.2]), np.array([50, 50, 1.0]), 3.2, box)
+    pos = np.vstack([low, rsa(rng, 340, np.array([0, 0, 4.5]), np.array([50, 50, 99.5]), 3.2, box)])
     z = np.ones(len(pos), np.int8)
     say("brownian", len(pos))
     s, c = pair(orc, dml, box, 2.0, pos, z, ermak=False, zmax=100.0, dif_sei=100.0, prob=0.5)
@@ -261,9 +262,9 @@ def _step_pair(orc, dml, s, c, rng, mode, nsteps, brown=True):
             assert st == 0
             c.overlap(u)
         else:
-            u = rng.random((8, c.total))
-            assert s.overlap_rounds(u[:, :s.n].reshape(-1), 8) == 0
-            c.overlap_rounds(u, 8)
+            u = rng.random((16, c.total))
+            assert s.overlap_rounds(u[:, :s.n].reshape(-1), 16) == 0
+            c.overlap_rounds(u, 16)
         st, r2 = s.test_update()
         assert st == 0
         assert r2 == c.test_update()[0], k
@@ -280,14 +281,16 @@ def _step_pair(orc, dml, s, c, rng, mode, nsteps, brown=True):
 def test_brownian_steps_with_overlap_match_oracle(orc, dml, mode):
     """several whole steps: the exact resolver follows the reference order
     bit for bit; the round schedule follows its own oracle."""
+    # h and the skin are chosen like the reference's own runs (skin 10 with
+    # steps of ~1 A): moved-back atoms then stay inside the list radius
     rng = np.random.default_rng(21)
     box = np.array([40.0, 40.0, 60.0])
-    el = electrode(3.51, 11, 11, 2, 1.755)
-    ions = rsa(rng, 700, np.array([0, 0, 5.0]), np.array([40, 40, 59.0]), 3.2, box)
+    el = electrode(3.51, 11, 11, 2, 3.51)
+    ions = rsa(rng, 700, np.array([0, 0, 6.0]), np.array([40, 40, 59.0]), 3.2, box)
     pos = np.vstack([el, ions])
     z = np.array([2] * len(el) + [1] * len(ions), np.int8)
     say("steps", mode, len(pos))
-    s, c = pair(orc, dml, box, 2.0, pos, z, ermak=False, zmax=60.0, z0=30.0,
+    s, c = pair(orc, dml, box, 10.0, pos, z, ermak=False, zmax=60.0, z0=30.0, h=1e-3,
                 overlap_mode=dml.OVERLAP_EXACT if mode == "exact" else dml.OVERLAP_ROUNDS)
     s.test_update(); c.test_update()
     so, sd = _step_pair(orc, dml, s, c, rng, mode, 12)
@@ -349,17 +352,17 @@ def test_run_graph_equals_eager_and_conserves(dml):
     count and species bookkeeping hold; every step advances."""
     rng = np.random.default_rng(41)
     box = np.array([60.0, 60.0, 80.0])
-    el = electrode(3.51, 17, 17, 2, 1.755)
-    ions = rsa(rng, 1500, np.array([0, 0, 5.0]), np.array([60, 60, 79.0]), 3.2, box)
+    el = electrode(3.51, 17, 17, 2, 3.51)
+    ions = rsa(rng, 1500, np.array([0, 0, 6.0]), np.array([60, 60, 79.0]), 3.2, box)
     pos = np.vstack([el, ions])
     z = np.array([2] * len(el) + [1] * len(ions), np.int8)
     outs = []
     for graph in (False, True):
         for ermak in (False, True):
             say("run graph", graph, "ermak", ermak)
-            cfg = dml.dana_config(box, 2.0, len(pos) + 64, integrator=dml.ERMAK if ermak else dml.BROWNIAN,
+            cfg = dml.dana_config(box, 10.0, len(pos) + 64, integrator=dml.ERMAK if ermak else dml.BROWNIAN,
                                   reservoir=dml.NO_RESERVOIR, overlap_mode=dml.OVERLAP_ROUNDS,
-                                  zmax=80.0, z0=40.0, seed=77)
+                                  zmax=80.0, z0=40.0, seed=77, h=1e-3)
             c = dml.Context(cfg)
             c.set_state(pos, species=z)
             c.run(20, graph=graph)
@@ -386,9 +389,9 @@ def test_ensemble_replicas_match_single_runs(dml):
         box = np.array([L, L, 60.0])
         pos = rsa(rng, n, np.array([0, 0, 3.0]), np.array([L, L, 59.0]), 3.2, box)
         systems.append((box, pos))
-    cfgs = [dml.dana_config(b, 2.0, len(p) + 64, integrator=dml.ERMAK if k % 2 else dml.BROWNIAN,
+    cfgs = [dml.dana_config(b, 10.0, len(p) + 64, integrator=dml.ERMAK if k % 2 else dml.BROWNIAN,
                             reservoir=dml.NO_RESERVOIR, overlap_mode=dml.OVERLAP_ROUNDS,
-                            zmax=60.0, z0=30.0, seed=1000 + k,
+                            zmax=60.0, z0=30.0, seed=1000 + k, h=1e-3,
                             precision=dml.FP32_FORCE if k == 1 else dml.FP64)
             for k, (b, p) in enumerate(systems)]
     say("ensemble of", len(cfgs))
