@@ -1,0 +1,223 @@
+#!/usr/bin/env python
+"""Throughput of the din-mol-Li hot path on one MI355X.
+
+Legs (synthetic, seeded systems; see din-mol-li_amd/dml/workloads.py):
+  plain    C2: one 100,000-particle system, FP64 forces, Ermak integrator,
+           kernels launched one by one (no graph).
+  batched  C3: 256 replicas, sizes log-uniform in 2k..50k, FP32 pair forces
+           with FP64 sums and positions, Brownian and Ermak alternating,
+           every phase one launch over all replicas, one hipGraph per step.
+value = batched particle-steps/s (particles of all replicas x steps / time),
+timed with HIP events around exactly --steps steps after --warmup steps.
+Also reported: the plain rate and the ratio, the FP32 force error against
+FP64 on the C2 configuration, the dominant kernel against the HBM roofline,
+and the CPU oracle on the reference's Ermak test (1,204 particles, 10^4
+steps) on one host core.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "din-mol-li_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def _replica(args):
+    n, seed = args
+    from dml import workloads as W
+    return W.system_replica(int(n), int(seed))
+
+
+def build_replicas(nrep, seed0=104012):
+    from dml import workloads as W
+    sizes = W.replica_sizes(nrep)
+    jobs = [(int(n), seed0 + 1000 + k) for k, n in enumerate(sizes)]
+    workers = min(16, os.cpu_count() or 1)
+    if workers > 1 and nrep > 4:
+        import multiprocessing as mp
+        with mp.get_context("fork").Pool(workers) as pool:
+            return pool.map(_replica, jobs, chunksize=2)
+    return [_replica(j) for j in jobs]
+
+
+def force_rel_err(f32, f64):
+    import numpy as np
+    na = np.linalg.norm(f32 - f64, axis=1)
+    nb = np.linalg.norm(f64, axis=1)
+    floor = 1e-3 * np.sqrt(np.mean(nb ** 2))
+    return float(np.max(na / np.maximum(nb, floor)))
+
+
+def plain_leg(dml, W, steps, warmup):
+    import numpy as np
+    c2 = W.system_c2()
+    n = len(c2["pos"])
+    mk = lambda prec: dml.dana_config(c2["box"], c2["skin"], n, h=c2["h"], z0=c2["z0"],
+                                      zmax=c2["zmax"], integrator=dml.ERMAK,
+                                      reservoir=dml.NO_RESERVOIR, precision=prec,
+                                      overlap_mode=dml.OVERLAP_ROUNDS, seed=104012)
+    ctx = dml.Context(mk(dml.FP64))
+    ctx.set_state(c2["pos"], species=c2["species"])
+    ctx.run(warmup, graph=False)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ms = ctx.run(steps, graph=False)
+    wall = time.perf_counter() - t0
+    mean_nn, reb, st = ctx.list_stats()
+    snap = ctx.state()
+    ctx.close()
+    # FP32 vs FP64 forces on the same configuration and the same lists
+    out = {}
+    for prec in (dml.FP64, dml.FP32_FORCE):
+        c = dml.Context(mk(prec))
+        c.set_state(snap["pos"], species=snap["species"])
+        c.test_update()
+        c.forces()
+        out[prec] = c.state()["force"]
+        c.close()
+    ions = snap["species"] != 2
+    err = force_rel_err(out[dml.FP32_FORCE][ions], out[dml.FP64][ions])
+    return dict(n=n, ms=ms, wall=wall, rate=n * steps / (ms * 1e-3), mean_nn=mean_nn,
+                rebuilds=reb, steps_done=st, force_err=err)
+
+
+def batched_leg(dml, nrep, steps, warmup):
+    t0 = time.perf_counter()
+    reps = build_replicas(nrep)
+    log("built %d replicas in %.1f s" % (nrep, time.perf_counter() - t0))
+    cfgs = []
+    for k, r in enumerate(reps):
+        n = len(r["pos"])
+        piston = (k % 3 == 0)
+        cfgs.append(dml.dana_config(r["box"], r["skin"], n, h=r["h"], z0=r["z0"], zmax=r["zmax"],
+                                    integrator=dml.ERMAK if k % 2 else dml.BROWNIAN,
+                                    reservoir=dml.PISTON if piston else dml.NO_RESERVOIR,
+                                    precision=dml.FP32_FORCE, overlap_mode=dml.OVERLAP_ROUNDS,
+                                    seed=104012 + 7919 * k))
+    ens = dml.Context(cfgs)
+    ntot = 0
+    for k, r in enumerate(reps):
+        ens.set_state(r["pos"], species=r["species"], rep=k)
+        ntot += len(r["pos"])
+    ens.run(warmup, graph=True)
+    ens.sync()
+    t0 = time.perf_counter()
+    ms = ens.run(steps, graph=True)
+    wall = time.perf_counter() - t0
+    nn = [ens.list_stats(k) for k in range(nrep)]
+    mean_nn = sum(a for a, b, c in nn) / nrep
+    reb = sum(b for a, b, c in nn)
+    stepsum = sum(c for a, b, c in nn)
+    # dominant kernel: the neighbour-row build (see profiles/), timed on the
+    # context's own stream
+    kl_ms = ens.time_kernel(dml.K_LISTBUILD, 10)
+    kf_ms = ens.time_kernel(dml.K_FORCE, 10)
+    ens.close()
+    return dict(n=ntot, ms=ms, wall=wall, rate=ntot * steps / (ms * 1e-3), mean_nn=mean_nn,
+                rebuilds_per_rep_step=reb / max(stepsum, 1), list_ms=kl_ms, force_ms=kf_ms)
+
+
+def cpu_baseline():
+    """The CPU oracle (a bit-exact restatement of the reference) on the
+    reference's own Ermak test: 1,204 particles, 10^4 steps, piston."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import orc
+    cfg = json.load(open(os.path.join(ROOT, "tests", "golden", "ref_runs.json")))["ermak"]
+    out = orc.dana_run(cfg["idum"], cfg["prob"], cfg["h"], cfg["nst"], cfg["nwr"], cfg["xi"],
+                       cfg["yi"], cfg["dist"], cfg["z0"], cfg["zmax"], cfg["dif_sc"],
+                       cfg["dif_sei"], cfg["nb_dcut"], cfg["ermak"], cfg["reservoir"])
+    n = out["nat"]
+    return dict(value=n * cfg["nst"] / out["loop_seconds"], unit="particle-steps/s", cores=1,
+                kind="port",
+                sample="reference tests/ermak workload: %d particles x %d Ermak steps, piston, "
+                       "skin 10, FP64 oracle on one host core, %.2f s" % (n, cfg["nst"], out["loop_seconds"]))
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--replicas", type=int, default=256)
+    ap.add_argument("--legs", default="plain,batched")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (replicas shard across GPUs with no "
+              "exchange; see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import dml
+    from dml import workloads as W
+    try:
+        import torch
+        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except Exception:
+        sync = lambda: None
+    legs = args.legs.split(",")
+    plain = batched = None
+    if "plain" in legs:
+        log("plain leg: C2 100k FP64")
+        sync()
+        plain = plain_leg(dml, W, args.steps, args.warmup)
+        sync()
+        log("plain", json.dumps(plain))
+    if "batched" in legs:
+        log("batched leg: C3 %d replicas FP32 forces" % args.replicas)
+        sync()
+        batched = batched_leg(dml, args.replicas, args.steps, args.warmup)
+        sync()
+        log("batched", json.dumps(batched))
+    head = batched or plain
+    res = {
+        "metric": "particle-steps/s batched vs plain at force err<=1e-5 vs FP64; p99 run latency",
+        "value": head["rate"],
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp64 (fp32 pair forces)" if batched else "fp64",
+        "data": "synthetic (seeded bcc electrode + random-sequential Li+ at 0.01/A^3)",
+        "config": {"workload": "C3 batched ensemble, %d replicas (2k-50k log-uniform), FP32 forces, "
+                               "hipGraph; plain = C2 single 100k system FP64" % args.replicas,
+                   "particles": head["n"], "parallelism": "replicas on one GPU"},
+    }
+    if plain:
+        res["plain_value"] = plain["rate"]
+        res["force_rel_err_fp32_vs_fp64"] = plain["force_err"]
+    if plain and batched:
+        res["batched_over_plain"] = batched["rate"] / plain["rate"]
+    res["p99_run_latency_s"] = None  # serving leg: serve.py
+    if batched:
+        # neighbour-row build: compulsory bytes per ref atom = own position
+        # (24) + row count (4) + 4 per list entry, plus per b atom its bucket
+        # slot and bucket-ordered position (4 + 24)
+        per_atom = 24 + 4 + 4 * batched["mean_nn"] + 28
+        traffic = per_atom * batched["n"]
+        ach = traffic / (batched["list_ms"] * 1e-3) / 1e9
+        res["roofline"] = {"kernel": "k_listbuild", "bound": "hbm", "achieved": ach,
+                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                           "traffic": None, "avg_launch_ms": batched["list_ms"],
+                           "algorithmic_bytes_per_launch": traffic}
+        res["kernel_ms"] = {"k_listbuild": batched["list_ms"], "k_force": batched["force_ms"]}
+        res["mean_list_length"] = batched["mean_nn"]
+        res["rebuilds_per_step"] = batched["rebuilds_per_rep_step"]
+    if not args.no_cpu:
+        log("cpu baseline: oracle on the reference Ermak test")
+        res["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -999,6 +999,57 @@ int dml_run(dml_ctx *c, int32_t nsteps, int32_t use_graph) {
   return finish(c);
 }
 
+int dml_time_kernel(dml_ctx *c, int32_t which, int32_t reps, double *avg_ms) {
+  if (!c || reps <= 0 || !avg_ms) return DML_E_ARG;
+  if (which == DML_K_LISTBUILD || which == DML_K_CELLS) {
+    // a rebuild at the current positions, then the phase alone, reps times
+    hipLaunchKernelGGL(k_force_rebuild, dim3((c->nrep + 63) / 64), dim3(64), 0, c->st, c->d);
+    enq_cells_lists(c);
+  }
+  HIPCHECK(hipEventRecord(c->e0, c->st));
+  for (int q = 0; q < reps; q++) {
+    switch (which) {
+      case DML_K_LISTBUILD:
+        hipLaunchKernelGGL(k_listbuild<4>, dim3((unsigned)(c->P / 4)), dim3(4 * 64), 0, c->st, c->d);
+        break;
+      case DML_K_FORCE:
+        launch_force_lpa<0>(c, c->lpa, 0);
+        break;
+      case DML_K_CELLS: {
+        dim3 gc(grid_cells(c), c->nrep);
+        hipLaunchKernelGGL(k_zero_cells, gc, dim3(DML_BLOCK), 0, c->st, c->d);
+        hipLaunchKernelGGL(k_cell_key, dim3(c->d.nblocks), dim3(DML_BLOCK), 0, c->st, c->d);
+        hipLaunchKernelGGL(k_cell_sort, gc, dim3(DML_BLOCK), 0, c->st, c->d);
+        break;
+      }
+      case DML_K_DISPMAX:
+        hipLaunchKernelGGL(k_pbc_dispmax, dim3(c->d.nblocks), dim3(DML_BLOCK), 0, c->st, c->d);
+        break;
+      default:
+        return DML_E_ARG;
+    }
+  }
+  HIPCHECK(hipEventRecord(c->e1, c->st));
+  HIPCHECK(hipEventSynchronize(c->e1));
+  float ms = 0.f;
+  HIPCHECK(hipEventElapsedTime(&ms, c->e0, c->e1));
+  *avg_ms = ms / reps;
+  return finish(c);
+}
+
+int dml_set_tuning(dml_ctx *c, int32_t lanes_per_atom, int32_t overlap_rounds) {
+  if (!c) return DML_E_ARG;
+  if (lanes_per_atom > 0) {
+    if (lanes_per_atom > 64 || (lanes_per_atom & (lanes_per_atom - 1))) return DML_E_ARG;
+    c->lpa = lanes_per_atom;
+  }
+  if (overlap_rounds > 0) {
+    if (overlap_rounds > kMaxRounds) return DML_E_ARG;
+    c->rounds = overlap_rounds;
+  }
+  return DML_OK;
+}
+
 int dml_last_run_ms(dml_ctx *c, double *ms) {
   if (!c || !ms) return DML_E_ARG;
   *ms = c->last_ms;

@@ -32,7 +32,8 @@ EXPORTS = ["dml_create", "dml_create_ensemble", "dml_destroy", "dml_sync", "dml_
            "dml_brownian", "dml_ermak_a", "dml_ermak_b", "dml_ref_offset",
            "dml_overlap_moveback", "dml_overlap_rounds", "dml_f_to_cg", "dml_gcmc",
            "dml_gcmc_trial_energy", "dml_run", "dml_get_counters", "dml_list_stats",
-           "dml_last_run_ms"]
+           "dml_last_run_ms", "dml_time_kernel", "dml_set_tuning"]
+K_LISTBUILD, K_FORCE, K_CELLS, K_DISPMAX = 0, 1, 2, 3
 
 
 class DmlError(RuntimeError):
@@ -100,6 +101,8 @@ def lib():
         L.dml_get_counters.argtypes = [vp, C.c_int32, C.POINTER(Counters)]
         L.dml_list_stats.argtypes = [vp, C.c_int32, vp, vp, vp]
         L.dml_last_run_ms.argtypes = [vp, vp]
+        L.dml_time_kernel.argtypes = [vp, C.c_int32, C.c_int32, vp]
+        L.dml_set_tuning.argtypes = [vp, C.c_int32, C.c_int32]
         L.dml_sync.argtypes = [vp]
         _lib = L
     return _lib
@@ -311,6 +314,14 @@ class Context:
         _chk(self.L.dml_list_stats(self.h, rep, C.byref(m), C.byref(rb), C.byref(st)),
              "dml_list_stats")
         return m.value, rb.value, st.value
+
+    def time_kernel(self, which, reps=10):
+        ms = C.c_double()
+        _chk(self.L.dml_time_kernel(self.h, which, reps, C.byref(ms)), "dml_time_kernel")
+        return ms.value
+
+    def set_tuning(self, lanes_per_atom=0, overlap_rounds=0):
+        _chk(self.L.dml_set_tuning(self.h, lanes_per_atom, overlap_rounds), "dml_set_tuning")
 
     def sync(self):
         _chk(self.L.dml_sync(self.h), "dml_sync")

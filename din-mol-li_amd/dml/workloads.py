@@ -1,0 +1,148 @@
+"""Synthetic, seeded systems of the benchmark configurations.
+
+The reference builds its initial state with an O(N^2) rejection loop
+(pos_inic, dana.F90:330-396) and has no explicit electrode; these generators
+place ions by random sequential addition on a hash grid (O(N)) with the same
+3.2 A hard-core rule, and add bcc(100) Li0 electrode layers (species CG).
+
+Decisions recorded where the configuration text is open:
+* C1: 2 bcc layers of 14 x 14 sites (a = 3.51) plus 8 adatoms -> 400 CG; the
+  piston reservoir with z0 = 50, zmax = 100.
+* C2: 4 bcc layers of 71 x 71 sites with 20% vacancies; N = 100,000 kept by
+  filling ions at 0.01 A^-3 in a slab above the electrode (0.01 x the whole
+  box would be 250k).
+* C3: replica sizes log-uniform in [2k, 50k]; each replica is a square box
+  L x L x 2L, L = (N / 0.01)^(1/3), 2 electrode layers with 20% vacancies,
+  ions at 0.01 A^-3 above; integrators alternate Brownian / Ermak.
+"""
+import math
+
+import numpy as np
+
+A_BCC = 3.51
+RHO_ION = 0.01
+DMIN = 3.2
+
+
+def bcc_layers(lx, ly, nl, z_bottom=0.0, vac=0.0, rng=None, a=A_BCC):
+    """nl bcc(100) layers, bottom layer at z_bottom, spacing a/2."""
+    nx, ny = int(lx // a), int(ly // a)
+    pts = []
+    for l in range(nl):
+        off = 0.5 * (l % 2)
+        ii, jj = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+        p = np.column_stack([((ii + off) * a).ravel(), ((jj + off) * a).ravel(),
+                             np.full(nx * ny, z_bottom + l * a / 2)])
+        if vac > 0:
+            p = p[rng.random(len(p)) >= vac]
+        pts.append(p)
+    return np.vstack(pts)
+
+
+def rsa_fill(rng, n, lo, hi, box, existing=None, dmin=DMIN, batch=None):
+    """n points uniform in [lo, hi) (x, y periodic in box) at least dmin from
+    each other and from `existing`, by batched random sequential addition."""
+    lo = np.asarray(lo, float); hi = np.asarray(hi, float)
+    g = dmin / math.sqrt(3.0) * 0.999
+    nx, ny = int(math.ceil(box[0] / g)), int(math.ceil(box[1] / g))
+    gx, gy = box[0] / nx, box[1] / ny
+    zlo = lo[2] - 2 * dmin
+    nz = int(math.ceil((hi[2] + 2 * dmin - zlo) / g)) + 1
+    grid = -np.ones((nx, ny, nz), np.int64)
+    pts = np.zeros((0, 3))
+    if existing is not None and len(existing):
+        e = existing[(existing[:, 2] >= zlo) & (existing[:, 2] < hi[2] + 2 * dmin)]
+        pts = e.copy()
+        cx = np.floor(e[:, 0] / gx).astype(int) % nx
+        cy = np.floor(e[:, 1] / gy).astype(int) % ny
+        cz = np.clip(np.floor((e[:, 2] - zlo) / g).astype(int), 0, nz - 1)
+        grid[cx, cy, cz] = np.arange(len(e))
+    n0 = len(pts)
+    offs = np.array([(a, b, c) for a in range(-2, 3) for b in range(-2, 3) for c in range(-2, 3)])
+    batch = batch or max(256, min(20000, n // 4 + 256))
+    stall = 0
+    while len(pts) - n0 < n and stall < 200:
+        cand = lo + rng.random((batch, 3)) * (hi - lo)
+        cx = np.floor(cand[:, 0] / gx).astype(int) % nx
+        cy = np.floor(cand[:, 1] / gy).astype(int) % ny
+        cz = np.floor((cand[:, 2] - zlo) / g).astype(int)
+        bad = np.zeros(batch, bool)
+        # earliest candidate of each cell (a cell holds at most one point)
+        tmp = np.full((nx, ny, nz), batch, np.int64)
+        np.minimum.at(tmp, (cx, cy, cz), np.arange(batch))
+        bad |= tmp[cx, cy, cz] != np.arange(batch)
+        for dx, dy, dz in offs:
+            qx, qy, qz = (cx + dx) % nx, (cy + dy) % ny, cz + dz
+            ok = (qz >= 0) & (qz < nz)
+            qz = np.clip(qz, 0, nz - 1)
+            j = np.where(ok, grid[qx, qy, qz], -1)
+            m = j >= 0
+            if m.any():
+                d = cand[m] - pts[j[m]]
+                d[:, :2] -= box[:2] * np.round(d[:, :2] / box[:2])
+                bad[np.nonzero(m)[0][(d * d).sum(1) < dmin * dmin]] = True
+            k = np.where(ok, tmp[qx, qy, qz], batch)
+            m = (k < np.arange(batch)) & ((dx, dy, dz) != (0, 0, 0))
+            if m.any():
+                d = cand[m] - cand[k[m]]
+                d[:, :2] -= box[:2] * np.round(d[:, :2] / box[:2])
+                bad[np.nonzero(m)[0][(d * d).sum(1) < dmin * dmin]] = True
+        good = np.nonzero(~bad)[0][: n - (len(pts) - n0)]
+        if len(good) == 0:
+            stall += 1
+            continue
+        grid[cx[good], cy[good], cz[good]] = len(pts) + np.arange(len(good))
+        pts = np.vstack([pts, cand[good]])
+    if len(pts) - n0 < n:
+        raise RuntimeError("placement stalled at %d of %d points" % (len(pts) - n0, n))
+    return pts[n0:]
+
+
+def system_c1(seed=104012):
+    """1,000 particles: 400 CG electrode atoms + 600 Li+, 50 x 50 x 100 A."""
+    rng = np.random.default_rng(seed)
+    box = np.array([50.0, 50.0, 100.0])
+    el = bcc_layers(50.0, 50.0, 2, 0.0)
+    # 8 adatoms on hollow sites above the top layer (the next layer's sites)
+    sites = np.array([[i * A_BCC, j * A_BCC, A_BCC] for i in range(14) for j in range(14)])
+    el = np.vstack([el, sites[rng.choice(len(sites), 8, replace=False)]])
+    ions = rsa_fill(rng, 600, [0, 0, A_BCC + DMIN], [50, 50, 100.0], box, existing=el)
+    pos = np.vstack([el, ions])
+    z = np.array([2] * len(el) + [1] * len(ions), np.int8)
+    return dict(name="C1", box=box, pos=pos, species=z, skin=10.0, h=1e-2, z0=50.0, zmax=100.0)
+
+
+def system_c2(seed=104012 + 2, n_total=100_000):
+    """100k particles: 4-layer bcc electrode with 20% vacancies + ions at 0.01."""
+    rng = np.random.default_rng(seed)
+    box = np.array([250.0, 250.0, 400.0])
+    el = bcc_layers(250.0, 250.0, 4, 0.0, vac=0.2, rng=rng)
+    ztop = el[:, 2].max()
+    n_ion = n_total - len(el)
+    h = n_ion / (RHO_ION * box[0] * box[1])
+    ions = rsa_fill(rng, n_ion, [0, 0, ztop + DMIN], [250, 250, ztop + DMIN + h], box, existing=el)
+    pos = np.vstack([el, ions])
+    z = np.array([2] * len(el) + [1] * len(ions), np.int8)
+    return dict(name="C2", box=box, pos=pos, species=z, skin=10.0, h=1e-2, z0=200.0, zmax=400.0)
+
+
+def replica_sizes(nrep=256, lo=2000, hi=50000, seed=104012 + 3):
+    rng = np.random.default_rng(seed)
+    return np.exp(rng.uniform(math.log(lo), math.log(hi), nrep)).astype(int)
+
+
+def system_replica(n_total, seed):
+    rng = np.random.default_rng(seed)
+    L = (n_total / RHO_ION) ** (1.0 / 3.0)
+    L = max(L, 4 * 13.3)
+    box = np.array([L, L, 2 * L])
+    el = bcc_layers(L, L, 2, 0.0, vac=0.2, rng=rng)
+    ztop = el[:, 2].max()
+    n_ion = max(n_total - len(el), 10)
+    hgt = n_ion / (RHO_ION * L * L)
+    ions = rsa_fill(rng, n_ion, [0, 0, ztop + DMIN], [L, L, min(ztop + DMIN + hgt, 2 * L - 1)], box,
+                    existing=el)
+    pos = np.vstack([el, ions])
+    z = np.array([2] * len(el) + [1] * len(ions), np.int8)
+    ion_top = ions[:, 2].max()
+    return dict(box=box, pos=pos, species=z, skin=10.0, h=1e-2, z0=0.5 * ion_top, zmax=2 * L)

@@ -183,8 +183,16 @@ int dml_get_counters(dml_ctx *ctx, int32_t rep, dml_counters *out);
 int dml_list_stats(dml_ctx *ctx, int32_t rep, double *mean_nn,
                    int64_t *rebuilds, int64_t *steps);
 
-/* --- timing of the last dml_run (HIP events on the context's stream) ---- */
+/* --- timing (HIP events on the context's stream) ---------------------- */
 int dml_last_run_ms(dml_ctx *ctx, double *ms);
+/* average duration of one launch of a phase, reps back-to-back launches;
+ * DML_K_LISTBUILD and DML_K_CELLS first rebuild at the current positions. */
+enum { DML_K_LISTBUILD = 0, DML_K_FORCE = 1, DML_K_CELLS = 2, DML_K_DISPMAX = 3 };
+int dml_time_kernel(dml_ctx *ctx, int32_t which, int32_t reps, double *avg_ms);
+/* launch shape of the row kernels (lanes per atom, power of two <= 64) and
+ * overlap rounds per step of dml_run; 0 keeps the current value.  No
+ * counterpart in the reference: used by the autotuner. */
+int dml_set_tuning(dml_ctx *ctx, int32_t lanes_per_atom, int32_t overlap_rounds);
 
 #ifdef __cplusplus
 }
