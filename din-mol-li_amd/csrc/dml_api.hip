@@ -49,6 +49,8 @@ struct dml_ctx {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   double last_ms = 0.0;
   int lpa = 8;
+  int lpa_ov = 8;  // lanes per atom of the overlap scan (1 = thread per atom)
+  int lb_mode = 0; // 0: cell-tile list build, 1: wave per atom
   int rounds = 8;
   int any_ermak = 0, any_brown = 0, any_rounds = 0, any_exact = 0, any_piston = 0;
 };
@@ -208,13 +210,29 @@ int grid_cells(const dml_ctx *c) {
 }
 
 // ---- phases -------------------------------------------------------------
+int max_cells(const dml_ctx *c) {
+  int64_t mx = 1;
+  for (auto &h : c->reps) mx = std::max<int64_t>(mx, h.p.ccap);
+  return (int)mx;
+}
+
+void enq_listbuild(dml_ctx *c) {
+  if (c->lb_mode == 0) {
+    hipLaunchKernelGGL(k_listbuild_cells, dim3(std::min(max_cells(c), 16384), c->nrep),
+                       dim3(DML_BLOCK), 0, c->st, c->d);
+  } else {
+    hipLaunchKernelGGL(k_listbuild<4>, dim3((unsigned)(c->P / 4)), dim3(4 * 64), 0, c->st, c->d);
+  }
+}
+
 void enq_cells_lists(dml_ctx *c) {
   dim3 gc(grid_cells(c), c->nrep);
+  dim3 gw(std::min((max_cells(c) + 3) / 4, 16384), c->nrep);
   int nb = c->d.nblocks;
   hipLaunchKernelGGL(k_zero_cells, gc, dim3(DML_BLOCK), 0, c->st, c->d);
   hipLaunchKernelGGL(k_cell_key, dim3(nb), dim3(DML_BLOCK), 0, c->st, c->d);
-  hipLaunchKernelGGL(k_cell_sort, gc, dim3(DML_BLOCK), 0, c->st, c->d);
-  hipLaunchKernelGGL(k_listbuild<4>, dim3((unsigned)(c->P / 4)), dim3(4 * 64), 0, c->st, c->d);
+  hipLaunchKernelGGL(k_cell_sort_w, gw, dim3(DML_BLOCK), 0, c->st, c->d);
+  enq_listbuild(c);
 }
 
 void enq_test_update(dml_ctx *c) {
@@ -253,8 +271,24 @@ void enq_overlap_rounds(dml_ctx *c, const double *u_round, int64_t stride, int r
   hipLaunchKernelGGL(k_zero_i32, dim3((unsigned)((kMaxRounds * nr + 255) / 256)), dim3(256), 0,
                      c->st, c->again, (int64_t)kMaxRounds * nr);
   for (int q = 0; q < rounds; q++) {
-    hipLaunchKernelGGL(k_ov_scan, dim3(c->d.nblocks), dim3(DML_BLOCK), 0, c->st, c->d,
-                       u_round, stride, q, c->again);
+    switch (c->lpa_ov) {
+      case 1:
+        hipLaunchKernelGGL(k_ov_scan, dim3(c->d.nblocks), dim3(DML_BLOCK), 0, c->st, c->d,
+                           u_round, stride, q, c->again);
+        break;
+      case 4:
+        hipLaunchKernelGGL(k_ov_scan_l<4>, dim3((unsigned)(c->P * 4 / DML_BLOCK)), dim3(DML_BLOCK),
+                           0, c->st, c->d, u_round, stride, q, c->again);
+        break;
+      case 16:
+        hipLaunchKernelGGL(k_ov_scan_l<16>, dim3((unsigned)(c->P * 16 / DML_BLOCK)), dim3(DML_BLOCK),
+                           0, c->st, c->d, u_round, stride, q, c->again);
+        break;
+      default:
+        hipLaunchKernelGGL(k_ov_scan_l<8>, dim3((unsigned)(c->P * 8 / DML_BLOCK)), dim3(DML_BLOCK),
+                           0, c->st, c->d, u_round, stride, q, c->again);
+        break;
+    }
     hipLaunchKernelGGL(k_ov_apply, dim3(c->d.nblocks), dim3(DML_BLOCK), 0, c->st, c->d, q,
                        (const int32_t *)c->again);
   }
@@ -551,6 +585,8 @@ int dml_create_ensemble(int32_t nrep, const dml_config *cfgs, dml_ctx **out) {
   }
   hipEventCreate(&c->e0);
   hipEventCreate(&c->e1);
+  if (const char *ev = getenv("DML_LISTBUILD")) c->lb_mode = atoi(ev);
+  if (const char *ev = getenv("DML_OVERLAP_LPA")) c->lpa_ov = atoi(ev);
   refresh_modes(c);
   if (hipStreamSynchronize(c->st) != hipSuccess) {
     dml_destroy(c);
@@ -1010,7 +1046,7 @@ int dml_time_kernel(dml_ctx *c, int32_t which, int32_t reps, double *avg_ms) {
   for (int q = 0; q < reps; q++) {
     switch (which) {
       case DML_K_LISTBUILD:
-        hipLaunchKernelGGL(k_listbuild<4>, dim3((unsigned)(c->P / 4)), dim3(4 * 64), 0, c->st, c->d);
+        enq_listbuild(c);
         break;
       case DML_K_FORCE:
         launch_force_lpa<0>(c, c->lpa, 0);
@@ -1019,7 +1055,8 @@ int dml_time_kernel(dml_ctx *c, int32_t which, int32_t reps, double *avg_ms) {
         dim3 gc(grid_cells(c), c->nrep);
         hipLaunchKernelGGL(k_zero_cells, gc, dim3(DML_BLOCK), 0, c->st, c->d);
         hipLaunchKernelGGL(k_cell_key, dim3(c->d.nblocks), dim3(DML_BLOCK), 0, c->st, c->d);
-        hipLaunchKernelGGL(k_cell_sort, gc, dim3(DML_BLOCK), 0, c->st, c->d);
+        dim3 gw(std::min((max_cells(c) + 3) / 4, 16384), c->nrep);
+        hipLaunchKernelGGL(k_cell_sort_w, gw, dim3(DML_BLOCK), 0, c->st, c->d);
         break;
       }
       case DML_K_DISPMAX:

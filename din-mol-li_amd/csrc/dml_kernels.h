@@ -7,6 +7,11 @@ __global__ void k_force_rebuild(DevState s);
 __global__ void k_zero_cells(DevState s);
 __global__ void k_cell_key(DevState s);
 __global__ void k_cell_sort(DevState s);
+__global__ void k_cell_sort_w(DevState s);
+__global__ void k_listbuild_cells(DevState s);
+template <int LPA>
+__global__ void k_ov_scan_l(DevState s, const double *u_round, int64_t u_stride, int round,
+                            int32_t *again);
 template <int WPB>
 __global__ void k_listbuild(DevState s);
 template <int LPA, int FUSEB>

@@ -8,6 +8,10 @@
 
 #define WAVE 64
 
+__device__ inline int same_pos(const DevState &s, int i) {
+  return s.px[i] == s.cx[i] && s.py[i] == s.cy[i] && s.pz[i] == s.cz[i];
+}
+
 __device__ inline const RepParams &rep_of_block(const DevState &s, int blk,
                                                 int *r) {
   *r = s.blk2rep[blk];
@@ -314,6 +318,332 @@ __global__ void __launch_bounds__(WPB *WAVE) k_listbuild(DevState s) {
 template __global__ void k_listbuild<4>(DevState s);
 
 // ------------------------------------------------------------------------
+// neighbour rows, cell-tile form: one workgroup per cell.  The candidates of
+// the 27 stencil cells (stencil order, chain order inside each cell) are
+// staged through LDS in tiles and shared by every ref atom of the centre
+// cell; each wave owns some centre atoms and appends to their rows by
+// ballot compaction, so every row keeps the order of ngroup_cells
+// (Neighbor.F90:509-535).  Same rows as k_listbuild, ~cell-occupancy times
+// fewer L2/HBM reads.
+// ------------------------------------------------------------------------
+#define LB_TILE 1024
+#define LB_CMAX 256
+__global__ void __launch_bounds__(DML_BLOCK) k_listbuild_cells(DevState s) {
+  __shared__ int32_t t_slot[LB_TILE];
+  __shared__ double t_x[LB_TILE], t_y[LB_TILE], t_z[LB_TILE];
+  __shared__ int32_t c_slot[LB_CMAX];
+  __shared__ double c_x[LB_CMAX], c_y[LB_CMAX], c_z[LB_CMAX];
+  __shared__ int32_t c_nn[LB_CMAX];
+  __shared__ int64_t st_base[27];
+  __shared__ int32_t st_pre[28];
+  __shared__ int32_t sh_A, sh_nrefc;
+  __shared__ unsigned long long sh_nnsum, sh_rows;
+  const int r = blockIdx.y;
+  const RepParams &P = s.rp[r];
+  if (!P.rebuild) return;
+  const int tid = threadIdx.x, w = tid / WAVE, lane = tid % WAVE;
+  const int nwav = DML_BLOCK / WAVE;
+  const int nc = ncell_total(P);
+  const double rl2 = P.rl2;
+  const int strict = P.use_cells;
+  const int maxnn = s.maxnn;
+  if (tid == 0) { sh_nnsum = 0; sh_rows = 0; }
+  for (int c = blockIdx.x; c < nc; c += gridDim.x) {
+    if (tid == 0) {
+      int A = s.cell_cnt[P.coff + c];
+      sh_A = A > P.bcap ? P.bcap : A;
+    }
+    __syncthreads();
+    const int Atot = sh_A;
+    const int64_t cbase = P.boff + (int64_t)c * P.bcap;
+    int ghost = 0;
+    int ci0 = 0, ci1 = 0, ci2 = 0;
+    if (P.use_cells) {
+      ci0 = c % (P.ncells[0] + 2);
+      ci1 = (c / (P.ncells[0] + 2)) % (P.ncells[1] + 2);
+      ci2 = c / ((P.ncells[0] + 2) * (P.ncells[1] + 2));
+      ghost = (ci0 < 1 || ci1 < 1 || ci2 < 1 || ci0 > P.ncells[0] || ci1 > P.ncells[1] ||
+               ci2 > P.ncells[2]);
+    }
+    for (int a0 = 0; a0 < Atot; a0 += LB_CMAX) {
+      const int A = (Atot - a0) < LB_CMAX ? (Atot - a0) : LB_CMAX;
+      if (tid == 0) sh_nrefc = 0;
+      __syncthreads();
+      for (int q = tid; q < A; q += DML_BLOCK) {
+        int v = s.bucket[cbase + a0 + q];
+        int isref = (s.flags[v] & DML_FL_REF) != 0;
+        c_slot[q] = isref ? v : -1;
+        c_x[q] = s.bx[cbase + a0 + q];
+        c_y[q] = s.by[cbase + a0 + q];
+        c_z[q] = s.bz[cbase + a0 + q];
+        c_nn[q] = 0;
+        if (isref) atomicAdd(&sh_nrefc, 1);
+      }
+      if (tid < 27) {
+        int cnt = 0;
+        int64_t base = 0;
+        if (P.use_cells) {
+          int n0 = cell_wrap(ci0 + kMap[tid][0], P.ncells[0]);
+          int n1 = cell_wrap(ci1 + kMap[tid][1], P.ncells[1]);
+          int n2 = cell_wrap(ci2 + kMap[tid][2], P.ncells[2]);
+          int cell = cell_lin(P, n0, n1, n2);
+          cnt = s.cell_cnt[P.coff + cell];
+          if (cnt > P.bcap) cnt = P.bcap;
+          base = P.boff + (int64_t)cell * P.bcap;
+        } else if (tid == 0) {
+          cnt = Atot;
+          base = P.boff;
+        }
+        st_base[tid] = base;
+        st_pre[tid + 1] = cnt;  // counts, scanned below
+      }
+      __syncthreads();
+      if (tid == 0) {
+        st_pre[0] = 0;
+        for (int k = 1; k <= 27; k++) st_pre[k] += st_pre[k - 1];
+      }
+      __syncthreads();
+      const int total = st_pre[27];
+      if (sh_nrefc == 0 || total == 0) {
+        __syncthreads();
+        continue;
+      }
+      for (int t0 = 0; t0 < total; t0 += LB_TILE) {
+        const int tn = (total - t0) < LB_TILE ? (total - t0) : LB_TILE;
+        for (int q = tid; q < tn; q += DML_BLOCK) {
+          int cidx = t0 + q;
+          int lo = 0, hi = 26;
+          while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (st_pre[mid] <= cidx) lo = mid; else hi = mid - 1;
+          }
+          int64_t pos = st_base[lo] + (cidx - st_pre[lo]);
+          t_slot[q] = s.bucket[pos];
+          t_x[q] = s.bx[pos];
+          t_y[q] = s.by[pos];
+          t_z[q] = s.bz[pos];
+        }
+        __syncthreads();
+        for (int a = w; a < A; a += nwav) {
+          const int i = c_slot[a];
+          if (i < 0) continue;
+          const double xi = c_x[a], yi = c_y[a], zi = c_z[a];
+          int nn = c_nn[a];
+          int32_t *row = s.nbr + (int64_t)i * maxnn;
+          for (int q0 = 0; q0 < tn; q0 += WAVE) {
+            int q = q0 + lane;
+            int hit = 0, j = -1;
+            if (q < tn) {
+              j = t_slot[q];
+              double dx, dy, dz;
+              vdist_mic(P, t_x[q], t_y[q], t_z[q], xi, yi, zi, &dx, &dy, &dz);
+              double rd = dot3(dx, dy, dz);
+              hit = (strict ? (rd < rl2) : !(rd > rl2)) && (j != i);
+            }
+            unsigned long long m = __ballot(hit);
+            int pr = __popcll(m & ((1ull << lane) - 1ull));
+            if (hit && nn + pr < maxnn) row[nn + pr] = j;
+            nn += __popcll(m);
+          }
+          if (lane == 0) c_nn[a] = nn;
+        }
+        __syncthreads();
+      }
+      for (int q = tid; q < A; q += DML_BLOCK) {
+        int i = c_slot[q];
+        if (i < 0) continue;
+        int nn = c_nn[q];
+        if (nn > maxnn) {
+          atomicOr(s.err, ERR_LIST);
+          atomicMax(&s.rp[r].max_nn_seen, nn);
+          nn = maxnn;
+        }
+        s.nn[i] = nn;
+        s.rowflag[i] = (uint8_t)ghost;
+        if (ghost) s.rp[r].any_ghost = 1;
+        atomicAdd(&sh_nnsum, (unsigned long long)nn);
+        atomicAdd(&sh_rows, 1ull);
+      }
+      __syncthreads();
+    }
+    __syncthreads();  // sh_A and the LDS tiles are rewritten by the next cell
+  }
+  __syncthreads();
+  if (tid == 0 && sh_rows) {
+    atomicAdd(&s.cnt[r].nn_sum, sh_nnsum);
+    atomicAdd(&s.cnt[r].nn_rows, sh_rows);
+  }
+}
+
+// bucket order by a rank sort inside one wave (cells of up to 64 atoms;
+// larger cells fall back to an insertion sort by lane 0)
+__global__ void __launch_bounds__(DML_BLOCK) k_cell_sort_w(DevState s) {
+  const int r = blockIdx.y;
+  const RepParams &P = s.rp[r];
+  if (!want_cells(P)) return;
+  const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+  const int nc = ncell_total(P);
+  const int desc = P.use_cells;
+  for (int c = blockIdx.x * (DML_BLOCK / WAVE) + w; c < nc; c += gridDim.x * (DML_BLOCK / WAVE)) {
+    int cnt = s.cell_cnt[P.coff + c];
+    if (cnt > P.bcap) cnt = P.bcap;
+    if (cnt == 0) continue;
+    const int64_t base = P.boff + (int64_t)c * P.bcap;
+    int32_t *b = s.bucket + base;
+    if (cnt <= WAVE) {
+      int v = lane < cnt ? b[lane] : 0;
+      int k = lane < cnt ? s.bkey[v] : 0;
+      int rank = 0;
+      for (int q = 0; q < cnt; q++) {
+        int kq = __shfl(k, q);
+        rank += desc ? (kq > k) : (kq < k);
+      }
+      if (lane < cnt) {
+        b[rank] = v;
+        s.bx[base + rank] = s.px[v];
+        s.by[base + rank] = s.py[v];
+        s.bz[base + rank] = s.pz[v];
+      }
+    } else {
+      if (lane == 0) {
+        for (int q = 1; q < cnt; q++) {
+          int v = b[q], kv = s.bkey[v];
+          int j = q - 1;
+          while (j >= 0) {
+            int kw = s.bkey[b[j]];
+            if (desc ? (kw >= kv) : (kw <= kv)) break;
+            b[j + 1] = b[j];
+            j--;
+          }
+          b[j + 1] = v;
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      for (int q = lane; q < cnt; q += WAVE) {
+        int v = __hip_atomic_load(&b[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        s.bx[base + q] = s.px[v];
+        s.by[base + q] = s.py[v];
+        s.bz[base + q] = s.pz[v];
+      }
+    }
+  }
+}
+
+// overlap rounds, read phase, one pass per row with LPA lanes per atom: hits
+// are rare (rcut << list radius), so each lane keeps its few hits in
+// registers; the CG contact is the first CG hit in row order, reverts are
+// the non-CG hits before it (dana.F90:875-935); domination as k_ov_scan.
+template <int LPA>
+__global__ void __launch_bounds__(DML_BLOCK) k_ov_scan_l(DevState s, const double *u_round,
+                                                          int64_t u_stride, int round,
+                                                          int32_t *again) {
+  const int gid = blockIdx.x * DML_BLOCK + threadIdx.x;
+  const int t = gid / LPA, sub = gid % LPA;
+  const int r = s.blk2rep[t / DML_BLOCK];
+  const RepParams &P = s.rp[r];
+  if (P.overlap_mode != DML_OVERLAP_ROUNDS) return;
+  if (round > 0 && !again[(round - 1) * s.nrep + r]) return;
+  const int local = t - P.poff;
+  const int active0 = local < P.nref;
+  int i = active0 ? s.ref_order[t] : 0;
+  int active = active0 && !(s.flags[i] & DML_FL_SKIP);
+  const int NH = 4;
+  int hj[NH], he[NH];
+  int nh = 0, over = 0, dom = 0, firstcg = 0x7fffffff;
+  double xi = 0, yi = 0, zi = 0;
+  int nn = 0;
+  const int32_t *row = s.nbr + (int64_t)i * s.maxnn;
+  if (active) {
+    xi = s.px[i]; yi = s.py[i]; zi = s.pz[i];
+    nn = s.nn[i];
+    for (int e = sub; e < nn; e += LPA) {
+      int j = row[e];
+      uint8_t fj = s.flags[j];
+      if (fj & DML_FL_LIMBO) continue;
+      double dx, dy, dz;
+      vdist_mic(P, xi, yi, zi, s.px[j], s.py[j], s.pz[j], &dx, &dy, &dz);
+      if (dot3(dx, dy, dz) > P.rcut2) continue;
+      if ((fj & DML_FL_REF) && !(fj & DML_FL_SKIP) && s.ref_pos[j] < local) dom = 1;
+      if (s.species[j] == SP_CG) {
+        if (e < firstcg) firstcg = e;
+      } else if (nh < NH) {
+        hj[nh] = j; he[nh] = e; nh++;
+      } else {
+        over = 1;
+      }
+    }
+  }
+  for (int off = LPA / 2; off > 0; off >>= 1) {
+    dom |= __shfl_xor(dom, off);
+    over |= __shfl_xor(over, off);
+    int o = __shfl_xor(firstcg, off);
+    firstcg = o < firstcg ? o : firstcg;
+  }
+  unsigned long long n_ch = 0, n_ch3 = 0;
+  int any = dom;
+  if (active && !dom) {
+    const int stamp = P.ov_round + round + 1;
+    if (!over) {
+      for (int q = 0; q < nh; q++) {
+        if (he[q] >= firstcg) continue;
+        int j = hj[q];
+        if (P.piston && same_pos(s, j) && same_pos(s, i)) { n_ch3++; continue; }
+        s.ov_rev[j] = stamp;
+        n_ch++;
+        any = 1;
+      }
+    } else {
+      for (int e = sub; e < nn && e < firstcg; e += LPA) {
+        int j = row[e];
+        if (s.flags[j] & DML_FL_LIMBO) continue;
+        double dx, dy, dz;
+        vdist_mic(P, xi, yi, zi, s.px[j], s.py[j], s.pz[j], &dx, &dy, &dz);
+        if (dot3(dx, dy, dz) > P.rcut2 || s.species[j] == SP_CG) continue;
+        if (P.piston && same_pos(s, j) && same_pos(s, i)) { n_ch3++; continue; }
+        s.ov_rev[j] = stamp;
+        n_ch++;
+        any = 1;
+      }
+    }
+  }
+  for (int off = LPA / 2; off > 0; off >>= 1) {
+    n_ch += __shfl_xor(n_ch, off);
+    n_ch3 += __shfl_xor(n_ch3, off);
+    any |= __shfl_xor(any, off);
+  }
+  if (active0 && sub == 0) {
+    uint8_t st = 0;
+    if (active) {
+      st = 1;
+      if (dom) {
+        st |= 2;
+      } else if (firstcg != 0x7fffffff) {
+        atomicAdd(&s.cnt[r].try_, 1ull);
+        double ne;
+        if (u_round) {
+          ne = u_round[(int64_t)round * u_stride + i];
+        } else {
+          u32x4 wv = dml_philox(P.seed, s.pid[i], (uint32_t)P.step, RS_CONTACT,
+                                (uint32_t)(1000 + round));
+          ne = u01_from(wv.x, wv.y);
+        }
+        st |= (ne < P.prob) ? 4 : 8;
+      }
+    }
+    s.ov_st[i] = st;
+    if (n_ch) atomicAdd(&s.cnt[r].choques, n_ch);
+    if (n_ch3) atomicAdd(&s.cnt[r].choques3, n_ch3);
+    if (any) again[round * s.nrep + r] = 1;
+  }
+}
+template __global__ void k_ov_scan_l<1>(DevState, const double *, int64_t, int, int32_t *);
+template __global__ void k_ov_scan_l<4>(DevState, const double *, int64_t, int, int32_t *);
+template __global__ void k_ov_scan_l<8>(DevState, const double *, int64_t, int, int32_t *);
+template __global__ void k_ov_scan_l<16>(DevState, const double *, int64_t, int, int32_t *);
+
+
+// ------------------------------------------------------------------------
 // fuerza (dana.F90:1055-1139) as a gather: the full list plus reaction of
 // the reference counts every ref-ref pair twice, so entry j of row i
 // weighs 2 when j is a ref atom (its row holds i) and 1 otherwise.  Rows
@@ -606,9 +936,6 @@ __global__ void __launch_bounds__(DML_BLOCK) k_ermak_a(DevState s, const double 
 // ------------------------------------------------------------------------
 // overlap_moveback (dana.F90:849-943)
 // ------------------------------------------------------------------------
-__device__ inline int same_pos(const DevState &s, int i) {
-  return s.px[i] == s.cx[i] && s.py[i] == s.cy[i] && s.pz[i] == s.cz[i];
-}
 
 // exact reference order: one wave per replica walks the ref list in order;
 // rows are scanned 64 entries at a time, hits acted on in list order and

@@ -104,10 +104,13 @@ def assert_lists_equal(c, s):
 NB = np.load(os.path.join(G, "nb_cases.npz"))
 
 
+@pytest.mark.parametrize("lb_mode", ["0", "1"])
 @pytest.mark.parametrize("name", list(NB["names"]))
-def test_cells_and_lists_equal_reference_modules(dml, name):
+def test_cells_and_lists_equal_reference_modules(dml, name, lb_mode, monkeypatch):
+    """lb_mode 0: cell-tile list build through LDS; 1: one wave per atom"""
+    monkeypatch.setenv("DML_LISTBUILD", lb_mode)
     d = {k.split("/", 1)[1]: NB[k] for k in NB.files if k.startswith(name + "/")}
-    say("case", name, len(d["z"]))
+    say("case", name, len(d["z"]), "lb_mode", lb_mode)
     rc, skin = d["rc_skin"]
     cfg = dml.dana_config(d["box"], skin, len(d["z"]) + 64, rcut=rc, integrator=dml.BROWNIAN,
                           reservoir=dml.NO_RESERVOIR)
@@ -277,8 +280,8 @@ def _step_pair(orc, dml, s, c, rng, mode, nsteps, brown=True):
     return so, sd
 
 
-@pytest.mark.parametrize("mode", ["exact", "rounds"])
-def test_brownian_steps_with_overlap_match_oracle(orc, dml, mode):
+@pytest.mark.parametrize("mode", ["exact", "rounds", "rounds_lpa1"])
+def test_brownian_steps_with_overlap_match_oracle(orc, dml, mode, monkeypatch):
     """several whole steps: the exact resolver follows the reference order
     bit for bit; the round schedule follows its own oracle."""
     # h and the skin are chosen like the reference's own runs (skin 10 with
@@ -290,10 +293,12 @@ def test_brownian_steps_with_overlap_match_oracle(orc, dml, mode):
     pos = np.vstack([el, ions])
     z = np.array([2] * len(el) + [1] * len(ions), np.int8)
     say("steps", mode, len(pos))
+    if mode == "rounds_lpa1":
+        monkeypatch.setenv("DML_OVERLAP_LPA", "1")
     s, c = pair(orc, dml, box, 10.0, pos, z, ermak=False, zmax=60.0, z0=30.0, h=1e-3,
                 overlap_mode=dml.OVERLAP_EXACT if mode == "exact" else dml.OVERLAP_ROUNDS)
     s.test_update(); c.test_update()
-    so, sd = _step_pair(orc, dml, s, c, rng, mode, 12)
+    so, sd = _step_pair(orc, dml, s, c, rng, "exact" if mode == "exact" else "rounds", 12)
     cnt_o, cnt_d = s.c, c.counters()
     assert cnt_o.choques == cnt_d["choques"]
     assert cnt_o.depo == cnt_d["depo"]
