@@ -102,7 +102,8 @@ def batched_leg(dml, nrep, steps, warmup):
     for k, r in enumerate(reps):
         n = len(r["pos"])
         piston = (k % 3 == 0)
-        cfgs.append(dml.dana_config(r["box"], r["skin"], n, h=r["h"], z0=r["z0"], zmax=r["zmax"],
+        cfgs.append(dml.dana_config(r["box"], r["skin"], n, h=r["h"],
+                                    z0=r["z0_piston"] if piston else r["z0"], zmax=r["zmax"],
                                     integrator=dml.ERMAK if k % 2 else dml.BROWNIAN,
                                     reservoir=dml.PISTON if piston else dml.NO_RESERVOIR,
                                     precision=dml.FP32_FORCE, overlap_mode=dml.OVERLAP_ROUNDS,
@@ -114,6 +115,7 @@ def batched_leg(dml, nrep, steps, warmup):
         ntot += len(r["pos"])
     ens.run(warmup, graph=True)
     ens.sync()
+    before = [ens.list_stats(k)[2] for k in range(nrep)]
     t0 = time.perf_counter()
     ms = ens.run(steps, graph=True)
     wall = time.perf_counter() - t0
@@ -121,13 +123,19 @@ def batched_leg(dml, nrep, steps, warmup):
     mean_nn = sum(a for a, b, c in nn) / nrep
     reb = sum(b for a, b, c in nn)
     stepsum = sum(c for a, b, c in nn)
+    # particle-steps actually taken: a replica halted by a fatal condition of
+    # the reference (e.g. a deposit above z0) stops advancing
+    sizes = [len(r["pos"]) for r in reps]
+    psteps = sum(n * (c - b0) for n, (a, bb, c), b0 in zip(sizes, nn, before))
+    halted = sum(1 for k in range(nrep) if ens.replica_status(k) != 0)
     # dominant kernel: the neighbour-row build (see profiles/), timed on the
     # context's own stream
     kl_ms = ens.time_kernel(dml.K_LISTBUILD, 10)
     kf_ms = ens.time_kernel(dml.K_FORCE, 10)
     ens.close()
-    return dict(n=ntot, ms=ms, wall=wall, rate=ntot * steps / (ms * 1e-3), mean_nn=mean_nn,
-                rebuilds_per_rep_step=reb / max(stepsum, 1), list_ms=kl_ms, force_ms=kf_ms)
+    return dict(n=ntot, ms=ms, wall=wall, rate=psteps / (ms * 1e-3), mean_nn=mean_nn,
+                rebuilds_per_rep_step=reb / max(stepsum, 1), list_ms=kl_ms, force_ms=kf_ms,
+                halted=halted)
 
 
 def cpu_baseline():
@@ -218,6 +226,7 @@ def main():
         res["kernel_ms"] = {"k_listbuild": batched["list_ms"], "k_force": batched["force_ms"]}
         res["mean_list_length"] = batched["mean_nn"]
         res["rebuilds_per_step"] = batched["rebuilds_per_rep_step"]
+        res["replicas_halted"] = batched["halted"]
     if not args.no_cpu:
         log("cpu baseline: oracle on the reference Ermak test")
         res["cpu_baseline"] = cpu_baseline()

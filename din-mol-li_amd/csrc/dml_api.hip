@@ -381,7 +381,9 @@ int handle_overflow(dml_ctx *c, int e) {
   return DML_OK;
 }
 
-int finish(dml_ctx *c) {
+// replica_fatal: report a replica's halt as the call's status (per-routine
+// entry points); dml_run leaves halted replicas to dml_replica_status()
+int finish(dml_ctx *c, int replica_fatal = 1) {
   int e = read_err(c, 0);
   if (e < 0) return DML_E_HIP;
   if (e & (ERR_LIST | ERR_BUCKET)) {
@@ -393,6 +395,12 @@ int finish(dml_ctx *c) {
     int z = 0;
     hipMemcpy(c->d.err, &z, sizeof(int), hipMemcpyHostToDevice);
     return status_from_err(e);
+  }
+  if (replica_fatal) {
+    int rc = fetch_params(c);
+    if (rc) return rc;
+    for (auto &h : c->reps)
+      if (h.p.err_bits) return status_from_err(h.p.err_bits);
   }
   return DML_OK;
 }
@@ -679,6 +687,8 @@ int dml_set_state(dml_ctx *c, int32_t rep, int32_t n, const double *pos, const d
   p.rebuild = 0;
   p.any_f = 0;
   p.step = 0;
+  p.halted = 0;
+  p.err_bits = 0;
   if (p.piston) {
     // initial calc_rho (dana.F90:145-146)
     int g = 0;
@@ -1002,10 +1012,10 @@ int dml_run(dml_ctx *c, int32_t nsteps, int32_t use_graph) {
   if (need_init) {
     // dana.F90:140-142: test_update, then fuerza for Ermak runs
     enq_test_update(c);
-    if ((rc = finish(c))) return rc;
+    if ((rc = finish(c, 0))) return rc;
     if (c->any_ermak) {
       enq_forces(c, 0, 1);
-      if ((rc = finish(c))) return rc;
+      if ((rc = finish(c, 0))) return rc;
     }
   }
   if (use_graph && (!c->gexec || c->graph_rounds != c->rounds)) {
@@ -1032,7 +1042,15 @@ int dml_run(dml_ctx *c, int32_t nsteps, int32_t use_graph) {
   float ms = 0.f;
   hipEventElapsedTime(&ms, c->e0, c->e1);
   c->last_ms = ms;
-  return finish(c);
+  return finish(c, 0);
+}
+
+int dml_replica_status(dml_ctx *c, int32_t rep, int32_t *status) {
+  if (!c || rep < 0 || rep >= c->nrep || !status) return DML_E_ARG;
+  int rc = fetch_params(c);
+  if (rc) return rc;
+  *status = status_from_err(c->reps[rep].p.err_bits);
+  return DML_OK;
 }
 
 int dml_time_kernel(dml_ctx *c, int32_t which, int32_t reps, double *avg_ms) {
@@ -1071,7 +1089,7 @@ int dml_time_kernel(dml_ctx *c, int32_t which, int32_t reps, double *avg_ms) {
   float ms = 0.f;
   HIPCHECK(hipEventElapsedTime(&ms, c->e0, c->e1));
   *avg_ms = ms / reps;
-  return finish(c);
+  return finish(c, 0);
 }
 
 int dml_set_tuning(dml_ctx *c, int32_t lanes_per_atom, int32_t overlap_rounds) {

@@ -61,6 +61,8 @@ struct RepParams {
   int32_t rho_count;
   int32_t max_nn_seen;
   int32_t nref_new;
+  int32_t halted;    // a fatal condition stopped this replica
+  int32_t err_bits;  // which one (ERR_*)
   uint64_t seed;
   double d1, d2;        // top-2 squared displacements of the last skin test
 };

@@ -12,6 +12,13 @@ __device__ inline int same_pos(const DevState &s, int i) {
   return s.px[i] == s.cx[i] && s.py[i] == s.cy[i] && s.pz[i] == s.cz[i];
 }
 
+// a fatal condition of the reference (werr/stop) ends this replica only: its
+// kernels skip it from now on and dml_replica_status() reports why
+__device__ inline void rep_fail(const DevState &s, int r, int bit) {
+  atomicOr(&s.rp[r].err_bits, bit);
+  s.rp[r].halted = 1;
+}
+
 __device__ inline const RepParams &rep_of_block(const DevState &s, int blk,
                                                 int *r) {
   *r = s.blk2rep[blk];
@@ -56,7 +63,9 @@ __global__ void __launch_bounds__(DML_BLOCK) k_pbc_dispmax(DevState s) {
   __shared__ int last;
   int r;
   const RepParams &P = rep_of_block(s, blockIdx.x, &r);
+  if (P.halted) return;
   RepParams *Pw = s.rp + r;
+  if (Pw->halted) return;
   int slot = blockIdx.x * DML_BLOCK + threadIdx.x;
   int local = slot - P.poff;
   double d = -1.0;
@@ -114,6 +123,7 @@ __global__ void k_force_rebuild(DevState s) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= s.nrep) return;
   RepParams *Pw = s.rp + r;
+  if (Pw->halted) return;
   Pw->rebuild = 1;
   Pw->listed = 1;
   Pw->any_ghost = 0;
@@ -135,6 +145,7 @@ __device__ inline int want_cells(const RepParams &P) {
 __global__ void k_zero_cells(DevState s) {
   int r = blockIdx.y;
   const RepParams &P = s.rp[r];
+  if (P.halted) return;
   if (!want_cells(P)) return;
   int nc = ncell_total(P);
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x)
@@ -144,6 +155,7 @@ __global__ void k_zero_cells(DevState s) {
 __global__ void __launch_bounds__(DML_BLOCK) k_cell_key(DevState s) {
   int r;
   const RepParams &P = rep_of_block(s, blockIdx.x, &r);
+  if (P.halted) return;
   if (!want_cells(P)) return;
   int slot = blockIdx.x * DML_BLOCK + threadIdx.x;
   int local = slot - P.poff;
@@ -167,7 +179,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_cell_key(DevState s) {
     int c2 = (int)(z / P.cell[2]) + 1;
     if (c0 < 0 || c1 < 0 || c2 < 0 || c0 > P.ncells[0] + 1 || c1 > P.ncells[1] + 1 ||
         c2 > P.ncells[2] + 1) {
-      atomicOr(s.err, ERR_TESS);
+      rep_fail(s, r, ERR_TESS);
       return;
     }
     cell = cell_lin(P, c0, c1, c2);
@@ -186,6 +198,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_cell_key(DevState s) {
 __global__ void k_cell_sort(DevState s) {
   int r = blockIdx.y;
   const RepParams &P = s.rp[r];
+  if (P.halted) return;
   if (!want_cells(P)) return;
   int nc = ncell_total(P);
   int desc = P.use_cells;
@@ -227,6 +240,7 @@ __global__ void __launch_bounds__(WPB *WAVE) k_listbuild(DevState s) {
   int t = blockIdx.x * WPB + w;
   int r = s.blk2rep[t / DML_BLOCK];
   const RepParams &P = s.rp[r];
+  if (P.halted) return;
   if (!P.rebuild) return;
   int local = t - P.poff;
   if (local >= P.nref) return;
@@ -340,6 +354,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_listbuild_cells(DevState s) {
   __shared__ unsigned long long sh_nnsum, sh_rows;
   const int r = blockIdx.y;
   const RepParams &P = s.rp[r];
+  if (P.halted) return;
   if (!P.rebuild) return;
   const int tid = threadIdx.x, w = tid / WAVE, lane = tid % WAVE;
   const int nwav = DML_BLOCK / WAVE;
@@ -480,6 +495,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_listbuild_cells(DevState s) {
 __global__ void __launch_bounds__(DML_BLOCK) k_cell_sort_w(DevState s) {
   const int r = blockIdx.y;
   const RepParams &P = s.rp[r];
+  if (P.halted) return;
   if (!want_cells(P)) return;
   const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
   const int nc = ncell_total(P);
@@ -542,6 +558,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_ov_scan_l(DevState s, const doubl
   const int t = gid / LPA, sub = gid % LPA;
   const int r = s.blk2rep[t / DML_BLOCK];
   const RepParams &P = s.rp[r];
+  if (P.halted) return;
   if (P.overlap_mode != DML_OVERLAP_ROUNDS) return;
   if (round > 0 && !again[(round - 1) * s.nrep + r]) return;
   const int local = t - P.poff;
@@ -683,6 +700,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_force(DevState s, int only_ermak)
   int t = gid / LPA, sub = gid % LPA;
   int r = s.blk2rep[t / DML_BLOCK];
   const RepParams &P = s.rp[r];
+  if (P.halted) return;
   if (only_ermak && P.integrator != DML_ERMAK) return;
   const int FP32 = P.precision == DML_FP32_FORCE;
   int local = t - P.poff;
@@ -775,6 +793,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_ghost_scatter(DevState s, int onl
   int t = blockIdx.x * DML_BLOCK + threadIdx.x;
   int r;
   const RepParams &P = rep_of_block(s, blockIdx.x, &r);
+  if (P.halted) return;
   if (!P.any_ghost) return;
   if (only_ermak && P.integrator != DML_ERMAK) return;
   if (t - P.poff >= P.nref) return;
@@ -821,6 +840,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_ermak_b(DevState s, int only_if_g
   int t = blockIdx.x * DML_BLOCK + threadIdx.x;
   int r;
   const RepParams &P = rep_of_block(s, blockIdx.x, &r);
+  if (P.halted) return;
   if (P.integrator != DML_ERMAK) return;
   if (only_if_ghost && !P.any_ghost) return;
   if (t - P.poff >= P.nref) return;
@@ -874,6 +894,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_brownian(DevState s, const double
   int t = blockIdx.x * DML_BLOCK + threadIdx.x;
   int r;
   const RepParams &P = rep_of_block(s, blockIdx.x, &r);
+  if (P.halted) return;
   if (P.integrator != DML_BROWNIAN) return;
   if (t - P.poff >= P.nref) return;
   int i = s.ref_order[t];
@@ -903,6 +924,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_ermak_a(DevState s, const double 
   int t = blockIdx.x * DML_BLOCK + threadIdx.x;
   int r;
   const RepParams &P = rep_of_block(s, blockIdx.x, &r);
+  if (P.halted) return;
   if (P.integrator != DML_ERMAK) return;
   if (t - P.poff >= P.nref) return;
   int i = s.ref_order[t];
@@ -946,6 +968,7 @@ __global__ void __launch_bounds__(WAVE) k_ov_exact(DevState s, const double *u,
   int r = blockIdx.x;
   if (r >= s.nrep) return;
   const RepParams &P = s.rp[r];
+  if (P.halted) return;
   if (P.overlap_mode != DML_OVERLAP_EXACT) return;
   int lane = threadIdx.x;
   const double *ur = u ? u + (int64_t)r * n_u : nullptr;
@@ -955,7 +978,7 @@ __global__ void __launch_bounds__(WAVE) k_ov_exact(DevState s, const double *u,
   int again = 1;
   for (int pass = 0; again; pass++) {
     if (pass >= max_passes) {
-      if (lane == 0) atomicOr(s.err, ERR_OVERLAP);
+      if (lane == 0) rep_fail(s, r, ERR_OVERLAP);
       break;
     }
     again = 0;
@@ -996,7 +1019,7 @@ __global__ void __launch_bounds__(WAVE) k_ov_exact(DevState s, const double *u,
               double ne;
               if (ur) {
                 ne = ucount < n_u ? ur[ucount] : 0.5;
-                if (ucount >= n_u && lane == 0) atomicOr(s.err, ERR_OVERLAP);
+                if (ucount >= n_u && lane == 0) rep_fail(s, r, ERR_OVERLAP);
               } else {
                 u32x4 w = dml_philox(P.seed, s.pid[ii], (uint32_t)P.step, RS_CONTACT,
                                      (uint32_t)ucount);
@@ -1007,7 +1030,7 @@ __global__ void __launch_bounds__(WAVE) k_ov_exact(DevState s, const double *u,
                 n_depo++;
                 s.species[ii] = SP_F;
                 if (lane == 0) s.rp[r].any_f = 1;
-                if (zi > P.z0 && lane == 0) atomicOr(s.err, ERR_DEPZ0);
+                if (zi > P.z0 && lane == 0) rep_fail(s, r, ERR_DEPZ0);
               } else {
                 s.px[ii] = s.cx[ii]; s.py[ii] = s.cy[ii]; s.pz[ii] = s.cz[ii];
                 s.flags[ii] &= (uint8_t)~DML_FL_SKIP;
@@ -1054,6 +1077,7 @@ __global__ void k_ov_begin(DevState s) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= s.nrep) return;
   RepParams *Pw = s.rp + r;
+  if (Pw->halted) return;
   Pw->ov_pending = 1;
   Pw->ov_round = 0;
 }
@@ -1064,6 +1088,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_ov_scan(DevState s, const double 
   int t = blockIdx.x * DML_BLOCK + threadIdx.x;
   int r;
   const RepParams &P = rep_of_block(s, blockIdx.x, &r);
+  if (P.halted) return;
   if (P.overlap_mode != DML_OVERLAP_ROUNDS) return;
   if (round > 0 && !again[(round - 1) * s.nrep + r]) return;
   int local = t - P.poff;
@@ -1136,6 +1161,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_ov_apply(DevState s, int round,
   int t = blockIdx.x * DML_BLOCK + threadIdx.x;
   int r;
   const RepParams &P = rep_of_block(s, blockIdx.x, &r);
+  if (P.halted) return;
   if (P.overlap_mode != DML_OVERLAP_ROUNDS) return;
   if (round > 0 && !again[(round - 1) * s.nrep + r]) return;
   if (t - P.poff >= P.nref) return;
@@ -1147,7 +1173,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_ov_apply(DevState s, int round,
     atomicAdd(&s.cnt[r].depo, 1ull);
     s.species[i] = SP_F;
     s.rp[r].any_f = 1;
-    if (s.pz[i] > P.z0) atomicOr(s.err, ERR_DEPZ0);
+    if (s.pz[i] > P.z0) rep_fail(s, r, ERR_DEPZ0);
   }
   if (st & 8) {
     s.px[i] = s.cx[i]; s.py[i] = s.cy[i]; s.pz[i] = s.cz[i];
@@ -1168,10 +1194,11 @@ __global__ void k_ov_end(DevState s, int rounds, const int32_t *again, int err_i
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= s.nrep) return;
   RepParams *Pw = s.rp + r;
+  if (Pw->halted) return;
   if (Pw->overlap_mode != DML_OVERLAP_ROUNDS) return;
   int pending = again[(rounds - 1) * s.nrep + r];
   if (pending) {
-    if (err_if_pending) atomicOr(s.err, ERR_OVERLAP);
+    if (err_if_pending) rep_fail(s, r, ERR_OVERLAP);
     atomicAdd(&s.cnt[r].ov_unfinished, 1ull);
   }
   Pw->ov_pending = pending;
@@ -1193,6 +1220,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_f2cg_mark(DevState s) {
   int t = blockIdx.x * DML_BLOCK + threadIdx.x;
   int r;
   const RepParams &P = rep_of_block(s, blockIdx.x, &r);
+  if (P.halted) return;
   if (threadIdx.x == 0) cnt = 0;
   __syncthreads();
   if (P.any_f && t - P.poff < P.nref) {
@@ -1214,6 +1242,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_f2cg_mark(DevState s) {
 __global__ void k_f2cg_scan(DevState s) {
   int r = blockIdx.x;
   RepParams *Pw = s.rp + r;
+  if (Pw->halted) return;
   if (!Pw->any_f) return;
   int lane = threadIdx.x;
   int b0 = Pw->poff / DML_BLOCK, nb = Pw->pcap / DML_BLOCK;
@@ -1237,6 +1266,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_f2cg_scatter(DevState s) {
   int t = blockIdx.x * DML_BLOCK + threadIdx.x;
   int r;
   const RepParams &P = rep_of_block(s, blockIdx.x, &r);
+  if (P.halted) return;
   if (!P.any_f) return;
   int local = t - P.poff;
   int i = local < P.nref ? s.ref_order[t] : -1;
@@ -1254,6 +1284,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_f2cg_copy(DevState s) {
   int t = blockIdx.x * DML_BLOCK + threadIdx.x;
   int r;
   const RepParams &P = rep_of_block(s, blockIdx.x, &r);
+  if (P.halted) return;
   if (!P.any_f) return;
   int nnew = P.nref_new;
   int local = t - P.poff;
@@ -1268,6 +1299,7 @@ __global__ void k_f2cg_end(DevState s) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= s.nrep) return;
   RepParams *Pw = s.rp + r;
+  if (Pw->halted) return;
   if (!Pw->any_f) return;
   Pw->nref = Pw->nref_new;
   Pw->any_f = 0;
@@ -1282,6 +1314,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_calc_rho(DevState s) {
   int slot = blockIdx.x * DML_BLOCK + threadIdx.x;
   int r;
   const RepParams &P = rep_of_block(s, blockIdx.x, &r);
+  if (P.halted) return;
   if (!P.piston) return;
   if (threadIdx.x == 0) cnt = 0;
   __syncthreads();
@@ -1293,6 +1326,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_calc_rho(DevState s) {
   __syncthreads();
   if (threadIdx.x == 0) {
     RepParams *Pw = s.rp + r;
+  if (Pw->halted) return;
     atomicAdd(&Pw->rho_count, cnt);
     __threadfence();
     int ticket = atomicAdd(&Pw->rho_done, 1);
@@ -1314,6 +1348,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_maxz(DevState s) {
   int slot = blockIdx.x * DML_BLOCK + threadIdx.x;
   int r;
   const RepParams &P = rep_of_block(s, blockIdx.x, &r);
+  if (P.halted) return;
   if (!P.piston) return;
   int local = slot - P.poff;
   if (local < P.nslots && (s.flags[slot] & DML_FL_ALIVE)) {
@@ -1328,6 +1363,7 @@ __global__ void k_step_end(DevState s) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= s.nrep) return;
   RepParams *Pw = s.rp + r;
+  if (Pw->halted) return;
   if (Pw->piston) Pw->zmax = Pw->zmax - Pw->lohi * (Pw->zmax - Pw->z0);
   Pw->step += 1;
   s.cnt[r].steps += 1;
@@ -1344,6 +1380,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_gcmc_energy(DevState s, int r, in
   int w = (blockIdx.x * DML_BLOCK + threadIdx.x) / WAVE, lane = threadIdx.x % WAVE;
   if (w >= ntrial) return;
   const RepParams &P = s.rp[r];
+  if (P.halted) return;
   double x = pos3[3 * w], y = pos3[3 * w + 1], z = pos3[3 * w + 2];
   double e = 0.0;
   int h = 0;

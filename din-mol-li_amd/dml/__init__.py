@@ -32,7 +32,8 @@ EXPORTS = ["dml_create", "dml_create_ensemble", "dml_destroy", "dml_sync", "dml_
            "dml_brownian", "dml_ermak_a", "dml_ermak_b", "dml_ref_offset",
            "dml_overlap_moveback", "dml_overlap_rounds", "dml_f_to_cg", "dml_gcmc",
            "dml_gcmc_trial_energy", "dml_run", "dml_get_counters", "dml_list_stats",
-           "dml_last_run_ms", "dml_time_kernel", "dml_set_tuning"]
+           "dml_last_run_ms", "dml_time_kernel", "dml_set_tuning",
+           "dml_replica_status"]
 K_LISTBUILD, K_FORCE, K_CELLS, K_DISPMAX = 0, 1, 2, 3
 
 
@@ -103,6 +104,7 @@ def lib():
         L.dml_last_run_ms.argtypes = [vp, vp]
         L.dml_time_kernel.argtypes = [vp, C.c_int32, C.c_int32, vp]
         L.dml_set_tuning.argtypes = [vp, C.c_int32, C.c_int32]
+        L.dml_replica_status.argtypes = [vp, C.c_int32, vp]
         L.dml_sync.argtypes = [vp]
         _lib = L
     return _lib
@@ -314,6 +316,11 @@ class Context:
         _chk(self.L.dml_list_stats(self.h, rep, C.byref(m), C.byref(rb), C.byref(st)),
              "dml_list_stats")
         return m.value, rb.value, st.value
+
+    def replica_status(self, rep=0):
+        st = C.c_int32()
+        _chk(self.L.dml_replica_status(self.h, rep, C.byref(st)), "dml_replica_status")
+        return st.value
 
     def time_kernel(self, which, reps=10):
         ms = C.c_double()

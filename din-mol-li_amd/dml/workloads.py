@@ -145,4 +145,7 @@ def system_replica(n_total, seed):
     pos = np.vstack([el, ions])
     z = np.array([2] * len(el) + [1] * len(ions), np.int8)
     ion_top = ions[:, 2].max()
-    return dict(box=box, pos=pos, species=z, skin=10.0, h=1e-2, z0=0.5 * ion_top, zmax=2 * L)
+    # z0: bottom of the piston band (used by piston replicas); others keep
+    # z0 = zmax, which only the deposit-above-z0 stop reads
+    return dict(box=box, pos=pos, species=z, skin=10.0, h=1e-2, z0_piston=0.75 * ion_top,
+                z0=2 * L, zmax=2 * L)

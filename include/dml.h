@@ -179,6 +179,11 @@ int dml_gcmc_trial_energy(dml_ctx *ctx, int32_t rep, int32_t ntrial,
  * hipGraph and replays it. */
 int dml_run(dml_ctx *ctx, int32_t nsteps, int32_t use_graph);
 int dml_get_counters(dml_ctx *ctx, int32_t rep, dml_counters *out);
+/* a fatal condition of the reference (werr / stop: out of tessellation,
+ * deposit above z0, overlap not resolved) halts only its replica: dml_run
+ * continues with the others and this reports the halted replica's status
+ * (DML_OK while it runs).  dml_set_state clears it. */
+int dml_replica_status(dml_ctx *ctx, int32_t rep, int32_t *status);
 /* mean list length and rebuild count of a replica since creation */
 int dml_list_stats(dml_ctx *ctx, int32_t rep, double *mean_nn,
                    int64_t *rebuilds, int64_t *steps);

@@ -413,3 +413,36 @@ def test_ensemble_replicas_match_single_runs(dml):
         np.testing.assert_array_equal(a["species"], o["species"])
         one.close()
     ens.close()
+
+
+def test_fatal_condition_halts_only_its_replica(dml):
+    """deposit above z0 (dana.F90:905-908) stops the reference; in an ensemble
+    it halts that replica only.  Replica 1 gets z0 below every ion, so its
+    first CG contact is fatal; replica 0 keeps stepping."""
+    rng = np.random.default_rng(61)
+    box = np.array([40.0, 40.0, 60.0])
+    el = electrode(3.51, 11, 11, 2, 3.51)
+    ions = rsa(rng, 500, np.array([0, 0, 6.5]), np.array([40, 40, 59.0]), 3.2, box)
+    pos = np.vstack([el, ions])
+    z = np.array([2] * len(el) + [1] * len(ions), np.int8)
+    cfgs = [dml.dana_config(box, 10.0, len(pos) + 64, integrator=dml.BROWNIAN, h=1e-2,
+                            reservoir=dml.NO_RESERVOIR, overlap_mode=dml.OVERLAP_ROUNDS,
+                            zmax=60.0, z0=z0, seed=5 + k) for k, z0 in enumerate((60.0, 0.5))]
+    say("halt test")
+    ens = dml.Context(cfgs)
+    ens.set_state(pos, species=z, rep=0)
+    ens.set_state(pos, species=z, rep=1)
+    ens.run(30, graph=True)
+    assert ens.replica_status(0) == 0
+    assert ens.replica_status(1) == -6  # DML_E_DEPOSIT_Z0
+    s0 = ens.list_stats(0)[2]
+    s1 = ens.list_stats(1)[2]
+    assert s0 == 30 and s1 < 30
+    frozen = ens.state(rep=1)["pos"]
+    ens.run(5, graph=True)
+    assert ens.list_stats(0)[2] == 35
+    np.testing.assert_array_equal(ens.state(rep=1)["pos"], frozen)
+    # set_state clears the halt
+    ens.set_state(pos, species=z, rep=1)
+    assert ens.replica_status(1) == 0
+    ens.close()
