@@ -42,7 +42,6 @@ struct dml_ctx {
   int64_t C = 0;  // cells
   int64_t B = 0;  // bucket entries
   hipStream_t st = nullptr;
-  int32_t *again = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   int graph_rounds = 0;
@@ -187,10 +186,13 @@ int alloc_cells(dml_ctx *c) {
   hipFree(c->d.bx);
   hipFree(c->d.by);
   hipFree(c->d.bz);
+  hipFree(c->d.bf4);
   c->d.cell_cnt = nullptr;
   c->d.bucket = nullptr;
   c->d.bx = c->d.by = c->d.bz = nullptr;
+  c->d.bf4 = nullptr;
   int e = 0;
+  e |= dalloc(&c->d.bf4, B);
   e |= dalloc(&c->d.cell_cnt, C);
   e |= dalloc(&c->d.bucket, B);
   e |= dalloc(&c->d.bx, B);
@@ -237,6 +239,7 @@ void enq_cells_lists(dml_ctx *c) {
 
 void enq_test_update(dml_ctx *c) {
   hipLaunchKernelGGL(k_pbc_dispmax, dim3(c->d.nblocks), dim3(DML_BLOCK), 0, c->st, c->d);
+  hipLaunchKernelGGL(k_dispmax_final, dim3(c->nrep), dim3(DML_BLOCK), 0, c->st, c->d);
   enq_cells_lists(c);
 }
 
@@ -267,33 +270,36 @@ void enq_forces(dml_ctx *c, int fuse_b, int only_ermak) {
 
 void enq_overlap_rounds(dml_ctx *c, const double *u_round, int64_t stride, int rounds,
                         int err_if_pending) {
-  int nr = c->nrep;
-  hipLaunchKernelGGL(k_zero_i32, dim3((unsigned)((kMaxRounds * nr + 255) / 256)), dim3(256), 0,
-                     c->st, c->again, (int64_t)kMaxRounds * nr);
+  hipLaunchKernelGGL(k_ov_begin, dim3(1), dim3(64), 0, c->st, c->d);
+  const int lpa = c->lpa_ov;
   for (int q = 0; q < rounds; q++) {
-    switch (c->lpa_ov) {
+    // round 0 covers every ref position; later rounds a short worklist
+    unsigned gs = q == 0 ? (unsigned)(c->P * lpa / DML_BLOCK)
+                         : (unsigned)std::min<int64_t>(c->P * lpa / DML_BLOCK, 1024);
+    unsigned ga = q == 0 ? (unsigned)(c->P / DML_BLOCK)
+                         : (unsigned)std::min<int64_t>(c->P / DML_BLOCK, 256);
+    switch (lpa) {
       case 1:
-        hipLaunchKernelGGL(k_ov_scan, dim3(c->d.nblocks), dim3(DML_BLOCK), 0, c->st, c->d,
-                           u_round, stride, q, c->again);
+        hipLaunchKernelGGL(k_ov_scan_l<1>, dim3(gs), dim3(DML_BLOCK), 0, c->st, c->d, u_round,
+                           stride, q, c->P);
         break;
       case 4:
-        hipLaunchKernelGGL(k_ov_scan_l<4>, dim3((unsigned)(c->P * 4 / DML_BLOCK)), dim3(DML_BLOCK),
-                           0, c->st, c->d, u_round, stride, q, c->again);
+        hipLaunchKernelGGL(k_ov_scan_l<4>, dim3(gs), dim3(DML_BLOCK), 0, c->st, c->d, u_round,
+                           stride, q, c->P);
         break;
       case 16:
-        hipLaunchKernelGGL(k_ov_scan_l<16>, dim3((unsigned)(c->P * 16 / DML_BLOCK)), dim3(DML_BLOCK),
-                           0, c->st, c->d, u_round, stride, q, c->again);
+        hipLaunchKernelGGL(k_ov_scan_l<16>, dim3(gs), dim3(DML_BLOCK), 0, c->st, c->d, u_round,
+                           stride, q, c->P);
         break;
       default:
-        hipLaunchKernelGGL(k_ov_scan_l<8>, dim3((unsigned)(c->P * 8 / DML_BLOCK)), dim3(DML_BLOCK),
-                           0, c->st, c->d, u_round, stride, q, c->again);
+        hipLaunchKernelGGL(k_ov_scan_l<8>, dim3(gs), dim3(DML_BLOCK), 0, c->st, c->d, u_round,
+                           stride, q, c->P);
         break;
     }
-    hipLaunchKernelGGL(k_ov_apply, dim3(c->d.nblocks), dim3(DML_BLOCK), 0, c->st, c->d, q,
-                       (const int32_t *)c->again);
+    hipLaunchKernelGGL(k_ov_apply, dim3(ga), dim3(DML_BLOCK), 0, c->st, c->d, q, c->P);
   }
-  hipLaunchKernelGGL(k_ov_end, dim3((nr + 63) / 64), dim3(64), 0, c->st, c->d, rounds,
-                     (const int32_t *)c->again, err_if_pending);
+  hipLaunchKernelGGL(k_ov_end, dim3(64), dim3(DML_BLOCK), 0, c->st, c->d, rounds, err_if_pending);
+  hipLaunchKernelGGL(k_ov_close, dim3((c->nrep + 63) / 64), dim3(64), 0, c->st, c->d, rounds);
 }
 
 void enq_overlap(dml_ctx *c) {
@@ -561,7 +567,12 @@ int dml_create_ensemble(int32_t nrep, const dml_config *cfgs, dml_ctx **out) {
   e |= dalloc(&d.rp, nrep);
   e |= dalloc(&d.cnt, nrep);
   e |= dalloc(&d.err, 1);
-  e |= dalloc(&c->again, (int64_t)kMaxRounds * nrep);
+  e |= dalloc(&d.wl[0], P);
+  e |= dalloc(&d.wl[1], P);
+  e |= dalloc(&d.wl_cnt, 4);
+  e |= dalloc(&d.wl_stamp, P);
+  e |= dalloc(&d.ov_more, 2 * nrep);
+  e |= dalloc(&d.pf4, P);
   if (e) {
     dml_destroy(c);
     return DML_E_NOMEM;
@@ -579,6 +590,9 @@ int dml_create_ensemble(int32_t nrep, const dml_config *cfgs, dml_ctx **out) {
   hipMemsetAsync(d.nn, 0, sizeof(int32_t) * P, c->st);
   hipMemsetAsync(d.ov_rev, 0, sizeof(int32_t) * P, c->st);
   hipMemsetAsync(d.ov_st, 0, P, c->st);
+  hipMemsetAsync(d.wl_stamp, 0, sizeof(int32_t) * P, c->st);
+  hipMemsetAsync(d.wl_cnt, 0, sizeof(int32_t) * 4, c->st);
+  hipMemsetAsync(d.pf4, 0, sizeof(float4) * P, c->st);
   hipMemsetAsync(d.ref_pos, 0xff, sizeof(int32_t) * P, c->st);
   hipMemsetAsync(d.cnt, 0, sizeof(DevCounters) * nrep, c->st);
   hipMemsetAsync(d.err, 0, sizeof(int32_t), c->st);
@@ -614,7 +628,8 @@ int dml_destroy(dml_ctx *c) {
                   d.epot, d.ox, d.oy, d.oz, d.cx, d.cy, d.cz, d.rvx, d.rvy, d.rvz,
                   d.species, d.flags, d.rowflag, d.bkey, d.pid, d.ref_order, d.ref_tmp,
                   d.ref_pos, d.nn, d.nbr, d.cell_cnt, d.bucket, d.bx, d.by, d.bz, d.ov_rev,
-                  d.ov_st, d.part_d, d.part_i, d.blk2rep, d.rp, d.cnt, d.err, c->again};
+                  d.ov_st, d.part_d, d.part_i, d.blk2rep, d.rp, d.cnt, d.err, d.wl[0],
+                  d.wl[1], d.wl_cnt, d.wl_stamp, d.ov_more, d.pf4, d.bf4};
   for (void *q : ptrs)
     if (q) hipFree(q);
   if (c->e0) hipEventDestroy(c->e0);
@@ -700,6 +715,8 @@ int dml_set_state(dml_ctx *c, int32_t rep, int32_t n, const double *pos, const d
   }
   rc = upload_params(c);
   if (rc) return rc;
+  hipLaunchKernelGGL(k_init_pf4, dim3((unsigned)((c->P + DML_BLOCK - 1) / DML_BLOCK)),
+                     dim3(DML_BLOCK), 0, c->st, c->d, c->P);
   HIPCHECK(hipStreamSynchronize(c->st));
   return DML_OK;
 }

@@ -196,6 +196,13 @@ struct DevState {
   // overlap rounds
   int32_t *ov_rev;        // round stamp of a revert
   uint8_t *ov_st;         // bit0 active bit1 dominated bit2 deposit bit3 self-revert
+  int32_t *wl[2];         // overlap worklists (global ref positions), by round parity
+  int32_t *wl_cnt;        // [nW0, nR0, nW1, nR1]: active and appended-revert counts
+  int32_t *wl_stamp;      // per slot: last round stamp it was listed with
+  int32_t *ov_more;       // [2][nrep]: the replica reverted or deferred in this round
+  // FP32 mirror of the positions, w = species bits: conservative prefilters
+  float4 *pf4;            // per slot
+  float4 *bf4;            // bucket-ordered
   // reductions
   double *part_d;         // 2 per particle block
   int32_t *part_i;        // per particle block

@@ -3,6 +3,8 @@
 #include "dml_internal.h"
 
 __global__ void k_pbc_dispmax(DevState s);
+__global__ void k_dispmax_final(DevState s);
+__global__ void k_init_pf4(DevState s, int64_t n);
 __global__ void k_force_rebuild(DevState s);
 __global__ void k_zero_cells(DevState s);
 __global__ void k_cell_key(DevState s);
@@ -11,7 +13,7 @@ __global__ void k_cell_sort_w(DevState s);
 __global__ void k_listbuild_cells(DevState s);
 template <int LPA>
 __global__ void k_ov_scan_l(DevState s, const double *u_round, int64_t u_stride, int round,
-                            int32_t *again);
+                            int64_t ptotal);
 template <int WPB>
 __global__ void k_listbuild(DevState s);
 template <int LPA, int FUSEB>
@@ -22,10 +24,9 @@ __global__ void k_brownian(DevState s, const double *noise, const double *udep);
 __global__ void k_ermak_a(DevState s, const double *noise, const double *udep);
 __global__ void k_ov_exact(DevState s, const double *u, int64_t n_u, int max_passes);
 __global__ void k_ov_begin(DevState s);
-__global__ void k_ov_scan(DevState s, const double *u_round, int64_t u_stride, int round,
-                          int32_t *again);
-__global__ void k_ov_apply(DevState s, int round, const int32_t *again);
-__global__ void k_ov_end(DevState s, int rounds, const int32_t *again, int err_if_pending);
+__global__ void k_ov_apply(DevState s, int round, int64_t ptotal);
+__global__ void k_ov_end(DevState s, int rounds, int err_if_pending);
+__global__ void k_ov_close(DevState s, int rounds);
 __global__ void k_zero_i32(int32_t *a, int64_t n);
 __global__ void k_f2cg_mark(DevState s);
 __global__ void k_f2cg_scan(DevState s);

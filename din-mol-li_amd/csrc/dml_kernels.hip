@@ -12,6 +12,26 @@ __device__ inline int same_pos(const DevState &s, int i) {
   return s.px[i] == s.cx[i] && s.py[i] == s.cy[i] && s.pz[i] == s.cz[i];
 }
 
+// FP32 mirror of a slot's position and species (prefilters only; every
+// decision near a threshold is re-taken in FP64)
+__device__ inline void put_pf4(const DevState &s, int i, double x, double y, double z, int sp) {
+  s.pf4[i] = make_float4((float)x, (float)y, (float)z, __int_as_float(sp));
+}
+__device__ inline void put_pf4_sp(const DevState &s, int i, int sp) {
+  reinterpret_cast<float *>(&s.pf4[i])[3] = __int_as_float(sp);
+}
+// squared minimum-image distance in FP32 and its safe margin: coordinates
+// below 2^10 A carry at most 2^-14 A of rounding, so for pairs within 16 A
+// |d2_f32 - d2_f64| < 2e-3 A^2; the margin keeps a factor 10.
+#define PF_MARGIN 0.02f
+__device__ inline float d2_f32(const RepParams &p, float4 a, float4 b, float bx, float by,
+                               float ibx, float iby) {
+  float vx = a.x - b.x, vy = a.y - b.y, vz = a.z - b.z;
+  vx = vx - bx * rintf(vx * ibx);
+  vy = vy - by * rintf(vy * iby);
+  return (vx * vx + vy * vy) + vz * vz;
+}
+
 // a fatal condition of the reference (werr/stop) ends this replica only: its
 // kernels skip it from now on and dml_replica_status() reports why
 __device__ inline void rep_fail(const DevState &s, int r, int bit) {
@@ -60,62 +80,61 @@ __device__ inline void block_top2(double &a1, double &a2, double *sh) {
 // ------------------------------------------------------------------------
 __global__ void __launch_bounds__(DML_BLOCK) k_pbc_dispmax(DevState s) {
   __shared__ double sh[2 * DML_BLOCK / WAVE];
-  __shared__ int last;
   int r;
   const RepParams &P = rep_of_block(s, blockIdx.x, &r);
   if (P.halted) return;
-  RepParams *Pw = s.rp + r;
-  if (Pw->halted) return;
   int slot = blockIdx.x * DML_BLOCK + threadIdx.x;
   int local = slot - P.poff;
   double d = -1.0;
   if (local < P.nslots && (s.flags[slot] & DML_FL_ALIVE)) {
     double x = s.px[slot], y = s.py[slot];
     double bx = P.box[0], by = P.box[1];
+    int moved = 0;
     if (x >= bx) {
-      x -= bx; s.px[slot] = x; s.ox[slot] -= bx;
+      x -= bx; s.px[slot] = x; s.ox[slot] -= bx; moved = 1;
     } else if (x < 0.0) {
-      x += bx; s.px[slot] = x; s.ox[slot] += bx;
+      x += bx; s.px[slot] = x; s.ox[slot] += bx; moved = 1;
     }
     if (y >= by) {
-      y -= by; s.py[slot] = y; s.oy[slot] -= by;
+      y -= by; s.py[slot] = y; s.oy[slot] -= by; moved = 1;
     } else if (y < 0.0) {
-      y += by; s.py[slot] = y; s.oy[slot] += by;
+      y += by; s.py[slot] = y; s.oy[slot] += by; moved = 1;
     }
-    d = dot3(x - s.ox[slot], y - s.oy[slot], s.pz[slot] - s.oz[slot]);
+    double z = s.pz[slot];
+    if (moved) put_pf4(s, slot, x, y, z, s.species[slot]);
+    d = dot3(x - s.ox[slot], y - s.oy[slot], z - s.oz[slot]);
   }
   double a1 = d, a2 = -1.0;
   block_top2(a1, a2, sh);
   if (threadIdx.x == 0) {
     s.part_d[2 * blockIdx.x] = a1;
     s.part_d[2 * blockIdx.x + 1] = a2;
-    __threadfence();
-    int nb = P.pcap / DML_BLOCK;
-    int ticket = atomicAdd(&Pw->dispmax_done, 1);
-    last = (ticket == nb - 1);
   }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  int b0 = P.poff / DML_BLOCK, nb = P.pcap / DML_BLOCK;
-  a1 = -1.0;
-  a2 = -1.0;
+}
+
+// one block per replica merges the block partials (a kernel boundary orders
+// them; no device-scope fences, which on this part write back per-XCD L2)
+__global__ void __launch_bounds__(DML_BLOCK) k_dispmax_final(DevState s) {
+  __shared__ double sh[2 * DML_BLOCK / WAVE];
+  const int r = blockIdx.x;
+  RepParams *Pw = s.rp + r;
+  if (Pw->halted) return;
+  const int b0 = Pw->poff / DML_BLOCK, nb = Pw->pcap / DML_BLOCK;
+  double a1 = -1.0, a2 = -1.0;
   for (int q = threadIdx.x; q < nb; q += blockDim.x)
-    top2_merge(a1, a2, atomic_load_d(&s.part_d[2 * (b0 + q)]),
-               atomic_load_d(&s.part_d[2 * (b0 + q) + 1]));
+    top2_merge(a1, a2, s.part_d[2 * (b0 + q)], s.part_d[2 * (b0 + q) + 1]);
   block_top2(a1, a2, sh);
   if (threadIdx.x == 0) {
     top2_merge(a1, a2, 1e-16, 1e-16);  // initial values of inq_dispmax
     Pw->d1 = a1;
     Pw->d2 = a2;
-    int reb = (!P.listed) || (sqrt(a1) + sqrt(a2) > P.skin);
+    int reb = (!Pw->listed) || (sqrt(a1) + sqrt(a2) > Pw->skin);
     Pw->rebuild = reb;
     if (reb) {
       Pw->listed = 1;
       Pw->any_ghost = 0;
       atomicAdd(&s.cnt[r].nupd, 1ull);
     }
-    Pw->dispmax_done = 0;
   }
 }
 
@@ -224,6 +243,7 @@ __global__ void k_cell_sort(DevState s) {
       s.bx[base + q] = s.px[v];
       s.by[base + q] = s.py[v];
       s.bz[base + q] = s.pz[v];
+      s.bf4[base + q] = s.pf4[v];
     }
   }
 }
@@ -344,7 +364,7 @@ template __global__ void k_listbuild<4>(DevState s);
 #define LB_CMAX 256
 __global__ void __launch_bounds__(DML_BLOCK) k_listbuild_cells(DevState s) {
   __shared__ int32_t t_slot[LB_TILE];
-  __shared__ double t_x[LB_TILE], t_y[LB_TILE], t_z[LB_TILE];
+  __shared__ float4 t_f[LB_TILE];
   __shared__ int32_t c_slot[LB_CMAX];
   __shared__ double c_x[LB_CMAX], c_y[LB_CMAX], c_z[LB_CMAX];
   __shared__ int32_t c_nn[LB_CMAX];
@@ -362,6 +382,9 @@ __global__ void __launch_bounds__(DML_BLOCK) k_listbuild_cells(DevState s) {
   const double rl2 = P.rl2;
   const int strict = P.use_cells;
   const int maxnn = s.maxnn;
+  const float rl2lo = (float)rl2 - PF_MARGIN, rl2hi = (float)rl2 + PF_MARGIN;
+  const float bxf = (float)P.box[0], byf = (float)P.box[1];
+  const float ibx = (float)P.one_box[0], iby = (float)P.one_box[1];
   if (tid == 0) { sh_nnsum = 0; sh_rows = 0; }
   for (int c = blockIdx.x; c < nc; c += gridDim.x) {
     if (tid == 0) {
@@ -434,15 +457,14 @@ __global__ void __launch_bounds__(DML_BLOCK) k_listbuild_cells(DevState s) {
           }
           int64_t pos = st_base[lo] + (cidx - st_pre[lo]);
           t_slot[q] = s.bucket[pos];
-          t_x[q] = s.bx[pos];
-          t_y[q] = s.by[pos];
-          t_z[q] = s.bz[pos];
+          t_f[q] = s.bf4[pos];
         }
         __syncthreads();
         for (int a = w; a < A; a += nwav) {
           const int i = c_slot[a];
           if (i < 0) continue;
           const double xi = c_x[a], yi = c_y[a], zi = c_z[a];
+          const float4 fi = make_float4((float)xi, (float)yi, (float)zi, 0.f);
           int nn = c_nn[a];
           int32_t *row = s.nbr + (int64_t)i * maxnn;
           for (int q0 = 0; q0 < tn; q0 += WAVE) {
@@ -450,10 +472,16 @@ __global__ void __launch_bounds__(DML_BLOCK) k_listbuild_cells(DevState s) {
             int hit = 0, j = -1;
             if (q < tn) {
               j = t_slot[q];
-              double dx, dy, dz;
-              vdist_mic(P, t_x[q], t_y[q], t_z[q], xi, yi, zi, &dx, &dy, &dz);
-              double rd = dot3(dx, dy, dz);
-              hit = (strict ? (rd < rl2) : !(rd > rl2)) && (j != i);
+              float df = d2_f32(P, t_f[q], fi, bxf, byf, ibx, iby);
+              if (df < rl2lo) {
+                hit = 1;  // inside by more than the FP32 error
+              } else if (df <= rl2hi) {
+                double dx, dy, dz;  // near the list radius: decide in FP64
+                vdist_mic(P, s.px[j], s.py[j], s.pz[j], xi, yi, zi, &dx, &dy, &dz);
+                double rd = dot3(dx, dy, dz);
+                hit = strict ? (rd < rl2) : !(rd > rl2);
+              }
+              hit = hit && (j != i);
             }
             unsigned long long m = __ballot(hit);
             int pr = __popcll(m & ((1ull << lane) - 1ull));
@@ -519,6 +547,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_cell_sort_w(DevState s) {
         s.bx[base + rank] = s.px[v];
         s.by[base + rank] = s.py[v];
         s.bz[base + rank] = s.pz[v];
+        s.bf4[base + rank] = s.pf4[v];
       }
     } else {
       if (lane == 0) {
@@ -541,123 +570,12 @@ __global__ void __launch_bounds__(DML_BLOCK) k_cell_sort_w(DevState s) {
         s.bx[base + q] = s.px[v];
         s.by[base + q] = s.py[v];
         s.bz[base + q] = s.pz[v];
+        s.bf4[base + q] = s.pf4[v];
       }
     }
   }
 }
 
-// overlap rounds, read phase, one pass per row with LPA lanes per atom: hits
-// are rare (rcut << list radius), so each lane keeps its few hits in
-// registers; the CG contact is the first CG hit in row order, reverts are
-// the non-CG hits before it (dana.F90:875-935); domination as k_ov_scan.
-template <int LPA>
-__global__ void __launch_bounds__(DML_BLOCK) k_ov_scan_l(DevState s, const double *u_round,
-                                                          int64_t u_stride, int round,
-                                                          int32_t *again) {
-  const int gid = blockIdx.x * DML_BLOCK + threadIdx.x;
-  const int t = gid / LPA, sub = gid % LPA;
-  const int r = s.blk2rep[t / DML_BLOCK];
-  const RepParams &P = s.rp[r];
-  if (P.halted) return;
-  if (P.overlap_mode != DML_OVERLAP_ROUNDS) return;
-  if (round > 0 && !again[(round - 1) * s.nrep + r]) return;
-  const int local = t - P.poff;
-  const int active0 = local < P.nref;
-  int i = active0 ? s.ref_order[t] : 0;
-  int active = active0 && !(s.flags[i] & DML_FL_SKIP);
-  const int NH = 4;
-  int hj[NH], he[NH];
-  int nh = 0, over = 0, dom = 0, firstcg = 0x7fffffff;
-  double xi = 0, yi = 0, zi = 0;
-  int nn = 0;
-  const int32_t *row = s.nbr + (int64_t)i * s.maxnn;
-  if (active) {
-    xi = s.px[i]; yi = s.py[i]; zi = s.pz[i];
-    nn = s.nn[i];
-    for (int e = sub; e < nn; e += LPA) {
-      int j = row[e];
-      uint8_t fj = s.flags[j];
-      if (fj & DML_FL_LIMBO) continue;
-      double dx, dy, dz;
-      vdist_mic(P, xi, yi, zi, s.px[j], s.py[j], s.pz[j], &dx, &dy, &dz);
-      if (dot3(dx, dy, dz) > P.rcut2) continue;
-      if ((fj & DML_FL_REF) && !(fj & DML_FL_SKIP) && s.ref_pos[j] < local) dom = 1;
-      if (s.species[j] == SP_CG) {
-        if (e < firstcg) firstcg = e;
-      } else if (nh < NH) {
-        hj[nh] = j; he[nh] = e; nh++;
-      } else {
-        over = 1;
-      }
-    }
-  }
-  for (int off = LPA / 2; off > 0; off >>= 1) {
-    dom |= __shfl_xor(dom, off);
-    over |= __shfl_xor(over, off);
-    int o = __shfl_xor(firstcg, off);
-    firstcg = o < firstcg ? o : firstcg;
-  }
-  unsigned long long n_ch = 0, n_ch3 = 0;
-  int any = dom;
-  if (active && !dom) {
-    const int stamp = P.ov_round + round + 1;
-    if (!over) {
-      for (int q = 0; q < nh; q++) {
-        if (he[q] >= firstcg) continue;
-        int j = hj[q];
-        if (P.piston && same_pos(s, j) && same_pos(s, i)) { n_ch3++; continue; }
-        s.ov_rev[j] = stamp;
-        n_ch++;
-        any = 1;
-      }
-    } else {
-      for (int e = sub; e < nn && e < firstcg; e += LPA) {
-        int j = row[e];
-        if (s.flags[j] & DML_FL_LIMBO) continue;
-        double dx, dy, dz;
-        vdist_mic(P, xi, yi, zi, s.px[j], s.py[j], s.pz[j], &dx, &dy, &dz);
-        if (dot3(dx, dy, dz) > P.rcut2 || s.species[j] == SP_CG) continue;
-        if (P.piston && same_pos(s, j) && same_pos(s, i)) { n_ch3++; continue; }
-        s.ov_rev[j] = stamp;
-        n_ch++;
-        any = 1;
-      }
-    }
-  }
-  for (int off = LPA / 2; off > 0; off >>= 1) {
-    n_ch += __shfl_xor(n_ch, off);
-    n_ch3 += __shfl_xor(n_ch3, off);
-    any |= __shfl_xor(any, off);
-  }
-  if (active0 && sub == 0) {
-    uint8_t st = 0;
-    if (active) {
-      st = 1;
-      if (dom) {
-        st |= 2;
-      } else if (firstcg != 0x7fffffff) {
-        atomicAdd(&s.cnt[r].try_, 1ull);
-        double ne;
-        if (u_round) {
-          ne = u_round[(int64_t)round * u_stride + i];
-        } else {
-          u32x4 wv = dml_philox(P.seed, s.pid[i], (uint32_t)P.step, RS_CONTACT,
-                                (uint32_t)(1000 + round));
-          ne = u01_from(wv.x, wv.y);
-        }
-        st |= (ne < P.prob) ? 4 : 8;
-      }
-    }
-    s.ov_st[i] = st;
-    if (n_ch) atomicAdd(&s.cnt[r].choques, n_ch);
-    if (n_ch3) atomicAdd(&s.cnt[r].choques3, n_ch3);
-    if (any) again[round * s.nrep + r] = 1;
-  }
-}
-template __global__ void k_ov_scan_l<1>(DevState, const double *, int64_t, int, int32_t *);
-template __global__ void k_ov_scan_l<4>(DevState, const double *, int64_t, int, int32_t *);
-template __global__ void k_ov_scan_l<8>(DevState, const double *, int64_t, int, int32_t *);
-template __global__ void k_ov_scan_l<16>(DevState, const double *, int64_t, int, int32_t *);
 
 
 // ------------------------------------------------------------------------
@@ -714,8 +632,16 @@ __global__ void __launch_bounds__(DML_BLOCK) k_force(DevState s, int only_ermak)
     int ghost = s.rowflag[i] & 1;
     int nn = s.nn[i];
     const int32_t *row = s.nbr + (int64_t)i * s.maxnn;
+    const float4 fi = s.pf4[i];
+    const float bxf = (float)P.box[0], byf = (float)P.box[1];
+    const float ibx = (float)P.one_box[0], iby = (float)P.one_box[1];
     for (int e = sub; e < nn; e += LPA) {
       int j = row[e];
+      const float4 fj4 = s.pf4[j];
+      {  // surely beyond the pair's r0: no force, no energy
+        float r0f = (float)P.r0[(k - 1) * 3 + (__float_as_int(fj4.w) - 1)];
+        if (d2_f32(P, fi, fj4, bxf, byf, ibx, iby) > r0f * r0f + PF_MARGIN) continue;
+      }
       uint8_t fj = s.flags[j];
       if (fj & DML_FL_LIMBO) continue;
       int m = s.species[j];
@@ -916,6 +842,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_brownian(DevState s, const double
   s.vz[i] = (nz - z) / P.h;
   int depos = atom_pbc(s, P, r, i, t, nx, ny, nz, udep);
   s.px[i] = nx; s.py[i] = ny; s.pz[i] = nz;
+  put_pf4(s, i, nx, ny, nz, s.species[i]);
   if (!depos) s.flags[i] &= (uint8_t)~DML_FL_SKIP;
 }
 
@@ -952,6 +879,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_ermak_a(DevState s, const double 
   s.rvx[i] = rv[0]; s.rvy[i] = rv[1]; s.rvz[i] = rv[2];
   int depos = atom_pbc(s, P, r, i, t, np[0], np[1], np[2], udep);
   s.px[i] = np[0]; s.py[i] = np[1]; s.pz[i] = np[2];
+  put_pf4(s, i, np[0], np[1], np[2], s.species[i]);
   if (!depos) s.flags[i] &= (uint8_t)~DML_FL_SKIP;
 }
 
@@ -1029,10 +957,12 @@ __global__ void __launch_bounds__(WAVE) k_ov_exact(DevState s, const double *u,
               if (ne < P.prob) {
                 n_depo++;
                 s.species[ii] = SP_F;
+                put_pf4_sp(s, ii, SP_F);
                 if (lane == 0) s.rp[r].any_f = 1;
                 if (zi > P.z0 && lane == 0) rep_fail(s, r, ERR_DEPZ0);
               } else {
                 s.px[ii] = s.cx[ii]; s.py[ii] = s.cy[ii]; s.pz[ii] = s.cz[ii];
+                put_pf4(s, ii, s.cx[ii], s.cy[ii], s.cz[ii], s.species[ii]);
                 s.flags[ii] &= (uint8_t)~DML_FL_SKIP;
                 reverted = 1;
               }
@@ -1044,6 +974,7 @@ __global__ void __launch_bounds__(WAVE) k_ov_exact(DevState s, const double *u,
               continue;
             }
             s.px[jj] = s.cx[jj]; s.py[jj] = s.cy[jj]; s.pz[jj] = s.cz[jj];
+            put_pf4(s, jj, s.cx[jj], s.cy[jj], s.cz[jj], s.species[jj]);
             s.vx[jj] = 0.0; s.vy[jj] = 0.0; s.vz[jj] = 0.0;
             s.ax[jj] = 0.0; s.ay[jj] = 0.0; s.az[jj] = 0.0;
             s.flags[jj] &= (uint8_t)~DML_FL_SKIP;
@@ -1071,139 +1002,214 @@ __global__ void __launch_bounds__(WAVE) k_ov_exact(DevState s, const double *u,
   }
 }
 
-// deterministic parallel rounds (this build's schedule): see orc.c
-// orc_overlap_rounds for the specification the kernels follow.
+// deterministic parallel rounds (this build's schedule; orc.c
+// orc_overlap_rounds is its specification).  Round 0 covers every ref
+// position; round r > 0 covers the worklist of atoms left with skip = 0 by
+// round r-1, plus the atoms reverted in round r.  Worklist entries are global
+// ref positions, deduplicated by a per-slot round stamp.
+__device__ inline int ov_stamp(const RepParams &P, int round) { return P.ov_round + round + 1; }
+
 __global__ void k_ov_begin(DevState s) {
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= s.nrep) return;
-  RepParams *Pw = s.rp + r;
-  if (Pw->halted) return;
-  Pw->ov_pending = 1;
-  Pw->ov_round = 0;
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < 4) s.wl_cnt[q] = 0;
+  for (int k = q; k < 2 * s.nrep; k += gridDim.x * blockDim.x) s.ov_more[k] = 0;
 }
 
-__global__ void __launch_bounds__(DML_BLOCK) k_ov_scan(DevState s, const double *u_round,
-                                                        int64_t u_stride, int round,
-                                                        int32_t *again) {
-  int t = blockIdx.x * DML_BLOCK + threadIdx.x;
-  int r;
-  const RepParams &P = rep_of_block(s, blockIdx.x, &r);
-  if (P.halted) return;
-  if (P.overlap_mode != DML_OVERLAP_ROUNDS) return;
-  if (round > 0 && !again[(round - 1) * s.nrep + r]) return;
-  int local = t - P.poff;
-  if (local >= P.nref) return;
-  int i = s.ref_order[t];
-  uint8_t fi = s.flags[i];
-  if (fi & DML_FL_SKIP) {
-    s.ov_st[i] = 0;
-    return;
-  }
-  double xi = s.px[i], yi = s.py[i], zi = s.pz[i];
-  int nn = s.nn[i];
-  const int32_t *row = s.nbr + (int64_t)i * s.maxnn;
-  int dom = 0;
-  for (int e = 0; e < nn; e++) {
-    int j = row[e];
-    uint8_t fj = s.flags[j];
-    if ((fj & DML_FL_LIMBO) || !(fj & DML_FL_REF) || (fj & DML_FL_SKIP)) continue;
-    if (s.ref_pos[j] >= local) continue;
-    double dx, dy, dz;
-    vdist_mic(P, xi, yi, zi, s.px[j], s.py[j], s.pz[j], &dx, &dy, &dz);
-    if (dot3(dx, dy, dz) <= P.rcut2) {
-      dom = 1;
-      break;
+template <int LPA>
+__global__ void __launch_bounds__(DML_BLOCK) k_ov_scan_l(DevState s, const double *u_round,
+                                                          int64_t u_stride, int round,
+                                                          int64_t ptotal) {
+  const int cur = round & 1, nxt = cur ^ 1;
+  if (blockIdx.x == 0) {  // next round's lists and progress flags start empty
+    if (threadIdx.x == 0) {
+      s.wl_cnt[2 * nxt] = 0;
+      s.wl_cnt[2 * nxt + 1] = 0;
     }
+    for (int q = threadIdx.x; q < s.nrep; q += DML_BLOCK) s.ov_more[nxt * s.nrep + q] = 0;
   }
-  uint8_t st = 1;
-  int any = dom;
-  unsigned long long n_ch = 0, n_ch3 = 0;
-  if (dom) {
-    st |= 2;
-  } else {
-    int stamp = P.ov_round + round + 1;
-    for (int e = 0; e < nn; e++) {
+  const int64_t nitems = round == 0 ? ptotal : s.wl_cnt[2 * cur];
+  const int gtid = blockIdx.x * DML_BLOCK + threadIdx.x;
+  const int sub = gtid % LPA;
+  const int64_t stride = (int64_t)gridDim.x * (DML_BLOCK / LPA);
+  for (int64_t item = gtid / LPA; item < nitems; item += stride) {
+    const int t = round == 0 ? (int)item : s.wl[cur][item];
+    const int r = s.blk2rep[t / DML_BLOCK];
+    const RepParams &P = s.rp[r];
+    if (P.halted || P.overlap_mode != DML_OVERLAP_ROUNDS) continue;
+    const int local = t - P.poff;
+    if (local >= P.nref) continue;
+    const int i = s.ref_order[t];
+    if (s.flags[i] & DML_FL_SKIP) {
+      if (sub == 0) s.ov_st[i] = 0;
+      continue;
+    }
+    const double xi = s.px[i], yi = s.py[i], zi = s.pz[i];
+    const float4 fi = s.pf4[i];
+    const float bxf = (float)P.box[0], byf = (float)P.box[1];
+    const float ibx = (float)P.one_box[0], iby = (float)P.one_box[1];
+    const float rcf = (float)P.rcut2 + PF_MARGIN;
+    const int nn = s.nn[i];
+    const int32_t *row = s.nbr + (int64_t)i * s.maxnn;
+    const int NH = 4;
+    int hj[NH], he[NH];
+    int nh = 0, over = 0, dom = 0, firstcg = 0x7fffffff;
+    for (int e = sub; e < nn; e += LPA) {
       int j = row[e];
-      if (s.flags[j] & DML_FL_LIMBO) continue;
+      if (d2_f32(P, fi, s.pf4[j], bxf, byf, ibx, iby) > rcf) continue;  // surely no hit
+      uint8_t fj = s.flags[j];
+      if (fj & DML_FL_LIMBO) continue;
       double dx, dy, dz;
       vdist_mic(P, xi, yi, zi, s.px[j], s.py[j], s.pz[j], &dx, &dy, &dz);
       if (dot3(dx, dy, dz) > P.rcut2) continue;
+      if ((fj & DML_FL_REF) && !(fj & DML_FL_SKIP) && s.ref_pos[j] < local) dom = 1;
       if (s.species[j] == SP_CG) {
+        if (e < firstcg) firstcg = e;
+      } else if (nh < NH) {
+        hj[nh] = j; he[nh] = e; nh++;
+      } else {
+        over = 1;
+      }
+    }
+    for (int off = LPA / 2; off > 0; off >>= 1) {
+      dom |= __shfl_xor(dom, off, LPA);
+      over |= __shfl_xor(over, off, LPA);
+      int o = __shfl_xor(firstcg, off, LPA);
+      firstcg = o < firstcg ? o : firstcg;
+    }
+    unsigned long long n_ch = 0, n_ch3 = 0;
+    const int stamp = ov_stamp(P, round);
+    auto revert = [&](int j) {
+      if (P.piston && same_pos(s, j) && same_pos(s, i)) { n_ch3++; return; }
+      s.ov_rev[j] = stamp;
+      n_ch++;
+      if (round > 0 && atomicExch(&s.wl_stamp[j], stamp) != stamp) {
+        int k = atomicAdd(&s.wl_cnt[2 * cur + 1], 1);
+        s.wl[cur][s.wl_cnt[2 * cur] + k] = P.poff + s.ref_pos[j];
+      }
+    };
+    if (!dom) {
+      if (!over) {
+        for (int q = 0; q < nh; q++)
+          if (he[q] < firstcg) revert(hj[q]);
+      } else {
+        for (int e = sub; e < nn && e < firstcg; e += LPA) {
+          int j = row[e];
+          if (s.flags[j] & DML_FL_LIMBO) continue;
+          double dx, dy, dz;
+          vdist_mic(P, xi, yi, zi, s.px[j], s.py[j], s.pz[j], &dx, &dy, &dz);
+          if (dot3(dx, dy, dz) > P.rcut2 || s.species[j] == SP_CG) continue;
+          revert(j);
+        }
+      }
+    }
+    for (int off = LPA / 2; off > 0; off >>= 1) {
+      n_ch += __shfl_xor(n_ch, off, LPA);
+      n_ch3 += __shfl_xor(n_ch3, off, LPA);
+    }
+    if (sub == 0) {
+      uint8_t st = 1;
+      if (dom) {
+        st |= 2;
+      } else if (firstcg != 0x7fffffff) {
         atomicAdd(&s.cnt[r].try_, 1ull);
         double ne;
         if (u_round) {
           ne = u_round[(int64_t)round * u_stride + i];
         } else {
-          u32x4 w = dml_philox(P.seed, s.pid[i], (uint32_t)P.step, RS_CONTACT,
-                               (uint32_t)(1000 + round));
-          ne = u01_from(w.x, w.y);
+          u32x4 wv = dml_philox(P.seed, s.pid[i], (uint32_t)P.step, RS_CONTACT,
+                                (uint32_t)(1000 + round));
+          ne = u01_from(wv.x, wv.y);
         }
         st |= (ne < P.prob) ? 4 : 8;
-        break;
       }
-      if (P.piston && same_pos(s, j) && same_pos(s, i)) {
-        n_ch3++;
-        continue;
-      }
-      s.ov_rev[j] = stamp;
-      n_ch++;
-      any = 1;
+      s.ov_st[i] = st;
+      if (n_ch) atomicAdd(&s.cnt[r].choques, n_ch);
+      if (n_ch3) atomicAdd(&s.cnt[r].choques3, n_ch3);
+      // a revert or a dominated atom: the sequential pass would recurse
+      if (dom || n_ch) s.ov_more[cur * s.nrep + r] = 1;
     }
   }
-  s.ov_st[i] = st;
-  if (n_ch) atomicAdd(&s.cnt[r].choques, n_ch);
-  if (n_ch3) atomicAdd(&s.cnt[r].choques3, n_ch3);
-  if (any) again[round * s.nrep + r] = 1;
+}
+template __global__ void k_ov_scan_l<1>(DevState, const double *, int64_t, int, int64_t);
+template __global__ void k_ov_scan_l<4>(DevState, const double *, int64_t, int, int64_t);
+template __global__ void k_ov_scan_l<8>(DevState, const double *, int64_t, int, int64_t);
+template __global__ void k_ov_scan_l<16>(DevState, const double *, int64_t, int, int64_t);
+
+__global__ void __launch_bounds__(DML_BLOCK) k_ov_apply(DevState s, int round, int64_t ptotal) {
+  const int cur = round & 1, nxt = cur ^ 1;
+  const int64_t nitems = round == 0 ? ptotal : (int64_t)s.wl_cnt[2 * cur] + s.wl_cnt[2 * cur + 1];
+  const int64_t stride = (int64_t)gridDim.x * DML_BLOCK;
+  for (int64_t item = blockIdx.x * DML_BLOCK + threadIdx.x; item < nitems; item += stride) {
+    const int t = round == 0 ? (int)item : s.wl[cur][item];
+    const int r = s.blk2rep[t / DML_BLOCK];
+    const RepParams &P = s.rp[r];
+    if (P.halted || P.overlap_mode != DML_OVERLAP_ROUNDS) continue;
+    if (t - P.poff >= P.nref) continue;
+    const int i = s.ref_order[t];
+    const uint8_t st = s.ov_st[i];
+    uint8_t f = s.flags[i];
+    int moved = 0;
+    if ((st & 1) && !(st & 2)) f |= DML_FL_SKIP;
+    if (st & 4) {
+      atomicAdd(&s.cnt[r].depo, 1ull);
+      s.species[i] = SP_F;
+      s.rp[r].any_f = 1;
+      moved = 1;
+      if (s.pz[i] > P.z0) rep_fail(s, r, ERR_DEPZ0);
+    }
+    if (st & 8) {
+      s.px[i] = s.cx[i]; s.py[i] = s.cy[i]; s.pz[i] = s.cz[i];
+      f &= (uint8_t)~DML_FL_SKIP;
+      moved = 1;
+    }
+    if (s.ov_rev[i] == ov_stamp(P, round)) {
+      s.px[i] = s.cx[i]; s.py[i] = s.cy[i]; s.pz[i] = s.cz[i];
+      s.vx[i] = 0.0; s.vy[i] = 0.0; s.vz[i] = 0.0;
+      s.ax[i] = 0.0; s.ay[i] = 0.0; s.az[i] = 0.0;
+      f &= (uint8_t)~DML_FL_SKIP;
+      moved = 1;
+    }
+    s.flags[i] = f;
+    s.ov_st[i] = 0;
+    if (moved) put_pf4(s, i, s.px[i], s.py[i], s.pz[i], s.species[i]);
+    // still to resolve: next round's worklist -- unless this replica made no
+    // progress (no revert, no dominated atom), where the reference's
+    // recursion ends and a failed-deposit atom keeps skip = 0 untouched
+    if (!(f & DML_FL_SKIP) && s.ov_more[cur * s.nrep + r]) {
+      const int st2 = ov_stamp(P, round + 1);
+      if (atomicExch(&s.wl_stamp[i], st2) != st2) {
+        int k = atomicAdd(&s.wl_cnt[2 * nxt], 1);
+        s.wl[nxt][k] = t;
+      }
+    }
+  }
 }
 
-__global__ void __launch_bounds__(DML_BLOCK) k_ov_apply(DevState s, int round,
-                                                         const int32_t *again) {
-  int t = blockIdx.x * DML_BLOCK + threadIdx.x;
-  int r;
-  const RepParams &P = rep_of_block(s, blockIdx.x, &r);
-  if (P.halted) return;
-  if (P.overlap_mode != DML_OVERLAP_ROUNDS) return;
-  if (round > 0 && !again[(round - 1) * s.nrep + r]) return;
-  if (t - P.poff >= P.nref) return;
-  int i = s.ref_order[t];
-  uint8_t st = s.ov_st[i];
-  uint8_t f = s.flags[i];
-  if ((st & 1) && !(st & 2)) f |= DML_FL_SKIP;
-  if (st & 4) {
-    atomicAdd(&s.cnt[r].depo, 1ull);
-    s.species[i] = SP_F;
-    s.rp[r].any_f = 1;
-    if (s.pz[i] > P.z0) rep_fail(s, r, ERR_DEPZ0);
-  }
-  if (st & 8) {
-    s.px[i] = s.cx[i]; s.py[i] = s.cy[i]; s.pz[i] = s.cz[i];
-    f &= (uint8_t)~DML_FL_SKIP;
-  }
-  if (s.ov_rev[i] == P.ov_round + round + 1) {
-    s.px[i] = s.cx[i]; s.py[i] = s.cy[i]; s.pz[i] = s.cz[i];
-    s.vx[i] = 0.0; s.vy[i] = 0.0; s.vz[i] = 0.0;
-    s.ax[i] = 0.0; s.ay[i] = 0.0; s.az[i] = 0.0;
-    f &= (uint8_t)~DML_FL_SKIP;
-  }
-  s.flags[i] = f;
-  s.ov_st[i] = 0;
-}
-
-// after the rounds: advance the stamp base, flag an unfinished replica
-__global__ void k_ov_end(DevState s, int rounds, const int32_t *again, int err_if_pending) {
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= s.nrep) return;
-  RepParams *Pw = s.rp + r;
-  if (Pw->halted) return;
-  if (Pw->overlap_mode != DML_OVERLAP_ROUNDS) return;
-  int pending = again[(rounds - 1) * s.nrep + r];
-  if (pending) {
+// after the rounds: replicas with atoms still listed did not finish
+__global__ void __launch_bounds__(DML_BLOCK) k_ov_end(DevState s, int rounds, int err_if_pending) {
+  const int cur = rounds & 1;
+  const int64_t n = s.wl_cnt[2 * cur];
+  for (int64_t q = blockIdx.x * DML_BLOCK + threadIdx.x; q < n; q += (int64_t)gridDim.x * DML_BLOCK) {
+    int t = s.wl[cur][q];
+    int r = s.blk2rep[t / DML_BLOCK];
     if (err_if_pending) rep_fail(s, r, ERR_OVERLAP);
     atomicAdd(&s.cnt[r].ov_unfinished, 1ull);
   }
-  Pw->ov_pending = pending;
-  Pw->ov_round += rounds + 1;
+}
+
+// advance every replica's stamp base past this call's rounds
+__global__ void k_ov_close(DevState s, int rounds) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= s.nrep) return;
+  RepParams *Pw = s.rp + r;
+  if (Pw->overlap_mode != DML_OVERLAP_ROUNDS) return;
+  Pw->ov_round += rounds + 2;
   atomicAdd(&s.cnt[r].passes, 1ull);
+}
+
+__global__ void k_init_pf4(DevState s, int64_t n) {
+  int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (q < n) put_pf4(s, (int)q, s.px[q], s.py[q], s.pz[q], s.species[q]);
 }
 
 __global__ void k_zero_i32(int32_t *a, int64_t n) {
@@ -1228,6 +1234,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_f2cg_mark(DevState s) {
     int keep = 1;
     if (s.species[i] == SP_F) {
       s.species[i] = SP_CG;
+      put_pf4_sp(s, i, SP_CG);
       s.flags[i] &= (uint8_t)~(DML_FL_REF | DML_FL_GCMC);
       s.ref_pos[i] = -1;
       keep = 0;
@@ -1310,7 +1317,6 @@ __global__ void k_f2cg_end(DevState s) {
 // ------------------------------------------------------------------------
 __global__ void __launch_bounds__(DML_BLOCK) k_calc_rho(DevState s) {
   __shared__ int cnt;
-  __shared__ int last;
   int slot = blockIdx.x * DML_BLOCK + threadIdx.x;
   int r;
   const RepParams &P = rep_of_block(s, blockIdx.x, &r);
@@ -1324,24 +1330,15 @@ __global__ void __launch_bounds__(DML_BLOCK) k_calc_rho(DevState s) {
     if (z > P.z0 && z < P.zmax) atomicAdd(&cnt, 1);
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    RepParams *Pw = s.rp + r;
-  if (Pw->halted) return;
-    atomicAdd(&Pw->rho_count, cnt);
-    __threadfence();
-    int ticket = atomicAdd(&Pw->rho_done, 1);
-    last = ticket == P.pcap / DML_BLOCK - 1;
-    if (last) {
-      __threadfence();
-      int g = __hip_atomic_load(&Pw->rho_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      double vol = P.box[0] * P.box[1] * (P.zmax - P.z0);
-      double rho = (double)g / vol;
-      const double tau = 0.1;
-      Pw->lohi = ((P.h / tau) * ((P.rho0 - rho) / rho));
-      Pw->rho_count = 0;
-      Pw->rho_done = 0;
-    }
-  }
+  if (threadIdx.x == 0 && cnt) atomicAdd(&s.rp[r].rho_count, cnt);
+}
+
+// the piston factor of maxz from the count of calc_rho (dana.F90:785)
+__device__ inline double piston_lohi(const RepParams &P) {
+  double vol = P.box[0] * P.box[1] * (P.zmax - P.z0);
+  double rho = (double)P.rho_count / vol;
+  const double tau = 0.1;
+  return ((P.h / tau) * ((P.rho0 - rho) / rho));
 }
 
 __global__ void __launch_bounds__(DML_BLOCK) k_maxz(DevState s) {
@@ -1353,7 +1350,12 @@ __global__ void __launch_bounds__(DML_BLOCK) k_maxz(DevState s) {
   int local = slot - P.poff;
   if (local < P.nslots && (s.flags[slot] & DML_FL_ALIVE)) {
     double z = s.pz[slot];
-    if (z > P.z0) s.pz[slot] = z - P.lohi * (z - P.z0);
+    if (z > P.z0) {
+      double lohi = piston_lohi(P);
+      double nz = z - lohi * (z - P.z0);
+      s.pz[slot] = nz;
+      put_pf4(s, slot, s.px[slot], s.py[slot], nz, s.species[slot]);
+    }
   }
 }
 
@@ -1364,7 +1366,12 @@ __global__ void k_step_end(DevState s) {
   if (r >= s.nrep) return;
   RepParams *Pw = s.rp + r;
   if (Pw->halted) return;
-  if (Pw->piston) Pw->zmax = Pw->zmax - Pw->lohi * (Pw->zmax - Pw->z0);
+  if (Pw->piston) {
+    double lohi = piston_lohi(*Pw);
+    Pw->lohi = lohi;
+    Pw->zmax = Pw->zmax - lohi * (Pw->zmax - Pw->z0);
+    Pw->rho_count = 0;
+  }
   Pw->step += 1;
   s.cnt[r].steps += 1;
 }
