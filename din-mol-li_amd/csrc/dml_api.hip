@@ -273,11 +273,8 @@ void enq_overlap_rounds(dml_ctx *c, const double *u_round, int64_t stride, int r
   hipLaunchKernelGGL(k_ov_begin, dim3(1), dim3(64), 0, c->st, c->d);
   const int lpa = c->lpa_ov;
   for (int q = 0; q < rounds; q++) {
-    // round 0 covers every ref position; later rounds a short worklist
-    unsigned gs = q == 0 ? (unsigned)(c->P * lpa / DML_BLOCK)
-                         : (unsigned)std::min<int64_t>(c->P * lpa / DML_BLOCK, 1024);
-    unsigned ga = q == 0 ? (unsigned)(c->P / DML_BLOCK)
-                         : (unsigned)std::min<int64_t>(c->P / DML_BLOCK, 256);
+    unsigned gs = (unsigned)(c->P * lpa / DML_BLOCK);
+    unsigned ga = (unsigned)(c->P / DML_BLOCK);
     switch (lpa) {
       case 1:
         hipLaunchKernelGGL(k_ov_scan_l<1>, dim3(gs), dim3(DML_BLOCK), 0, c->st, c->d, u_round,
@@ -298,7 +295,8 @@ void enq_overlap_rounds(dml_ctx *c, const double *u_round, int64_t stride, int r
     }
     hipLaunchKernelGGL(k_ov_apply, dim3(ga), dim3(DML_BLOCK), 0, c->st, c->d, q, c->P);
   }
-  hipLaunchKernelGGL(k_ov_end, dim3(64), dim3(DML_BLOCK), 0, c->st, c->d, rounds, err_if_pending);
+  hipLaunchKernelGGL(k_ov_end, dim3((c->nrep + 63) / 64), dim3(64), 0, c->st, c->d, rounds,
+                     err_if_pending);
   hipLaunchKernelGGL(k_ov_close, dim3((c->nrep + 63) / 64), dim3(64), 0, c->st, c->d, rounds);
 }
 
@@ -571,7 +569,7 @@ int dml_create_ensemble(int32_t nrep, const dml_config *cfgs, dml_ctx **out) {
   e |= dalloc(&d.wl[1], P);
   e |= dalloc(&d.wl_cnt, 4);
   e |= dalloc(&d.wl_stamp, P);
-  e |= dalloc(&d.ov_more, 2 * nrep);
+  e |= dalloc(&d.ov_more, 3 * nrep);
   e |= dalloc(&d.pf4, P);
   if (e) {
     dml_destroy(c);

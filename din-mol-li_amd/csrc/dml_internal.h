@@ -199,7 +199,7 @@ struct DevState {
   int32_t *wl[2];         // overlap worklists (global ref positions), by round parity
   int32_t *wl_cnt;        // [nW0, nR0, nW1, nR1]: active and appended-revert counts
   int32_t *wl_stamp;      // per slot: last round stamp it was listed with
-  int32_t *ov_more;       // [2][nrep]: the replica reverted or deferred in this round
+  int32_t *ov_more;       // [3][nrep]: the replica reverted or deferred in a round
   // FP32 mirror of the positions, w = species bits: conservative prefilters
   float4 *pf4;            // per slot
   float4 *bf4;            // bucket-ordered

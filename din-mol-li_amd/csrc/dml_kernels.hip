@@ -1011,31 +1011,29 @@ __device__ inline int ov_stamp(const RepParams &P, int round) { return P.ov_roun
 
 __global__ void k_ov_begin(DevState s) {
   int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q < 4) s.wl_cnt[q] = 0;
-  for (int k = q; k < 2 * s.nrep; k += gridDim.x * blockDim.x) s.ov_more[k] = 0;
+  for (int k = q; k < 3 * s.nrep; k += gridDim.x * blockDim.x) s.ov_more[k] = 0;
 }
 
 template <int LPA>
 __global__ void __launch_bounds__(DML_BLOCK) k_ov_scan_l(DevState s, const double *u_round,
                                                           int64_t u_stride, int round,
                                                           int64_t ptotal) {
-  const int cur = round & 1, nxt = cur ^ 1;
-  if (blockIdx.x == 0) {  // next round's lists and progress flags start empty
-    if (threadIdx.x == 0) {
-      s.wl_cnt[2 * nxt] = 0;
-      s.wl_cnt[2 * nxt + 1] = 0;
-    }
+  // progress flags rotate over three buffers: this round writes round % 3,
+  // is gated by (round - 1) % 3, and clears (round + 1) % 3 for the next one
+  const int cur = round % 3, prv = (round + 2) % 3, nxt = (round + 1) % 3;
+  if (blockIdx.x == 0)
     for (int q = threadIdx.x; q < s.nrep; q += DML_BLOCK) s.ov_more[nxt * s.nrep + q] = 0;
-  }
-  const int64_t nitems = round == 0 ? ptotal : s.wl_cnt[2 * cur];
+  const int64_t nitems = ptotal;
   const int gtid = blockIdx.x * DML_BLOCK + threadIdx.x;
   const int sub = gtid % LPA;
   const int64_t stride = (int64_t)gridDim.x * (DML_BLOCK / LPA);
   for (int64_t item = gtid / LPA; item < nitems; item += stride) {
-    const int t = round == 0 ? (int)item : s.wl[cur][item];
+    const int t = (int)item;
     const int r = s.blk2rep[t / DML_BLOCK];
     const RepParams &P = s.rp[r];
     if (P.halted || P.overlap_mode != DML_OVERLAP_ROUNDS) continue;
+    // the sequential pass recurses only after a revert or a deferred atom
+    if (round > 0 && !s.ov_more[prv * s.nrep + r]) continue;
     const int local = t - P.poff;
     if (local >= P.nref) continue;
     const int i = s.ref_order[t];
@@ -1082,10 +1080,6 @@ __global__ void __launch_bounds__(DML_BLOCK) k_ov_scan_l(DevState s, const doubl
       if (P.piston && same_pos(s, j) && same_pos(s, i)) { n_ch3++; return; }
       s.ov_rev[j] = stamp;
       n_ch++;
-      if (round > 0 && atomicExch(&s.wl_stamp[j], stamp) != stamp) {
-        int k = atomicAdd(&s.wl_cnt[2 * cur + 1], 1);
-        s.wl[cur][s.wl_cnt[2 * cur] + k] = P.poff + s.ref_pos[j];
-      }
     };
     if (!dom) {
       if (!over) {
@@ -1136,14 +1130,14 @@ template __global__ void k_ov_scan_l<8>(DevState, const double *, int64_t, int, 
 template __global__ void k_ov_scan_l<16>(DevState, const double *, int64_t, int, int64_t);
 
 __global__ void __launch_bounds__(DML_BLOCK) k_ov_apply(DevState s, int round, int64_t ptotal) {
-  const int cur = round & 1, nxt = cur ^ 1;
-  const int64_t nitems = round == 0 ? ptotal : (int64_t)s.wl_cnt[2 * cur] + s.wl_cnt[2 * cur + 1];
+  const int prv = (round + 2) % 3;
   const int64_t stride = (int64_t)gridDim.x * DML_BLOCK;
-  for (int64_t item = blockIdx.x * DML_BLOCK + threadIdx.x; item < nitems; item += stride) {
-    const int t = round == 0 ? (int)item : s.wl[cur][item];
+  for (int64_t item = blockIdx.x * DML_BLOCK + threadIdx.x; item < ptotal; item += stride) {
+    const int t = (int)item;
     const int r = s.blk2rep[t / DML_BLOCK];
     const RepParams &P = s.rp[r];
     if (P.halted || P.overlap_mode != DML_OVERLAP_ROUNDS) continue;
+    if (round > 0 && !s.ov_more[prv * s.nrep + r]) continue;  // scan skipped it
     if (t - P.poff >= P.nref) continue;
     const int i = s.ref_order[t];
     const uint8_t st = s.ov_st[i];
@@ -1169,29 +1163,21 @@ __global__ void __launch_bounds__(DML_BLOCK) k_ov_apply(DevState s, int round, i
       f &= (uint8_t)~DML_FL_SKIP;
       moved = 1;
     }
-    s.flags[i] = f;
-    s.ov_st[i] = 0;
-    if (moved) put_pf4(s, i, s.px[i], s.py[i], s.pz[i], s.species[i]);
-    // still to resolve: next round's worklist -- unless this replica made no
-    // progress (no revert, no dominated atom), where the reference's
-    // recursion ends and a failed-deposit atom keeps skip = 0 untouched
-    if (!(f & DML_FL_SKIP) && s.ov_more[cur * s.nrep + r]) {
-      const int st2 = ov_stamp(P, round + 1);
-      if (atomicExch(&s.wl_stamp[i], st2) != st2) {
-        int k = atomicAdd(&s.wl_cnt[2 * nxt], 1);
-        s.wl[nxt][k] = t;
-      }
+    if (st || moved) {
+      s.flags[i] = f;
+      s.ov_st[i] = 0;
     }
+    if (moved) put_pf4(s, i, s.px[i], s.py[i], s.pz[i], s.species[i]);
   }
 }
 
-// after the rounds: replicas with atoms still listed did not finish
-__global__ void __launch_bounds__(DML_BLOCK) k_ov_end(DevState s, int rounds, int err_if_pending) {
-  const int cur = rounds & 1;
-  const int64_t n = s.wl_cnt[2 * cur];
-  for (int64_t q = blockIdx.x * DML_BLOCK + threadIdx.x; q < n; q += (int64_t)gridDim.x * DML_BLOCK) {
-    int t = s.wl[cur][q];
-    int r = s.blk2rep[t / DML_BLOCK];
+// after the rounds: a replica whose last round still made progress would
+// need more rounds
+__global__ void k_ov_end(DevState s, int rounds, int err_if_pending) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= s.nrep) return;
+  if (s.rp[r].overlap_mode != DML_OVERLAP_ROUNDS || s.rp[r].halted) return;
+  if (s.ov_more[((rounds - 1) % 3) * s.nrep + r]) {
     if (err_if_pending) rep_fail(s, r, ERR_OVERLAP);
     atomicAdd(&s.cnt[r].ov_unfinished, 1ull);
   }
