@@ -51,7 +51,7 @@ struct dml_ctx {
   int lpa_ov = 8;  // lanes per atom of the overlap scan (1 = thread per atom)
   int lb_mode = 0; // 0: cell-tile list build, 1: wave per atom
   int rounds = 8;
-  int any_ermak = 0, any_brown = 0, any_rounds = 0, any_exact = 0, any_piston = 0;
+  int any_ermak = 0, any_brown = 0, any_rounds = 0, any_exact = 0, any_piston = 0, any_gcmc = 0;
 };
 
 namespace {
@@ -131,6 +131,7 @@ int status_from_err(int e) {
   if (e & ERR_OVERLAP) return DML_E_OVERLAP;
   if (e & ERR_LIST) return DML_E_LIST_FULL;
   if (e & ERR_BUCKET) return DML_E_NOMEM;
+  if (e & ERR_UNSUP) return DML_E_UNSUPPORTED;
   return DML_OK;
 }
 
@@ -330,6 +331,9 @@ void enq_step(dml_ctx *c) {
   enq_overlap(c);
   enq_test_update(c);
   enq_f2cg(c);
+  if (c->any_gcmc)  // gcmc_run after F -> CG (dana.F90:238-241)
+    hipLaunchKernelGGL(k_gcmc_run, dim3(c->nrep), dim3(DML_BLOCK), 0, c->st, c->d, -1,
+                       (const double *)nullptr, (const double *)nullptr, 0.0, 0);
   if (c->any_piston) {
     hipLaunchKernelGGL(k_calc_rho, dim3(nb), dim3(DML_BLOCK), 0, c->st, c->d);
     hipLaunchKernelGGL(k_maxz, dim3(nb), dim3(DML_BLOCK), 0, c->st, c->d);
@@ -419,8 +423,9 @@ double *upload_tmp(dml_ctx *c, const double *h, int64_t n) {
 }
 
 void refresh_modes(dml_ctx *c) {
-  c->any_ermak = c->any_brown = c->any_rounds = c->any_exact = c->any_piston = 0;
+  c->any_ermak = c->any_brown = c->any_rounds = c->any_exact = c->any_piston = c->any_gcmc = 0;
   for (auto &h : c->reps) {
+    if (h.p.reservoir == DML_GCMC) c->any_gcmc = 1;
     if (h.p.integrator == DML_ERMAK) c->any_ermak = 1; else c->any_brown = 1;
     if (h.p.overlap_mode == DML_OVERLAP_ROUNDS) c->any_rounds = 1; else c->any_exact = 1;
     if (h.p.piston) c->any_piston = 1;
@@ -473,7 +478,8 @@ int dml_create_ensemble(int32_t nrep, const dml_config *cfgs, dml_ctx **out) {
   int lpa = 0;
   for (int r = 0; r < nrep; r++) {
     const dml_config &g = cfgs[r];
-    if (g.capacity <= 0 || g.box[0] <= 0 || g.box[1] <= 0 || g.box[2] <= 0 || g.rcut <= 0) {
+    if (g.capacity <= 0 || g.box[0] <= 0 || g.box[1] <= 0 || g.box[2] <= 0 || g.rcut <= 0 ||
+        g.nadj < 0 || g.nadj > (1 << 21)) {
       delete c;
       return DML_E_ARG;
     }
@@ -571,6 +577,9 @@ int dml_create_ensemble(int32_t nrep, const dml_config *cfgs, dml_ctx **out) {
   e |= dalloc(&d.wl_stamp, P);
   e |= dalloc(&d.ov_more, 3 * nrep);
   e |= dalloc(&d.pf4, P);
+  e |= dalloc(&d.gc_order, P);
+  e |= dalloc(&d.b2slot, P);
+  e |= dalloc(&d.seq, P);
   if (e) {
     dml_destroy(c);
     return DML_E_NOMEM;
@@ -592,6 +601,9 @@ int dml_create_ensemble(int32_t nrep, const dml_config *cfgs, dml_ctx **out) {
   hipMemsetAsync(d.wl_cnt, 0, sizeof(int32_t) * 4, c->st);
   hipMemsetAsync(d.pf4, 0, sizeof(float4) * P, c->st);
   hipMemsetAsync(d.ref_pos, 0xff, sizeof(int32_t) * P, c->st);
+  hipMemsetAsync(d.b2slot, 0xff, sizeof(int32_t) * P, c->st);
+  hipMemsetAsync(d.gc_order, 0xff, sizeof(int32_t) * P, c->st);
+  hipMemsetAsync(d.seq, 0, sizeof(int32_t) * P, c->st);
   hipMemsetAsync(d.cnt, 0, sizeof(DevCounters) * nrep, c->st);
   hipMemsetAsync(d.err, 0, sizeof(int32_t), c->st);
   std::vector<int32_t> b2r(P / DML_BLOCK);
@@ -627,7 +639,8 @@ int dml_destroy(dml_ctx *c) {
                   d.species, d.flags, d.rowflag, d.bkey, d.pid, d.ref_order, d.ref_tmp,
                   d.ref_pos, d.nn, d.nbr, d.cell_cnt, d.bucket, d.bx, d.by, d.bz, d.ov_rev,
                   d.ov_st, d.part_d, d.part_i, d.blk2rep, d.rp, d.cnt, d.err, d.wl[0],
-                  d.wl[1], d.wl_cnt, d.wl_stamp, d.ov_more, d.pf4, d.bf4};
+                  d.wl[1], d.wl_cnt, d.wl_stamp, d.ov_more, d.pf4, d.bf4, d.gc_order,
+                  d.b2slot, d.seq};
   for (void *q : ptrs)
     if (q) hipFree(q);
   if (c->e0) hipEventDestroy(c->e0);
@@ -694,6 +707,33 @@ int dml_set_state(dml_ctx *c, int32_t rep, int32_t n, const double *pos, const d
     HIPCHECK(hipMemcpy(c->d.ref_order + o, rord.data(), sizeof(int32_t) * rord.size(),
                        hipMemcpyHostToDevice));
   HIPCHECK(hipMemset(c->d.nn + o, 0, sizeof(int32_t) * p.pcap));
+  // hs%b index table, the gcmc group in sys order (dana.F90:505-513) and
+  // the creation numbers
+  {
+    std::vector<int32_t> b2s(p.pcap, -1), gco, sq(std::max(n, 1));
+    int nat = 0;
+    for (int i = 0; i < n; i++) {
+      sq[i] = i;
+      if (fl[i] & DML_FL_ALIVE) {
+        nat++;
+        b2s[i] = (int32_t)(o + i);
+      }
+      if (fl[i] & DML_FL_GCMC) gco.push_back((int32_t)(o + i));
+    }
+    HIPCHECK(hipMemcpy(c->d.b2slot + o, b2s.data(), sizeof(int32_t) * p.pcap, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(c->d.seq + o, sq.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    if (!gco.empty())
+      HIPCHECK(hipMemcpy(c->d.gc_order + o, gco.data(), sizeof(int32_t) * gco.size(),
+                         hipMemcpyHostToDevice));
+    p.nat = nat;
+    p.nlimbo = 0;
+    p.b_amax = n;
+    p.b_nat = nat;
+    p.ngc = (int32_t)gco.size();
+    p.pid_next = (uint32_t)n;
+    p.seq_next = n;
+    p.gcmc_calls = 0;
+  }
   p.nslots = n;
   p.nref = (int32_t)rord.size();
   p.listed = 0;
@@ -739,7 +779,21 @@ int dml_set_order(dml_ctx *c, int32_t rep, int32_t nref, const int32_t *ref_orde
     HIPCHECK(hipMemcpy(c->d.ref_pos + o, rpos.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
     p.nref = nref;
   }
-  if (bkey) HIPCHECK(hipMemcpy(c->d.bkey + o, bkey, sizeof(int32_t) * n, hipMemcpyHostToDevice));
+  if (bkey) {
+    std::vector<uint8_t> fl(std::max(n, 1));
+    HIPCHECK(hipMemcpy(fl.data(), c->d.flags + o, n, hipMemcpyDeviceToHost));
+    std::vector<int32_t> b2s(p.pcap, -1);
+    int amax = 0;
+    for (int i = 0; i < n; i++) {
+      if (!(fl[i] & DML_FL_ALIVE)) continue;
+      if (bkey[i] < 0 || bkey[i] >= p.pcap || b2s[bkey[i]] >= 0) return DML_E_ARG;
+      b2s[bkey[i]] = (int32_t)(o + i);
+      amax = std::max(amax, bkey[i] + 1);
+    }
+    HIPCHECK(hipMemcpy(c->d.bkey + o, bkey, sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(c->d.b2slot + o, b2s.data(), sizeof(int32_t) * p.pcap, hipMemcpyHostToDevice));
+    p.b_amax = amax;
+  }
   return upload_params(c);
 }
 
@@ -982,8 +1036,65 @@ int dml_f_to_cg(dml_ctx *c) {
 }
 
 int dml_gcmc(dml_ctx *c, int32_t rep, double act, int32_t nadj, const double *u, const double *g) {
-  (void)c; (void)rep; (void)act; (void)nadj; (void)u; (void)g;
-  return DML_E_UNSUPPORTED;
+  if (!c || rep < 0 || rep >= c->nrep || nadj < 0 || nadj > (1 << 21) || (!u) != (!g))
+    return DML_E_ARG;
+  int rc = fetch_params(c);
+  if (rc) return rc;
+  const RepParams &p = c->reps[rep].p;
+  if (p.reservoir != DML_GCMC) return DML_E_STATE;
+  if (!p.listed) return DML_E_STATE;  // cells and rows come from test_update
+  if (nadj == 0) return DML_OK;
+  double *du = upload_tmp(c, u, 5 * (int64_t)nadj);
+  double *dg = upload_tmp(c, g, 3 * (int64_t)nadj);
+  if ((u && !du) || (g && !dg)) {
+    hipFree(du);
+    hipFree(dg);
+    return DML_E_NOMEM;
+  }
+  hipLaunchKernelGGL(k_gcmc_run, dim3(1), dim3(DML_BLOCK), 0, c->st, c->d, rep,
+                     (const double *)du, (const double *)dg, act, nadj);
+  rc = finish(c);
+  hipFree(du);
+  hipFree(dg);
+  return rc;
+}
+
+int dml_get_gcmc_order(dml_ctx *c, int32_t rep, int32_t *ngc, int32_t *order) {
+  if (!c || rep < 0 || rep >= c->nrep || !ngc) return DML_E_ARG;
+  int rc = fetch_params(c);
+  if (rc) return rc;
+  const RepParams &p = c->reps[rep].p;
+  // atoms turned CG since the last gcmc_run are still listed on the device
+  // (dropped at its start); report the group as gcmc%detach leaves it
+  std::vector<int32_t> g(std::max(p.ngc, 1));
+  std::vector<uint8_t> fl(std::max(p.pcap, 1));
+  if (p.ngc > 0) {
+    HIPCHECK(hipMemcpy(g.data(), c->d.gc_order + p.poff, sizeof(int32_t) * p.ngc,
+                       hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(fl.data(), c->d.flags + p.poff, p.pcap, hipMemcpyDeviceToHost));
+  }
+  int m = 0;
+  for (int t = 0; t < p.ngc; t++) {
+    int i = g[t] - p.poff;
+    if (i < 0 || i >= p.pcap || !(fl[i] & DML_FL_GCMC)) continue;
+    if (order) order[m] = i;
+    m++;
+  }
+  *ngc = m;
+  return DML_OK;
+}
+
+int dml_get_slots(dml_ctx *c, int32_t rep, int32_t *nslots, int32_t *bkey, uint32_t *pid) {
+  if (!c || rep < 0 || rep >= c->nrep || !nslots) return DML_E_ARG;
+  int rc = fetch_params(c);
+  if (rc) return rc;
+  const RepParams &p = c->reps[rep].p;
+  *nslots = p.nslots;
+  if (bkey && p.nslots > 0)
+    HIPCHECK(hipMemcpy(bkey, c->d.bkey + p.poff, sizeof(int32_t) * p.nslots, hipMemcpyDeviceToHost));
+  if (pid && p.nslots > 0)
+    HIPCHECK(hipMemcpy(pid, c->d.pid + p.poff, sizeof(uint32_t) * p.nslots, hipMemcpyDeviceToHost));
+  return DML_OK;
 }
 
 int dml_gcmc_trial_energy(dml_ctx *c, int32_t rep, int32_t ntrial, const double *pos3, double *du,

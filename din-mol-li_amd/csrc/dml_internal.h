@@ -29,6 +29,7 @@
 #define ERR_NOPART 8
 #define ERR_OVERLAP 16
 #define ERR_BUCKET 32
+#define ERR_UNSUP 64
 
 struct RepParams {
   double box[3], one_box[3];
@@ -63,6 +64,13 @@ struct RepParams {
   int32_t nref_new;
   int32_t halted;    // a fatal condition stopped this replica
   int32_t err_bits;  // which one (ERR_*)
+  // group bookkeeping of the GCMC reservoir (Groups.F90 igroup semantics)
+  int32_t nat, nlimbo;     // hs members alive / in limbo
+  int32_t b_amax, b_nat;   // hs%b index range and members
+  int32_t ngc;             // members of the gcmc group (list order in gc_order)
+  uint32_t pid_next;       // next particle id (Philox counter)
+  int32_t seq_next;        // next creation number (sys list order)
+  int32_t gcmc_calls;
   uint64_t seed;
   double d1, d2;        // top-2 squared displacements of the last skin test
 };
@@ -200,6 +208,9 @@ struct DevState {
   int32_t *wl_cnt;        // [nW0, nR0, nW1, nR1]: active and appended-revert counts
   int32_t *wl_stamp;      // per slot: last round stamp it was listed with
   int32_t *ov_more;       // [3][nrep]: the replica reverted or deferred in a round
+  int32_t *gc_order;      // gcmc group in list order (global slots)
+  int32_t *b2slot;        // hs%b index -> global slot, -1 when free
+  int32_t *seq;           // creation number: the order of sys%alist
   // FP32 mirror of the positions, w = species bits: conservative prefilters
   float4 *pf4;            // per slot
   float4 *bf4;            // bucket-ordered

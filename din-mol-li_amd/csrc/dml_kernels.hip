@@ -133,6 +133,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_dispmax_final(DevState s) {
     if (reb) {
       Pw->listed = 1;
       Pw->any_ghost = 0;
+      Pw->nlimbo = 0;  // the rebuild cleans the limbo (Groups.F90:1036-1053)
       atomicAdd(&s.cnt[r].nupd, 1ull);
     }
   }
@@ -146,6 +147,7 @@ __global__ void k_force_rebuild(DevState s) {
   Pw->rebuild = 1;
   Pw->listed = 1;
   Pw->any_ghost = 0;
+  Pw->nlimbo = 0;
   atomicAdd(&s.cnt[r].nupd, 1ull);
 }
 
@@ -1417,5 +1419,395 @@ __global__ void __launch_bounds__(DML_BLOCK) k_gcmc_energy(DevState s, int r, in
   if (lane == 0) {
     du[w] = e;
     hard[w] = hm != 0;
+  }
+}
+
+// ------------------------------------------------------------------------
+// GCMC reservoir: gcmc_run (dana.F90:590-713) with the group bookkeeping it
+// drives (Groups.F90:730-766, 1058-1135; Cells.F90:105-175, 281-352;
+// Neighbor.F90:173-318).  Trials are sequential, as in the reference: one
+// workgroup per replica, each trial's scans spread over the block.
+// ------------------------------------------------------------------------
+__device__ inline int blk_sum(int v, int *sh) {
+  __syncthreads();
+  for (int off = WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  if (threadIdx.x % WAVE == 0) sh[threadIdx.x / WAVE] = v;
+  __syncthreads();
+  int t = 0;
+  for (int q = 0; q < (int)(blockDim.x / WAVE); q++) t += sh[q];
+  __syncthreads();
+  return t;
+}
+__device__ inline int blk_min(int v, int *sh) {
+  __syncthreads();
+  for (int off = WAVE / 2; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off));
+  if (threadIdx.x % WAVE == 0) sh[threadIdx.x / WAVE] = v;
+  __syncthreads();
+  int t = sh[0];
+  for (int q = 1; q < (int)(blockDim.x / WAVE); q++) t = min(t, sh[q]);
+  __syncthreads();
+  return t;
+}
+// exclusive prefix of a 0/1 flag over the block, and the block total
+__device__ inline int blk_scan01(int f, int *sh, int *total) {
+  __syncthreads();
+  unsigned long long m = __ballot(f);
+  int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
+  if (lane == 0) sh[w] = __popcll(m);
+  __syncthreads();
+  int off = 0, tot = 0;
+  for (int q = 0; q < (int)(blockDim.x / WAVE); q++) {
+    if (q < w) off += sh[q];
+    tot += sh[q];
+  }
+  *total = tot;
+  __syncthreads();
+  return off + __popcll(m & ((1ull << lane) - 1ull));
+}
+
+__device__ inline double gc_unif(const RepParams &P, const double *u, int t, int k, int call) {
+  if (u) return u[5 * t + k];
+  u32x4 w = dml_philox(P.seed, 0xFFFFFFFFu - (uint32_t)call, (uint32_t)P.step, RS_GCMC,
+                       (uint32_t)(8 * t + k));
+  return u01_from(w.x, w.y);
+}
+__device__ inline double gc_gauss(const RepParams &P, const double *g, int t, int k, int call) {
+  if (g) return g[3 * t + k];
+  double a, b;
+  gauss2(dml_philox(P.seed, 0xFFFFFFFFu - (uint32_t)call, (uint32_t)P.step, RS_GCMC,
+                    (uint32_t)(8 * t + 5 + (k >> 1))), &a, &b);
+  return (k & 1) ? b : a;
+}
+
+__global__ void __launch_bounds__(DML_BLOCK) k_gcmc_run(DevState s, int rep_only, const double *u,
+                                                         const double *g, double act_in,
+                                                         int nadj_in) {
+  __shared__ int sh[DML_BLOCK / WAVE];
+  __shared__ int sh_v[4];
+  const int r = rep_only >= 0 ? rep_only : (int)blockIdx.x;
+  if (r >= s.nrep) return;
+  RepParams *Pw = s.rp + r;
+  const RepParams &P = *Pw;
+  if (P.halted || (rep_only < 0 && P.reservoir != DML_GCMC)) return;
+  const int tid = threadIdx.x;
+  const int poff = P.poff;
+  const double act = rep_only >= 0 ? act_in : P.act;
+  const int nadj = rep_only >= 0 ? nadj_in : P.nadj;
+  const int call = P.gcmc_calls;
+  const double rc2 = P.rcut * P.rcut;
+  const double z0 = P.z0, zmax = P.zmax;
+  const double vol = P.box[0] * P.box[1] * (zmax - z0);
+  const double rl2 = P.rl2;
+  const int maxnn = s.maxnn;
+  int *gc = s.gc_order + poff;
+  // gcmc%detach of the atoms turned CG at step end (dana.F90:235): stable
+  // compaction of the list by membership
+  int ngc = P.ngc;
+  {
+    int w = 0;
+    for (int q0 = 0; q0 < ngc; q0 += DML_BLOCK) {
+      int q = q0 + tid;
+      int v = q < ngc ? gc[q] : -1;
+      int keep = v >= 0 && (s.flags[v] & DML_FL_GCMC);
+      int tot;
+      int pos = blk_scan01(keep, sh, &tot);
+      if (keep) gc[w + pos] = v;
+      w += tot;
+      __syncthreads();
+    }
+    ngc = w;
+  }
+  // count in the control volume (dana.F90:609-615)
+  int n = 0;
+  for (int q = tid; q < ngc; q += DML_BLOCK) {
+    double z = s.pz[gc[q]];
+    n += !(z < z0 || z > zmax);
+  }
+  n = blk_sum(n, sh);
+  // bookkeeping registers (uniform across the block)
+  int nslots = P.nslots, nat = P.nat, nlimbo = P.nlimbo, b_amax = P.b_amax, b_nat = P.b_nat;
+  int nref = P.nref;
+  uint32_t pid_next = P.pid_next;
+  int seq_next = P.seq_next;
+  const int listed = P.listed;
+  unsigned long long n_ins = 0, n_del = 0;
+  int fail = 0;
+  // constants of gems_constants (Constants.F90:100-167) for beta
+  const double ui_ev = 1.0e2 * 1.0e2 * 1.6605402e-27 * (1.0 / 1.60219e-19);
+  const double kB_ui = 8.617385e-05 * (1.0 / ui_ev);
+  for (int t = 0; t < nadj && !fail; t++) {
+    __syncthreads();  // the previous trial's single-thread writes
+    if (ngc == 0) { fail = ERR_NOPART; break; }
+    const int tmpl = gc[0];
+    const double beta = sqrt(kB_ui * 300.0 / P.mass[s.species[tmpl] - 1]);
+    if (gc_unif(P, u, t, 0, call) < 0.5) {
+      // ---- creation attempt (dana.F90:627-677)
+      if (act * vol / (double)(n + 1) < gc_unif(P, u, t, 1, call)) continue;
+      const double rx = gc_unif(P, u, t, 2, call) * P.box[0];
+      const double ry = gc_unif(P, u, t, 3, call) * P.box[1];
+      const double rz = gc_unif(P, u, t, 4, call) * (zmax - z0) + z0;
+      int hit = 0;
+      for (int q = tid; q < ngc && !hit; q += DML_BLOCK) {
+        int o = gc[q];
+        double dx, dy, dz;  // distance(o%pos, r): r - o%pos
+        vdist_mic(P, rx, ry, rz, s.px[o], s.py[o], s.pz[o], &dx, &dy, &dz);
+        if (dot3(dx, dy, dz) < rc2) hit = 1;
+      }
+      if (blk_sum(hit, sh)) continue;
+      n++;
+      const int tail = gc[ngc - 1];
+      if (tid == 0) {  // the velocity lands on the list's last atom (dana.F90:666-668)
+        s.vx[tail] = beta * gc_gauss(P, g, t, 0, call);
+        s.vy[tail] = beta * gc_gauss(P, g, t, 1, call);
+        s.vz[tail] = beta * gc_gauss(P, g, t, 2, call);
+      }
+      // hs slot: first free slot or append (Groups.F90:1083-1093)
+      int hs;
+      if (nslots >= nat + 1 + nlimbo) {
+        int cand = 0x7fffffff;
+        for (int q = tid; q < nslots; q += DML_BLOCK)
+          if (!(s.flags[poff + q] & (DML_FL_ALIVE | DML_FL_LIMBO))) cand = min(cand, q);
+        hs = blk_min(cand, sh);
+        if (hs == 0x7fffffff) { fail = ERR_UNSUP; break; }
+      } else {
+        if (nslots >= P.pcap) { fail = ERR_BUCKET; break; }
+        hs = nslots++;
+      }
+      // hs%b slot the same way
+      int bs;
+      if (b_amax >= b_nat + 1) {
+        int cand = 0x7fffffff;
+        for (int q = tid; q < b_amax; q += DML_BLOCK)
+          if (s.b2slot[poff + q] < 0) cand = min(cand, q);
+        bs = blk_min(cand, sh);
+        if (bs == 0x7fffffff) { fail = ERR_UNSUP; break; }
+      } else {
+        bs = b_amax++;
+      }
+      const int i = poff + hs;
+      if (tid == 0) {
+        s.species[i] = s.species[tmpl];  // atom_asign (Groups.F90:484-502)
+        s.vx[i] = s.vx[tmpl]; s.vy[i] = s.vy[tmpl]; s.vz[i] = s.vz[tmpl];
+        s.fx[i] = s.fx[tmpl]; s.fy[i] = s.fy[tmpl]; s.fz[i] = s.fz[tmpl];
+        s.ax[i] = s.ax[tmpl]; s.ay[i] = s.ay[tmpl]; s.az[i] = s.az[tmpl];
+        s.epot[i] = s.epot[tmpl];
+        s.px[i] = rx; s.py[i] = ry; s.pz[i] = rz;
+        s.ox[i] = rx; s.oy[i] = ry; s.oz[i] = rz;
+        s.cx[i] = 1e8; s.cy[i] = 1e8; s.cz[i] = 1e8;
+        s.flags[i] = DML_FL_ALIVE | DML_FL_REF | DML_FL_GCMC;
+        s.bkey[i] = bs;
+        s.b2slot[poff + bs] = i;
+        s.pid[i] = pid_next;
+        s.seq[i] = seq_next;
+        s.rowflag[i] = 0;  // set below for a ghost-layer cell
+        s.nn[i] = 0;
+        s.ref_order[poff + nref] = i;
+        s.ref_pos[i] = nref;
+        put_pf4(s, i, rx, ry, rz, s.species[tmpl]);
+      }
+      pid_next++;
+      seq_next++;
+      nat++;
+      b_nat++;
+      nref++;
+      // cgroup_sort_atom: push-front into its cell chain (Cells.F90:299-300);
+      // the Verlet bucket keeps ascending hs%b order
+      int cell = 0, ci0 = 0, ci1 = 0, ci2 = 0;
+      if (P.use_cells) {
+        ci0 = (int)(rx / P.cell[0]) + 1;
+        ci1 = (int)(ry / P.cell[1]) + 1;
+        ci2 = (int)(rz / P.cell[2]) + 1;
+        if (ci0 < 0 || ci1 < 0 || ci2 < 0 || ci0 > P.ncells[0] + 1 || ci1 > P.ncells[1] + 1 ||
+            ci2 > P.ncells[2] + 1) { fail = ERR_UNSUP; break; }
+        cell = cell_lin(P, ci0, ci1, ci2);
+      }
+      {
+        const int64_t base = P.boff + (int64_t)cell * P.bcap;
+        const int cnt = s.cell_cnt[P.coff + cell];
+        if (cnt >= P.bcap) { fail = ERR_BUCKET; break; }
+        if (tid == 0) {
+          int at = 0;
+          if (!P.use_cells)
+            while (at < cnt && s.bkey[s.bucket[base + at]] < bs) at++;
+          for (int q = cnt; q > at; q--) {
+            s.bucket[base + q] = s.bucket[base + q - 1];
+            s.bx[base + q] = s.bx[base + q - 1];
+            s.by[base + q] = s.by[base + q - 1];
+            s.bz[base + q] = s.bz[base + q - 1];
+            s.bf4[base + q] = s.bf4[base + q - 1];
+          }
+          s.bucket[base + at] = i;
+          s.bx[base + at] = rx; s.by[base + at] = ry; s.bz[base + at] = rz;
+          s.bf4[base + at] = s.pf4[i];
+          s.cell_cnt[P.coff + cell] = cnt + 1;
+        }
+      }
+      __syncthreads();
+      if (listed) {
+        // ngroup_sort_atom (Neighbor.F90:271-318): own row by the cell
+        // search of ngroup_cells_atom (strict '<') or the Verlet scan
+        int32_t *row = s.nbr + (int64_t)i * maxnn;
+        int nn = 0;
+        const int nst = P.use_cells ? 27 : 1;
+        for (int k = 0; k < nst; k++) {
+          int cc = 0;
+          if (P.use_cells) {
+            int n0 = cell_wrap(ci0 + kMap[k][0], P.ncells[0]);
+            int n1 = cell_wrap(ci1 + kMap[k][1], P.ncells[1]);
+            int n2 = cell_wrap(ci2 + kMap[k][2], P.ncells[2]);
+            cc = cell_lin(P, n0, n1, n2);
+          }
+          const int64_t base = P.boff + (int64_t)cc * P.bcap;
+          int cnt = s.cell_cnt[P.coff + cc];
+          if (cnt > P.bcap) cnt = P.bcap;
+          for (int q0 = 0; q0 < cnt; q0 += DML_BLOCK) {
+            int q = q0 + tid, hitq = 0, j = -1;
+            if (q < cnt) {
+              j = s.bucket[base + q];
+              if (j != i) {
+                double dx, dy, dz;
+                vdist_mic(P, s.px[j], s.py[j], s.pz[j], rx, ry, rz, &dx, &dy, &dz);
+                double rd = dot3(dx, dy, dz);
+                hitq = P.use_cells ? (rd < rl2) : !(rd > rl2);
+              }
+            }
+            int tot;
+            int pos = blk_scan01(hitq, sh, &tot);
+            if (hitq && nn + pos < maxnn) row[nn + pos] = j;
+            nn += tot;
+          }
+        }
+        if (nn > maxnn) { fail = ERR_LIST; break; }
+        const int ghost = P.use_cells && (ci2 == 0 || ci2 == P.ncells[2] + 1 || ci0 == 0 ||
+                                          ci0 == P.ncells[0] + 1 || ci1 == 0 ||
+                                          ci1 == P.ncells[1] + 1);
+        if (tid == 0) {
+          s.nn[i] = nn;
+          s.rowflag[i] = (uint8_t)ghost;
+          if (ghost) Pw->any_ghost = 1;
+        }
+        // and i joins the row of every ref atom within the list radius
+        int full = 0;
+        for (int q = tid; q < nref; q += DML_BLOCK) {
+          int aj = s.ref_order[poff + q];
+          if (aj == i) continue;
+          double dx, dy, dz;
+          vdist_mic(P, rx, ry, rz, s.px[aj], s.py[aj], s.pz[aj], &dx, &dy, &dz);
+          if (dot3(dx, dy, dz) > rl2) continue;
+          int m = s.nn[aj];
+          if (m < maxnn) {
+            s.nbr[(int64_t)aj * maxnn + m] = i;
+            s.nn[aj] = m + 1;
+          } else {
+            full = 1;
+          }
+        }
+        if (blk_sum(full, sh)) { fail = ERR_LIST; break; }
+      }
+      if (tid == 0) gc[ngc] = i;
+      ngc++;
+      n_ins++;
+    } else {
+      // ---- destruction attempt (dana.F90:680-708)
+      if ((double)n / (vol * act) < gc_unif(P, u, t, 1, call)) continue;
+      int m = (int)floor(gc_unif(P, u, t, 2, call) * n) + 1;
+      if (m > n) m = n;
+      int found = -1, running = 0;
+      for (int q0 = 0; q0 < ngc && found < 0; q0 += DML_BLOCK) {
+        int q = q0 + tid;
+        int inv = 0;
+        if (q < ngc) {
+          double z = s.pz[gc[q]];
+          inv = !(z < z0 || z > zmax);
+        }
+        int tot;
+        int pos = blk_scan01(inv, sh, &tot);
+        if (running + tot >= m) {
+          if (inv && running + pos + 1 == m) sh_v[0] = q;
+          __syncthreads();
+          found = sh_v[0];
+          __syncthreads();
+        }
+        running += tot;
+      }
+      if (found < 0) { fail = ERR_NOPART; break; }
+      const int o = gc[found];
+      n--;
+      // LIFO detach: gcmc, then hs (ref, b), then sys (Groups.F90:433-467)
+      for (int q0 = found; q0 < ngc - 1; q0 += DML_BLOCK) {
+        int q = q0 + tid;
+        int v = q + 1 < ngc ? gc[q + 1] : -1;
+        __syncthreads();
+        if (q < ngc - 1) gc[q] = v;
+      }
+      ngc--;
+      const int rp = s.ref_pos[o];
+      for (int q0 = rp; q0 < nref - 1; q0 += DML_BLOCK) {
+        int q = q0 + tid;
+        int v = q + 1 < nref ? s.ref_order[poff + q + 1] : -1;
+        __syncthreads();
+        if (q < nref - 1) {
+          s.ref_order[poff + q] = v;
+          s.ref_pos[v] = q;
+        }
+      }
+      nref--;
+      __syncthreads();
+      // cgroup_unsort_atom (Cells.F90:304-352)
+      int cell = 0;
+      if (P.use_cells) {
+        int c0 = (int)(s.px[o] / P.cell[0]) + 1;
+        int c1 = (int)(s.py[o] / P.cell[1]) + 1;
+        int c2 = (int)(s.pz[o] / P.cell[2]) + 1;
+        if (c0 < 0 || c1 < 0 || c2 < 0 || c0 > P.ncells[0] || c1 > P.ncells[1] ||
+            c2 > P.ncells[2]) { fail = ERR_UNSUP; break; }
+        cell = cell_lin(P, c0, c1, c2);
+      }
+      if (tid == 0) {
+        const int64_t base = P.boff + (int64_t)cell * P.bcap;
+        int cnt = min(s.cell_cnt[P.coff + cell], P.bcap);
+        int at = -1;
+        for (int q = 0; q < cnt; q++)
+          if (s.bucket[base + q] == o) { at = q; break; }
+        if (at >= 0) {
+          for (int q = at; q < cnt - 1; q++) {
+            s.bucket[base + q] = s.bucket[base + q + 1];
+            s.bx[base + q] = s.bx[base + q + 1];
+            s.by[base + q] = s.by[base + q + 1];
+            s.bz[base + q] = s.bz[base + q + 1];
+            s.bf4[base + q] = s.bf4[base + q + 1];
+          }
+          s.cell_cnt[P.coff + cell] = cnt - 1;
+        }
+        sh_v[1] = at < 0;  // 'Unsort cgroup particle fail' (Cells.F90:350)
+        s.b2slot[poff + s.bkey[o]] = -1;
+        if (listed) s.nn[o] = 0;
+        // a destroyed atom keeps its skip value (atom_destroy, Groups.F90:433-467)
+        s.flags[o] = (uint8_t)((listed ? DML_FL_LIMBO : 0) | (s.flags[o] & DML_FL_SKIP));
+        s.ref_pos[o] = -1;
+      }
+      b_nat--;
+      nat--;
+      if (listed) nlimbo++;
+      n_del++;
+      __syncthreads();
+      if (sh_v[1]) { fail = ERR_UNSUP; break; }
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    Pw->ngc = ngc;
+    Pw->nslots = nslots;
+    Pw->nat = nat;
+    Pw->nlimbo = nlimbo;
+    Pw->b_amax = b_amax;
+    Pw->b_nat = b_nat;
+    Pw->nref = nref;
+    Pw->pid_next = pid_next;
+    Pw->seq_next = seq_next;
+    Pw->gcmc_calls = call + 1;
+    if (n_ins) atomicAdd(&s.cnt[r].ins, n_ins);
+    if (n_del) atomicAdd(&s.cnt[r].del, n_del);
+    if (fail) rep_fail(s, r, fail);
   }
 }

@@ -33,7 +33,7 @@ EXPORTS = ["dml_create", "dml_create_ensemble", "dml_destroy", "dml_sync", "dml_
            "dml_overlap_moveback", "dml_overlap_rounds", "dml_f_to_cg", "dml_gcmc",
            "dml_gcmc_trial_energy", "dml_run", "dml_get_counters", "dml_list_stats",
            "dml_last_run_ms", "dml_time_kernel", "dml_set_tuning",
-           "dml_replica_status"]
+           "dml_replica_status", "dml_get_gcmc_order", "dml_get_slots"]
 K_LISTBUILD, K_FORCE, K_CELLS, K_DISPMAX = 0, 1, 2, 3
 
 
@@ -106,6 +106,8 @@ def lib():
         L.dml_set_tuning.argtypes = [vp, C.c_int32, C.c_int32]
         L.dml_replica_status.argtypes = [vp, C.c_int32, vp]
         L.dml_sync.argtypes = [vp]
+        L.dml_get_gcmc_order.argtypes = [vp, C.c_int32, vp, vp]
+        L.dml_get_slots.argtypes = [vp, C.c_int32, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -194,8 +196,41 @@ class Context:
         _chk(self.L.dml_set_order(self.h, rep, 0 if ro is None else len(ro), _p(ro), _p(bk)),
              "dml_set_order")
 
+    def nslots(self, rep=0):
+        """Slots in use (hs%amax): grows when GCMC appends atoms."""
+        n = C.c_int32()
+        _chk(self.L.dml_get_slots(self.h, rep, C.byref(n), None, None), "dml_get_slots")
+        self.n[rep] = n.value
+        return n.value
+
+    def slots(self, rep=0):
+        """(bkey, pid) per slot: the hs%b index and the particle id."""
+        n = self.nslots(rep)
+        bk = np.zeros(max(n, 1), np.int32)
+        pid = np.zeros(max(n, 1), np.uint32)
+        nn = C.c_int32()
+        _chk(self.L.dml_get_slots(self.h, rep, C.byref(nn), _p(bk), _p(pid)), "dml_get_slots")
+        return bk[:n], pid[:n]
+
+    def gcmc_order(self, rep=0):
+        ng = C.c_int32()
+        _chk(self.L.dml_get_gcmc_order(self.h, rep, C.byref(ng), None), "dml_get_gcmc_order")
+        out = np.zeros(max(ng.value, 1), np.int32)
+        _chk(self.L.dml_get_gcmc_order(self.h, rep, C.byref(ng), _p(out)), "dml_get_gcmc_order")
+        return out[:ng.value]
+
+    def gcmc(self, act, nadj, u=None, g=None, rep=0):
+        """nadj GCMC trials (gcmc_run, dana.F90:590-713) on replica rep; u is
+        (nadj, 5) uniforms, g (nadj, 3) normals, or both None for Philox."""
+        uu = None if u is None else np.ascontiguousarray(u, np.float64).reshape(-1)
+        gg = None if g is None else np.ascontiguousarray(g, np.float64).reshape(-1)
+        if uu is not None and (len(uu) < 5 * nadj or gg is None or len(gg) < 3 * nadj):
+            raise ValueError("u needs 5 and g 3 values per trial")
+        _chk(self.L.dml_gcmc(self.h, rep, float(act), int(nadj), _p(uu), _p(gg)), "dml_gcmc")
+        self.nslots(rep)
+
     def state(self, rep=0):
-        n = self.n[rep]
+        n = self.nslots(rep)
         d = {k: np.zeros(3 * n) for k in ("pos", "vel", "acel", "force", "pos_old", "old_cg")}
         d["epot"] = np.zeros(n)
         d["species"] = np.zeros(n, np.int8)
@@ -237,13 +272,13 @@ class Context:
 
     def cells(self, rep=0):
         nc = np.zeros(3, np.int32); cell = np.zeros(3); uc = C.c_int32()
-        n = self.n[rep]
+        n = self.nslots(rep)
         ci = np.zeros(3 * max(n, 1), np.int32)
         _chk(self.L.dml_cells(self.h, rep, _p(nc), _p(cell), C.byref(uc), _p(ci)), "dml_cells")
         return nc, cell, bool(uc.value), ci[:3 * n].reshape(3, n).T.copy()
 
     def lists(self, rep=0):
-        n = self.n[rep]
+        n = self.nslots(rep)
         tot = C.c_int64()
         nn = np.zeros(max(n, 1), np.int32)
         _chk(self.L.dml_neighbors(self.h, rep, _p(nn), None, None, 0, C.byref(tot)), "dml_neighbors")

@@ -167,6 +167,13 @@ int dml_f_to_cg(dml_ctx *ctx);
  * 2..4 coords | 2 pick}] and velocity normals g[t*3 + k]; NULL = Philox. */
 int dml_gcmc(dml_ctx *ctx, int32_t rep, double act, int32_t nadj,
              const double *u, const double *g);
+/* the gcmc group in list order (slots of rep; the template is order[0], the
+ * velocity of an insertion lands on order[ngc-1]: dana.F90:660-668) */
+int dml_get_gcmc_order(dml_ctx *ctx, int32_t rep, int32_t *ngc, int32_t *order);
+/* per slot: hs%b index (cell-chain order key) and particle id; *nslots is the
+ * hs%amax of igroup_attach_atom (Groups.F90:1083-1093) */
+int dml_get_slots(dml_ctx *ctx, int32_t rep, int32_t *nslots, int32_t *bkey,
+                  uint32_t *pid);
 /* batched trial energies: WCA sum (FP64) of an inserted Li at pos3[t] against
  * every atom of its 27 cells, and the hard-core indicator of
  * dana.F90:638-653 (gcmc atom closer than rcut). */

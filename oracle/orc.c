@@ -1239,6 +1239,25 @@ int orc_gcmc(orc_sys *s, orc_rng *r, double act, int nadj, orc_counters *c) {
   return r->err ? ORC_E_INDEX : 0;
 }
 
+void orc_gcmc_init(orc_sys *s) {
+  /* gcmc group: the Li atoms in sys order (dana.F90:505-513) */
+  s->gc.n = 0;
+  for (int q = 0; q < s->sysl.n; q++) {
+    int i = s->sysl.v[q];
+    s->in_gcmc[i] = 0;
+    if (s->zs[i] == ORC_LI && s->st[i] == ORC_ALIVE) {
+      ivec_push(&s->gc, i);
+      s->in_gcmc[i] = 1;
+    }
+  }
+}
+
+int orc_gcmc_order(const orc_sys *s, int32_t *out) {
+  if (out)
+    for (int j = 0; j < s->gc.n; j++) out[j] = s->gc.v[j];
+  return s->gc.n;
+}
+
 void orc_gcmc_trial_energy(const orc_sys *s, int ntrial, const double *pos3,
                            double *du, int32_t *hard) {
   /* Trial energy of an inserted Li at pos3[t]: the reference's acceptance

@@ -170,6 +170,10 @@ int orc_overlap_rounds(orc_sys *s, const double *u_round, int max_rounds,
 void orc_f_to_cg(orc_sys *s, int gcmc);
 /* gcmc_run (dana.F90:590-713) */
 int orc_gcmc(orc_sys *s, orc_rng *r, double act, int nadj, orc_counters *c);
+/* gcmc group of config_inic: Li atoms in sys order (dana.F90:505-513) */
+void orc_gcmc_init(orc_sys *s);
+/* gcmc group in list order; returns its size */
+int orc_gcmc_order(const orc_sys *s, int32_t *out);
 /* GCMC insertion trial energy: WCA sum over neighbours within 27 cells and
  * the hard-core indicator of dana.F90:638-653 (over the gcmc group). */
 void orc_gcmc_trial_energy(const orc_sys *s, int ntrial, const double *pos3,

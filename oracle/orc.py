@@ -266,6 +266,19 @@ class System:
     def f_to_cg(self, gcmc=False):
         self.L.orc_f_to_cg(C.c_void_p(self.s), int(gcmc))
 
+    def gcmc_init(self):
+        self.L.orc_gcmc_init(C.c_void_p(self.s))
+
+    def gcmc_order(self):
+        n = self.L.orc_gcmc_order(C.c_void_p(self.s), None)
+        out = np.zeros(max(n, 1), np.int32)
+        self.L.orc_gcmc_order(C.c_void_p(self.s), _p(out))
+        return out[:n]
+
+    def b_slots(self):
+        return np.array([self.L.orc_get_b_slot(C.c_void_p(self.s), i) for i in range(self.n)],
+                        np.int32)
+
     def gcmc(self, act, nadj, u, g):
         r, keep = self._arr_rng(gcmc_u=u, gcmc_g=g)
         return self.L.orc_gcmc(self.s, C.byref(r), act, nadj, C.byref(self.c))

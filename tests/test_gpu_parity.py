@@ -95,8 +95,13 @@ def c1_system(seed=3, n_ion=600, skin=10.0):
 
 
 def assert_lists_equal(c, s):
+    """rows of the ref atoms, entry for entry (the only rows any routine
+    reads; an atom that left ref keeps a stale row in the reference)"""
     nn_d, rp_d, idx_d = c.lists()
     nn_o, rp_o, idx_o = s.lists()
+    ref = (s.state()["flags"] & 4) != 0
+    nn_o = np.where(ref, nn_o, 0)
+    idx_o = np.concatenate([idx_o[rp_o[i]:rp_o[i + 1]] for i in np.nonzero(ref)[0]] + [idx_o[:0]])
     np.testing.assert_array_equal(nn_d, nn_o)
     np.testing.assert_array_equal(idx_d, idx_o)
 
@@ -239,7 +244,7 @@ def test_ermak_step_matches_oracle(orc, dml):
     c.close()
 
 
-def _step_pair(orc, dml, s, c, rng, mode, nsteps, brown=True):
+def _step_pair(orc, dml, s, c, rng, mode, nsteps, brown=True, gcmc=False):
     """integrator + test_update + overlap + test_update + F->CG, on both"""
     for k in range(nsteps):
         nref = s.nref
@@ -271,7 +276,7 @@ def _step_pair(orc, dml, s, c, rng, mode, nsteps, brown=True):
         st, r2 = s.test_update()
         assert st == 0
         assert r2 == c.test_update()[0], k
-        s.f_to_cg(); c.f_to_cg()
+        s.f_to_cg(gcmc); c.f_to_cg()
         so, sd = s.state(), c.state()
         np.testing.assert_array_equal(sd["species"], so["species"], err_msg="step %d" % k)
         np.testing.assert_array_equal(sd["flags"], so["flags"], err_msg="step %d" % k)
@@ -446,3 +451,100 @@ def test_fatal_condition_halts_only_its_replica(dml):
     ens.set_state(pos, species=z, rep=1)
     assert ens.replica_status(1) == 0
     ens.close()
+
+
+def _assert_same_groups(s, c, tag):
+    so, sd = s.state(), c.state()
+    assert c.nslots() == s.n, tag
+    np.testing.assert_array_equal(sd["flags"], so["flags"], err_msg=tag)
+    np.testing.assert_array_equal(sd["species"], so["species"], err_msg=tag)
+    for k in ("pos", "pos_old", "vel", "old_cg"):
+        assert rel_err(sd[k], so[k]) <= 1e-12, (tag, k)
+    np.testing.assert_array_equal(c.ref_order(), s.ref_order(), err_msg=tag)
+    np.testing.assert_array_equal(c.gcmc_order(), s.gcmc_order(), err_msg=tag)
+    alive = (so["flags"] & 1) != 0
+    np.testing.assert_array_equal(c.slots()[0][alive], s.b_slots()[alive], err_msg=tag)
+    assert_lists_equal(c, s)
+
+
+@pytest.mark.parametrize("geom", ["cells", "verlet"])
+def test_gcmc_trials_match_oracle(orc, dml, geom):
+    """gcmc_run with injected uniforms/normals: accept/reject decisions, slot
+    and hs%b reuse, limbo, push-front into the cell chains, incremental rows
+    and the gcmc list order, then Brownian steps over the changed system."""
+    rng = np.random.default_rng(41 if geom == "cells" else 42)
+    if geom == "cells":
+        box = np.array([40.0, 40.0, 60.0]); nx, n_ion, nadj = 11, 160, 80   # 7 x 7 x 11 cells
+    else:
+        box = np.array([15.0, 15.0, 15.0]); nx, n_ion, nadj = 4, 12, 30     # 2 x 2 x 2: Verlet
+    el = electrode(3.51, nx, nx, 2, 1.755)
+    ions = rsa(rng, n_ion, np.array([0, 0, 4.0]), np.array([box[0], box[1], box[2] - 1.0]), 3.3, box)
+    pos = np.vstack([el, ions])
+    z = np.array([2] * len(el) + [1] * len(ions), np.int8)
+    # z0 above every possible deposit (electrode top + rcut): no fatal stop
+    z0, zmax = max(box[2] / 3.0, 7.0), box[2] - 0.5
+    say("gcmc", geom, len(pos))
+    # h = 5e-5 with skin 2, as in the reference's gcmc run
+    s = orc.System(orc.dana_params(box, 2.0, h=5e-5, z0=z0, zmax=zmax, ermak=False, piston=False),
+                   pos, z)
+    cfg = dml.dana_config(box, 2.0, len(pos) + 512, h=5e-5, z0=z0, zmax=zmax,
+                          integrator=dml.BROWNIAN, reservoir=dml.GCMC,
+                          overlap_mode=dml.OVERLAP_EXACT)
+    c = dml.Context(cfg)
+    c.set_state(pos, species=z)
+    s.test_update(); c.test_update()
+    s.gcmc_init()
+    assert c.cells()[2] == (geom == "cells")
+    _assert_same_groups(s, c, "initial")
+    vol = box[0] * box[1] * (zmax - z0)
+    n_in = int(((ions[:, 2] >= z0) & (ions[:, 2] <= zmax)).sum())
+    act = max(n_in, 1) / vol  # accept about half of each kind
+    for it in range(4):
+        u = rng.random((nadj, 5))
+        g = rng.standard_normal((nadj, 3))
+        assert s.gcmc(act, nadj, u, g) == 0
+        c.gcmc(act, nadj, u, g)
+        _assert_same_groups(s, c, "gcmc call %d" % it)
+        assert (s.c.ins, s.c.del_) == (c.counters()["ins"], c.counters()["del_"])
+        _step_pair(orc, dml, s, c, rng, "exact", 2, gcmc=True)
+        _assert_same_groups(s, c, "steps after call %d" % it)
+    assert s.c.ins > 5 and s.c.del_ > 5, (s.c.ins, s.c.del_)
+    c.close()
+
+
+def test_gcmc_run_alone_equals_in_ensemble_and_graph(dml):
+    """whole steps with the GCMC reservoir on Philox draws: the replica's
+    result does not depend on its neighbours or on graph replay."""
+    rng = np.random.default_rng(43)
+    box = np.array([40.0, 40.0, 60.0])
+    el = electrode(3.51, 11, 11, 2, 1.755)
+    ions = rsa(rng, 160, np.array([0, 0, 4.0]), np.array([40, 40, 59.0]), 3.3, box)
+    pos = np.vstack([el, ions])
+    z = np.array([2] * len(el) + [1] * len(ions), np.int8)
+    z0, zmax = 20.0, 59.5
+    act = 100.0 / (box[0] * box[1] * (zmax - z0))
+
+    def cfg(seed, reservoir=dml.GCMC):
+        return dml.dana_config(box, 2.0, len(pos) + 1024, h=5e-5, z0=z0, zmax=zmax,
+                               integrator=dml.BROWNIAN, reservoir=reservoir, seed=seed,
+                               overlap_mode=dml.OVERLAP_ROUNDS, act=act, nadj=16)
+    say("gcmc run", len(pos))
+    a = dml.Context(cfg(7))
+    a.set_state(pos, species=z)
+    a.run(25, graph=False)
+    assert a.replica_status(0) == 0
+    sa = a.state()
+    ca = a.counters()
+    assert ca["ins"] > 0 and ca["del_"] > 0, ca
+    b = dml.Context([cfg(99, dml.NO_RESERVOIR), cfg(7)])
+    b.set_state(pos, species=z, rep=0)
+    b.set_state(pos, species=z, rep=1)
+    b.run(25, graph=True)
+    assert b.replica_status(1) == 0
+    sb = b.state(1)
+    assert b.nslots(1) == a.nslots(0)
+    for k in ("pos", "vel", "species", "flags"):
+        np.testing.assert_array_equal(sb[k], sa[k], err_msg=k)
+    np.testing.assert_array_equal(b.gcmc_order(1), a.gcmc_order(0))
+    assert b.counters(1) == ca
+    a.close(); b.close()
