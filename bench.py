@@ -24,6 +24,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "din-mol-li_amd"))
 
 HBM_PEAK_GBS = 8000.0
+# reservoir of replica k is RESERVOIR_CYCLE[k % 3]
+RESERVOIR_CYCLE = ("PISTON", "NO_RESERVOIR", "GCMC")
 
 
 def log(*a):
@@ -101,13 +103,21 @@ def batched_leg(dml, nrep, steps, warmup):
     cfgs = []
     for k, r in enumerate(reps):
         n = len(r["pos"])
-        piston = (k % 3 == 0)
+        res = RESERVOIR_CYCLE[k % 3]
+        # GCMC: control volume from 3/4 of the ion slab to the box top at the
+        # ion density, 10 trials per step as in the reference's gcmc run
+        kw = dict(reservoir=getattr(dml, res))
+        if res == "PISTON":
+            kw.update(z0=r["z0_piston"], zmax=r["zmax"])
+        elif res == "GCMC":
+            kw.update(z0=r["z0_piston"], zmax=r["zmax"], act=0.01, nadj=10)
+            n += max(1024, n // 4)
+        else:
+            kw.update(z0=r["z0"], zmax=r["zmax"])
         cfgs.append(dml.dana_config(r["box"], r["skin"], n, h=r["h"],
-                                    z0=r["z0_piston"] if piston else r["z0"], zmax=r["zmax"],
                                     integrator=dml.ERMAK if k % 2 else dml.BROWNIAN,
-                                    reservoir=dml.PISTON if piston else dml.NO_RESERVOIR,
                                     precision=dml.FP32_FORCE, overlap_mode=dml.OVERLAP_ROUNDS,
-                                    seed=104012 + 7919 * k))
+                                    seed=104012 + 7919 * k, **kw))
     ens = dml.Context(cfgs)
     ntot = 0
     for k, r in enumerate(reps):
@@ -115,7 +125,7 @@ def batched_leg(dml, nrep, steps, warmup):
         ntot += len(r["pos"])
     ens.run(warmup, graph=True)
     ens.sync()
-    before = [ens.list_stats(k)[2] for k in range(nrep)]
+    before = [ens.counters(k) for k in range(nrep)]
     t0 = time.perf_counter()
     ms = ens.run(steps, graph=True)
     wall = time.perf_counter() - t0
@@ -123,10 +133,12 @@ def batched_leg(dml, nrep, steps, warmup):
     mean_nn = sum(a for a, b, c in nn) / nrep
     reb = sum(b for a, b, c in nn)
     stepsum = sum(c for a, b, c in nn)
-    # particle-steps actually taken: a replica halted by a fatal condition of
-    # the reference (e.g. a deposit above z0) stops advancing
-    sizes = [len(r["pos"]) for r in reps]
-    psteps = sum(n * (c - b0) for n, (a, bb, c), b0 in zip(sizes, nn, before))
+    # particle-steps actually taken (atoms alive in each completed step): a
+    # replica halted by a fatal condition of the reference (e.g. a deposit
+    # above z0) stops advancing, a GCMC replica grows and shrinks
+    after = [ens.counters(k) for k in range(nrep)]
+    psteps = sum(a["particle_steps"] - b["particle_steps"] for a, b in zip(after, before))
+    trials = sum(a["ins"] + a["del_"] - b["ins"] - b["del_"] for a, b in zip(after, before))
     halted = sum(1 for k in range(nrep) if ens.replica_status(k) != 0)
     # dominant kernel: the neighbour-row build (see profiles/), timed on the
     # context's own stream
@@ -135,7 +147,9 @@ def batched_leg(dml, nrep, steps, warmup):
     ens.close()
     return dict(n=ntot, ms=ms, wall=wall, rate=psteps / (ms * 1e-3), mean_nn=mean_nn,
                 rebuilds_per_rep_step=reb / max(stepsum, 1), list_ms=kl_ms, force_ms=kf_ms,
-                halted=halted)
+                halted=halted, gcmc_accepted=trials,
+                reservoirs={x: sum(1 for k in range(nrep) if RESERVOIR_CYCLE[k % 3] == x)
+                            for x in RESERVOIR_CYCLE})
 
 
 def cpu_baseline():
@@ -154,6 +168,18 @@ def cpu_baseline():
                        "skin 10, FP64 oracle on one host core, %.2f s" % (n, cfg["nst"], out["loop_seconds"]))
 
 
+def calibrate_copy(nbytes=1 << 30):
+    """One plain device copy of known size: under rocprofv3 --pmc the
+    FETCH_SIZE / WRITE_SIZE this kernel reads give the counter's scale."""
+    import torch
+    src = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    log("copy calibration: %d bytes read, %d bytes written" % (nbytes, nbytes))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -162,6 +188,8 @@ def main():
     ap.add_argument("--replicas", type=int, default=256)
     ap.add_argument("--legs", default="plain,batched")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run one 1 GiB device copy (scale of the rocprofv3 byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (replicas shard across GPUs with no "
@@ -175,6 +203,8 @@ def main():
     except Exception:
         sync = lambda: None
     legs = args.legs.split(",")
+    if args.calibrate_copy:
+        calibrate_copy()
     plain = batched = None
     if "plain" in legs:
         log("plain leg: C2 100k FP64")
@@ -219,14 +249,22 @@ def main():
         per_atom = 24 + 4 + 4 * batched["mean_nn"] + 28
         traffic = per_atom * batched["n"]
         ach = traffic / (batched["list_ms"] * 1e-3) / 1e9
-        res["roofline"] = {"kernel": "k_listbuild", "bound": "hbm", "achieved": ach,
+        # HBM bytes per launch from the two rocprofv3 PMC passes (FETCH_SIZE,
+        # WRITE_SIZE, scaled by a 1 GiB copy) of the same 256-replica launch
+        measured = None
+        tf = os.path.join(ROOT, "profiles", "k_listbuild_cells_c3_traffic.json")
+        if args.replicas == 256 and os.path.exists(tf):
+            measured = json.load(open(tf))["hbm_bytes_per_launch"]
+        res["roofline"] = {"kernel": "k_listbuild_cells", "bound": "hbm", "achieved": ach,
                            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                           "traffic": None, "avg_launch_ms": batched["list_ms"],
+                           "traffic": measured, "avg_launch_ms": batched["list_ms"],
                            "algorithmic_bytes_per_launch": traffic}
-        res["kernel_ms"] = {"k_listbuild": batched["list_ms"], "k_force": batched["force_ms"]}
+        res["kernel_ms"] = {"k_listbuild_cells": batched["list_ms"], "k_force": batched["force_ms"]}
         res["mean_list_length"] = batched["mean_nn"]
         res["rebuilds_per_step"] = batched["rebuilds_per_rep_step"]
         res["replicas_halted"] = batched["halted"]
+        res["reservoir_mix"] = batched["reservoirs"]
+        res["gcmc_moves_accepted"] = batched["gcmc_accepted"]
     if not args.no_cpu:
         log("cpu baseline: oracle on the reference Ermak test")
         res["cpu_baseline"] = cpu_baseline()

@@ -1249,6 +1249,7 @@ int dml_get_counters(dml_ctx *c, int32_t rep, dml_counters *out) {
   out->ins = (int64_t)h.ins;
   out->del = (int64_t)h.del;
   out->passes = (int64_t)h.passes;
+  out->particle_steps = (int64_t)h.particle_steps;
   return DML_OK;
 }
 

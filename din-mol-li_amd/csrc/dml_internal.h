@@ -78,6 +78,7 @@ struct RepParams {
 struct DevCounters {
   unsigned long long try_, depo, choques, choques3, nupd, ins, del, passes;
   unsigned long long nn_sum, nn_rows, steps, ov_unfinished;
+  unsigned long long particle_steps;  // sum over steps of the atoms alive
 };
 
 // ------------------------------------------------------------------------

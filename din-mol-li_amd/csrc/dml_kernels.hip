@@ -1362,6 +1362,7 @@ __global__ void k_step_end(DevState s) {
   }
   Pw->step += 1;
   s.cnt[r].steps += 1;
+  s.cnt[r].particle_steps += (unsigned long long)Pw->nat;
 }
 
 // ------------------------------------------------------------------------
