@@ -13,3 +13,14 @@ hipcc=${HIPCC:-/opt/rocm/bin/hipcc}
 [[ -f "$out.tmp" ]] || { echo "build failed" >&2; exit 1; }
 mv "$out.tmp" "$out"
 echo "built $out"
+
+# the Fortran side: ISO_C_BINDING interfaces and the dana driver on libdml
+fc=${FC_AMD:-/opt/rocm/bin/amdflang}
+if [[ -x "$fc" ]]; then
+  mod="$here/build/fmod"
+  mkdir -p "$mod"
+  "$fc" -O2 -c "$here/fortran/dml_f.F90" -module-dir "$mod" -o "$mod/dml_f.o"
+  "$fc" -O2 "$here/fortran/dana_dml.F90" "$mod/dml_f.o" -module-dir "$mod" -I "$mod" \
+    -L "$here/dml" -ldml -Wl,-rpath,'$ORIGIN' -o "$here/dml/dana_dml"
+  echo "built $here/dml/dana_dml"
+fi

@@ -342,8 +342,11 @@ void enq_step(dml_ctx *c) {
 }
 
 // grow neighbour rows or buckets after an overflow and redo the rebuild of
-// the replicas whose rebuild flag is still set
-int handle_overflow(dml_ctx *c, int e) {
+// the replicas whose rebuild flag is still set.  An overflowing replica halts
+// in the kernel; with resume it is released and rebuilt (the call that
+// overflowed was the rebuild itself), without it (after whole steps, which
+// went on without it) it stays halted with its status.
+int handle_overflow(dml_ctx *c, int e, int resume) {
   for (int attempt = 0; attempt < 8 && (e & (ERR_LIST | ERR_BUCKET)); attempt++) {
     int rc = fetch_params(c);
     if (rc) return rc;
@@ -369,7 +372,9 @@ int handle_overflow(dml_ctx *c, int e) {
       }
       int rc2 = alloc_cells(c);
       if (rc2) return rc2;
-      // only the layout fields changed: patch them into the device table
+    }
+    {
+      // patch the layout fields and the overflow halts into the device table
       std::vector<RepParams> v(c->nrep);
       HIPCHECK(hipMemcpy(v.data(), c->d.rp, sizeof(RepParams) * c->nrep, hipMemcpyDeviceToHost));
       for (int r = 0; r < c->nrep; r++) {
@@ -377,6 +382,12 @@ int handle_overflow(dml_ctx *c, int e) {
         v[r].ccap = c->reps[r].p.ccap;
         v[r].boff = c->reps[r].p.boff;
         v[r].bcap = c->reps[r].p.bcap;
+        const int ovf = ERR_LIST | ERR_BUCKET;
+        if (resume && v[r].err_bits && !(v[r].err_bits & ~ovf)) {
+          v[r].err_bits = 0;
+          v[r].halted = 0;
+        }
+        c->reps[r].p = v[r];
       }
       HIPCHECK(hipMemcpy(c->d.rp, v.data(), sizeof(RepParams) * c->nrep, hipMemcpyHostToDevice));
     }
@@ -391,11 +402,11 @@ int handle_overflow(dml_ctx *c, int e) {
 
 // replica_fatal: report a replica's halt as the call's status (per-routine
 // entry points); dml_run leaves halted replicas to dml_replica_status()
-int finish(dml_ctx *c, int replica_fatal = 1) {
+int finish(dml_ctx *c, int replica_fatal = 1, int resume = 1) {
   int e = read_err(c, 0);
   if (e < 0) return DML_E_HIP;
   if (e & (ERR_LIST | ERR_BUCKET)) {
-    int rc = handle_overflow(c, e);
+    int rc = handle_overflow(c, e, resume);
     if (rc) return rc;
     e = read_err(c, 0);
   }
@@ -1059,6 +1070,36 @@ int dml_gcmc(dml_ctx *c, int32_t rep, double act, int32_t nadj, const double *u,
   return rc;
 }
 
+int dml_calc_rho(dml_ctx *c, double *rho) {
+  if (!c) return DML_E_ARG;
+  int nb = c->d.nblocks;
+  int rb = (c->nrep + 63) / 64;
+  hipLaunchKernelGGL(k_rep_op, dim3(rb), dim3(64), 0, c->st, c->d, 0);
+  hipLaunchKernelGGL(k_calc_rho, dim3(nb), dim3(DML_BLOCK), 0, c->st, c->d);
+  int rc = finish(c);
+  if (rc || !rho) return rc;
+  rc = fetch_params(c);
+  if (rc) return rc;
+  for (int r = 0; r < c->nrep; r++) {
+    const RepParams &p = c->reps[r].p;
+    rho[r] = (double)p.rho_count / (p.box[0] * p.box[1] * (p.zmax - p.z0));
+  }
+  return DML_OK;
+}
+
+int dml_maxz(dml_ctx *c) {
+  if (!c) return DML_E_ARG;
+  hipLaunchKernelGGL(k_maxz, dim3(c->d.nblocks), dim3(DML_BLOCK), 0, c->st, c->d);
+  hipLaunchKernelGGL(k_rep_op, dim3((c->nrep + 63) / 64), dim3(64), 0, c->st, c->d, 1);
+  return finish(c);
+}
+
+int dml_end_step(dml_ctx *c) {
+  if (!c) return DML_E_ARG;
+  hipLaunchKernelGGL(k_rep_op, dim3((c->nrep + 63) / 64), dim3(64), 0, c->st, c->d, 2);
+  return finish(c);
+}
+
 int dml_get_gcmc_order(dml_ctx *c, int32_t rep, int32_t *ngc, int32_t *order) {
   if (!c || rep < 0 || rep >= c->nrep || !ngc) return DML_E_ARG;
   int rc = fetch_params(c);
@@ -1084,7 +1125,8 @@ int dml_get_gcmc_order(dml_ctx *c, int32_t rep, int32_t *ngc, int32_t *order) {
   return DML_OK;
 }
 
-int dml_get_slots(dml_ctx *c, int32_t rep, int32_t *nslots, int32_t *bkey, uint32_t *pid) {
+int dml_get_slots(dml_ctx *c, int32_t rep, int32_t *nslots, int32_t *bkey, uint32_t *pid,
+                  int32_t *seq) {
   if (!c || rep < 0 || rep >= c->nrep || !nslots) return DML_E_ARG;
   int rc = fetch_params(c);
   if (rc) return rc;
@@ -1094,6 +1136,8 @@ int dml_get_slots(dml_ctx *c, int32_t rep, int32_t *nslots, int32_t *bkey, uint3
     HIPCHECK(hipMemcpy(bkey, c->d.bkey + p.poff, sizeof(int32_t) * p.nslots, hipMemcpyDeviceToHost));
   if (pid && p.nslots > 0)
     HIPCHECK(hipMemcpy(pid, c->d.pid + p.poff, sizeof(uint32_t) * p.nslots, hipMemcpyDeviceToHost));
+  if (seq && p.nslots > 0)
+    HIPCHECK(hipMemcpy(seq, c->d.seq + p.poff, sizeof(int32_t) * p.nslots, hipMemcpyDeviceToHost));
   return DML_OK;
 }
 
@@ -1132,6 +1176,8 @@ int dml_run(dml_ctx *c, int32_t nsteps, int32_t use_graph) {
   if (!c || nsteps < 0) return DML_E_ARG;
   int rc = fetch_params(c);
   if (rc) return rc;
+  for (auto &h : c->reps)
+    if (h.p.reservoir == DML_CHUNKS) return DML_E_UNSUPPORTED;  // bloques is not on the device
   refresh_modes(c);
   int need_init = 0;
   for (auto &h : c->reps) need_init |= !h.p.listed;
@@ -1168,7 +1214,7 @@ int dml_run(dml_ctx *c, int32_t nsteps, int32_t use_graph) {
   float ms = 0.f;
   hipEventElapsedTime(&ms, c->e0, c->e1);
   c->last_ms = ms;
-  return finish(c, 0);
+  return finish(c, 0, 0);
 }
 
 int dml_replica_status(dml_ctx *c, int32_t rep, int32_t *status) {

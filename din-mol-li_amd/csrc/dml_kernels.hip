@@ -208,6 +208,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_cell_key(DevState s) {
   int rank = atomicAdd(&s.cell_cnt[P.coff + cell], 1);
   if (rank >= P.bcap) {
     atomicOr(s.err, ERR_BUCKET);
+    rep_fail(s, r, ERR_BUCKET);
     return;
   }
   s.bucket[P.boff + (int64_t)cell * P.bcap + rank] = slot;
@@ -341,6 +342,7 @@ __global__ void __launch_bounds__(WPB *WAVE) k_listbuild(DevState s) {
     if (nn > maxnn) {
       atomicOr(s.err, ERR_LIST);
       atomicMax(&s.rp[r].max_nn_seen, nn);
+      rep_fail(s, r, ERR_LIST);
       nn = maxnn;
     }
     s.nn[i] = nn;
@@ -501,6 +503,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_listbuild_cells(DevState s) {
         if (nn > maxnn) {
           atomicOr(s.err, ERR_LIST);
           atomicMax(&s.rp[r].max_nn_seen, nn);
+          rep_fail(s, r, ERR_LIST);
           nn = maxnn;
         }
         s.nn[i] = nn;
@@ -1309,7 +1312,6 @@ __global__ void __launch_bounds__(DML_BLOCK) k_calc_rho(DevState s) {
   int r;
   const RepParams &P = rep_of_block(s, blockIdx.x, &r);
   if (P.halted) return;
-  if (!P.piston) return;
   if (threadIdx.x == 0) cnt = 0;
   __syncthreads();
   int local = slot - P.poff;
@@ -1347,6 +1349,28 @@ __global__ void __launch_bounds__(DML_BLOCK) k_maxz(DevState s) {
   }
 }
 
+// per-replica bookkeeping of the per-routine entry points: 0 = clear the
+// calc_rho count, 1 = piston zmax (dana.F90:792), 2 = end of step
+__global__ void k_rep_op(DevState s, int op) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= s.nrep) return;
+  RepParams *Pw = s.rp + r;
+  if (Pw->halted) return;
+  if (op == 0) {
+    Pw->rho_count = 0;
+  } else if (op == 1) {
+    if (Pw->piston) {
+      double lohi = piston_lohi(*Pw);
+      Pw->lohi = lohi;
+      Pw->zmax = Pw->zmax - lohi * (Pw->zmax - Pw->z0);
+    }
+  } else {
+    Pw->step += 1;
+    s.cnt[r].steps += 1;
+    s.cnt[r].particle_steps += (unsigned long long)Pw->nat;
+  }
+}
+
 // step end: advance the per-replica step counter (and the piston zmax,
 // which no kernel of k_maxz reads)
 __global__ void k_step_end(DevState s) {
@@ -1358,8 +1382,8 @@ __global__ void k_step_end(DevState s) {
     double lohi = piston_lohi(*Pw);
     Pw->lohi = lohi;
     Pw->zmax = Pw->zmax - lohi * (Pw->zmax - Pw->z0);
-    Pw->rho_count = 0;
   }
+  Pw->rho_count = 0;
   Pw->step += 1;
   s.cnt[r].steps += 1;
   s.cnt[r].particle_steps += (unsigned long long)Pw->nat;

@@ -33,7 +33,8 @@ EXPORTS = ["dml_create", "dml_create_ensemble", "dml_destroy", "dml_sync", "dml_
            "dml_overlap_moveback", "dml_overlap_rounds", "dml_f_to_cg", "dml_gcmc",
            "dml_gcmc_trial_energy", "dml_run", "dml_get_counters", "dml_list_stats",
            "dml_last_run_ms", "dml_time_kernel", "dml_set_tuning",
-           "dml_replica_status", "dml_get_gcmc_order", "dml_get_slots"]
+           "dml_replica_status", "dml_get_gcmc_order", "dml_get_slots",
+           "dml_calc_rho", "dml_maxz", "dml_end_step"]
 K_LISTBUILD, K_FORCE, K_CELLS, K_DISPMAX = 0, 1, 2, 3
 
 
@@ -107,7 +108,10 @@ def lib():
         L.dml_replica_status.argtypes = [vp, C.c_int32, vp]
         L.dml_sync.argtypes = [vp]
         L.dml_get_gcmc_order.argtypes = [vp, C.c_int32, vp, vp]
-        L.dml_get_slots.argtypes = [vp, C.c_int32, vp, vp, vp]
+        L.dml_get_slots.argtypes = [vp, C.c_int32, vp, vp, vp, vp]
+        L.dml_calc_rho.argtypes = [vp, vp]
+        L.dml_maxz.argtypes = [vp]
+        L.dml_end_step.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -199,18 +203,21 @@ class Context:
     def nslots(self, rep=0):
         """Slots in use (hs%amax): grows when GCMC appends atoms."""
         n = C.c_int32()
-        _chk(self.L.dml_get_slots(self.h, rep, C.byref(n), None, None), "dml_get_slots")
+        _chk(self.L.dml_get_slots(self.h, rep, C.byref(n), None, None, None), "dml_get_slots")
         self.n[rep] = n.value
         return n.value
 
     def slots(self, rep=0):
-        """(bkey, pid) per slot: the hs%b index and the particle id."""
+        """(bkey, pid, seq) per slot: the hs%b index, the particle id and the
+        creation number (sys list order)."""
         n = self.nslots(rep)
         bk = np.zeros(max(n, 1), np.int32)
         pid = np.zeros(max(n, 1), np.uint32)
+        seq = np.zeros(max(n, 1), np.int32)
         nn = C.c_int32()
-        _chk(self.L.dml_get_slots(self.h, rep, C.byref(nn), _p(bk), _p(pid)), "dml_get_slots")
-        return bk[:n], pid[:n]
+        _chk(self.L.dml_get_slots(self.h, rep, C.byref(nn), _p(bk), _p(pid), _p(seq)),
+             "dml_get_slots")
+        return bk[:n], pid[:n], seq[:n]
 
     def gcmc_order(self, rep=0):
         ng = C.c_int32()
@@ -326,6 +333,19 @@ class Context:
 
     def f_to_cg(self):
         _chk(self.L.dml_f_to_cg(self.h), "dml_f_to_cg")
+
+    def calc_rho(self):
+        """calc_rho (dana.F90:521-549) of every replica"""
+        rho = np.zeros(self.nrep)
+        _chk(self.L.dml_calc_rho(self.h, _p(rho)), "dml_calc_rho")
+        return rho
+
+    def maxz(self):
+        """maxz (dana.F90:776-794) of the piston replicas"""
+        _chk(self.L.dml_maxz(self.h), "dml_maxz")
+
+    def end_step(self):
+        _chk(self.L.dml_end_step(self.h), "dml_end_step")
 
     def gcmc_trial_energy(self, pos3, rep=0):
         p = np.ascontiguousarray(pos3, np.float64)

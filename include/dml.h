@@ -163,6 +163,18 @@ int dml_overlap_rounds(dml_ctx *ctx, const double *u_round, int32_t max_rounds);
 /* F -> CG and removal from the ref list (dana.F90:228-236) */
 int dml_f_to_cg(dml_ctx *ctx);
 
+/* --- piston reservoir and the step counter -----------------------------
+ * calc_rho (dana.F90:521-549): count of atoms with z0 < z < zmax per replica,
+ * rho[rep] = count / (Lx Ly (zmax - z0)) (may be NULL); piston replicas keep
+ * the count for dml_maxz. */
+int dml_calc_rho(dml_ctx *ctx, double *rho);
+/* maxz (dana.F90:776-794) of the piston replicas, with the count of the last
+ * dml_calc_rho */
+int dml_maxz(dml_ctx *ctx);
+/* end of one main-loop step (dana.F90:262): advances the step number that
+ * keys the Philox streams; dml_run does this itself */
+int dml_end_step(dml_ctx *ctx);
+
 /* --- GCMC reservoir (dana.F90:590-713) ----------------------------------
  * nadj trials of replica rep with uniforms u[t*5 + {0 coin,1 accept,
  * 2..4 coords | 2 pick}] and velocity normals g[t*3 + k]; NULL = Philox. */
@@ -171,10 +183,11 @@ int dml_gcmc(dml_ctx *ctx, int32_t rep, double act, int32_t nadj,
 /* the gcmc group in list order (slots of rep; the template is order[0], the
  * velocity of an insertion lands on order[ngc-1]: dana.F90:660-668) */
 int dml_get_gcmc_order(dml_ctx *ctx, int32_t rep, int32_t *ngc, int32_t *order);
-/* per slot: hs%b index (cell-chain order key) and particle id; *nslots is the
- * hs%amax of igroup_attach_atom (Groups.F90:1083-1093) */
+/* per slot: hs%b index (cell-chain order key), particle id and creation
+ * number (the order of sys%alist, which salida writes); *nslots is the
+ * hs%amax of igroup_attach_atom (Groups.F90:1083-1093).  Arrays may be NULL. */
 int dml_get_slots(dml_ctx *ctx, int32_t rep, int32_t *nslots, int32_t *bkey,
-                  uint32_t *pid);
+                  uint32_t *pid, int32_t *seq);
 /* batched trial energies: WCA sum (FP64) of an inserted Li at pos3[t] against
  * every atom of its 27 cells, and the hard-core indicator of
  * dana.F90:638-653 (gcmc atom closer than rcut). */

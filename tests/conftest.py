@@ -20,3 +20,11 @@ def orc():
     import orc as _orc
     _orc.build()
     return _orc
+
+
+@pytest.fixture(scope="session")
+def dml():
+    """the ctypes binding of libdml (raises if the library is not built)"""
+    import dml as _dml
+    _dml.lib()
+    return _dml

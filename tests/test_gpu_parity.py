@@ -548,3 +548,23 @@ def test_gcmc_run_alone_equals_in_ensemble_and_graph(dml):
     np.testing.assert_array_equal(b.gcmc_order(1), a.gcmc_order(0))
     assert b.counters(1) == ca
     a.close(); b.close()
+
+
+def test_row_overflow_grows_and_rebuilds(orc, dml):
+    """rows longer than max_neighbors: a per-routine rebuild halts the
+    replica in the kernel, then the host grows the rows and replays it"""
+    box, skin, pos, z = c1_system(seed=9)
+    s = orc.System(orc.dana_params(box, skin, ermak=False, piston=False), pos, z)
+    cfg = dml.dana_config(box, skin, len(pos) + 64, integrator=dml.BROWNIAN,
+                          reservoir=dml.NO_RESERVOIR, max_neighbors=16)
+    say("overflow", len(pos))
+    c = dml.Context(cfg)
+    c.set_state(pos, species=z)
+    s.test_update(); c.test_update()
+    assert c.replica_status(0) == 0
+    assert_lists_equal(c, s)
+    nn, _, _ = c.lists()
+    assert nn.max() > 16
+    s.fuerza(); c.forces()  # the grown rows serve the force loop
+    assert rel_err(c.state()["force"], s.state()["force"]) <= 1e-12
+    c.close()
