@@ -33,15 +33,16 @@ def log(*a):
 
 
 def _replica(args):
-    n, seed = args
+    n, seed, band = args
     from dml import workloads as W
-    return W.system_replica(int(n), int(seed))
+    return W.system_replica(int(n), int(seed), reservoir_band=band)
 
 
 def build_replicas(nrep, seed0=104012):
     from dml import workloads as W
     sizes = W.replica_sizes(nrep)
-    jobs = [(int(n), seed0 + 1000 + k) for k, n in enumerate(sizes)]
+    jobs = [(int(n), seed0 + 1000 + k, RESERVOIR_CYCLE[k % 3] != "NO_RESERVOIR")
+            for k, n in enumerate(sizes)]
     workers = min(16, os.cpu_count() or 1)
     if workers > 1 and nrep > 4:
         import multiprocessing as mp
@@ -107,13 +108,10 @@ def batched_leg(dml, nrep, steps, warmup):
         # GCMC: control volume from 3/4 of the ion slab to the box top at the
         # ion density, 10 trials per step as in the reference's gcmc run
         kw = dict(reservoir=getattr(dml, res))
-        if res == "PISTON":
-            kw.update(z0=r["z0_piston"], zmax=r["zmax"])
-        elif res == "GCMC":
-            kw.update(z0=r["z0_piston"], zmax=r["zmax"], act=0.01, nadj=10)
+        kw.update(z0=r["z0"], zmax=r["zmax"])
+        if res == "GCMC":
+            kw.update(act=0.01, nadj=10)
             n += max(1024, n // 4)
-        else:
-            kw.update(z0=r["z0"], zmax=r["zmax"])
         cfgs.append(dml.dana_config(r["box"], r["skin"], n, h=r["h"],
                                     integrator=dml.ERMAK if k % 2 else dml.BROWNIAN,
                                     precision=dml.FP32_FORCE, overlap_mode=dml.OVERLAP_ROUNDS,
@@ -152,6 +150,69 @@ def batched_leg(dml, nrep, steps, warmup):
                             for x in RESERVOIR_CYCLE})
 
 
+def _request_system(args):
+    n, seed, band = args
+    from dml import workloads as W
+    return W.system_replica(int(n), int(seed), reservoir_band=band)
+
+
+def serving_leg(dml, duration, rate=20.0, step_scale=0.01, seed=104012 + 4):
+    """C4 on one GPU: Poisson arrivals at `rate` per second for `duration`
+    seconds; sizes lognormal (median 10k, sigma 1) clipped to 1k..200k; step
+    counts uniform in 1e3..1e5 times `step_scale`.  Requests are admitted into
+    a slot pool reserved at start and served by continuous batching."""
+    import math
+    import numpy as np
+    from dml import serve, workloads as W
+    rng = np.random.default_rng(seed)
+    t, arr = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > duration:
+            break
+        arr.append(t)
+    m = len(arr)
+    sizes = np.clip(np.exp(rng.normal(math.log(10000.0), 1.0, m)), 1000, 200000).astype(int)
+    steps = np.maximum(1, np.round(rng.integers(1000, 100001, m) * step_scale)).astype(int)
+    t0 = time.perf_counter()
+    jobs = [(int(n), seed + 17 * k, RESERVOIR_CYCLE[k % 3] != "NO_RESERVOIR")
+            for k, n in enumerate(sizes)]
+    workers = min(16, os.cpu_count() or 1)
+    import multiprocessing as mp
+    with mp.get_context("fork").Pool(workers) as pool:
+        systems = pool.map(_request_system, jobs, chunksize=4)
+    log("built %d requests in %.1f s" % (m, time.perf_counter() - t0))
+    reqs = []
+    for k, (sysd, st) in enumerate(zip(systems, steps)):
+        res = RESERVOIR_CYCLE[k % 3]
+        n = len(sysd["pos"])
+        kw = dict(reservoir=getattr(dml, res), z0=sysd["z0"], zmax=sysd["zmax"])
+        need = n
+        if res == "GCMC":
+            kw.update(act=0.01, nadj=10)
+            need = n + min(max(1024, n // 4), 10 * int(st) + 64)
+        cfg = dml.dana_config(sysd["box"], sysd["skin"], need, h=sysd["h"],
+                              integrator=dml.ERMAK if k % 2 else dml.BROWNIAN,
+                              precision=dml.FP32_FORCE, overlap_mode=dml.OVERLAP_ROUNDS,
+                              seed=104012 + 7919 * k, **kw)
+        reqs.append(serve.Request(k, arr[k], cfg, sysd["pos"], sysd["species"], st, need=need))
+    # slot classes: the HBM reserved for serving (state + rows)
+    classes = []
+    for cap, count in ((8192, 48), (32768, 16), (131072, 4), (204800 + 8192, 2)):
+        classes.append((cap, count, W.system_replica_box(cap)))
+    pool = serve.SlotPool(classes)
+    log("serving pool: %d slots, %.1f GB reserved" % (sum(c[1] for c in classes),
+                                                     pool.hbm_bytes() / 1e9))
+    done, wall = serve.serve(pool, reqs, segment=8)
+    pool.close()
+    out = serve.summarize(done, wall)
+    offered = sum(q.n * q.steps for q in reqs) / duration
+    out.update(requests=m, arrival_rate_per_s=rate, duration_s=duration, step_scale=step_scale,
+               offered_particle_steps_per_s=offered, slots=sum(c[1] for c in classes),
+               reserved_gb=pool.hbm_bytes() / 1e9)
+    return out
+
+
 def cpu_baseline():
     """The CPU oracle (a bit-exact restatement of the reference) on the
     reference's own Ermak test: 1,204 particles, 10^4 steps, piston."""
@@ -186,7 +247,9 @@ def main():
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--replicas", type=int, default=256)
-    ap.add_argument("--legs", default="plain,batched")
+    ap.add_argument("--legs", default="plain,batched,serving")
+    ap.add_argument("--serve-seconds", type=float, default=30.0,
+                    help="arrival window of the serving leg")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 1 GiB device copy (scale of the rocprofv3 byte counters)")
@@ -218,7 +281,17 @@ def main():
         batched = batched_leg(dml, args.replicas, args.steps, args.warmup)
         sync()
         log("batched", json.dumps(batched))
+    serving = None
+    if "serving" in legs:
+        log("serving leg: Poisson 20/s for %.0f s" % args.serve_seconds)
+        sync()
+        serving = serving_leg(dml, args.serve_seconds)
+        sync()
+        log("serving", json.dumps(serving))
     head = batched or plain
+    if head is None:  # serving only
+        print(json.dumps({"serving": serving}), flush=True)
+        return
     res = {
         "metric": "particle-steps/s batched vs plain at force err<=1e-5 vs FP64; p99 run latency",
         "value": head["rate"],
@@ -241,7 +314,9 @@ def main():
         res["force_rel_err_fp32_vs_fp64"] = plain["force_err"]
     if plain and batched:
         res["batched_over_plain"] = batched["rate"] / plain["rate"]
-    res["p99_run_latency_s"] = None  # serving leg: serve.py
+    res["p99_run_latency_s"] = serving["p99_latency_s"] if serving else None
+    if serving:
+        res["serving"] = serving
     if batched:
         # neighbour-row build: compulsory bytes per ref atom = own position
         # (24) + row count (4) + 4 per list entry, plus per b atom its bucket

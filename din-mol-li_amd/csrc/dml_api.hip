@@ -29,7 +29,9 @@ constexpr int kMaxRounds = 32;
 
 struct HostRep {
   dml_config cfg;
-  RepParams p;  // host mirror of the static parameters
+  RepParams p;         // host mirror of the static parameters
+  int64_t creg = 0;    // cell-count entries reserved for this replica
+  int64_t breg = 0;    // bucket entries reserved for this replica
 };
 
 }  // namespace
@@ -52,6 +54,7 @@ struct dml_ctx {
   int lb_mode = 0; // 0: cell-tile list build, 1: wave per atom
   int rounds = 8;
   int any_ermak = 0, any_brown = 0, any_rounds = 0, any_exact = 0, any_piston = 0, any_gcmc = 0;
+  int64_t graph_modes = -1;  // mode signature the captured step was built for
 };
 
 namespace {
@@ -95,6 +98,60 @@ int tessellate(RepParams &p, int first) {
   }
   p.use_cells = 1;
   return DML_OK;
+}
+
+int bucket_cap(const RepParams &p);
+void set_ermak_consts(RepParams &p);
+
+// the static parameters of one replica from its config (everything but its
+// slot range and cell layout; p.pcap must be set: a Verlet bucket holds it)
+int setup_rep(RepParams &p, const dml_config &g) {
+  for (int k = 0; k < 3; k++) {
+    p.box[k] = g.box[k];
+    p.one_box[k] = 1.0 / g.box[k];
+  }
+  p.rcut = g.rcut;
+  p.rcut2 = g.rcut * g.rcut;
+  p.skin = g.skin;
+  double rl = g.rcut + g.skin;
+  p.rl2 = rl * rl;
+  p.zmax = g.zmax;
+  p.z0 = g.z0;
+  p.prob = g.prob;
+  p.h = g.h;
+  p.z_switch = g.z_switch;
+  p.dif_sc = g.dif_sc;
+  p.dif_sei = g.dif_sei;
+  for (int q = 0; q < 9; q++) {
+    p.eps[q] = g.eps[q];
+    p.r0[q] = g.r0[q];
+  }
+  for (int q = 0; q < 3; q++) p.mass[q] = g.mass[q];
+  set_ermak_consts(p);
+  p.act = g.act;
+  p.nadj = g.nadj;
+  p.integrator = g.integrator;
+  p.reservoir = g.reservoir;
+  p.piston = g.reservoir == DML_PISTON;
+  p.precision = g.precision;
+  p.overlap_mode = g.overlap_mode;
+  p.seed = g.seed;
+  p.use_cells = 0;
+  if (tessellate(p, 1)) return DML_E_UNSUPPORTED;
+  p.bcap = bucket_cap(p);
+  return DML_OK;
+}
+
+// neighbour-row capacity a config asks for (default: 0.06 atoms/A^3 in the
+// list sphere, +30 %)
+int rows_for(const dml_config &g) {
+  int mnn = g.max_neighbors;
+  if (mnn <= 0) {
+    double rl = g.rcut + g.skin;
+    double vol = 4.0 / 3.0 * M_PI * rl * rl * rl;
+    mnn = next_pow2((int)(0.06 * vol * 1.3) + 32);
+  }
+  return mnn;
 }
 
 int ncells_total(const RepParams &p) {
@@ -179,8 +236,10 @@ int alloc_cells(dml_ctx *c) {
     h.p.coff = (int32_t)C;
     h.p.ccap = ncells_total(h.p);
     h.p.boff = B;
-    C += h.p.ccap;
-    B += (int64_t)h.p.ccap * h.p.bcap;
+    h.creg = std::max<int64_t>(h.creg, h.p.ccap);
+    h.breg = std::max<int64_t>(h.breg, (int64_t)h.p.ccap * h.p.bcap);
+    C += h.creg;
+    B += h.breg;
   }
   hipFree(c->d.cell_cnt);
   hipFree(c->d.bucket);
@@ -385,7 +444,7 @@ int handle_overflow(dml_ctx *c, int e, int resume) {
         const int ovf = ERR_LIST | ERR_BUCKET;
         if (resume && v[r].err_bits && !(v[r].err_bits & ~ovf)) {
           v[r].err_bits = 0;
-          v[r].halted = 0;
+          v[r].halted &= ~HALT_FATAL;
         }
         c->reps[r].p = v[r];
       }
@@ -498,50 +557,14 @@ int dml_create_ensemble(int32_t nrep, const dml_config *cfgs, dml_ctx **out) {
     h.cfg = g;
     RepParams &p = h.p;
     memset(&p, 0, sizeof(p));
-    for (int k = 0; k < 3; k++) {
-      p.box[k] = g.box[k];
-      p.one_box[k] = 1.0 / g.box[k];
-    }
-    p.rcut = g.rcut;
-    p.rcut2 = g.rcut * g.rcut;
-    p.skin = g.skin;
-    double rl = g.rcut + g.skin;
-    p.rl2 = rl * rl;
-    p.zmax = g.zmax;
-    p.z0 = g.z0;
-    p.prob = g.prob;
-    p.h = g.h;
-    p.z_switch = g.z_switch;
-    p.dif_sc = g.dif_sc;
-    p.dif_sei = g.dif_sei;
-    for (int q = 0; q < 9; q++) {
-      p.eps[q] = g.eps[q];
-      p.r0[q] = g.r0[q];
-    }
-    for (int q = 0; q < 3; q++) p.mass[q] = g.mass[q];
-    set_ermak_consts(p);
-    p.act = g.act;
-    p.nadj = g.nadj;
-    p.integrator = g.integrator;
-    p.reservoir = g.reservoir;
-    p.piston = g.reservoir == DML_PISTON;
-    p.precision = g.precision;
-    p.overlap_mode = g.overlap_mode;
-    p.seed = g.seed;
-    p.pcap = (int32_t)round_up(g.capacity, DML_BLOCK);
-    p.poff = (int32_t)P;
-    P += p.pcap;
-    if (tessellate(p, 1)) {
+    p.pcap = (int32_t)round_up(g.capacity, DML_BLOCK);  // before setup_rep: Verlet buckets hold pcap
+    if (setup_rep(p, g)) {
       delete c;
       return DML_E_UNSUPPORTED;
     }
-    p.bcap = bucket_cap(p);
-    int mnn = g.max_neighbors;
-    if (mnn <= 0) {
-      double vol = 4.0 / 3.0 * M_PI * rl * rl * rl;
-      mnn = next_pow2((int)(0.06 * vol * 1.3) + 32);
-    }
-    maxnn = std::max(maxnn, mnn);
+    p.poff = (int32_t)P;
+    P += p.pcap;
+    maxnn = std::max(maxnn, rows_for(g));
     if (g.lanes_per_atom > 0) lpa = g.lanes_per_atom;
   }
   if (P > INT32_MAX / 2) {
@@ -718,6 +741,22 @@ int dml_set_state(dml_ctx *c, int32_t rep, int32_t n, const double *pos, const d
     HIPCHECK(hipMemcpy(c->d.ref_order + o, rord.data(), sizeof(int32_t) * rord.size(),
                        hipMemcpyHostToDevice));
   HIPCHECK(hipMemset(c->d.nn + o, 0, sizeof(int32_t) * p.pcap));
+  // a reused replica (serving) starts like a fresh one: no overlap stamps,
+  // no flags beyond n, zero forces
+  HIPCHECK(hipMemset(c->d.ov_rev + o, 0, sizeof(int32_t) * p.pcap));
+  HIPCHECK(hipMemset(c->d.ov_st + o, 0, p.pcap));
+  HIPCHECK(hipMemset(c->d.rowflag + o, 0, p.pcap));
+  if (p.pcap > n) HIPCHECK(hipMemset(c->d.flags + o + n, 0, p.pcap - n));
+  {
+    double *z8[] = {c->d.fx, c->d.fy, c->d.fz, c->d.epot, c->d.rvx, c->d.rvy, c->d.rvz};
+    for (double *q : z8) HIPCHECK(hipMemset(q + o, 0, sizeof(double) * p.pcap));
+  }
+  p.ov_round = 0;
+  p.ov_pending = 0;
+  p.rho_count = 0;
+  p.any_ghost = 0;
+  p.lohi = 0.0;
+  p.d1 = p.d2 = 0.0;
   // hs%b index table, the gcmc group in sys order (dana.F90:505-513) and
   // the creation numbers
   {
@@ -1180,17 +1219,24 @@ int dml_run(dml_ctx *c, int32_t nsteps, int32_t use_graph) {
     if (h.p.reservoir == DML_CHUNKS) return DML_E_UNSUPPORTED;  // bloques is not on the device
   refresh_modes(c);
   int need_init = 0;
-  for (auto &h : c->reps) need_init |= !h.p.listed;
+  for (auto &h : c->reps) need_init |= !h.p.listed && !h.p.halted;
   if (need_init) {
-    // dana.F90:140-142: test_update, then fuerza for Ermak runs
+    // dana.F90:140-142: test_update, then fuerza for Ermak runs, of the
+    // replicas that have no lists yet; the others sit this phase out
+    const int rb = (c->nrep + 63) / 64;
+    hipLaunchKernelGGL(k_mask_listed, dim3(rb), dim3(64), 0, c->st, c->d, 1);
     enq_test_update(c);
+    if (c->any_ermak) enq_forces(c, 0, 1);
+    hipLaunchKernelGGL(k_mask_listed, dim3(rb), dim3(64), 0, c->st, c->d, 0);
     if ((rc = finish(c, 0))) return rc;
-    if (c->any_ermak) {
-      enq_forces(c, 0, 1);
-      if ((rc = finish(c, 0))) return rc;
-    }
   }
-  if (use_graph && (!c->gexec || c->graph_rounds != c->rounds)) {
+  // what the captured step depends on: the phases present and the launch
+  // shapes (rounds <= 32, lanes <= 64)
+  const int64_t modes = (int64_t)c->any_ermak | c->any_brown << 1 | c->any_rounds << 2 |
+                        c->any_exact << 3 | c->any_piston << 4 | c->any_gcmc << 5 |
+                        (int64_t)c->rounds << 6 | (int64_t)c->lpa << 12 |
+                        (int64_t)c->lpa_ov << 19 | (int64_t)c->lb_mode << 26;
+  if (use_graph && (!c->gexec || c->graph_modes != modes)) {
     if (c->gexec) {
       hipGraphExecDestroy(c->gexec);
       hipGraphDestroy(c->graph);
@@ -1201,6 +1247,7 @@ int dml_run(dml_ctx *c, int32_t nsteps, int32_t use_graph) {
     HIPCHECK(hipStreamEndCapture(c->st, &c->graph));
     HIPCHECK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
     c->graph_rounds = c->rounds;
+    c->graph_modes = modes;
   }
   HIPCHECK(hipEventRecord(c->e0, c->st));
   for (int q = 0; q < nsteps; q++) {
@@ -1264,17 +1311,113 @@ int dml_time_kernel(dml_ctx *c, int32_t which, int32_t reps, double *avg_ms) {
   return finish(c, 0);
 }
 
-int dml_set_tuning(dml_ctx *c, int32_t lanes_per_atom, int32_t overlap_rounds) {
+int dml_set_tuning(dml_ctx *c, int32_t lanes_per_atom, int32_t overlap_rounds,
+                   int32_t overlap_lanes, int32_t list_kernel) {
   if (!c) return DML_E_ARG;
-  if (lanes_per_atom > 0) {
-    if (lanes_per_atom > 64 || (lanes_per_atom & (lanes_per_atom - 1))) return DML_E_ARG;
-    c->lpa = lanes_per_atom;
+  auto pow2_le64 = [](int v) { return v > 0 && v <= 64 && !(v & (v - 1)); };
+  if ((lanes_per_atom && !pow2_le64(lanes_per_atom)) || (overlap_lanes && !pow2_le64(overlap_lanes)) ||
+      overlap_rounds < 0 || overlap_rounds > kMaxRounds || list_kernel < 0 || list_kernel > 2)
+    return DML_E_ARG;
+  if (lanes_per_atom) c->lpa = lanes_per_atom;
+  if (overlap_rounds) c->rounds = overlap_rounds;
+  if (overlap_lanes) c->lpa_ov = overlap_lanes;
+  if (list_kernel) c->lb_mode = list_kernel - 1;
+  return DML_OK;
+}
+
+int dml_get_tuning(dml_ctx *c, int32_t *lanes_per_atom, int32_t *overlap_rounds,
+                   int32_t *overlap_lanes, int32_t *list_kernel) {
+  if (!c) return DML_E_ARG;
+  if (lanes_per_atom) *lanes_per_atom = c->lpa;
+  if (overlap_rounds) *overlap_rounds = c->rounds;
+  if (overlap_lanes) *overlap_lanes = c->lpa_ov;
+  if (list_kernel) *list_kernel = c->lb_mode + 1;
+  return DML_OK;
+}
+
+int dml_progress(dml_ctx *c, int64_t *steps, int32_t *status, int32_t *running) {
+  if (!c) return DML_E_ARG;
+  int rc = fetch_params(c);
+  if (rc) return rc;
+  std::vector<DevCounters> h(c->nrep);
+  if (steps) {
+    HIPCHECK(hipMemcpy(h.data(), c->d.cnt, sizeof(DevCounters) * c->nrep, hipMemcpyDeviceToHost));
   }
-  if (overlap_rounds > 0) {
-    if (overlap_rounds > kMaxRounds) return DML_E_ARG;
-    c->rounds = overlap_rounds;
+  for (int r = 0; r < c->nrep; r++) {
+    const RepParams &p = c->reps[r].p;
+    if (steps) steps[r] = (int64_t)p.step;
+    if (status) status[r] = status_from_err(p.err_bits);
+    if (running) running[r] = p.halted == 0;
   }
   return DML_OK;
+}
+
+int dml_set_active(dml_ctx *c, int32_t rep, int32_t active) {
+  if (!c || rep < 0 || rep >= c->nrep) return DML_E_ARG;
+  int rc = fetch_params(c);
+  if (rc) return rc;
+  RepParams &p = c->reps[rep].p;
+  if (active) p.halted &= ~HALT_PAUSED;
+  else p.halted |= HALT_PAUSED;
+  return upload_params(c);
+}
+
+int dml_set_step_limit(dml_ctx *c, int32_t rep, int64_t limit) {
+  if (!c || rep < 0 || rep >= c->nrep || limit < 0 || limit > INT32_MAX) return DML_E_ARG;
+  int rc = fetch_params(c);
+  if (rc) return rc;
+  RepParams &p = c->reps[rep].p;
+  p.step_limit = (int32_t)limit;
+  if (limit > 0 && p.step >= limit) p.halted |= HALT_PAUSED;
+  return upload_params(c);
+}
+
+int dml_reconfigure(dml_ctx *c, int32_t rep, const dml_config *cfg) {
+  if (!c || rep < 0 || rep >= c->nrep || !cfg) return DML_E_ARG;
+  const dml_config &g = *cfg;
+  if (g.capacity <= 0 || g.box[0] <= 0 || g.box[1] <= 0 || g.box[2] <= 0 || g.rcut <= 0 ||
+      g.nadj < 0 || g.nadj > (1 << 21))
+    return DML_E_ARG;
+  int rc = fetch_params(c);
+  if (rc) return rc;
+  HostRep &h = c->reps[rep];
+  if (round_up(g.capacity, DML_BLOCK) > h.p.pcap) return DML_E_NOMEM;
+  RepParams p;
+  memset(&p, 0, sizeof(p));
+  p.pcap = h.p.pcap;
+  if (setup_rep(p, g)) return DML_E_UNSUPPORTED;
+  // keep the slot range and the reserved cell regions
+  p.poff = h.p.poff;
+  p.coff = h.p.coff;
+  p.boff = h.p.boff;
+  p.ccap = ncells_total(p);
+  if (p.ccap > h.creg) return DML_E_NOMEM;
+  int64_t fit = h.breg / p.ccap;
+  if (fit < 8) return DML_E_NOMEM;
+  // the whole reserved bucket region serves the new cell count
+  p.bcap = (int32_t)std::min<int64_t>(fit, std::max<int64_t>(p.bcap, std::min<int64_t>(fit, 4096)));
+  p.nslots = 0;
+  p.nref = 0;
+  p.halted = HALT_PAUSED;  // until dml_set_state
+  h.cfg = g;
+  h.p = p;
+  if (rows_for(g) > c->d.maxnn) {
+    // wider rows than the context has: grow them (keeps every row)
+    int nm = next_pow2(rows_for(g));
+    int32_t *nbr2 = nullptr;
+    if (dalloc(&nbr2, c->P * (int64_t)nm)) return DML_E_NOMEM;
+    HIPCHECK(hipMemcpy2DAsync(nbr2, sizeof(int32_t) * nm, c->d.nbr, sizeof(int32_t) * c->d.maxnn,
+                              sizeof(int32_t) * c->d.maxnn, (size_t)c->P, hipMemcpyDeviceToDevice,
+                              c->st));
+    HIPCHECK(hipStreamSynchronize(c->st));
+    hipFree(c->d.nbr);
+    c->d.nbr = nbr2;
+    c->d.maxnn = nm;
+  }
+  hipMemsetAsync(c->d.cell_cnt + p.coff, 0, sizeof(int32_t) * p.ccap, c->st);
+  hipMemsetAsync(c->d.cnt + rep, 0, sizeof(DevCounters), c->st);
+  refresh_modes(c);
+  return upload_params(c);
 }
 
 int dml_last_run_ms(dml_ctx *c, double *ms) {

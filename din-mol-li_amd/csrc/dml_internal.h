@@ -31,6 +31,11 @@
 #define ERR_BUCKET 32
 #define ERR_UNSUP 64
 
+// RepParams::halted bits
+#define HALT_FATAL 1   // a fatal condition of the reference (err_bits says which)
+#define HALT_PAUSED 2  // not running: free serving slot, or step_limit reached
+#define HALT_MASK 4    // held out of one phase (dml_run's first rebuild)
+
 struct RepParams {
   double box[3], one_box[3];
   double rcut, rcut2, rl2, skin;
@@ -62,7 +67,8 @@ struct RepParams {
   int32_t rho_count;
   int32_t max_nn_seen;
   int32_t nref_new;
-  int32_t halted;    // a fatal condition stopped this replica
+  int32_t halted;    // HALT_* bits: any set bit keeps every kernel off this replica
+  int32_t step_limit; // > 0: pause (HALT_PAUSED) once step reaches it
   int32_t err_bits;  // which one (ERR_*)
   // group bookkeeping of the GCMC reservoir (Groups.F90 igroup semantics)
   int32_t nat, nlimbo;     // hs members alive / in limbo

@@ -36,6 +36,7 @@ __global__ void k_f2cg_end(DevState s);
 __global__ void k_calc_rho(DevState s);
 __global__ void k_maxz(DevState s);
 __global__ void k_rep_op(DevState s, int op);
+__global__ void k_mask_listed(DevState s, int on);
 __global__ void k_step_end(DevState s);
 __global__ void k_gcmc_run(DevState s, int rep_only, const double *u, const double *g,
                            double act_in, int nadj_in);

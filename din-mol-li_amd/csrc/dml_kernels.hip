@@ -36,7 +36,7 @@ __device__ inline float d2_f32(const RepParams &p, float4 a, float4 b, float bx,
 // kernels skip it from now on and dml_replica_status() reports why
 __device__ inline void rep_fail(const DevState &s, int r, int bit) {
   atomicOr(&s.rp[r].err_bits, bit);
-  s.rp[r].halted = 1;
+  atomicOr(&s.rp[r].halted, HALT_FATAL);
 }
 
 __device__ inline const RepParams &rep_of_block(const DevState &s, int blk,
@@ -1368,6 +1368,7 @@ __global__ void k_rep_op(DevState s, int op) {
     Pw->step += 1;
     s.cnt[r].steps += 1;
     s.cnt[r].particle_steps += (unsigned long long)Pw->nat;
+    if (Pw->step_limit > 0 && Pw->step >= Pw->step_limit) Pw->halted |= HALT_PAUSED;
   }
 }
 
@@ -1387,6 +1388,20 @@ __global__ void k_step_end(DevState s) {
   Pw->step += 1;
   s.cnt[r].steps += 1;
   s.cnt[r].particle_steps += (unsigned long long)Pw->nat;
+  if (Pw->step_limit > 0 && Pw->step >= Pw->step_limit) Pw->halted |= HALT_PAUSED;
+}
+
+// hold the replicas that already have lists out of the first rebuild of
+// dml_run (on = 1), or release them (on = 0)
+__global__ void k_mask_listed(DevState s, int on) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= s.nrep) return;
+  RepParams *Pw = s.rp + r;
+  if (on) {
+    if (Pw->listed) Pw->halted |= HALT_MASK;
+  } else {
+    Pw->halted &= ~HALT_MASK;
+  }
 }
 
 // ------------------------------------------------------------------------

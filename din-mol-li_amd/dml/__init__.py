@@ -34,7 +34,9 @@ EXPORTS = ["dml_create", "dml_create_ensemble", "dml_destroy", "dml_sync", "dml_
            "dml_gcmc_trial_energy", "dml_run", "dml_get_counters", "dml_list_stats",
            "dml_last_run_ms", "dml_time_kernel", "dml_set_tuning",
            "dml_replica_status", "dml_get_gcmc_order", "dml_get_slots",
-           "dml_calc_rho", "dml_maxz", "dml_end_step"]
+           "dml_calc_rho", "dml_maxz", "dml_end_step",
+           "dml_get_tuning", "dml_reconfigure", "dml_set_step_limit", "dml_set_active",
+           "dml_progress"]
 K_LISTBUILD, K_FORCE, K_CELLS, K_DISPMAX = 0, 1, 2, 3
 
 
@@ -104,7 +106,12 @@ def lib():
         L.dml_list_stats.argtypes = [vp, C.c_int32, vp, vp, vp]
         L.dml_last_run_ms.argtypes = [vp, vp]
         L.dml_time_kernel.argtypes = [vp, C.c_int32, C.c_int32, vp]
-        L.dml_set_tuning.argtypes = [vp, C.c_int32, C.c_int32]
+        L.dml_set_tuning.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32]
+        L.dml_get_tuning.argtypes = [vp, vp, vp, vp, vp]
+        L.dml_reconfigure.argtypes = [vp, C.c_int32, C.POINTER(Config)]
+        L.dml_set_step_limit.argtypes = [vp, C.c_int32, C.c_int64]
+        L.dml_set_active.argtypes = [vp, C.c_int32, C.c_int32]
+        L.dml_progress.argtypes = [vp, vp, vp, vp]
         L.dml_replica_status.argtypes = [vp, C.c_int32, vp]
         L.dml_sync.argtypes = [vp]
         L.dml_get_gcmc_order.argtypes = [vp, C.c_int32, vp, vp]
@@ -382,8 +389,35 @@ class Context:
         _chk(self.L.dml_time_kernel(self.h, which, reps, C.byref(ms)), "dml_time_kernel")
         return ms.value
 
-    def set_tuning(self, lanes_per_atom=0, overlap_rounds=0):
-        _chk(self.L.dml_set_tuning(self.h, lanes_per_atom, overlap_rounds), "dml_set_tuning")
+    def set_tuning(self, lanes_per_atom=0, overlap_rounds=0, overlap_lanes=0, list_kernel=0):
+        """launch shapes (0 keeps a value); list_kernel 1 = cell tiles, 2 = wave per atom"""
+        _chk(self.L.dml_set_tuning(self.h, lanes_per_atom, overlap_rounds, overlap_lanes,
+                                   list_kernel), "dml_set_tuning")
+
+    def tuning(self):
+        v = [C.c_int32() for _ in range(4)]
+        _chk(self.L.dml_get_tuning(self.h, *[C.byref(x) for x in v]), "dml_get_tuning")
+        return dict(lanes_per_atom=v[0].value, overlap_rounds=v[1].value,
+                    overlap_lanes=v[2].value, list_kernel=v[3].value)
+
+    def reconfigure(self, cfg, rep):
+        """a new system description for replica rep (paused until set_state)"""
+        _chk(self.L.dml_reconfigure(self.h, rep, C.byref(cfg)), "dml_reconfigure")
+        self.n[rep] = 0
+
+    def set_step_limit(self, limit, rep):
+        _chk(self.L.dml_set_step_limit(self.h, rep, int(limit)), "dml_set_step_limit")
+
+    def progress(self):
+        """(steps, status, running) arrays over the replicas"""
+        st = np.zeros(self.nrep, np.int64)
+        ss = np.zeros(self.nrep, np.int32)
+        ru = np.zeros(self.nrep, np.int32)
+        _chk(self.L.dml_progress(self.h, _p(st), _p(ss), _p(ru)), "dml_progress")
+        return st, ss, ru.astype(bool)
+
+    def set_active(self, active, rep):
+        _chk(self.L.dml_set_active(self.h, rep, int(bool(active))), "dml_set_active")
 
     def sync(self):
         _chk(self.L.dml_sync(self.h), "dml_sync")

@@ -13,7 +13,8 @@ Decisions recorded where the configuration text is open:
   box would be 250k).
 * C3: replica sizes log-uniform in [2k, 50k]; each replica is a square box
   L x L x 2L, L = (N / 0.01)^(1/3), 2 electrode layers with 20% vacancies,
-  ions at 0.01 A^-3 above; integrators alternate Brownian / Ermak.
+  ions at 0.01 A^-3 above (piston and GCMC replicas: over the whole height,
+  band z0 = 1.5 L); integrators alternate Brownian / Ermak.
 """
 import math
 
@@ -131,7 +132,15 @@ def replica_sizes(nrep=256, lo=2000, hi=50000, seed=104012 + 3):
     return np.exp(rng.uniform(math.log(lo), math.log(hi), nrep)).astype(int)
 
 
-def system_replica(n_total, seed):
+def system_replica(n_total, seed, reservoir_band=False):
+    """One replica: box L x L x 2L with L = (N / 0.01)^(1/3) (at least 4 cells
+    of 13.3 A), 2 bcc layers with 20 % vacancies, ions by RSA.  Without a
+    reservoir band the ions fill a slab above the electrode at 0.01 A^-3 and
+    z0 = zmax = 2L (z0 is then read only by the deposit-above-z0 stop).  With
+    reservoir_band (piston and GCMC runs) the same ions fill the whole height
+    and the band is the top quarter, z0 = 1.5 L: at 0.01 A^-3 and prob = 1 the
+    deposit front otherwise reaches a band inside the slab within a few
+    hundred steps and stops the run (dana.F90:905-908)."""
     rng = np.random.default_rng(seed)
     L = (n_total / RHO_ION) ** (1.0 / 3.0)
     L = max(L, 4 * 13.3)
@@ -140,12 +149,15 @@ def system_replica(n_total, seed):
     ztop = el[:, 2].max()
     n_ion = max(n_total - len(el), 10)
     hgt = n_ion / (RHO_ION * L * L)
-    ions = rsa_fill(rng, n_ion, [0, 0, ztop + DMIN], [L, L, min(ztop + DMIN + hgt, 2 * L - 1)], box,
-                    existing=el)
+    top = 2 * L - 1 if reservoir_band else min(ztop + DMIN + hgt, 2 * L - 1)
+    ions = rsa_fill(rng, n_ion, [0, 0, ztop + DMIN], [L, L, top], box, existing=el)
     pos = np.vstack([el, ions])
     z = np.array([2] * len(el) + [1] * len(ions), np.int8)
-    ion_top = ions[:, 2].max()
-    # z0: bottom of the piston band (used by piston replicas); others keep
-    # z0 = zmax, which only the deposit-above-z0 stop reads
-    return dict(box=box, pos=pos, species=z, skin=10.0, h=1e-2, z0_piston=0.75 * ion_top,
-                z0=2 * L, zmax=2 * L)
+    z0 = 1.5 * L if reservoir_band else 2 * L
+    return dict(box=box, pos=pos, species=z, skin=10.0, h=1e-2, z0=z0, zmax=2 * L)
+
+
+def system_replica_box(n_total):
+    """the box system_replica(n_total, ...) uses"""
+    L = max((n_total / RHO_ION) ** (1.0 / 3.0), 4 * 13.3)
+    return np.array([L, L, 2 * L])

@@ -215,10 +215,30 @@ int dml_last_run_ms(dml_ctx *ctx, double *ms);
  * DML_K_LISTBUILD and DML_K_CELLS first rebuild at the current positions. */
 enum { DML_K_LISTBUILD = 0, DML_K_FORCE = 1, DML_K_CELLS = 2, DML_K_DISPMAX = 3 };
 int dml_time_kernel(dml_ctx *ctx, int32_t which, int32_t reps, double *avg_ms);
-/* launch shape of the row kernels (lanes per atom, power of two <= 64) and
- * overlap rounds per step of dml_run; 0 keeps the current value.  No
- * counterpart in the reference: used by the autotuner. */
-int dml_set_tuning(dml_ctx *ctx, int32_t lanes_per_atom, int32_t overlap_rounds);
+/* launch shapes: lanes per ref atom of the force kernel and of the overlap
+ * scan (powers of two <= 64), overlap rounds per step of dml_run (<= 32),
+ * list kernel (1 = cell tiles through LDS, 2 = one wave per atom); 0 keeps
+ * the current value.  Results do not depend on them beyond FP64 summation
+ * order.  No counterpart in the reference: used by the autotuner. */
+int dml_set_tuning(dml_ctx *ctx, int32_t lanes_per_atom, int32_t overlap_rounds,
+                   int32_t overlap_lanes, int32_t list_kernel);
+int dml_get_tuning(dml_ctx *ctx, int32_t *lanes_per_atom, int32_t *overlap_rounds,
+                   int32_t *overlap_lanes, int32_t *list_kernel);
+
+/* --- serving (continuous batching over the replicas of one context) ----
+ * No counterpart in the reference.  A replica is a slot that runs one
+ * request at a time: dml_reconfigure gives it a new system description
+ * (within the capacity and cell memory it was created with; DML_E_NOMEM
+ * otherwise) and pauses it; dml_set_state loads the atoms and starts it;
+ * dml_set_step_limit pauses it after that many steps of dml_run;
+ * dml_set_active pauses / resumes it.  A paused replica costs no kernel
+ * work, and a replica's results do not depend on the others. */
+int dml_reconfigure(dml_ctx *ctx, int32_t rep, const dml_config *cfg);
+int dml_set_step_limit(dml_ctx *ctx, int32_t rep, int64_t limit);
+int dml_set_active(dml_ctx *ctx, int32_t rep, int32_t active);
+/* per replica, in one call: steps since dml_set_state, status (as
+ * dml_replica_status) and whether it is running (arrays of nrep; NULL skips) */
+int dml_progress(dml_ctx *ctx, int64_t *steps, int32_t *status, int32_t *running);
 
 #ifdef __cplusplus
 }
