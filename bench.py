@@ -116,7 +116,12 @@ def batched_leg(dml, nrep, steps, warmup):
                                     integrator=dml.ERMAK if k % 2 else dml.BROWNIAN,
                                     precision=dml.FP32_FORCE, overlap_mode=dml.OVERLAP_ROUNDS,
                                     seed=104012 + 7919 * k, **kw))
+    from dml import autotune
+    t0 = time.perf_counter()
+    choice = autotune.tune(cfgs, [(r["pos"], r["species"]) for r in reps], log=log)
+    log("autotune %.1f s%s" % (time.perf_counter() - t0, " (cached)" if choice.get("cached") else ""))
     ens = dml.Context(cfgs)
+    autotune.apply(ens, choice)
     ntot = 0
     for k, r in enumerate(reps):
         ens.set_state(r["pos"], species=r["species"], rep=k)
@@ -146,6 +151,7 @@ def batched_leg(dml, nrep, steps, warmup):
     return dict(n=ntot, ms=ms, wall=wall, rate=psteps / (ms * 1e-3), mean_nn=mean_nn,
                 rebuilds_per_rep_step=reb / max(stepsum, 1), list_ms=kl_ms, force_ms=kf_ms,
                 halted=halted, gcmc_accepted=trials,
+                tuning={k: choice[k] for k in ("lanes_per_atom", "list_kernel", "overlap_lanes")},
                 reservoirs={x: sum(1 for k in range(nrep) if RESERVOIR_CYCLE[k % 3] == x)
                             for x in RESERVOIR_CYCLE})
 
@@ -339,6 +345,7 @@ def main():
         res["rebuilds_per_step"] = batched["rebuilds_per_rep_step"]
         res["replicas_halted"] = batched["halted"]
         res["reservoir_mix"] = batched["reservoirs"]
+        res["tuning"] = batched["tuning"]
         res["gcmc_moves_accepted"] = batched["gcmc_accepted"]
     if not args.no_cpu:
         log("cpu baseline: oracle on the reference Ermak test")

@@ -1315,7 +1315,9 @@ int dml_set_tuning(dml_ctx *c, int32_t lanes_per_atom, int32_t overlap_rounds,
                    int32_t overlap_lanes, int32_t list_kernel) {
   if (!c) return DML_E_ARG;
   auto pow2_le64 = [](int v) { return v > 0 && v <= 64 && !(v & (v - 1)); };
-  if ((lanes_per_atom && !pow2_le64(lanes_per_atom)) || (overlap_lanes && !pow2_le64(overlap_lanes)) ||
+  // the overlap scan is instantiated for 1, 4, 8 and 16 lanes
+  auto ov_ok = [](int v) { return v == 1 || v == 4 || v == 8 || v == 16; };
+  if ((lanes_per_atom && !pow2_le64(lanes_per_atom)) || (overlap_lanes && !ov_ok(overlap_lanes)) ||
       overlap_rounds < 0 || overlap_rounds > kMaxRounds || list_kernel < 0 || list_kernel > 2)
     return DML_E_ARG;
   if (lanes_per_atom) c->lpa = lanes_per_atom;
