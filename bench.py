@@ -25,7 +25,10 @@ sys.path.insert(0, os.path.join(ROOT, "din-mol-li_amd"))
 
 HBM_PEAK_GBS = 8000.0
 # reservoir of replica k is RESERVOIR_CYCLE[k % 3]
-RESERVOIR_CYCLE = ("PISTON", "NO_RESERVOIR", "GCMC")
+RESERVOIR_CYCLE = ("PISTON", "CHUNKS", "GCMC")
+# serving requests cycle without chunks: a chunks run grows its box by one
+# block step per firing, beyond the cell memory of a slot class
+SERVING_CYCLE = ("PISTON", "NO_RESERVOIR", "GCMC")
 
 
 def log(*a):
@@ -33,16 +36,17 @@ def log(*a):
 
 
 def _replica(args):
-    n, seed, band = args
+    n, seed, res = args
     from dml import workloads as W
-    return W.system_replica(int(n), int(seed), reservoir_band=band)
+    if res == "CHUNKS":
+        return W.system_chunks(int(n), int(seed))
+    return W.system_replica(int(n), int(seed), reservoir_band=res != "NO_RESERVOIR")
 
 
 def build_replicas(nrep, seed0=104012):
     from dml import workloads as W
     sizes = W.replica_sizes(nrep)
-    jobs = [(int(n), seed0 + 1000 + k, RESERVOIR_CYCLE[k % 3] != "NO_RESERVOIR")
-            for k, n in enumerate(sizes)]
+    jobs = [(int(n), seed0 + 1000 + k, RESERVOIR_CYCLE[k % 3]) for k, n in enumerate(sizes)]
     workers = min(16, os.cpu_count() or 1)
     if workers > 1 and nrep > 4:
         import multiprocessing as mp
@@ -112,19 +116,25 @@ def batched_leg(dml, nrep, steps, warmup):
         if res == "GCMC":
             kw.update(act=0.01, nadj=10)
             n += max(1024, n // 4)
+        elif res == "CHUNKS":
+            n += 4 * len(r["chunk"])  # room for four blocks (the sensor holds after ~3)
         cfgs.append(dml.dana_config(r["box"], r["skin"], n, h=r["h"],
                                     integrator=dml.ERMAK if k % 2 else dml.BROWNIAN,
                                     precision=dml.FP32_FORCE, overlap_mode=dml.OVERLAP_ROUNDS,
                                     seed=104012 + 7919 * k, **kw))
     from dml import autotune
     t0 = time.perf_counter()
-    choice = autotune.tune(cfgs, [(r["pos"], r["species"]) for r in reps], log=log)
+    choice = autotune.tune(cfgs, [(r["pos"], r["species"],
+                                   (r["chunk"], r["dist"], r["z1"]) if "chunk" in r else None)
+                                  for r in reps], log=log)
     log("autotune %.1f s%s" % (time.perf_counter() - t0, " (cached)" if choice.get("cached") else ""))
     ens = dml.Context(cfgs)
     autotune.apply(ens, choice)
     ntot = 0
     for k, r in enumerate(reps):
         ens.set_state(r["pos"], species=r["species"], rep=k)
+        if "chunk" in r:
+            ens.set_chunk(r["chunk"], r["dist"], r["z1"], rep=k)
         ntot += len(r["pos"])
     ens.run(warmup, graph=True)
     ens.sync()
@@ -157,9 +167,9 @@ def batched_leg(dml, nrep, steps, warmup):
 
 
 def _request_system(args):
-    n, seed, band = args
+    n, seed, res = args
     from dml import workloads as W
-    return W.system_replica(int(n), int(seed), reservoir_band=band)
+    return W.system_replica(int(n), int(seed), reservoir_band=res != "NO_RESERVOIR")
 
 
 def serving_leg(dml, duration, rate=20.0, step_scale=0.01, seed=104012 + 4):
@@ -181,8 +191,7 @@ def serving_leg(dml, duration, rate=20.0, step_scale=0.01, seed=104012 + 4):
     sizes = np.clip(np.exp(rng.normal(math.log(10000.0), 1.0, m)), 1000, 200000).astype(int)
     steps = np.maximum(1, np.round(rng.integers(1000, 100001, m) * step_scale)).astype(int)
     t0 = time.perf_counter()
-    jobs = [(int(n), seed + 17 * k, RESERVOIR_CYCLE[k % 3] != "NO_RESERVOIR")
-            for k, n in enumerate(sizes)]
+    jobs = [(int(n), seed + 17 * k, SERVING_CYCLE[k % 3]) for k, n in enumerate(sizes)]
     workers = min(16, os.cpu_count() or 1)
     import multiprocessing as mp
     with mp.get_context("fork").Pool(workers) as pool:
@@ -190,7 +199,7 @@ def serving_leg(dml, duration, rate=20.0, step_scale=0.01, seed=104012 + 4):
     log("built %d requests in %.1f s" % (m, time.perf_counter() - t0))
     reqs = []
     for k, (sysd, st) in enumerate(zip(systems, steps)):
-        res = RESERVOIR_CYCLE[k % 3]
+        res = SERVING_CYCLE[k % 3]
         n = len(sysd["pos"])
         kw = dict(reservoir=getattr(dml, res), z0=sysd["z0"], zmax=sysd["zmax"])
         need = n

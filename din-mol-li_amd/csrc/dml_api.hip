@@ -32,6 +32,7 @@ struct HostRep {
   RepParams p;         // host mirror of the static parameters
   int64_t creg = 0;    // cell-count entries reserved for this replica
   int64_t breg = 0;    // bucket entries reserved for this replica
+  std::vector<double> chunk;  // block template (config_chunk), 6 x nchunk SoA
 };
 
 }  // namespace
@@ -51,10 +52,21 @@ struct dml_ctx {
   double last_ms = 0.0;
   int lpa = 8;
   int lpa_ov = 8;  // lanes per atom of the overlap scan (1 = thread per atom)
-  int lb_mode = 0; // 0: cell-tile list build, 1: wave per atom
+  // list kernel: 0 cell tiles, 1 wave per atom, 2 per replica by density
+  // (0 or 1), 3 wave per cell
+  int lb_mode = 3;
+  int any_lb[3] = {0, 0, 1};  // some replica uses list kernel 0 / 1 / 2 (wave per cell)
   int rounds = 8;
   int any_ermak = 0, any_brown = 0, any_rounds = 0, any_exact = 0, any_piston = 0, any_gcmc = 0;
+  int any_chunks = 0;
+  int64_t chunk_cap = 0;    // template atoms the device chunk pool holds
+  int64_t chunk_used = 0;   // ... and has handed out (append only)
+  int cells_dirty = 0;      // reserved cell regions grew: reallocate before use
+  uint64_t map_sig = 0;     // layout the flat work lists were built for
+  int64_t map_version = 0;  // bumped at every rebuild of the work lists
+  int64_t cap_cm = 0, cap_tm = 0, cap_wm = 0;
   int64_t graph_modes = -1;  // mode signature the captured step was built for
+  uint64_t graph_dsig = 0;   // device-state bytes the captured step was built with
 };
 
 namespace {
@@ -69,35 +81,7 @@ int next_pow2(int x) {
 
 // cgroup_tessellate (Cells.F90:180-265) on the host
 int tessellate(RepParams &p, int first) {
-  double brcut = p.rcut + p.skin;
-  if (!first && p.use_cells) {
-    int ok = 1, ok2 = 1;
-    for (int k = 0; k < 3; k++)
-      if (!((double)(p.ncells[k] + 1) >= p.box[k] / brcut)) ok = 0;
-    for (int k = 0; k < 3; k++)
-      if (!(brcut < p.box[k] / (double)p.ncells[k])) ok2 = 0;
-    if (ok && ok2) {
-      for (int k = 0; k < 3; k++) p.cell[k] = p.box[k] / (double)p.ncells[k];
-      return DML_OK;
-    }
-  }
-  int nc[3];
-  for (int k = 0; k < 3; k++) nc[k] = (int)(p.box[k] / brcut);
-  if (nc[0] < 4 && nc[1] < 4 && nc[2] < 4) {
-    if (!first && p.use_cells) return DML_E_UNSUPPORTED;  // stale-cell quirk
-    for (int k = 0; k < 3; k++) {
-      p.ncells[k] = nc[k];
-      p.cell[k] = p.box[k];
-    }
-    p.use_cells = 0;
-    return DML_OK;
-  }
-  for (int k = 0; k < 3; k++) {
-    p.ncells[k] = nc[k];
-    p.cell[k] = p.box[k] / (double)nc[k];
-  }
-  p.use_cells = 1;
-  return DML_OK;
+  return tessellate_rep(p, first) ? DML_E_UNSUPPORTED : DML_OK;
 }
 
 int bucket_cap(const RepParams &p);
@@ -212,7 +196,65 @@ int read_err(dml_ctx *c, int reset) {
   return e;
 }
 
+void assign_list_kernels(dml_ctx *c);
+
+// the flat work lists of the cell and list kernels: every reserved cell of
+// every replica (sort), the cells of cell-tile replicas (list build) and the
+// particle blocks of wave-per-atom replicas; rebuilt when the layout changes
+int build_maps(dml_ctx *c) {
+  uint64_t sig = 1469598103934665603ull;
+  auto mix = [&](uint64_t v) { sig = (sig ^ v) * 1099511628211ull; };
+  for (auto &h : c->reps) {
+    mix((uint64_t)h.p.coff); mix((uint64_t)h.p.creg); mix((uint64_t)h.p.lb_kind);
+    mix((uint64_t)h.p.poff); mix((uint64_t)h.p.pcap);
+  }
+  if (sig == c->map_sig && c->d.cm_rep) return DML_OK;
+  std::vector<int32_t> cr, cc, tr, tc, wb;
+  for (int r = 0; r < c->nrep; r++) {
+    const RepParams &p = c->reps[r].p;
+    for (int64_t q = 0; q < p.creg; q++) {
+      cr.push_back(r);
+      cc.push_back((int32_t)q);
+      if (p.lb_kind == 0) {
+        tr.push_back(r);
+        tc.push_back((int32_t)q);
+      }
+    }
+    if (p.lb_kind == 1)
+      for (int b = p.poff / DML_BLOCK; b < (p.poff + p.pcap) / DML_BLOCK; b++) wb.push_back(b);
+  }
+  auto put = [&](int32_t **dst, int64_t *cap, const std::vector<int32_t> &v) -> int {
+    if ((int64_t)v.size() > *cap || !*dst) {
+      hipFree(*dst);
+      *dst = nullptr;
+      *cap = std::max<int64_t>((int64_t)v.size(), 1);
+      if (dalloc(dst, *cap)) return DML_E_NOMEM;
+    }
+    if (!v.empty())
+      HIPCHECK(hipMemcpy(*dst, v.data(), sizeof(int32_t) * v.size(), hipMemcpyHostToDevice));
+    return DML_OK;
+  };
+  int64_t capc = c->cap_cm, capt = c->cap_tm;
+  int rc;
+  if ((rc = put(&c->d.cm_rep, &capc, cr))) return rc;
+  capc = c->cap_cm;
+  if ((rc = put(&c->d.cm_cell, &capc, cc))) return rc;
+  c->cap_cm = capc;
+  if ((rc = put(&c->d.tm_rep, &capt, tr))) return rc;
+  capt = c->cap_tm;
+  if ((rc = put(&c->d.tm_cell, &capt, tc))) return rc;
+  c->cap_tm = capt;
+  if ((rc = put(&c->d.wm_blk, &c->cap_wm, wb))) return rc;
+  c->d.n_cm = (int64_t)cr.size();
+  c->d.n_tm = (int64_t)tr.size();
+  c->d.n_wm = (int64_t)wb.size();
+  c->map_sig = sig;
+  c->map_version++;
+  return DML_OK;
+}
+
 int upload_params(dml_ctx *c) {
+  assign_list_kernels(c);
   std::vector<RepParams> v(c->nrep);
   for (int r = 0; r < c->nrep; r++) v[r] = c->reps[r].p;
   HIPCHECK(hipMemcpyAsync(c->d.rp, v.data(), sizeof(RepParams) * c->nrep, hipMemcpyHostToDevice,
@@ -238,6 +280,8 @@ int alloc_cells(dml_ctx *c) {
     h.p.boff = B;
     h.creg = std::max<int64_t>(h.creg, h.p.ccap);
     h.breg = std::max<int64_t>(h.breg, (int64_t)h.p.ccap * h.p.bcap);
+    h.p.creg = h.creg;
+    h.p.breg = h.breg;
     C += h.creg;
     B += h.breg;
   }
@@ -265,31 +309,43 @@ int alloc_cells(dml_ctx *c) {
   return DML_OK;
 }
 
-int grid_cells(const dml_ctx *c) {
-  int64_t mx = 1;
-  for (auto &h : c->reps) mx = std::max<int64_t>(mx, h.p.ccap);
-  return (int)std::min<int64_t>((mx + DML_BLOCK - 1) / DML_BLOCK, 4096);
-}
-
 // ---- phases -------------------------------------------------------------
-int max_cells(const dml_ctx *c) {
-  int64_t mx = 1;
-  for (auto &h : c->reps) mx = std::max<int64_t>(mx, h.p.ccap);
-  return (int)mx;
-}
+unsigned clamp_grid(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>(n, 65535)); }
 
 void enq_listbuild(dml_ctx *c) {
-  if (c->lb_mode == 0) {
-    hipLaunchKernelGGL(k_listbuild_cells, dim3(std::min(max_cells(c), 16384), c->nrep),
-                       dim3(DML_BLOCK), 0, c->st, c->d);
-  } else {
-    hipLaunchKernelGGL(k_listbuild<4>, dim3((unsigned)(c->P / 4)), dim3(4 * 64), 0, c->st, c->d);
+  if (c->d.n_tm > 0)
+    hipLaunchKernelGGL(k_listbuild_cells, dim3(clamp_grid(c->d.n_tm)), dim3(DML_BLOCK), 0, c->st,
+                       c->d);
+  if (c->d.n_wm > 0)
+    hipLaunchKernelGGL(k_listbuild<4>, dim3((unsigned)(c->d.n_wm * (DML_BLOCK / 4))),
+                       dim3(4 * 64), 0, c->st, c->d);
+  if (c->any_lb[2])
+    hipLaunchKernelGGL(k_listbuild_wc, dim3(clamp_grid((c->d.n_cm + 3) / 4)), dim3(DML_BLOCK), 0,
+                       c->st, c->d);
+}
+
+// the list kernel of each replica (host mirror; the caller uploads): cell
+// tiles pay off from a few atoms per cell, below that one wave per atom
+// wastes less on staging empty neighbour cells
+void assign_list_kernels(dml_ctx *c) {
+  c->any_lb[0] = c->any_lb[1] = c->any_lb[2] = 0;
+  for (auto &h : c->reps) {
+    RepParams &p = h.p;
+    int kind = c->lb_mode;
+    if (kind == 2) {
+      double cells = p.use_cells ? (double)p.ncells[0] * p.ncells[1] * p.ncells[2] : 1.0;
+      kind = (p.use_cells && (double)p.nslots / cells >= 4.0) ? 0 : 1;
+    } else if (kind == 3) {
+      kind = 2;
+    }
+    p.lb_kind = kind;
+    c->any_lb[kind] = 1;
   }
 }
 
 void enq_cells_lists(dml_ctx *c) {
-  dim3 gc(grid_cells(c), c->nrep);
-  dim3 gw(std::min((max_cells(c) + 3) / 4, 16384), c->nrep);
+  dim3 gc(clamp_grid((c->d.n_cm + DML_BLOCK - 1) / DML_BLOCK));
+  dim3 gw(clamp_grid((c->d.n_cm + 3) / 4));
   int nb = c->d.nblocks;
   hipLaunchKernelGGL(k_zero_cells, gc, dim3(DML_BLOCK), 0, c->st, c->d);
   hipLaunchKernelGGL(k_cell_key, dim3(nb), dim3(DML_BLOCK), 0, c->st, c->d);
@@ -376,6 +432,15 @@ void enq_f2cg(dml_ctx *c) {
   hipLaunchKernelGGL(k_f2cg_end, dim3((c->nrep + 63) / 64), dim3(64), 0, c->st, c->d);
 }
 
+// bloques, then test_update of the replicas that added a block (dana.F90:771)
+void enq_bloques(dml_ctx *c) {
+  const int rb = (c->nrep + 63) / 64;
+  hipLaunchKernelGGL(k_chunk_run, dim3(c->nrep), dim3(DML_BLOCK), 0, c->st, c->d);
+  hipLaunchKernelGGL(k_mask_fired, dim3(rb), dim3(64), 0, c->st, c->d, 1);
+  enq_test_update(c);
+  hipLaunchKernelGGL(k_mask_fired, dim3(rb), dim3(64), 0, c->st, c->d, 0);
+}
+
 void enq_step(dml_ctx *c) {
   int nb = c->d.nblocks;
   if (c->any_ermak) {
@@ -393,10 +458,11 @@ void enq_step(dml_ctx *c) {
   if (c->any_gcmc)  // gcmc_run after F -> CG (dana.F90:238-241)
     hipLaunchKernelGGL(k_gcmc_run, dim3(c->nrep), dim3(DML_BLOCK), 0, c->st, c->d, -1,
                        (const double *)nullptr, (const double *)nullptr, 0.0, 0);
-  if (c->any_piston) {
+  // calc_rho (dana.F90:244), bloques (:247), maxz (:261)
+  if (c->any_piston || c->any_chunks)
     hipLaunchKernelGGL(k_calc_rho, dim3(nb), dim3(DML_BLOCK), 0, c->st, c->d);
-    hipLaunchKernelGGL(k_maxz, dim3(nb), dim3(DML_BLOCK), 0, c->st, c->d);
-  }
+  if (c->any_chunks) enq_bloques(c);
+  if (c->any_piston) hipLaunchKernelGGL(k_maxz, dim3(nb), dim3(DML_BLOCK), 0, c->st, c->d);
   hipLaunchKernelGGL(k_step_end, dim3((c->nrep + 63) / 64), dim3(64), 0, c->st, c->d);
 }
 
@@ -494,8 +560,10 @@ double *upload_tmp(dml_ctx *c, const double *h, int64_t n) {
 
 void refresh_modes(dml_ctx *c) {
   c->any_ermak = c->any_brown = c->any_rounds = c->any_exact = c->any_piston = c->any_gcmc = 0;
+  c->any_chunks = 0;
   for (auto &h : c->reps) {
     if (h.p.reservoir == DML_GCMC) c->any_gcmc = 1;
+    if (h.p.reservoir == DML_CHUNKS && h.p.nchunk > 0) c->any_chunks = 1;
     if (h.p.integrator == DML_ERMAK) c->any_ermak = 1; else c->any_brown = 1;
     if (h.p.overlap_mode == DML_OVERLAP_ROUNDS) c->any_rounds = 1; else c->any_exact = 1;
     if (h.p.piston) c->any_piston = 1;
@@ -674,7 +742,8 @@ int dml_destroy(dml_ctx *c) {
                   d.ref_pos, d.nn, d.nbr, d.cell_cnt, d.bucket, d.bx, d.by, d.bz, d.ov_rev,
                   d.ov_st, d.part_d, d.part_i, d.blk2rep, d.rp, d.cnt, d.err, d.wl[0],
                   d.wl[1], d.wl_cnt, d.wl_stamp, d.ov_more, d.pf4, d.bf4, d.gc_order,
-                  d.b2slot, d.seq};
+                  d.b2slot, d.seq, d.chx, d.chy, d.chz, d.chox, d.choy, d.choz,
+                  d.cm_rep, d.cm_cell, d.tm_rep, d.tm_cell, d.wm_blk};
   for (void *q : ptrs)
     if (q) hipFree(q);
   if (c->e0) hipEventDestroy(c->e0);
@@ -936,8 +1005,11 @@ int dml_set_box(dml_ctx *c, int32_t rep, const double box[3]) {
   return upload_params(c);
 }
 
+static int ensure_cells(dml_ctx *c);
+
 int dml_test_update(dml_ctx *c, int32_t *rebuilt) {
   if (!c) return DML_E_ARG;
+  if (int rc0 = ensure_cells(c)) return rc0;
   enq_test_update(c);
   int rc = finish(c);
   if (rc) return rc;
@@ -950,6 +1022,7 @@ int dml_test_update(dml_ctx *c, int32_t *rebuilt) {
 
 int dml_update(dml_ctx *c) {
   if (!c) return DML_E_ARG;
+  if (int rc0 = ensure_cells(c)) return rc0;
   hipLaunchKernelGGL(k_force_rebuild, dim3((c->nrep + 63) / 64), dim3(64), 0, c->st, c->d);
   enq_cells_lists(c);
   return finish(c);
@@ -1088,6 +1161,7 @@ int dml_f_to_cg(dml_ctx *c) {
 int dml_gcmc(dml_ctx *c, int32_t rep, double act, int32_t nadj, const double *u, const double *g) {
   if (!c || rep < 0 || rep >= c->nrep || nadj < 0 || nadj > (1 << 21) || (!u) != (!g))
     return DML_E_ARG;
+  if (int rc0 = ensure_cells(c)) return rc0;
   int rc = fetch_params(c);
   if (rc) return rc;
   const RepParams &p = c->reps[rep].p;
@@ -1121,9 +1195,91 @@ int dml_calc_rho(dml_ctx *c, double *rho) {
   if (rc) return rc;
   for (int r = 0; r < c->nrep; r++) {
     const RepParams &p = c->reps[r].p;
-    rho[r] = (double)p.rho_count / (p.box[0] * p.box[1] * (p.zmax - p.z0));
+    const double top = p.reservoir == DML_CHUNKS ? p.z1 : p.zmax;
+    rho[r] = (double)p.rho_count / (p.box[0] * p.box[1] * (top - p.z0));
   }
   return DML_OK;
+}
+
+int dml_set_chunk(dml_ctx *c, int32_t rep, int32_t n, const double *pos, double dist, double z1) {
+  if (!c || rep < 0 || rep >= c->nrep || n < 0 || (n > 0 && !pos) || !(dist > 0)) return DML_E_ARG;
+  int rc = fetch_params(c);
+  if (rc) return rc;
+  HostRep &h = c->reps[rep];
+  RepParams &p = h.p;
+  if (p.reservoir != DML_CHUNKS) return DML_E_STATE;
+  // config_chunk (dana.F90:552-587): the template at zmax + z, its pos_old
+  // another zmax higher
+  h.chunk.assign(6 * (size_t)n, 0.0);
+  for (int j = 0; j < n; j++) {
+    for (int k = 0; k < 3; k++) {
+      h.chunk[(size_t)k * n + j] = pos[(int64_t)k * n + j];
+      h.chunk[(size_t)(3 + k) * n + j] = pos[(int64_t)k * n + j];
+    }
+    h.chunk[(size_t)2 * n + j] = h.chunk[(size_t)2 * n + j] + p.zmax;
+    h.chunk[(size_t)5 * n + j] = h.chunk[(size_t)2 * n + j] + p.zmax;
+  }
+  p.nchunk = n;
+  p.dist = dist;
+  p.z1 = z1;
+  // cell memory for the tallest box the slots allow: each block adds dist
+  if (n > 0) {
+    int64_t blocks = std::max<int64_t>(0, (p.pcap - p.nslots) / n);
+    RepParams t = p;
+    t.box[2] = p.zmax + (double)blocks * dist;
+    t.use_cells = 0;
+    if (tessellate_rep(t, 1) == 0) {
+      int64_t cneed = ncells_total(t);
+      int64_t bneed = cneed * std::max<int64_t>(p.bcap, bucket_cap(t));
+      if (cneed > h.creg || bneed > h.breg) {
+        // grown at the next call that sorts cells (the buckets are refilled
+        // at each replica's next sort; rows stay)
+        h.creg = std::max(h.creg, cneed);
+        h.breg = std::max(h.breg, bneed);
+        c->cells_dirty = 1;
+      }
+    }
+  }
+  // the device pool: templates appended, the pool doubled when full
+  double **arr[6] = {&c->d.chx, &c->d.chy, &c->d.chz, &c->d.chox, &c->d.choy, &c->d.choz};
+  if (c->chunk_used + n > c->chunk_cap) {
+    int64_t cap = std::max<int64_t>(2 * c->chunk_cap, c->chunk_used + n);
+    for (auto a : arr) {
+      double *q = nullptr;
+      if (dalloc(&q, cap)) return DML_E_NOMEM;
+      if (*a && c->chunk_used)
+        HIPCHECK(hipMemcpy(q, *a, sizeof(double) * c->chunk_used, hipMemcpyDeviceToDevice));
+      hipFree(*a);
+      *a = q;
+    }
+    c->chunk_cap = cap;
+  }
+  p.chunk_off = c->chunk_used;
+  c->chunk_used += n;
+  for (int k = 0; k < 6 && n > 0; k++)
+    HIPCHECK(hipMemcpy(*arr[k] + p.chunk_off, h.chunk.data() + (size_t)k * n, sizeof(double) * n,
+                       hipMemcpyHostToDevice));
+  refresh_modes(c);
+  return upload_params(c);
+}
+
+// reallocate the cell regions that dml_set_chunk enlarged
+static int ensure_cells(dml_ctx *c) {
+  if (c->cells_dirty) {
+    int rc = fetch_params(c);
+    if (rc) return rc;
+    if ((rc = alloc_cells(c))) return rc;
+    c->cells_dirty = 0;
+    if ((rc = upload_params(c))) return rc;
+  }
+  return build_maps(c);
+}
+
+int dml_bloques(dml_ctx *c) {
+  if (!c) return DML_E_ARG;
+  if (int rc0 = ensure_cells(c)) return rc0;
+  enq_bloques(c);
+  return finish(c);
 }
 
 int dml_maxz(dml_ctx *c) {
@@ -1213,10 +1369,9 @@ int dml_gcmc_trial_energy(dml_ctx *c, int32_t rep, int32_t ntrial, const double 
 
 int dml_run(dml_ctx *c, int32_t nsteps, int32_t use_graph) {
   if (!c || nsteps < 0) return DML_E_ARG;
+  if (int rc0 = ensure_cells(c)) return rc0;
   int rc = fetch_params(c);
   if (rc) return rc;
-  for (auto &h : c->reps)
-    if (h.p.reservoir == DML_CHUNKS) return DML_E_UNSUPPORTED;  // bloques is not on the device
   refresh_modes(c);
   int need_init = 0;
   for (auto &h : c->reps) need_init |= !h.p.listed && !h.p.halted;
@@ -1234,9 +1389,19 @@ int dml_run(dml_ctx *c, int32_t nsteps, int32_t use_graph) {
   // shapes (rounds <= 32, lanes <= 64)
   const int64_t modes = (int64_t)c->any_ermak | c->any_brown << 1 | c->any_rounds << 2 |
                         c->any_exact << 3 | c->any_piston << 4 | c->any_gcmc << 5 |
+                        (int64_t)c->any_chunks << 27 |
                         (int64_t)c->rounds << 6 | (int64_t)c->lpa << 12 |
-                        (int64_t)c->lpa_ov << 19 | (int64_t)c->lb_mode << 26;
-  if (use_graph && (!c->gexec || c->graph_modes != modes)) {
+                        (int64_t)c->lpa_ov << 19 | (int64_t)c->lb_mode << 26 |
+                        (int64_t)c->any_lb[0] << 28 | (int64_t)c->any_lb[1] << 29 |
+                        (int64_t)c->any_lb[2] << 30;
+  // a captured step holds the device state by value: any reallocation or
+  // resize (rows, cells, work lists, chunk pool) needs a new capture
+  uint64_t dsig = 1469598103934665603ull;
+  {
+    const unsigned char *b = (const unsigned char *)&c->d;
+    for (size_t q = 0; q < sizeof(DevState); q++) dsig = (dsig ^ b[q]) * 1099511628211ull;
+  }
+  if (use_graph && (!c->gexec || c->graph_modes != modes || c->graph_dsig != dsig)) {
     if (c->gexec) {
       hipGraphExecDestroy(c->gexec);
       hipGraphDestroy(c->graph);
@@ -1248,6 +1413,7 @@ int dml_run(dml_ctx *c, int32_t nsteps, int32_t use_graph) {
     HIPCHECK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
     c->graph_rounds = c->rounds;
     c->graph_modes = modes;
+    c->graph_dsig = dsig;
   }
   HIPCHECK(hipEventRecord(c->e0, c->st));
   for (int q = 0; q < nsteps; q++) {
@@ -1274,6 +1440,7 @@ int dml_replica_status(dml_ctx *c, int32_t rep, int32_t *status) {
 
 int dml_time_kernel(dml_ctx *c, int32_t which, int32_t reps, double *avg_ms) {
   if (!c || reps <= 0 || !avg_ms) return DML_E_ARG;
+  if (int rc0 = ensure_cells(c)) return rc0;
   if (which == DML_K_LISTBUILD || which == DML_K_CELLS) {
     // a rebuild at the current positions, then the phase alone, reps times
     hipLaunchKernelGGL(k_force_rebuild, dim3((c->nrep + 63) / 64), dim3(64), 0, c->st, c->d);
@@ -1289,10 +1456,10 @@ int dml_time_kernel(dml_ctx *c, int32_t which, int32_t reps, double *avg_ms) {
         launch_force_lpa<0>(c, c->lpa, 0);
         break;
       case DML_K_CELLS: {
-        dim3 gc(grid_cells(c), c->nrep);
+        dim3 gc(clamp_grid((c->d.n_cm + DML_BLOCK - 1) / DML_BLOCK));
         hipLaunchKernelGGL(k_zero_cells, gc, dim3(DML_BLOCK), 0, c->st, c->d);
         hipLaunchKernelGGL(k_cell_key, dim3(c->d.nblocks), dim3(DML_BLOCK), 0, c->st, c->d);
-        dim3 gw(std::min((max_cells(c) + 3) / 4, 16384), c->nrep);
+        dim3 gw(clamp_grid((c->d.n_cm + 3) / 4));
         hipLaunchKernelGGL(k_cell_sort_w, gw, dim3(DML_BLOCK), 0, c->st, c->d);
         break;
       }
@@ -1318,12 +1485,17 @@ int dml_set_tuning(dml_ctx *c, int32_t lanes_per_atom, int32_t overlap_rounds,
   // the overlap scan is instantiated for 1, 4, 8 and 16 lanes
   auto ov_ok = [](int v) { return v == 1 || v == 4 || v == 8 || v == 16; };
   if ((lanes_per_atom && !pow2_le64(lanes_per_atom)) || (overlap_lanes && !ov_ok(overlap_lanes)) ||
-      overlap_rounds < 0 || overlap_rounds > kMaxRounds || list_kernel < 0 || list_kernel > 2)
+      overlap_rounds < 0 || overlap_rounds > kMaxRounds || list_kernel < 0 || list_kernel > 4)
     return DML_E_ARG;
   if (lanes_per_atom) c->lpa = lanes_per_atom;
   if (overlap_rounds) c->rounds = overlap_rounds;
   if (overlap_lanes) c->lpa_ov = overlap_lanes;
-  if (list_kernel) c->lb_mode = list_kernel - 1;
+  if (list_kernel) {
+    c->lb_mode = list_kernel - 1;
+    int rc = fetch_params(c);
+    if (rc) return rc;
+    return upload_params(c);
+  }
   return DML_OK;
 }
 
@@ -1392,6 +1564,8 @@ int dml_reconfigure(dml_ctx *c, int32_t rep, const dml_config *cfg) {
   p.poff = h.p.poff;
   p.coff = h.p.coff;
   p.boff = h.p.boff;
+  p.creg = h.creg;
+  p.breg = h.breg;
   p.ccap = ncells_total(p);
   if (p.ccap > h.creg) return DML_E_NOMEM;
   int64_t fit = h.breg / p.ccap;

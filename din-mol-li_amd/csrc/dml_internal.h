@@ -79,7 +79,50 @@ struct RepParams {
   int32_t gcmc_calls;
   uint64_t seed;
   double d1, d2;        // top-2 squared displacements of the last skin test
+  // chunks reservoir (dana.F90:437-441, 552-587, 716-773)
+  double z1, dist;         // sensor top, block step (dist + rcut)
+  int64_t chunk_off;       // first template atom in the chunk pool
+  int32_t nchunk;          // template atoms
+  int32_t chunk_fired;     // bloques added a block in this step
+  int64_t creg, breg;      // reserved cell counts / bucket entries (grow room)
+  int32_t lb_kind;         // list kernel of this replica: 0 cell tiles, 1 wave per atom
 };
+
+// cgroup_tessellate (Cells.F90:180-265) on the replica's box: the fast path
+// keeps ncells when the box barely changed; all(ncells < 4) selects the
+// O(N^2) Verlet scan.  Returns 1 for the reference's stale-cell quirk (the
+// fast path declines while cells were in use and the new grid is all < 4).
+__host__ __device__ inline int tessellate_rep(RepParams &p, int first) {
+  double brcut = p.rcut + p.skin;
+  if (!first && p.use_cells) {
+    int ok = 1, ok2 = 1;
+    for (int k = 0; k < 3; k++)
+      if (!((double)(p.ncells[k] + 1) >= p.box[k] / brcut)) ok = 0;
+    for (int k = 0; k < 3; k++)
+      if (!(brcut < p.box[k] / (double)p.ncells[k])) ok2 = 0;
+    if (ok && ok2) {
+      for (int k = 0; k < 3; k++) p.cell[k] = p.box[k] / (double)p.ncells[k];
+      return 0;
+    }
+  }
+  int nc[3];
+  for (int k = 0; k < 3; k++) nc[k] = (int)(p.box[k] / brcut);
+  if (nc[0] < 4 && nc[1] < 4 && nc[2] < 4) {
+    if (!first && p.use_cells) return 1;
+    for (int k = 0; k < 3; k++) {
+      p.ncells[k] = nc[k];
+      p.cell[k] = p.box[k];
+    }
+    p.use_cells = 0;
+    return 0;
+  }
+  for (int k = 0; k < 3; k++) {
+    p.ncells[k] = nc[k];
+    p.cell[k] = p.box[k] / (double)nc[k];
+  }
+  p.use_cells = 1;
+  return 0;
+}
 
 struct DevCounters {
   unsigned long long try_, depo, choques, choques3, nupd, ins, del, passes;
@@ -218,6 +261,13 @@ struct DevState {
   int32_t *gc_order;      // gcmc group in list order (global slots)
   int32_t *b2slot;        // hs%b index -> global slot, -1 when free
   int32_t *seq;           // creation number: the order of sys%alist
+  double *chx, *chy, *chz;   // chunk templates (pool over the replicas)
+  double *chox, *choy, *choz;
+  // flat work lists (rebuilt by the host when the layout changes)
+  int32_t *cm_rep, *cm_cell; // every reserved cell of every replica
+  int32_t *tm_rep, *tm_cell; // cells of the replicas built by cell tiles
+  int32_t *wm_blk;           // particle blocks of the replicas built wave per atom
+  int64_t n_cm, n_tm, n_wm;
   // FP32 mirror of the positions, w = species bits: conservative prefilters
   float4 *pf4;            // per slot
   float4 *bf4;            // bucket-ordered

@@ -37,6 +37,9 @@ __global__ void k_calc_rho(DevState s);
 __global__ void k_maxz(DevState s);
 __global__ void k_rep_op(DevState s, int op);
 __global__ void k_mask_listed(DevState s, int on);
+__global__ void k_chunk_run(DevState s);
+__global__ void k_mask_fired(DevState s, int on);
+__global__ void k_listbuild_wc(DevState s);
 __global__ void k_step_end(DevState s);
 __global__ void k_gcmc_run(DevState s, int rep_only, const double *u, const double *g,
                            double act_in, int nadj_in);

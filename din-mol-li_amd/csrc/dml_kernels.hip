@@ -164,13 +164,13 @@ __device__ inline int want_cells(const RepParams &P) {
 }
 
 __global__ void k_zero_cells(DevState s) {
-  int r = blockIdx.y;
-  const RepParams &P = s.rp[r];
-  if (P.halted) return;
-  if (!want_cells(P)) return;
-  int nc = ncell_total(P);
-  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x)
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < s.n_cm;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    const RepParams &P = s.rp[s.cm_rep[g]];
+    const int c = s.cm_cell[g];
+    if (P.halted || !want_cells(P) || c >= ncell_total(P)) continue;
     s.cell_cnt[P.coff + c] = 0;
+  }
 }
 
 __global__ void __launch_bounds__(DML_BLOCK) k_cell_key(DevState s) {
@@ -260,11 +260,13 @@ __global__ void k_cell_sort(DevState s) {
 template <int WPB>
 __global__ void __launch_bounds__(WPB *WAVE) k_listbuild(DevState s) {
   int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-  int t = blockIdx.x * WPB + w;
+  // launch block b covers WPB atoms of particle block wm_blk[b / (256 / WPB)]
+  const int per = DML_BLOCK / WPB;
+  int t = s.wm_blk[blockIdx.x / per] * DML_BLOCK + (blockIdx.x % per) * WPB + w;
   int r = s.blk2rep[t / DML_BLOCK];
   const RepParams &P = s.rp[r];
   if (P.halted) return;
-  if (!P.rebuild) return;
+  if (!P.rebuild || P.lb_kind != 1) return;
   int local = t - P.poff;
   if (local >= P.nref) return;
   int i = s.ref_order[t];
@@ -376,21 +378,21 @@ __global__ void __launch_bounds__(DML_BLOCK) k_listbuild_cells(DevState s) {
   __shared__ int32_t st_pre[28];
   __shared__ int32_t sh_A, sh_nrefc;
   __shared__ unsigned long long sh_nnsum, sh_rows;
-  const int r = blockIdx.y;
-  const RepParams &P = s.rp[r];
-  if (P.halted) return;
-  if (!P.rebuild) return;
   const int tid = threadIdx.x, w = tid / WAVE, lane = tid % WAVE;
   const int nwav = DML_BLOCK / WAVE;
-  const int nc = ncell_total(P);
-  const double rl2 = P.rl2;
-  const int strict = P.use_cells;
   const int maxnn = s.maxnn;
-  const float rl2lo = (float)rl2 - PF_MARGIN, rl2hi = (float)rl2 + PF_MARGIN;
-  const float bxf = (float)P.box[0], byf = (float)P.box[1];
-  const float ibx = (float)P.one_box[0], iby = (float)P.one_box[1];
   if (tid == 0) { sh_nnsum = 0; sh_rows = 0; }
-  for (int c = blockIdx.x; c < nc; c += gridDim.x) {
+  // one cell of the flat list of cell-tile replicas per iteration
+  for (int64_t g = blockIdx.x; g < s.n_tm; g += gridDim.x) {
+    const int r = s.tm_rep[g];
+    const int c = s.tm_cell[g];
+    const RepParams &P = s.rp[r];
+    if (P.halted || !P.rebuild || P.lb_kind != 0 || c >= ncell_total(P)) continue;
+    const double rl2 = P.rl2;
+    const int strict = P.use_cells;
+    const float rl2lo = (float)rl2 - PF_MARGIN, rl2hi = (float)rl2 + PF_MARGIN;
+    const float bxf = (float)P.box[0], byf = (float)P.box[1];
+    const float ibx = (float)P.one_box[0], iby = (float)P.one_box[1];
     if (tid == 0) {
       int A = s.cell_cnt[P.coff + c];
       sh_A = A > P.bcap ? P.bcap : A;
@@ -515,25 +517,195 @@ __global__ void __launch_bounds__(DML_BLOCK) k_listbuild_cells(DevState s) {
       __syncthreads();
     }
     __syncthreads();  // sh_A and the LDS tiles are rewritten by the next cell
+    if (tid == 0 && sh_rows) {
+      atomicAdd(&s.cnt[r].nn_sum, sh_nnsum);
+      atomicAdd(&s.cnt[r].nn_rows, sh_rows);
+      sh_nnsum = 0;
+      sh_rows = 0;
+    }
   }
-  __syncthreads();
-  if (tid == 0 && sh_rows) {
-    atomicAdd(&s.cnt[r].nn_sum, sh_nnsum);
-    atomicAdd(&s.cnt[r].nn_rows, sh_rows);
+}
+
+// ------------------------------------------------------------------------
+// neighbour rows, one wave per cell: the wave holds up to 64 centre atoms of
+// its cell in lanes (position, row length) and streams the candidates of the
+// 27 stencil cells (stencil order, chain order inside each cell), 64 at a
+// time, one per lane; every ref centre atom in turn tests the whole chunk
+// (FP32 prefilter, FP64 inside the band) and appends its hits by ballot
+// compaction, so rows keep the order of ngroup_cells (Neighbor.F90:509-535).
+// No LDS and no block barriers: candidate loads are shared by all atoms of
+// the cell through registers, and dense and sparse cells cost in proportion
+// to their work.
+// ------------------------------------------------------------------------
+__device__ inline float rdlane_f(float v, int a) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), a));
+}
+__device__ inline double rdlane_d(double v, int a) {
+  int64_t b = __double_as_longlong(v);
+  int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), a);
+  int hi = __builtin_amdgcn_readlane((int)(b >> 32), a);
+  return __longlong_as_double(((int64_t)(uint32_t)lo) | ((int64_t)hi << 32));
+}
+
+__global__ void __launch_bounds__(DML_BLOCK) k_listbuild_wc(DevState s) {
+  const int lane = threadIdx.x % WAVE;
+  const int64_t nw = (int64_t)gridDim.x * (DML_BLOCK / WAVE);
+  const int maxnn = s.maxnn;
+  int cur_r = -1;
+  unsigned long long acc_nn = 0, acc_rows = 0;
+  for (int64_t g = (int64_t)blockIdx.x * (DML_BLOCK / WAVE) + threadIdx.x / WAVE; g < s.n_cm;
+       g += nw) {
+    const int r = s.cm_rep[g];
+    const int c = s.cm_cell[g];
+    const RepParams &P = s.rp[r];
+    if (P.halted || !P.rebuild || P.lb_kind != 2 || c >= ncell_total(P)) continue;
+    int ccnt = s.cell_cnt[P.coff + c];
+    if (ccnt > P.bcap) ccnt = P.bcap;
+    if (ccnt == 0) continue;
+    if (r != cur_r) {
+      if (cur_r >= 0 && lane == 0 && acc_rows) {
+        atomicAdd(&s.cnt[cur_r].nn_sum, acc_nn);
+        atomicAdd(&s.cnt[cur_r].nn_rows, acc_rows);
+      }
+      cur_r = r;
+      acc_nn = acc_rows = 0;
+    }
+    const int64_t cbase = P.boff + (int64_t)c * P.bcap;
+    int ghost = 0, ci0 = 0, ci1 = 0, ci2 = 0;
+    if (P.use_cells) {
+      ci0 = c % (P.ncells[0] + 2);
+      ci1 = (c / (P.ncells[0] + 2)) % (P.ncells[1] + 2);
+      ci2 = c / ((P.ncells[0] + 2) * (P.ncells[1] + 2));
+      ghost = (ci0 < 1 || ci1 < 1 || ci2 < 1 || ci0 > P.ncells[0] || ci1 > P.ncells[1] ||
+               ci2 > P.ncells[2]);
+    }
+    // the 27 stencil cells: lane k holds count and first entry of cell k
+    int kcnt = 0;
+    int64_t kbase = 0;
+    if (P.use_cells) {
+      if (lane < 27) {
+        int n0 = cell_wrap(ci0 + kMap[lane][0], P.ncells[0]);
+        int n1 = cell_wrap(ci1 + kMap[lane][1], P.ncells[1]);
+        int n2 = cell_wrap(ci2 + kMap[lane][2], P.ncells[2]);
+        int cell = cell_lin(P, n0, n1, n2);
+        kcnt = s.cell_cnt[P.coff + cell];
+        if (kcnt > P.bcap) kcnt = P.bcap;
+        kbase = P.boff + (int64_t)cell * P.bcap;
+      }
+    } else if (lane == 0) {
+      kcnt = ccnt;
+      kbase = P.boff;
+    }
+    int incl = kcnt;
+    for (int off = 1; off < 32; off <<= 1) {
+      int v = __shfl_up(incl, off);
+      if (lane >= off) incl += v;
+    }
+    const int kpre = incl - kcnt;
+    const int total = __shfl(incl, 26);
+    const double rl2 = P.rl2;
+    const int strict = P.use_cells;
+    const float rl2lo = (float)rl2 - PF_MARGIN, rl2hi = (float)rl2 + PF_MARGIN;
+    const float bxf = (float)P.box[0], byf = (float)P.box[1];
+    const float ibx = (float)P.one_box[0], iby = (float)P.one_box[1];
+    for (int a0 = 0; a0 < ccnt; a0 += WAVE) {
+      // centre atoms a0 .. a0+63 of this cell, one per lane
+      const int na = (ccnt - a0) < WAVE ? (ccnt - a0) : WAVE;
+      int my_i = -1, my_ref = 0;
+      double my_x = 0.0, my_y = 0.0, my_z = 0.0;
+      float4 my_f = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (lane < na) {
+        const int64_t e = cbase + a0 + lane;
+        my_i = s.bucket[e];
+        my_ref = (s.flags[my_i] & DML_FL_REF) != 0;
+        my_x = s.bx[e];
+        my_y = s.by[e];
+        my_z = s.bz[e];
+        my_f = make_float4((float)my_x, (float)my_y, (float)my_z, 0.f);
+      }
+      const unsigned long long refm = __ballot(my_ref);
+      if (!refm) continue;
+      int my_nn = 0;
+      for (int q0 = 0; q0 < total; q0 += WAVE) {
+        const int cidx = q0 + lane;
+        const int cc = cidx < total ? cidx : total - 1;
+        int lo = 0, hi = 26;  // the stencil cell holding candidate cc
+        for (int it = 0; it < 5; it++) {
+          int mid = (lo + hi + 1) >> 1;
+          int pm = __shfl(kpre, mid);
+          if (lo < hi) {
+            if (pm <= cc) lo = mid; else hi = mid - 1;
+          }
+        }
+        const int64_t pos = __shfl(kbase, lo) + (cc - __shfl(kpre, lo));
+        const int cj = s.bucket[pos];
+        const float4 cf = s.bf4[pos];
+        const int valid = cidx < total;
+        unsigned long long m = refm;
+        while (m) {
+          const int a = __builtin_ctzll(m);
+          m &= m - 1;
+          const int i = __builtin_amdgcn_readlane(my_i, a);
+          const float4 fi = make_float4(rdlane_f(my_f.x, a), rdlane_f(my_f.y, a),
+                                        rdlane_f(my_f.z, a), 0.f);
+          int hit = 0;
+          if (valid) {
+            const float df = d2_f32(P, cf, fi, bxf, byf, ibx, iby);
+            if (df < rl2lo) {
+              hit = 1;
+            } else if (df <= rl2hi) {
+              double dx, dy, dz;  // near the list radius: decide in FP64
+              vdist_mic(P, s.px[cj], s.py[cj], s.pz[cj], rdlane_d(my_x, a), rdlane_d(my_y, a),
+                        rdlane_d(my_z, a), &dx, &dy, &dz);
+              const double rd = dot3(dx, dy, dz);
+              hit = strict ? (rd < rl2) : !(rd > rl2);
+            }
+            hit = hit && (cj != i);
+          }
+          const unsigned long long hm = __ballot(hit);
+          if (hm) {
+            const int nn0 = __builtin_amdgcn_readlane(my_nn, a);
+            const int pr = __popcll(hm & ((1ull << lane) - 1ull));
+            if (hit && nn0 + pr < maxnn) s.nbr[(int64_t)i * maxnn + nn0 + pr] = cj;
+            if (lane == a) my_nn = nn0 + __popcll(hm);
+          }
+        }
+      }
+      if (my_ref) {
+        int nn = my_nn;
+        if (nn > maxnn) {
+          atomicOr(s.err, ERR_LIST);
+          atomicMax(&s.rp[r].max_nn_seen, nn);
+          rep_fail(s, r, ERR_LIST);
+          nn = maxnn;
+        }
+        s.nn[my_i] = nn;
+        s.rowflag[my_i] = (uint8_t)ghost;
+        if (ghost) s.rp[r].any_ghost = 1;
+      }
+      // row statistics of this group
+      unsigned long long v = my_ref ? (unsigned long long)my_nn : 0ull;
+      for (int off = WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
+      acc_nn += v;
+      acc_rows += (unsigned long long)__popcll(refm);
+    }
+  }
+  if (cur_r >= 0 && lane == 0 && acc_rows) {
+    atomicAdd(&s.cnt[cur_r].nn_sum, acc_nn);
+    atomicAdd(&s.cnt[cur_r].nn_rows, acc_rows);
   }
 }
 
 // bucket order by a rank sort inside one wave (cells of up to 64 atoms;
 // larger cells fall back to an insertion sort by lane 0)
 __global__ void __launch_bounds__(DML_BLOCK) k_cell_sort_w(DevState s) {
-  const int r = blockIdx.y;
-  const RepParams &P = s.rp[r];
-  if (P.halted) return;
-  if (!want_cells(P)) return;
   const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-  const int nc = ncell_total(P);
-  const int desc = P.use_cells;
-  for (int c = blockIdx.x * (DML_BLOCK / WAVE) + w; c < nc; c += gridDim.x * (DML_BLOCK / WAVE)) {
+  for (int64_t g = (int64_t)blockIdx.x * (DML_BLOCK / WAVE) + w; g < s.n_cm;
+       g += (int64_t)gridDim.x * (DML_BLOCK / WAVE)) {
+    const RepParams &P = s.rp[s.cm_rep[g]];
+    const int c = s.cm_cell[g];
+    if (P.halted || !want_cells(P) || c >= ncell_total(P)) continue;
+    const int desc = P.use_cells;
     int cnt = s.cell_cnt[P.coff + c];
     if (cnt > P.bcap) cnt = P.bcap;
     if (cnt == 0) continue;
@@ -1317,7 +1489,8 @@ __global__ void __launch_bounds__(DML_BLOCK) k_calc_rho(DevState s) {
   int local = slot - P.poff;
   if (local < P.nslots && (s.flags[slot] & DML_FL_ALIVE)) {
     double z = s.pz[slot];
-    if (z > P.z0 && z < P.zmax) atomicAdd(&cnt, 1);
+    const double zt = P.reservoir == DML_CHUNKS ? P.z1 : P.zmax;  // the chunks sensor
+    if (z > P.z0 && z < zt) atomicAdd(&cnt, 1);
   }
   __syncthreads();
   if (threadIdx.x == 0 && cnt) atomicAdd(&s.rp[r].rho_count, cnt);
@@ -1849,5 +2022,139 @@ __global__ void __launch_bounds__(DML_BLOCK) k_gcmc_run(DevState s, int rep_only
     if (n_ins) atomicAdd(&s.cnt[r].ins, n_ins);
     if (n_del) atomicAdd(&s.cnt[r].del, n_del);
     if (fail) rep_fail(s, r, fail);
+  }
+}
+
+// ------------------------------------------------------------------------
+// chunks reservoir: bloques (dana.F90:716-773).  When the sensor density of
+// calc_rho leaves its band, the reservoir moves up by dist, a copy of the
+// block template joins sys, hs%b, hs%ref and hs (slots and hs%b indices
+// reused as igroup_attach_atom does, Groups.F90:1058-1100), the template
+// moves up, the box grows to zmax and the lists are rebuilt (the caller runs
+// test_update on the replicas whose chunk_fired is set).
+// ------------------------------------------------------------------------
+__global__ void __launch_bounds__(DML_BLOCK) k_chunk_run(DevState s) {
+  __shared__ int sh[DML_BLOCK / WAVE];
+  const int r = blockIdx.x;
+  if (r >= s.nrep) return;
+  RepParams *Pw = s.rp + r;
+  if (Pw->halted || Pw->reservoir != DML_CHUNKS || Pw->nchunk <= 0) return;
+  const RepParams P = *Pw;
+  const int tid = threadIdx.x;
+  const double rhomedia = 5.775329e-4;
+  const double rho = (double)P.rho_count / (P.box[0] * P.box[1] * (P.z1 - P.z0));
+  if (fabs(rho - rhomedia) < rhomedia * (double)0.186f) return;
+  const int poff = P.poff, nch = P.nchunk;
+  int32_t *hs_of = s.ref_tmp + poff;  // chosen hs slot of template atom j
+  int32_t *b_of = s.wl[0] + poff;     // chosen hs%b index of template atom j
+  // hs slots: atom j reuses the j-th null slot while nslots >= nat + j + 1 +
+  // nlimbo, i.e. for j < nslots - nat - nlimbo; the rest are appended
+  const int take = max(0, min(nch, P.nslots - P.nat - P.nlimbo));
+  int got = 0;
+  for (int q0 = 0; q0 < P.nslots && got < take; q0 += DML_BLOCK) {
+    int q = q0 + tid;
+    int fr = q < P.nslots && !(s.flags[poff + q] & (DML_FL_ALIVE | DML_FL_LIMBO));
+    int tot;
+    int pos = blk_scan01(fr, sh, &tot);
+    if (fr && got + pos < take) hs_of[got + pos] = q;
+    got += tot;
+  }
+  int fail = 0;
+  if (got < take) fail = ERR_UNSUP;
+  if (P.nslots + (nch - take) > P.pcap) fail = ERR_BUCKET;
+  // hs%b indices: the first free ones while b_amax >= b_nat + j + 1
+  const int takeb = max(0, min(nch, P.b_amax - P.b_nat));
+  got = 0;
+  for (int q0 = 0; q0 < P.b_amax && got < takeb; q0 += DML_BLOCK) {
+    int q = q0 + tid;
+    int fr = q < P.b_amax && s.b2slot[poff + q] < 0;
+    int tot;
+    int pos = blk_scan01(fr, sh, &tot);
+    if (fr && got + pos < takeb) b_of[got + pos] = q;
+    got += tot;
+  }
+  if (got < takeb) fail = ERR_UNSUP;
+  if (P.b_amax + (nch - takeb) > P.pcap) fail = ERR_BUCKET;
+  // the grown box and its tessellation
+  RepParams Q = P;
+  Q.z0 = P.z0 + P.dist;
+  Q.z1 = P.z1 + P.dist;
+  Q.zmax = P.zmax + P.dist;
+  Q.box[2] = Q.zmax;
+  Q.one_box[2] = 1.0 / Q.box[2];
+  if (tessellate_rep(Q, 0)) fail = ERR_UNSUP;
+  Q.ccap = Q.use_cells ? (Q.ncells[0] + 2) * (Q.ncells[1] + 2) * (Q.ncells[2] + 2) : 1;
+  if (Q.ccap > P.creg) fail = ERR_BUCKET;
+  if (!Q.use_cells && P.use_cells) Q.bcap = (int32_t)min<int64_t>(P.breg, (int64_t)P.pcap);
+  if ((int64_t)Q.ccap * Q.bcap > P.breg) Q.bcap = (int32_t)(P.breg / Q.ccap);
+  if (Q.bcap < 8) fail = ERR_BUCKET;
+  if (fail) {
+    if (tid == 0) rep_fail(s, r, fail);
+    return;
+  }
+  for (int j = tid; j < nch; j += DML_BLOCK) {
+    const int hs = j < take ? hs_of[j] : P.nslots + (j - take);
+    const int bs = j < takeb ? b_of[j] : P.b_amax + (j - takeb);
+    const int i = poff + hs;
+    const int64_t c = P.chunk_off + j;
+    // atom_allocate defaults (Groups.F90:370-387), then the template's place
+    s.species[i] = SP_LI;
+    s.flags[i] = DML_FL_ALIVE | DML_FL_REF;
+    s.rowflag[i] = 0;
+    s.px[i] = s.chx[c]; s.py[i] = s.chy[c]; s.pz[i] = s.chz[c];
+    s.ox[i] = s.chox[c]; s.oy[i] = s.choy[c]; s.oz[i] = s.choz[c];
+    s.cx[i] = 1e8; s.cy[i] = 1e8; s.cz[i] = 1e8;
+    s.vx[i] = 0.0; s.vy[i] = 0.0; s.vz[i] = 0.0;
+    s.ax[i] = 0.0; s.ay[i] = 0.0; s.az[i] = 0.0;
+    s.fx[i] = 0.0; s.fy[i] = 0.0; s.fz[i] = 0.0;
+    s.epot[i] = 0.0;
+    s.nn[i] = 0;
+    s.bkey[i] = bs;
+    s.b2slot[poff + bs] = i;
+    s.pid[i] = P.pid_next + (uint32_t)j;
+    s.seq[i] = P.seq_next + j;
+    s.ref_order[poff + P.nref + j] = i;
+    s.ref_pos[i] = P.nref + j;
+    put_pf4(s, i, s.chx[c], s.chy[c], s.chz[c], SP_LI);
+    // and the template moves up with the reservoir
+    s.chz[c] = s.chz[c] + P.dist;
+    s.choz[c] = s.choz[c] + P.dist;
+  }
+  if (tid == 0) {
+    Pw->z0 = Q.z0;
+    Pw->z1 = Q.z1;
+    Pw->zmax = Q.zmax;
+    Pw->box[2] = Q.box[2];
+    Pw->one_box[2] = Q.one_box[2];
+    for (int k = 0; k < 3; k++) {
+      Pw->ncells[k] = Q.ncells[k];
+      Pw->cell[k] = Q.cell[k];
+    }
+    Pw->use_cells = Q.use_cells;
+    Pw->ccap = Q.ccap;
+    Pw->bcap = Q.bcap;
+    Pw->nslots = P.nslots + (nch - take);
+    Pw->b_amax = P.b_amax + (nch - takeb);
+    Pw->b_nat = P.b_nat + nch;
+    Pw->nat = P.nat + nch;
+    Pw->nref = P.nref + nch;
+    Pw->pid_next = P.pid_next + (uint32_t)nch;
+    Pw->seq_next = P.seq_next + nch;
+    Pw->listed = 0;  // hs%listed = .false.: the next test_update rebuilds
+    Pw->chunk_fired = 1;
+  }
+}
+
+// hold every replica that added no block out of the test_update that
+// follows bloques (on = 1); release them and clear the marks (on = 0)
+__global__ void k_mask_fired(DevState s, int on) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= s.nrep) return;
+  RepParams *Pw = s.rp + r;
+  if (on) {
+    if (!Pw->chunk_fired) Pw->halted |= HALT_MASK;
+  } else {
+    Pw->halted &= ~HALT_MASK;
+    Pw->chunk_fired = 0;
   }
 }

@@ -36,7 +36,7 @@ EXPORTS = ["dml_create", "dml_create_ensemble", "dml_destroy", "dml_sync", "dml_
            "dml_replica_status", "dml_get_gcmc_order", "dml_get_slots",
            "dml_calc_rho", "dml_maxz", "dml_end_step",
            "dml_get_tuning", "dml_reconfigure", "dml_set_step_limit", "dml_set_active",
-           "dml_progress"]
+           "dml_progress", "dml_set_chunk", "dml_bloques"]
 K_LISTBUILD, K_FORCE, K_CELLS, K_DISPMAX = 0, 1, 2, 3
 
 
@@ -112,6 +112,8 @@ def lib():
         L.dml_set_step_limit.argtypes = [vp, C.c_int32, C.c_int64]
         L.dml_set_active.argtypes = [vp, C.c_int32, C.c_int32]
         L.dml_progress.argtypes = [vp, vp, vp, vp]
+        L.dml_set_chunk.argtypes = [vp, C.c_int32, C.c_int32, vp, C.c_double, C.c_double]
+        L.dml_bloques.argtypes = [vp]
         L.dml_replica_status.argtypes = [vp, C.c_int32, vp]
         L.dml_sync.argtypes = [vp]
         L.dml_get_gcmc_order.argtypes = [vp, C.c_int32, vp, vp]
@@ -350,6 +352,17 @@ class Context:
     def maxz(self):
         """maxz (dana.F90:776-794) of the piston replicas"""
         _chk(self.L.dml_maxz(self.h), "dml_maxz")
+
+    def set_chunk(self, pos, dist, z1, rep=0):
+        """config_chunk: the block template (chunk.xyz positions, (n, 3)),
+        the reservoir step dist (+ rcut) and the sensor top z1"""
+        p = np.ascontiguousarray(np.asarray(pos, np.float64).T.reshape(-1))
+        _chk(self.L.dml_set_chunk(self.h, rep, len(pos), _p(p), float(dist), float(z1)),
+             "dml_set_chunk")
+
+    def bloques(self):
+        """bloques (dana.F90:716-773) of every chunks replica"""
+        _chk(self.L.dml_bloques(self.h), "dml_bloques")
 
     def end_step(self):
         _chk(self.L.dml_end_step(self.h), "dml_end_step")

@@ -4,7 +4,8 @@ Picks, for one ensemble (a list of replica configs and their initial
 states), the launch shapes that make a step fastest:
 
 * lanes per ref atom of the force kernel (2..32),
-* the list kernel: cell tiles staged through LDS (1) or one wave per atom (2),
+* the list kernel: cell tiles staged through LDS (1), one wave per atom (2),
+  each replica by its atoms per cell (3) or one wave per cell (4),
 * lanes per ref atom of the overlap round scan (4, 8, 16).
 
 Everything runs on a scratch context holding the same replicas, so the
@@ -24,7 +25,7 @@ import numpy as np
 from . import K_FORCE, K_LISTBUILD, Context
 
 FORCE_LANES = (2, 4, 8, 16, 32)
-LIST_KERNELS = (1, 2)
+LIST_KERNELS = (1, 2, 3, 4)
 OVERLAP_LANES = (4, 8, 16)
 
 
@@ -62,17 +63,21 @@ def _save(path, cache):
 
 
 def tune(configs, states, steps=4, reps=10, cache_path=None, use_cache=True, log=None):
-    """configs: list of dml Config; states: list of (pos, species) per replica.
-    Returns dict(lanes_per_atom, list_kernel, overlap_lanes, timings)."""
+    """configs: list of dml Config; states: list of (pos, species) or
+    (pos, species, chunk) per replica, chunk = (template, dist, z1) of a
+    chunks replica.  Returns dict(lanes_per_atom, list_kernel, overlap_lanes,
+    timings)."""
     path = _cache_path(cache_path)
-    sig = signature(configs, [len(p) for p, _ in states])
+    sig = signature(configs, [len(st[0]) for st in states])
     cache = _load(path) if use_cache else {}
     if sig in cache:
         return dict(cache[sig], cached=True)
     ctx = Context(list(configs))
     try:
-        for k, (pos, sp) in enumerate(states):
-            ctx.set_state(pos, species=sp, rep=k)
+        for k, st in enumerate(states):
+            ctx.set_state(st[0], species=st[1], rep=k)
+            if len(st) > 2 and st[2] is not None:
+                ctx.set_chunk(*st[2], rep=k)
         ctx.run(2, graph=False)  # lists, forces, a first overlap pass
         t_force = {}
         for lanes in FORCE_LANES:

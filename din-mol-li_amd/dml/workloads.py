@@ -161,3 +161,27 @@ def system_replica_box(n_total):
     """the box system_replica(n_total, ...) uses"""
     L = max((n_total / RHO_ION) ** (1.0 / 3.0), 4 * 13.3)
     return np.array([L, L, 2 * L])
+
+
+RHO_CHUNK = 6.0e-4   # 1 M, the concentration the reference's chunks sensor expects
+
+
+def system_chunks(n_total, seed, rho=RHO_CHUNK):
+    """A chunks-reservoir replica in the regime of the reference's brown run
+    (tests/brown: ions at 1 M, the sensor band of bloques around rhomedia):
+    ions by RSA in an L x L x L column (no electrode: deposition at z <= 0),
+    z0 = L, the step dist = L/2 + rcut, z1 = z0 + dist, zmax = box(3) =
+    z1 + dist, and a block template of the same density, L x L x L/2."""
+    rng = np.random.default_rng(seed)
+    L = max((n_total / rho) ** (1.0 / 3.0), 4 * 13.3)
+    dist = 0.5 * L + 3.2
+    z0 = L
+    z1 = z0 + dist
+    zmax = z1 + dist
+    box = np.array([L, L, zmax])
+    ions = rsa_fill(rng, int(n_total), [0, 0, 0.0], [L, L, L], box)
+    nch = max(1, int(round(rho * L * L * (0.5 * L))))
+    chunk = rsa_fill(rng, nch, [0, 0, 0.0], [L, L, 0.5 * L], box)
+    z = np.ones(len(ions), np.int8)
+    return dict(box=box, pos=ions, species=z, skin=10.0, h=1e-2, z0=z0, zmax=zmax, z1=z1,
+                dist=dist, chunk=chunk)

@@ -164,9 +164,9 @@ int dml_overlap_rounds(dml_ctx *ctx, const double *u_round, int32_t max_rounds);
 int dml_f_to_cg(dml_ctx *ctx);
 
 /* --- piston reservoir and the step counter -----------------------------
- * calc_rho (dana.F90:521-549): count of atoms with z0 < z < zmax per replica,
- * rho[rep] = count / (Lx Ly (zmax - z0)) (may be NULL); piston replicas keep
- * the count for dml_maxz. */
+ * calc_rho (dana.F90:521-549): count of atoms with z0 < z < zmax (z1 for a
+ * chunks replica) per replica, rho[rep] = count / (Lx Ly (top - z0)) (may be
+ * NULL); the count stays for dml_maxz and dml_bloques. */
 int dml_calc_rho(dml_ctx *ctx, double *rho);
 /* maxz (dana.F90:776-794) of the piston replicas, with the count of the last
  * dml_calc_rho */
@@ -174,6 +174,19 @@ int dml_maxz(dml_ctx *ctx);
 /* end of one main-loop step (dana.F90:262): advances the step number that
  * keys the Philox streams; dml_run does this itself */
 int dml_end_step(dml_ctx *ctx);
+
+/* --- chunks reservoir ---------------------------------------------------
+ * config_chunk (dana.F90:552-587) for replica rep: n template atoms at the
+ * chunk.xyz positions pos (n x 3 SoA; placed at zmax + z as the reference
+ * does), dist = the reservoir step (dist + rcut, dana.F90:437-439) and the
+ * sensor top z1 (z0 + dist).  Reserves cell memory for the box growth the
+ * replica's capacity allows. */
+int dml_set_chunk(dml_ctx *ctx, int32_t rep, int32_t n, const double *pos,
+                  double dist, double z1);
+/* bloques (dana.F90:716-773) of every chunks replica with the count of the
+ * last dml_calc_rho: adds a block when the sensor density leaves its band,
+ * moves the reservoir up, grows the box and rebuilds that replica's lists */
+int dml_bloques(dml_ctx *ctx);
 
 /* --- GCMC reservoir (dana.F90:590-713) ----------------------------------
  * nadj trials of replica rep with uniforms u[t*5 + {0 coin,1 accept,
@@ -217,8 +230,9 @@ enum { DML_K_LISTBUILD = 0, DML_K_FORCE = 1, DML_K_CELLS = 2, DML_K_DISPMAX = 3 
 int dml_time_kernel(dml_ctx *ctx, int32_t which, int32_t reps, double *avg_ms);
 /* launch shapes: lanes per ref atom of the force kernel (power of two <= 64)
  * and of the overlap scan (1, 4, 8 or 16), overlap rounds per step of dml_run (<= 32),
- * list kernel (1 = cell tiles through LDS, 2 = one wave per atom); 0 keeps
- * the current value.  Results do not depend on them beyond FP64 summation
+ * list kernel (1 = cell tiles through LDS, 2 = one wave per atom, 3 = 1 or 2
+ * per replica by atoms per cell, 4 = one wave per cell, the default); 0
+ * keeps the current value.  Results do not depend on them beyond FP64 summation
  * order.  No counterpart in the reference: used by the autotuner. */
 int dml_set_tuning(dml_ctx *ctx, int32_t lanes_per_atom, int32_t overlap_rounds,
                    int32_t overlap_lanes, int32_t list_kernel);

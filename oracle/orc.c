@@ -1310,6 +1310,84 @@ static double calc_rho(orc_sys *s, int chunks) {
   return (double)g / vol;
 }
 
+double orc_calc_rho(orc_sys *s, int chunks) { return calc_rho(s, chunks); }
+
+void orc_set_z1(orc_sys *s, double z1) { s->z1 = z1; }
+
+void orc_get_reservoir(const orc_sys *s, double out[4]) {
+  out[0] = s->p.z0; out[1] = s->z1; out[2] = s->p.zmax; out[3] = s->box[2];
+}
+
+int orc_chunk_init(const orc_sys *s, orc_chunk *ch, int n, const double *pos3, double dist) {
+  /* config_chunk (dana.F90:552-587): the block template sits at zmax + z, and
+   * its pos_old another zmax higher */
+  ch->n = n;
+  ch->dist = dist;
+  ch->pos = (double(*)[3])malloc(sizeof(double[3]) * (size_t)(n > 0 ? n : 1));
+  ch->old = (double(*)[3])malloc(sizeof(double[3]) * (size_t)(n > 0 ? n : 1));
+  if (!ch->pos || !ch->old) return ORC_E_ALLOC;
+  for (int j = 0; j < n; j++) {
+    for (int k = 0; k < 3; k++) {
+      ch->pos[j][k] = pos3[3 * j + k];
+      ch->old[j][k] = ch->pos[j][k];
+    }
+    ch->pos[j][2] = ch->pos[j][2] + s->p.zmax;
+    ch->old[j][2] = ch->pos[j][2] + s->p.zmax;
+  }
+  return 0;
+}
+
+void orc_chunk_free(orc_chunk *ch) {
+  free(ch->pos);
+  free(ch->old);
+  ch->pos = ch->old = NULL;
+  ch->n = 0;
+}
+
+int orc_bloques(orc_sys *s, orc_chunk *ch, double rho, int *fired, orc_counters *c) {
+  /* bloques (dana.F90:716-773) */
+  const double rhomedia = 5.775329e-4;
+  double dist = ch->dist;
+  double drho = rho - rhomedia;
+  *fired = 0;
+  if (fabs(drho) < (rhomedia * (double)0.186f)) return 0;
+  *fired = 1;
+  s->p.z0 = s->p.z0 + dist;
+  s->z1 = s->z1 + dist;
+  s->p.zmax = s->p.zmax + dist;
+  s->listed = 0;
+  for (int j = 0; j < ch->n; j++) {
+    int hs, slot, e;
+    if ((e = hs_new_slot(s, &hs))) return e;
+    atom_default(s, hs);
+    s->st[hs] = ORC_ALIVE;
+    if ((e = igrp_attach(&s->sys, hs, 0, &slot))) return e;
+    s->sys_of[hs] = slot;
+    ivec_push(&s->sysl, hs);
+    if ((e = igrp_attach(&s->b, hs, 0, &slot))) return e;
+    s->b_of[hs] = slot;
+    if (s->bnext_n < s->b.cap) {
+      s->bnext = (int *)realloc(s->bnext, sizeof(int) * (size_t)s->b.cap);
+      s->bnext_n = s->b.cap;
+    }
+    /* sorted while still at the origin (pos set after attach) */
+    if (s->tessellated) b_push_front(s, slot);
+    ivec_push(&s->ref, hs);
+    s->in_ref[hs] = 1;
+    s->nat++;
+    for (int k = 0; k < 3; k++) {
+      s->pos[hs][k] = ch->pos[j][k];
+      s->pos_old[hs][k] = ch->old[j][k];
+    }
+    s->zs[hs] = ORC_LI;
+    ch->pos[j][2] = ch->pos[j][2] + dist;
+    ch->old[j][2] = ch->old[j][2] + dist;
+  }
+  double nb[3] = {s->box[0], s->box[1], s->p.zmax};
+  orc_set_box(s, nb);
+  return orc_test_update(s, NULL, c);
+}
+
 int orc_dana_run(const orc_run_cfg *cfg, orc_run_out *out) {
   orc_params p;
   memset(&p, 0, sizeof(p));
@@ -1337,6 +1415,8 @@ int orc_dana_run(const orc_run_cfg *cfg, orc_run_out *out) {
   if (!s) return ORC_E_ALLOC;
   int status = 0;
   double dist = cfg->dist;
+  orc_chunk ch;
+  memset(&ch, 0, sizeof ch);
 
   /* pos_inic (dana.F90:330-396); box = (xi, yi, zmax) */
   int n;
@@ -1400,22 +1480,8 @@ int orc_dana_run(const orc_run_cfg *cfg, orc_run_out *out) {
   double rho = calc_rho(s, chunks);
   double rho0 = rho;
   /* config_chunk (dana.F90:552-587) */
-  int nchunk = chunks ? cfg->nchunk : 0;
-  double (*chunk)[3] = NULL, (*chunk_old)[3] = NULL;
-  if (nchunk > 0) {
-    chunk = (double(*)[3])malloc(sizeof(double[3]) * (size_t)nchunk);
-    chunk_old = (double(*)[3])malloc(sizeof(double[3]) * (size_t)nchunk);
-    for (int j = 0; j < nchunk; j++) {
-      for (int k = 0; k < 3; k++) {
-        chunk[j][k] = cfg->chunk_pos[3 * j + k];
-        chunk_old[j][k] = chunk[j][k];
-      }
-      chunk[j][2] = chunk[j][2] + s->p.zmax;
-      chunk_old[j][2] = chunk[j][2] + s->p.zmax;
-    }
-  }
+  if (chunks && (status = orc_chunk_init(s, &ch, cfg->nchunk, cfg->chunk_pos, dist))) goto done;
   const double tau = 0.1;
-  const double rhomedia = 5.775329e-4;
   double t0 = now_s();
   int64_t psteps = 0;
   for (int step = 1; step <= cfg->nst; step++) {
@@ -1435,44 +1501,8 @@ int orc_dana_run(const orc_run_cfg *cfg, orc_run_out *out) {
       if ((status = orc_gcmc(s, &rng, cfg->act, cfg->nadj, &cnt))) goto done;
     rho = calc_rho(s, chunks);
     if (chunks) {
-      /* bloques (dana.F90:716-773) */
-      double drho = rho - rhomedia;
-      if (!(fabs(drho) < (rhomedia * (double)0.186f))) {
-        s->p.z0 = s->p.z0 + dist;
-        s->z1 = s->z1 + dist;
-        s->p.zmax = s->p.zmax + dist;
-        s->listed = 0;
-        for (int j = 0; j < nchunk; j++) {
-          int hs, slot, e;
-          if ((e = hs_new_slot(s, &hs))) { status = e; goto done; }
-          atom_default(s, hs);
-          s->st[hs] = ORC_ALIVE;
-          if ((e = igrp_attach(&s->sys, hs, 0, &slot))) { status = e; goto done; }
-          s->sys_of[hs] = slot;
-          ivec_push(&s->sysl, hs);
-          if ((e = igrp_attach(&s->b, hs, 0, &slot))) { status = e; goto done; }
-          s->b_of[hs] = slot;
-          if (s->bnext_n < s->b.cap) {
-            s->bnext = (int *)realloc(s->bnext, sizeof(int) * (size_t)s->b.cap);
-            s->bnext_n = s->b.cap;
-          }
-          /* sorted while still at the origin (pos set after attach) */
-          if (s->tessellated) b_push_front(s, slot);
-          ivec_push(&s->ref, hs);
-          s->in_ref[hs] = 1;
-          s->nat++;
-          for (int k = 0; k < 3; k++) {
-            s->pos[hs][k] = chunk[j][k];
-            s->pos_old[hs][k] = chunk_old[j][k];
-          }
-          s->zs[hs] = ORC_LI;
-          chunk[j][2] = chunk[j][2] + dist;
-          chunk_old[j][2] = chunk_old[j][2] + dist;
-        }
-        double nb[3] = {s->box[0], s->box[1], s->p.zmax};
-        orc_set_box(s, nb);
-        if ((status = orc_test_update(s, NULL, &cnt))) goto done;
-      }
+      int fired;
+      if ((status = orc_bloques(s, &ch, rho, &fired, &cnt))) goto done;
     }
     if (cfg->nwr > 0 && step % cfg->nwr == 0) {
       /* salida() writes the frame before the piston move (dana.F90:250-261) */
@@ -1508,9 +1538,8 @@ int orc_dana_run(const orc_run_cfg *cfg, orc_run_out *out) {
     }
   }
   out->c = cnt;
-  free(chunk);
-  free(chunk_old);
 done:
+  orc_chunk_free(&ch);
   free(r);
   free(zinit);
   orc_sys_free(s);

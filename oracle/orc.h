@@ -170,6 +170,24 @@ int orc_overlap_rounds(orc_sys *s, const double *u_round, int max_rounds,
 void orc_f_to_cg(orc_sys *s, int gcmc);
 /* gcmc_run (dana.F90:590-713) */
 int orc_gcmc(orc_sys *s, orc_rng *r, double act, int nadj, orc_counters *c);
+/* chunks reservoir: the block template of config_chunk and bloques */
+typedef struct orc_chunk {
+  int n;
+  double dist;            /* dist + rcut (config_inic, dana.F90:437-441) */
+  double (*pos)[3], (*old)[3];
+} orc_chunk;
+/* config_chunk (dana.F90:552-587) from chunk.xyz positions pos3[3*j+k] */
+int orc_chunk_init(const orc_sys *s, orc_chunk *ch, int n, const double *pos3, double dist);
+void orc_chunk_free(orc_chunk *ch);
+/* bloques (dana.F90:716-773) with the rho of calc_rho; *fired = 1 when the
+ * block was added (then the box grew and test_update ran) */
+int orc_bloques(orc_sys *s, orc_chunk *ch, double rho, int *fired, orc_counters *c);
+/* calc_rho (dana.F90:521-549): chunks = 1 counts up to z1 */
+double orc_calc_rho(orc_sys *s, int chunks);
+void orc_set_z1(orc_sys *s, double z1);
+/* z0, z1, zmax, box(3) */
+void orc_get_reservoir(const orc_sys *s, double out[4]);
+
 /* gcmc group of config_inic: Li atoms in sys order (dana.F90:505-513) */
 void orc_gcmc_init(orc_sys *s);
 /* gcmc group in list order; returns its size */

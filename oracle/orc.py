@@ -97,6 +97,7 @@ def lib():
         L.orc_overlap_rounds.argtypes = [vp, vp, C.c_int, C.POINTER(Counters)]
         L.orc_gcmc.argtypes = [vp, C.POINTER(Rng), C.c_double, C.c_int, C.POINTER(Counters)]
         L.orc_dana_run.argtypes = [C.POINTER(RunCfg), C.POINTER(RunOut)]
+        L.orc_calc_rho.restype = C.c_double
         L.orc_nr_ran.restype = C.c_double
         L.orc_nr_ran.argtypes = [C.POINTER(Rng)]
         L.orc_nr_gasdev.restype = C.c_double
@@ -133,6 +134,10 @@ def dana_params(box, skin, h=1e-2, prob=1.0, z0=100.0, zmax=200.0, dif_sc=250.0,
     return p
 
 
+class Chunk(C.Structure):
+    _fields_ = [("n", C.c_int), ("dist", C.c_double), ("pos", C.c_void_p), ("old", C.c_void_p)]
+
+
 class System:
     """An oracle system built like config_inic from positions and species."""
 
@@ -152,6 +157,8 @@ class System:
 
     def __del__(self):
         try:
+            if getattr(self, "chunk", None) is not None:
+                self.L.orc_chunk_free(C.byref(self.chunk))
             self.L.orc_sys_free(self.s)
         except Exception:
             pass
@@ -265,6 +272,33 @@ class System:
 
     def f_to_cg(self, gcmc=False):
         self.L.orc_f_to_cg(C.c_void_p(self.s), int(gcmc))
+
+    def chunk_init(self, pos3, dist):
+        """config_chunk: the block template from chunk.xyz positions (n, 3);
+        dist is the reservoir step (dist + rcut of config_inic)"""
+        p3 = np.ascontiguousarray(pos3, np.float64)
+        self.chunk = Chunk()
+        r = self.L.orc_chunk_init(C.c_void_p(self.s), C.byref(self.chunk), len(p3), _p(p3),
+                                  C.c_double(dist))
+        assert r == 0, r
+
+    def bloques(self, rho):
+        fired = C.c_int()
+        st = self.L.orc_bloques(C.c_void_p(self.s), C.byref(self.chunk), C.c_double(rho),
+                                C.byref(fired), C.byref(self.c))
+        return st, fired.value
+
+    def calc_rho(self, chunks=False):
+        return self.L.orc_calc_rho(C.c_void_p(self.s), int(chunks))
+
+    def set_z1(self, z1):
+        self.L.orc_set_z1(C.c_void_p(self.s), C.c_double(z1))
+
+    def reservoir(self):
+        """z0, z1, zmax, box(3)"""
+        out = np.zeros(4)
+        self.L.orc_get_reservoir(C.c_void_p(self.s), _p(out))
+        return out
 
     def gcmc_init(self):
         self.L.orc_gcmc_init(C.c_void_p(self.s))

@@ -109,10 +109,11 @@ def assert_lists_equal(c, s):
 NB = np.load(os.path.join(G, "nb_cases.npz"))
 
 
-@pytest.mark.parametrize("lb_mode", ["0", "1"])
+@pytest.mark.parametrize("lb_mode", ["0", "1", "3"])
 @pytest.mark.parametrize("name", list(NB["names"]))
 def test_cells_and_lists_equal_reference_modules(dml, name, lb_mode, monkeypatch):
-    """lb_mode 0: cell-tile list build through LDS; 1: one wave per atom"""
+    """lb_mode 0: cell-tile list build through LDS; 1: one wave per atom;
+    3: one wave per cell"""
     monkeypatch.setenv("DML_LISTBUILD", lb_mode)
     d = {k.split("/", 1)[1]: NB[k] for k in NB.files if k.startswith(name + "/")}
     say("case", name, len(d["z"]), "lb_mode", lb_mode)
@@ -567,4 +568,60 @@ def test_row_overflow_grows_and_rebuilds(orc, dml):
     assert nn.max() > 16
     s.fuerza(); c.forces()  # the grown rows serve the force loop
     assert rel_err(c.state()["force"], s.state()["force"]) <= 1e-12
+    c.close()
+
+
+@pytest.mark.parametrize("band", ["fires", "holds"])
+def test_chunks_reservoir_matches_oracle(orc, dml, band):
+    """calc_rho up to the sensor z1 and bloques: the density test, slot and
+    hs%b reuse of the new block, the reservoir and box moving up by dist,
+    the regrown tessellation and the rebuilt rows, over whole steps."""
+    rng = np.random.default_rng(51 if band == "fires" else 52)
+    box0 = np.array([40.0, 40.0, 0.0])
+    z0, dist = 20.0, 10.0 + 3.2  # dist + rcut (config_inic)
+    z1 = z0 + dist
+    zmax = z1 + dist
+    box = np.array([40.0, 40.0, zmax])
+    el = electrode(3.51, 11, 11, 2, 1.755)
+    if band == "fires":   # sensor density far above rhomedia
+        ions = rsa(rng, 200, np.array([0, 0, 4.0]), np.array([40, 40, zmax - 1.0]), 3.3, box)
+    else:                 # 12 ions in (z0, z1): rho within 18.6 % of rhomedia
+        low = rsa(rng, 60, np.array([0, 0, 4.0]), np.array([40, 40, z0 - 1.0]), 3.3, box)
+        mid = rsa(rng, 12, np.array([0, 0, z0 + 2.0]), np.array([40, 40, z1 - 2.0]), 3.3, box)
+        ions = np.vstack([low, mid])
+    pos = np.vstack([el, ions])
+    z = np.array([2] * len(el) + [1] * len(ions), np.int8)
+    chunk = rsa(rng, 30, np.zeros(3), np.array([40, 40, dist - 0.5]), 3.3, box)
+    say("chunks", band, len(pos), len(chunk))
+    s = orc.System(orc.dana_params(box, 2.0, h=5e-5, z0=z0, zmax=zmax, ermak=False, piston=False),
+                   pos, z)
+    s.set_z1(z1)
+    s.chunk_init(chunk, dist)
+    cfg = dml.dana_config(box, 2.0, len(pos) + 8 * len(chunk) + 64, h=5e-5, z0=z0, zmax=zmax,
+                          integrator=dml.BROWNIAN, reservoir=dml.CHUNKS,
+                          overlap_mode=dml.OVERLAP_EXACT)
+    c = dml.Context(cfg)
+    c.set_state(pos, species=z)
+    c.set_chunk(chunk, dist, z1)
+    s.test_update(); c.test_update()
+    fired = 0
+    for it in range(3):
+        _step_pair(orc, dml, s, c, rng, "exact", 1)
+        rho_o = s.calc_rho(True)
+        rho_d = c.calc_rho()[0]
+        assert rho_d == rho_o, (it, rho_d, rho_o)
+        st, f = s.bloques(rho_o)
+        assert st == 0
+        c.bloques()
+        fired += f
+        res = s.reservoir()
+        assert c.zmax() == res[2], it
+        nc_o, cell_o, uc_o, ci_o = s.cells()
+        nc_d, cell_d, uc_d, ci_d = c.cells()
+        np.testing.assert_array_equal(nc_d, nc_o)
+        np.testing.assert_array_equal(cell_d, cell_o)
+        if f:  # the device sorts its buckets when it rebuilds (the reference at every test_update)
+            np.testing.assert_array_equal(ci_d, ci_o)
+        _assert_same_groups(s, c, "bloques %d" % it)
+    assert fired == (3 if band == "fires" else 0)
     c.close()
