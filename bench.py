@@ -154,11 +154,19 @@ def batched_leg(dml, nrep, steps, warmup):
     trials = sum(a["ins"] + a["del_"] - b["ins"] - b["del_"] for a, b in zip(after, before))
     halted = sum(1 for k in range(nrep) if ens.replica_status(k) != 0)
     # dominant kernel: the neighbour-row build (see profiles/), timed on the
-    # context's own stream
+    # context's own stream over every replica; its rows are those of the ref
+    # atoms now (GCMC and chunks replicas have grown)
+    c0 = [ens.counters(k) for k in range(nrep)]
     kl_ms = ens.time_kernel(dml.K_LISTBUILD, 10)
+    c1 = [ens.counters(k) for k in range(nrep)]
+    # the timing rebuilds once before its 10 launches: 11 builds counted
+    rows_now = sum(b["rows"] - a["rows"] for a, b in zip(c0, c1)) / 11.0
+    entries_now = sum(b["row_entries"] - a["row_entries"] for a, b in zip(c0, c1)) / 11.0
+    mean_nn_w = entries_now / max(rows_now, 1.0)
     kf_ms = ens.time_kernel(dml.K_FORCE, 10)
     ens.close()
     return dict(n=ntot, ms=ms, wall=wall, rate=psteps / (ms * 1e-3), mean_nn=mean_nn,
+                rows_at_timing=rows_now, mean_nn_rows=mean_nn_w,
                 rebuilds_per_rep_step=reb / max(stepsum, 1), list_ms=kl_ms, force_ms=kf_ms,
                 halted=halted, gcmc_accepted=trials,
                 tuning={k: choice[k] for k in ("lanes_per_atom", "list_kernel", "overlap_lanes")},
@@ -336,20 +344,20 @@ def main():
         # neighbour-row build: compulsory bytes per ref atom = own position
         # (24) + row count (4) + 4 per list entry, plus per b atom its bucket
         # slot and bucket-ordered position (4 + 24)
-        per_atom = 24 + 4 + 4 * batched["mean_nn"] + 28
-        traffic = per_atom * batched["n"]
+        per_atom = 24 + 4 + 4 * batched["mean_nn_rows"] + 28
+        traffic = per_atom * batched["rows_at_timing"]
         ach = traffic / (batched["list_ms"] * 1e-3) / 1e9
         # HBM bytes per launch from the two rocprofv3 PMC passes (FETCH_SIZE,
         # WRITE_SIZE, scaled by a 1 GiB copy) of the same 256-replica launch
         measured = None
-        tf = os.path.join(ROOT, "profiles", "k_listbuild_cells_c3_traffic.json")
+        tf = os.path.join(ROOT, "profiles", "k_listbuild_wc_c3_traffic.json")
         if args.replicas == 256 and os.path.exists(tf):
             measured = json.load(open(tf))["hbm_bytes_per_launch"]
-        res["roofline"] = {"kernel": "k_listbuild_cells", "bound": "hbm", "achieved": ach,
+        res["roofline"] = {"kernel": "k_listbuild_wc", "bound": "hbm", "achieved": ach,
                            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
                            "traffic": measured, "avg_launch_ms": batched["list_ms"],
                            "algorithmic_bytes_per_launch": traffic}
-        res["kernel_ms"] = {"k_listbuild_cells": batched["list_ms"], "k_force": batched["force_ms"]}
+        res["kernel_ms"] = {"k_listbuild_wc": batched["list_ms"], "k_force": batched["force_ms"]}
         res["mean_list_length"] = batched["mean_nn"]
         res["rebuilds_per_step"] = batched["rebuilds_per_rep_step"]
         res["replicas_halted"] = batched["halted"]

@@ -319,9 +319,12 @@ void enq_listbuild(dml_ctx *c) {
   if (c->d.n_wm > 0)
     hipLaunchKernelGGL(k_listbuild<4>, dim3((unsigned)(c->d.n_wm * (DML_BLOCK / 4))),
                        dim3(4 * 64), 0, c->st, c->d);
-  if (c->any_lb[2])
-    hipLaunchKernelGGL(k_listbuild_wc, dim3(clamp_grid((c->d.n_cm + 3) / 4)), dim3(DML_BLOCK), 0,
-                       c->st, c->d);
+  if (c->any_lb[2]) {
+    // a multiple of the 8 XCDs (the kernel deals its cell runs by XCD)
+    unsigned g = clamp_grid((c->d.n_cm + 3) / 4);
+    g = std::max(8u, g / 8 * 8);
+    hipLaunchKernelGGL(k_listbuild_wc, dim3(g), dim3(DML_BLOCK), 0, c->st, c->d);
+  }
 }
 
 // the list kernel of each replica (host mirror; the caller uploads): cell
@@ -1615,6 +1618,8 @@ int dml_get_counters(dml_ctx *c, int32_t rep, dml_counters *out) {
   out->del = (int64_t)h.del;
   out->passes = (int64_t)h.passes;
   out->particle_steps = (int64_t)h.particle_steps;
+  out->row_entries = (int64_t)h.nn_sum;
+  out->rows = (int64_t)h.nn_rows;
   return DML_OK;
 }
 

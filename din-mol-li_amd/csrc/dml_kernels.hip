@@ -549,12 +549,20 @@ __device__ inline double rdlane_d(double v, int a) {
 
 __global__ void __launch_bounds__(DML_BLOCK) k_listbuild_wc(DevState s) {
   const int lane = threadIdx.x % WAVE;
-  const int64_t nw = (int64_t)gridDim.x * (DML_BLOCK / WAVE);
   const int maxnn = s.maxnn;
+  // XCD-aware: workgroups are dealt round-robin to the 8 XCDs, so
+  // workgroup b works on slice (b % 8) of the cell list and, inside it,
+  // every wave on one contiguous run of cells; the waves of one XCD then
+  // sweep neighbouring cells together and share their candidates in its L2
+  const int nb = gridDim.x;  // a multiple of 8
+  const int64_t vb = (int64_t)(blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8;
+  const int64_t wid = vb * (DML_BLOCK / WAVE) + threadIdx.x / WAVE;
+  const int64_t nwv = (int64_t)nb * (DML_BLOCK / WAVE);
+  const int64_t run = (s.n_cm + nwv - 1) / nwv;
+  const int64_t g_end = min(s.n_cm, (wid + 1) * run);
   int cur_r = -1;
   unsigned long long acc_nn = 0, acc_rows = 0;
-  for (int64_t g = (int64_t)blockIdx.x * (DML_BLOCK / WAVE) + threadIdx.x / WAVE; g < s.n_cm;
-       g += nw) {
+  for (int64_t g = wid * run; g < g_end; g++) {
     const int r = s.cm_rep[g];
     const int c = s.cm_cell[g];
     const RepParams &P = s.rp[r];

@@ -60,7 +60,8 @@ class Config(C.Structure):
 
 class Counters(C.Structure):
     _fields_ = [(n, C.c_int64) for n in ("try_", "depo", "choques", "choques3", "nupd",
-                                         "ins", "del_", "passes", "particle_steps")]
+                                         "ins", "del_", "passes", "particle_steps",
+                                         "row_entries", "rows")]
 
 
 _lib = None

@@ -43,7 +43,7 @@ module dml_f
   ! struct dml_counters
   type, bind(C) :: dml_counters
     integer(c_int64_t) :: try_, depo, choques, choques3, nupd, ins, del, passes
-    integer(c_int64_t) :: particle_steps
+    integer(c_int64_t) :: particle_steps, row_entries, rows
   end type dml_counters
 
   interface

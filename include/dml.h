@@ -87,6 +87,7 @@ typedef struct dml_config {
 typedef struct dml_counters {
   int64_t try_, depo, choques, choques3, nupd, ins, del, passes;
   int64_t particle_steps; /* sum over completed steps of the atoms alive */
+  int64_t row_entries, rows; /* neighbour-row entries and rows built so far */
 } dml_counters;
 
 /* --- lifetime ---------------------------------------------------------- */
