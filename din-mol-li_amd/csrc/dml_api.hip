@@ -459,7 +459,7 @@ void enq_step(dml_ctx *c) {
   enq_test_update(c);
   enq_f2cg(c);
   if (c->any_gcmc)  // gcmc_run after F -> CG (dana.F90:238-241)
-    hipLaunchKernelGGL(k_gcmc_run, dim3(c->nrep), dim3(DML_BLOCK), 0, c->st, c->d, -1,
+    hipLaunchKernelGGL(k_gcmc_run, dim3(c->nrep), dim3(DML_GCMC_BLOCK), 0, c->st, c->d, -1,
                        (const double *)nullptr, (const double *)nullptr, 0.0, 0);
   // calc_rho (dana.F90:244), bloques (:247), maxz (:261)
   if (c->any_piston || c->any_chunks)
@@ -1178,7 +1178,7 @@ int dml_gcmc(dml_ctx *c, int32_t rep, double act, int32_t nadj, const double *u,
     hipFree(dg);
     return DML_E_NOMEM;
   }
-  hipLaunchKernelGGL(k_gcmc_run, dim3(1), dim3(DML_BLOCK), 0, c->st, c->d, rep,
+  hipLaunchKernelGGL(k_gcmc_run, dim3(1), dim3(DML_GCMC_BLOCK), 0, c->st, c->d, rep,
                      (const double *)du, (const double *)dg, act, nadj);
   rc = finish(c);
   hipFree(du);

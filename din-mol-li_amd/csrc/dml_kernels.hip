@@ -1700,17 +1700,83 @@ __device__ inline double gc_gauss(const RepParams &P, const double *g, int t, in
   return (k & 1) ? b : a;
 }
 
-__global__ void __launch_bounds__(DML_BLOCK) k_gcmc_run(DevState s, int rep_only, const double *u,
+#define GC_TRACK 256
+#define GC_BLOCK DML_GCMC_BLOCK  // 16 waves for the list passes
+// stable block compaction of src[0, len) into dst (dst may not alias src):
+// keeps entries v with keep(q, v); each wave takes a contiguous quarter in
+// chunks of 64 (ballot counts, no barriers), one block scan of the wave
+// totals, then the same sweep writes.  Returns the kept count.
+template <class F>
+__device__ int blk_compact(const int32_t *src, int len, int32_t *dst, int *sh, F keep) {
+  __syncthreads();  // src may just have been written by other threads
+  const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
+  const int nwv = blockDim.x / WAVE;
+  const int part = (len + nwv - 1) / nwv;
+  const int a = min(len, w * part), b = min(len, a + part);
+  int cnt = 0;
+  for (int q0 = a; q0 < b; q0 += WAVE) {
+    int q = q0 + lane;
+    int k = q < b ? (keep(q, src[q]) ? 1 : 0) : 0;
+    cnt += __popcll(__ballot(k));
+  }
+  __syncthreads();
+  if (lane == 0) sh[w] = cnt;
+  __syncthreads();
+  int off = 0, tot = 0;
+  for (int q = 0; q < nwv; q++) {
+    if (q < w) off += sh[q];
+    tot += sh[q];
+  }
+  for (int q0 = a; q0 < b; q0 += WAVE) {
+    int q = q0 + lane;
+    int v = q < b ? src[q] : -1;
+    int k = q < b ? (keep(q, v) ? 1 : 0) : 0;
+    unsigned long long m = __ballot(k);
+    if (k) dst[off + __popcll(m & ((1ull << lane) - 1ull))] = v;
+    off += __popcll(m);
+  }
+  __syncthreads();
+  return tot;
+}
+
+// the cells within one cell step of (c0, c1, c2): x, y periodic, z not (its
+// ghost layers included), each cell once (ncells of 1 or 2 repeat a cell)
+__device__ inline int near_cells(const RepParams &P, int c0, int c1, int c2, int *out) {
+  int xs[3], ys[3], zs[3], nx = 0, ny = 0, nz = 0;
+  for (int d = -1; d <= 1; d++) {
+    int x = cell_wrap(c0 + d, P.ncells[0]);
+    bool dup = false;
+    for (int q = 0; q < nx; q++) dup |= xs[q] == x;
+    if (!dup) xs[nx++] = x;
+    int y = cell_wrap(c1 + d, P.ncells[1]);
+    dup = false;
+    for (int q = 0; q < ny; q++) dup |= ys[q] == y;
+    if (!dup) ys[ny++] = y;
+    int z = c2 + d;
+    if (z >= 0 && z <= P.ncells[2] + 1) zs[nz++] = z;
+  }
+  int n = 0;
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) out[n++] = cell_lin(P, xs[i], ys[j], zs[k]);
+  return n;
+}
+
+__global__ void __launch_bounds__(GC_BLOCK) k_gcmc_run(DevState s, int rep_only, const double *u,
                                                          const double *g, double act_in,
                                                          int nadj_in) {
-  __shared__ int sh[DML_BLOCK / WAVE];
-  __shared__ int sh_v[4];
+  __shared__ int sh[GC_BLOCK / WAVE];
+  __shared__ int sh_cells[27];
+  __shared__ int sh_ncell;
+  __shared__ int sh_hit, sh_full, sh_flag;
+  __shared__ int sh_freed_b[GC_TRACK];  // hs%b indices freed in this call (ascending)
+  __shared__ int sh_tomb[GC_TRACK];     // deleted positions of invl (ascending)
   const int r = rep_only >= 0 ? rep_only : (int)blockIdx.x;
   if (r >= s.nrep) return;
   RepParams *Pw = s.rp + r;
   const RepParams &P = *Pw;
   if (P.halted || (rep_only < 0 && P.reservoir != DML_GCMC)) return;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid % WAVE, wv = tid / WAVE;
   const int poff = P.poff;
   const double act = rep_only >= 0 ? act_in : P.act;
   const int nadj = rep_only >= 0 ? nadj_in : P.nadj;
@@ -1720,46 +1786,39 @@ __global__ void __launch_bounds__(DML_BLOCK) k_gcmc_run(DevState s, int rep_only
   const double vol = P.box[0] * P.box[1] * (zmax - z0);
   const double rl2 = P.rl2;
   const int maxnn = s.maxnn;
-  int *gc = s.gc_order + poff;
-  // gcmc%detach of the atoms turned CG at step end (dana.F90:235): stable
-  // compaction of the list by membership
-  int ngc = P.ngc;
-  {
-    int w = 0;
-    for (int q0 = 0; q0 < ngc; q0 += DML_BLOCK) {
-      int q = q0 + tid;
-      int v = q < ngc ? gc[q] : -1;
-      int keep = v >= 0 && (s.flags[v] & DML_FL_GCMC);
-      int tot;
-      int pos = blk_scan01(keep, sh, &tot);
-      if (keep) gc[w + pos] = v;
-      w += tot;
-      __syncthreads();
-    }
-    ngc = w;
-  }
-  // count in the control volume (dana.F90:609-615)
-  int n = 0;
-  for (int q = tid; q < ngc; q += DML_BLOCK) {
-    double z = s.pz[gc[q]];
-    n += !(z < z0 || z > zmax);
-  }
-  n = blk_sum(n, sh);
-  // bookkeeping registers (uniform across the block)
+  const int listed = P.listed;
+  int32_t *gc = s.gc_order + poff;    // the gcmc list (tombstones -1 during the call)
+  int32_t *tmp = s.wl[1] + poff;      // scratch (pcap entries)
+  int32_t *invl = s.ref_tmp + poff;   // in-volume gcmc atoms, list order
+  int32_t *rord = s.ref_order + poff;
+  // gcmc%detach of the atoms turned CG at step end (dana.F90:235)
+  int glen = blk_compact(gc, P.ngc, tmp, sh,
+                         [&](int, int v) { return v >= 0 && (s.flags[v] & DML_FL_GCMC); });
+  for (int q = tid; q < glen; q += GC_BLOCK) gc[q] = tmp[q];
+  __syncthreads();
+  // the control volume's atoms in list order (dana.F90:609-615); n = their count
+  int ilen = blk_compact(gc, glen, invl, sh, [&](int, int v) {
+    double z = s.pz[v];
+    return !(z < z0 || z > zmax);
+  });
+  int n = ilen;
+  int idel = 0;  // in-volume entries deleted so far (tombstones in invl)
   int nslots = P.nslots, nat = P.nat, nlimbo = P.nlimbo, b_amax = P.b_amax, b_nat = P.b_nat;
-  int nref = P.nref;
+  int rlen = P.nref;  // ref list length with tombstones
+  int head = 0, tail = glen - 1;  // first / last live gcmc entry
   uint32_t pid_next = P.pid_next;
   int seq_next = P.seq_next;
-  const int listed = P.listed;
   unsigned long long n_ins = 0, n_del = 0;
+  int nfreed_b = 0;
+  int hs_scan = 0, b_scan = 0;  // lowest slot / index not yet known to be taken
   int fail = 0;
-  // constants of gems_constants (Constants.F90:100-167) for beta
   const double ui_ev = 1.0e2 * 1.0e2 * 1.6605402e-27 * (1.0 / 1.60219e-19);
-  const double kB_ui = 8.617385e-05 * (1.0 / ui_ev);
+  const double kB_ui = 8.617385e-05 * (1.0 / ui_ev);  // Constants.F90:100-167
   for (int t = 0; t < nadj && !fail; t++) {
     __syncthreads();  // the previous trial's single-thread writes
-    if (ngc == 0) { fail = ERR_NOPART; break; }
-    const int tmpl = gc[0];
+    while (head <= tail && (gc[head] < 0 || !(s.flags[gc[head]] & DML_FL_GCMC))) head++;
+    if (head > tail) { fail = ERR_NOPART; break; }
+    const int tmpl = gc[head];
     const double beta = sqrt(kB_ui * 300.0 / P.mass[s.species[tmpl] - 1]);
     if (gc_unif(P, u, t, 0, call) < 0.5) {
       // ---- creation attempt (dana.F90:627-677)
@@ -1767,46 +1826,93 @@ __global__ void __launch_bounds__(DML_BLOCK) k_gcmc_run(DevState s, int rep_only
       const double rx = gc_unif(P, u, t, 2, call) * P.box[0];
       const double ry = gc_unif(P, u, t, 3, call) * P.box[1];
       const double rz = gc_unif(P, u, t, 4, call) * (zmax - z0) + z0;
-      int hit = 0;
-      for (int q = tid; q < ngc && !hit; q += DML_BLOCK) {
-        int o = gc[q];
-        double dx, dy, dz;  // distance(o%pos, r): r - o%pos
-        vdist_mic(P, rx, ry, rz, s.px[o], s.py[o], s.pz[o], &dx, &dy, &dz);
-        if (dot3(dx, dy, dz) < rc2) hit = 1;
+      int ci0 = 0, ci1 = 0, ci2 = 0;
+      if (P.use_cells) {
+        ci0 = (int)(rx / P.cell[0]) + 1;
+        ci1 = (int)(ry / P.cell[1]) + 1;
+        ci2 = (int)(rz / P.cell[2]) + 1;
+        if (ci0 < 0 || ci1 < 0 || ci2 < 0 || ci0 > P.ncells[0] + 1 || ci1 > P.ncells[1] + 1 ||
+            ci2 > P.ncells[2] + 1) { fail = ERR_UNSUP; break; }
       }
-      if (blk_sum(hit, sh)) continue;
+      if (tid == 0) {
+        sh_ncell = P.use_cells ? near_cells(P, ci0, ci1, ci2, sh_cells) : 1;
+        if (!P.use_cells) sh_cells[0] = 0;
+        sh_hit = 0;
+        sh_full = 0;
+      }
+      __syncthreads();
+      const int ncell = sh_ncell;
+      // hard core against the gcmc group (dana.F90:638-653): every gcmc atom
+      // closer than rc lies in a neighbouring cell
+      for (int k = wv; k < ncell; k += GC_BLOCK / WAVE) {
+        const int cc = sh_cells[k];
+        const int64_t base = P.boff + (int64_t)cc * P.bcap;
+        const int cnt = min(s.cell_cnt[P.coff + cc], P.bcap);
+        for (int q = lane; q < cnt; q += WAVE) {
+          const int o = s.bucket[base + q];
+          if (!(s.flags[o] & DML_FL_GCMC)) continue;
+          double dx, dy, dz;  // distance(o%pos, r): r - o%pos
+          vdist_mic(P, rx, ry, rz, s.px[o], s.py[o], s.pz[o], &dx, &dy, &dz);
+          if (dot3(dx, dy, dz) < rc2) sh_hit = 1;
+        }
+      }
+      __syncthreads();
+      if (sh_hit) continue;
       n++;
-      const int tail = gc[ngc - 1];
-      if (tid == 0) {  // the velocity lands on the list's last atom (dana.F90:666-668)
-        s.vx[tail] = beta * gc_gauss(P, g, t, 0, call);
-        s.vy[tail] = beta * gc_gauss(P, g, t, 1, call);
-        s.vz[tail] = beta * gc_gauss(P, g, t, 2, call);
-      }
+      const int tail_atom = gc[tail];
       // hs slot: first free slot or append (Groups.F90:1083-1093)
       int hs;
       if (nslots >= nat + 1 + nlimbo) {
+        // 16 flags per load from the 16-aligned block holding hs_scan
         int cand = 0x7fffffff;
-        for (int q = tid; q < nslots; q += DML_BLOCK)
-          if (!(s.flags[poff + q] & (DML_FL_ALIVE | DML_FL_LIMBO))) cand = min(cand, q);
+        const uint8_t *fl = s.flags + poff;
+        for (int q16 = (hs_scan & ~15) + 16 * tid; q16 < nslots && cand == 0x7fffffff;
+             q16 += 16 * GC_BLOCK) {
+          const uint4 v = *reinterpret_cast<const uint4 *>(fl + q16);
+          for (int k = 0; k < 16; k++) {
+            const int q = q16 + k;
+            const uint32_t wd = k < 4 ? v.x : k < 8 ? v.y : k < 12 ? v.z : v.w;
+            const uint32_t f = (wd >> (8 * (k & 3))) & 0xffu;
+            if (q >= hs_scan && q < nslots && !(f & (DML_FL_ALIVE | DML_FL_LIMBO))) {
+              cand = q;
+              break;
+            }
+          }
+        }
         hs = blk_min(cand, sh);
         if (hs == 0x7fffffff) { fail = ERR_UNSUP; break; }
+        hs_scan = hs + 1;
       } else {
         if (nslots >= P.pcap) { fail = ERR_BUCKET; break; }
         hs = nslots++;
       }
-      // hs%b slot the same way
+      // hs%b index the same way (indices freed in this call count too)
       int bs;
       if (b_amax >= b_nat + 1) {
         int cand = 0x7fffffff;
-        for (int q = tid; q < b_amax; q += DML_BLOCK)
-          if (s.b2slot[poff + q] < 0) cand = min(cand, q);
+        for (int q = b_scan + tid; q < b_amax && cand == 0x7fffffff; q += GC_BLOCK)
+          if (s.b2slot[poff + q] < 0) cand = q;
         bs = blk_min(cand, sh);
+        if (nfreed_b && sh_freed_b[0] < bs) bs = sh_freed_b[0];
         if (bs == 0x7fffffff) { fail = ERR_UNSUP; break; }
+        const int from_freed = nfreed_b && bs == sh_freed_b[0];
+        __syncthreads();
+        if (from_freed) {
+          if (tid == 0)
+            for (int q = 1; q < nfreed_b; q++) sh_freed_b[q - 1] = sh_freed_b[q];
+          nfreed_b--;
+        } else {
+          b_scan = bs + 1;
+        }
       } else {
         bs = b_amax++;
       }
       const int i = poff + hs;
       if (tid == 0) {
+        // the velocity lands on the list's last atom (dana.F90:666-668)
+        s.vx[tail_atom] = beta * gc_gauss(P, g, t, 0, call);
+        s.vy[tail_atom] = beta * gc_gauss(P, g, t, 1, call);
+        s.vz[tail_atom] = beta * gc_gauss(P, g, t, 2, call);
         s.species[i] = s.species[tmpl];  // atom_asign (Groups.F90:484-502)
         s.vx[i] = s.vx[tmpl]; s.vy[i] = s.vy[tmpl]; s.vz[i] = s.vz[tmpl];
         s.fx[i] = s.fx[tmpl]; s.fy[i] = s.fy[tmpl]; s.fz[i] = s.fz[tmpl];
@@ -1820,33 +1926,20 @@ __global__ void __launch_bounds__(DML_BLOCK) k_gcmc_run(DevState s, int rep_only
         s.b2slot[poff + bs] = i;
         s.pid[i] = pid_next;
         s.seq[i] = seq_next;
-        s.rowflag[i] = 0;  // set below for a ghost-layer cell
+        s.rowflag[i] = 0;
         s.nn[i] = 0;
-        s.ref_order[poff + nref] = i;
-        s.ref_pos[i] = nref;
+        rord[rlen] = i;
+        s.ref_pos[i] = rlen;
         put_pf4(s, i, rx, ry, rz, s.species[tmpl]);
-      }
-      pid_next++;
-      seq_next++;
-      nat++;
-      b_nat++;
-      nref++;
-      // cgroup_sort_atom: push-front into its cell chain (Cells.F90:299-300);
-      // the Verlet bucket keeps ascending hs%b order
-      int cell = 0, ci0 = 0, ci1 = 0, ci2 = 0;
-      if (P.use_cells) {
-        ci0 = (int)(rx / P.cell[0]) + 1;
-        ci1 = (int)(ry / P.cell[1]) + 1;
-        ci2 = (int)(rz / P.cell[2]) + 1;
-        if (ci0 < 0 || ci1 < 0 || ci2 < 0 || ci0 > P.ncells[0] + 1 || ci1 > P.ncells[1] + 1 ||
-            ci2 > P.ncells[2] + 1) { fail = ERR_UNSUP; break; }
-        cell = cell_lin(P, ci0, ci1, ci2);
-      }
-      {
+        // cgroup_sort_atom: push-front into its cell chain (Cells.F90:299-300);
+        // the Verlet bucket keeps ascending hs%b order
+        const int cell = P.use_cells ? cell_lin(P, ci0, ci1, ci2) : 0;
         const int64_t base = P.boff + (int64_t)cell * P.bcap;
         const int cnt = s.cell_cnt[P.coff + cell];
-        if (cnt >= P.bcap) { fail = ERR_BUCKET; break; }
-        if (tid == 0) {
+        if (cnt >= P.bcap) {
+          sh_flag = ERR_BUCKET;
+        } else {
+          sh_flag = 0;
           int at = 0;
           if (!P.use_cells)
             while (at < cnt && s.bkey[s.bucket[base + at]] < bs) at++;
@@ -1864,116 +1957,114 @@ __global__ void __launch_bounds__(DML_BLOCK) k_gcmc_run(DevState s, int rep_only
         }
       }
       __syncthreads();
+      if (sh_flag) { fail = sh_flag; break; }
+      pid_next++;
+      seq_next++;
+      nat++;
+      b_nat++;
+      rlen++;
       if (listed) {
-        // ngroup_sort_atom (Neighbor.F90:271-318): own row by the cell
-        // search of ngroup_cells_atom (strict '<') or the Verlet scan
-        int32_t *row = s.nbr + (int64_t)i * maxnn;
-        int nn = 0;
-        const int nst = P.use_cells ? 27 : 1;
-        for (int k = 0; k < nst; k++) {
-          int cc = 0;
-          if (P.use_cells) {
-            int n0 = cell_wrap(ci0 + kMap[k][0], P.ncells[0]);
-            int n1 = cell_wrap(ci1 + kMap[k][1], P.ncells[1]);
-            int n2 = cell_wrap(ci2 + kMap[k][2], P.ncells[2]);
-            cc = cell_lin(P, n0, n1, n2);
-          }
-          const int64_t base = P.boff + (int64_t)cc * P.bcap;
-          int cnt = s.cell_cnt[P.coff + cc];
-          if (cnt > P.bcap) cnt = P.bcap;
-          for (int q0 = 0; q0 < cnt; q0 += DML_BLOCK) {
-            int q = q0 + tid, hitq = 0, j = -1;
-            if (q < cnt) {
-              j = s.bucket[base + q];
-              if (j != i) {
-                double dx, dy, dz;
-                vdist_mic(P, s.px[j], s.py[j], s.pz[j], rx, ry, rz, &dx, &dy, &dz);
-                double rd = dot3(dx, dy, dz);
-                hitq = P.use_cells ? (rd < rl2) : !(rd > rl2);
-              }
-            }
-            int tot;
-            int pos = blk_scan01(hitq, sh, &tot);
-            if (hitq && nn + pos < maxnn) row[nn + pos] = j;
-            nn += tot;
-          }
-        }
-        if (nn > maxnn) { fail = ERR_LIST; break; }
         const int ghost = P.use_cells && (ci2 == 0 || ci2 == P.ncells[2] + 1 || ci0 == 0 ||
                                           ci0 == P.ncells[0] + 1 || ci1 == 0 ||
                                           ci1 == P.ncells[1] + 1);
-        if (tid == 0) {
-          s.nn[i] = nn;
-          s.rowflag[i] = (uint8_t)ghost;
-          if (ghost) Pw->any_ghost = 1;
-        }
-        // and i joins the row of every ref atom within the list radius
-        int full = 0;
-        for (int q = tid; q < nref; q += DML_BLOCK) {
-          int aj = s.ref_order[poff + q];
-          if (aj == i) continue;
-          double dx, dy, dz;
-          vdist_mic(P, rx, ry, rz, s.px[aj], s.py[aj], s.pz[aj], &dx, &dy, &dz);
-          if (dot3(dx, dy, dz) > rl2) continue;
-          int m = s.nn[aj];
-          if (m < maxnn) {
-            s.nbr[(int64_t)aj * maxnn + m] = i;
-            s.nn[aj] = m + 1;
-          } else {
-            full = 1;
+        if (wv == 0) {
+          // ngroup_sort_atom (Neighbor.F90:271-318): the own row in the
+          // order of ngroup_cells_atom (stencil, chain; strict '<') or the
+          // Verlet scan ('>' rejects)
+          int32_t *row = s.nbr + (int64_t)i * maxnn;
+          int nn = 0;
+          const int nst = P.use_cells ? 27 : 1;
+          for (int k = 0; k < nst; k++) {
+            int cc = 0;
+            if (P.use_cells) {
+              int n0 = cell_wrap(ci0 + kMap[k][0], P.ncells[0]);
+              int n1 = cell_wrap(ci1 + kMap[k][1], P.ncells[1]);
+              int n2 = cell_wrap(ci2 + kMap[k][2], P.ncells[2]);
+              cc = cell_lin(P, n0, n1, n2);
+            }
+            const int64_t base = P.boff + (int64_t)cc * P.bcap;
+            const int cnt = min(s.cell_cnt[P.coff + cc], P.bcap);
+            for (int q0 = 0; q0 < cnt; q0 += WAVE) {
+              const int q = q0 + lane;
+              int hitq = 0, j = -1;
+              if (q < cnt) {
+                j = s.bucket[base + q];
+                if (j != i) {
+                  double dx, dy, dz;
+                  vdist_mic(P, s.px[j], s.py[j], s.pz[j], rx, ry, rz, &dx, &dy, &dz);
+                  const double rd = dot3(dx, dy, dz);
+                  hitq = P.use_cells ? (rd < rl2) : !(rd > rl2);
+                }
+              }
+              const unsigned long long m = __ballot(hitq);
+              const int pr = __popcll(m & ((1ull << lane) - 1ull));
+              if (hitq && nn + pr < maxnn) row[nn + pr] = j;
+              nn += __popcll(m);
+            }
+          }
+          if (lane == 0) {
+            if (nn > maxnn) sh_full = 1;
+            s.nn[i] = min(nn, maxnn);
+            s.rowflag[i] = (uint8_t)ghost;
+            if (ghost) Pw->any_ghost = 1;
+          }
+        } else {
+          // and i joins the row of every ref atom within the list radius
+          // (all of them lie in a neighbouring cell)
+          const int nw3 = GC_BLOCK / WAVE - 1;
+          for (int k = wv - 1; k < ncell; k += nw3) {
+            const int cc = sh_cells[k];
+            const int64_t base = P.boff + (int64_t)cc * P.bcap;
+            const int cnt = min(s.cell_cnt[P.coff + cc], P.bcap);
+            for (int q = lane; q < cnt; q += WAVE) {
+              const int aj = s.bucket[base + q];
+              if (aj == i || !(s.flags[aj] & DML_FL_REF)) continue;
+              double dx, dy, dz;
+              vdist_mic(P, rx, ry, rz, s.px[aj], s.py[aj], s.pz[aj], &dx, &dy, &dz);
+              if (dot3(dx, dy, dz) > rl2) continue;
+              const int m = s.nn[aj];
+              if (m < maxnn) {
+                s.nbr[(int64_t)aj * maxnn + m] = i;
+                s.nn[aj] = m + 1;
+              } else {
+                sh_full = 1;
+              }
+            }
           }
         }
-        if (blk_sum(full, sh)) { fail = ERR_LIST; break; }
+        __syncthreads();
+        if (sh_full) { fail = ERR_LIST; break; }
       }
-      if (tid == 0) gc[ngc] = i;
-      ngc++;
+      if (tid == 0) {
+        gc[glen] = i;
+        invl[ilen] = i;  // inside the control volume by construction
+      }
+      tail = glen;
+      glen++;
+      ilen++;
       n_ins++;
     } else {
       // ---- destruction attempt (dana.F90:680-708)
       if ((double)n / (vol * act) < gc_unif(P, u, t, 1, call)) continue;
       int m = (int)floor(gc_unif(P, u, t, 2, call) * n) + 1;
       if (m > n) m = n;
-      int found = -1, running = 0;
-      for (int q0 = 0; q0 < ngc && found < 0; q0 += DML_BLOCK) {
-        int q = q0 + tid;
-        int inv = 0;
-        if (q < ngc) {
-          double z = s.pz[gc[q]];
-          inv = !(z < z0 || z > zmax);
-        }
-        int tot;
-        int pos = blk_scan01(inv, sh, &tot);
-        if (running + tot >= m) {
-          if (inv && running + pos + 1 == m) sh_v[0] = q;
-          __syncthreads();
-          found = sh_v[0];
-          __syncthreads();
-        }
-        running += tot;
+      // the m-th in-volume atom in list order: position m-1 among the
+      // entries not deleted in this call (sorted tombstones)
+      if (idel == GC_TRACK) {  // many deletions: drop the tombstones
+        __syncthreads();
+        ilen = blk_compact(invl, ilen, tmp, sh, [&](int, int v) { return v >= 0; });
+        for (int q = tid; q < ilen; q += GC_BLOCK) invl[q] = tmp[q];
+        idel = 0;
+        __syncthreads();
       }
-      if (found < 0) { fail = ERR_NOPART; break; }
-      const int o = gc[found];
+      int p = m - 1;
+      for (int q = 0; q < idel; q++) {
+        if (sh_tomb[q] <= p) p++;
+        else break;
+      }
+      if (p < 0 || p >= ilen) { fail = ERR_NOPART; break; }
+      const int o = invl[p];
       n--;
-      // LIFO detach: gcmc, then hs (ref, b), then sys (Groups.F90:433-467)
-      for (int q0 = found; q0 < ngc - 1; q0 += DML_BLOCK) {
-        int q = q0 + tid;
-        int v = q + 1 < ngc ? gc[q + 1] : -1;
-        __syncthreads();
-        if (q < ngc - 1) gc[q] = v;
-      }
-      ngc--;
-      const int rp = s.ref_pos[o];
-      for (int q0 = rp; q0 < nref - 1; q0 += DML_BLOCK) {
-        int q = q0 + tid;
-        int v = q + 1 < nref ? s.ref_order[poff + q + 1] : -1;
-        __syncthreads();
-        if (q < nref - 1) {
-          s.ref_order[poff + q] = v;
-          s.ref_pos[v] = q;
-        }
-      }
-      nref--;
-      __syncthreads();
       // cgroup_unsort_atom (Cells.F90:304-352)
       int cell = 0;
       if (P.use_cells) {
@@ -1984,7 +2075,27 @@ __global__ void __launch_bounds__(DML_BLOCK) k_gcmc_run(DevState s, int rep_only
             c2 > P.ncells[2]) { fail = ERR_UNSUP; break; }
         cell = cell_lin(P, c0, c1, c2);
       }
+      const int bso = s.bkey[o];
       if (tid == 0) {
+        invl[p] = -1;
+        {
+          int q = idel;  // keep the tombstones sorted
+          while (q > 0 && sh_tomb[q - 1] > p) {
+            sh_tomb[q] = sh_tomb[q - 1];
+            q--;
+          }
+          sh_tomb[q] = p;
+        }
+        // LIFO detach: gcmc, then hs (ref, b), then sys (Groups.F90:433-467).
+        // The gcmc entry of a destroyed atom is dead by its flags; outside the
+        // limbo its slot may come back in this call, so mark the entry too
+        if (!listed) {
+          int gq = tail;
+          while (gq >= head && gc[gq] != o) gq--;
+          if (gq >= head) gc[gq] = -1;
+        }
+        rord[s.ref_pos[o]] = -1;
+        s.ref_pos[o] = -1;
         const int64_t base = P.boff + (int64_t)cell * P.bcap;
         int cnt = min(s.cell_cnt[P.coff + cell], P.bcap);
         int at = -1;
@@ -2000,30 +2111,60 @@ __global__ void __launch_bounds__(DML_BLOCK) k_gcmc_run(DevState s, int rep_only
           }
           s.cell_cnt[P.coff + cell] = cnt - 1;
         }
-        sh_v[1] = at < 0;  // 'Unsort cgroup particle fail' (Cells.F90:350)
-        s.b2slot[poff + s.bkey[o]] = -1;
+        sh_flag = at < 0;  // 'Unsort cgroup particle fail' (Cells.F90:350)
+        s.b2slot[poff + bso] = -1;
         if (listed) s.nn[o] = 0;
         // a destroyed atom keeps its skip value (atom_destroy, Groups.F90:433-467)
         s.flags[o] = (uint8_t)((listed ? DML_FL_LIMBO : 0) | (s.flags[o] & DML_FL_SKIP));
-        s.ref_pos[o] = -1;
+        // the freed hs%b index, kept sorted (b_scan may have passed it)
+        if (nfreed_b < GC_TRACK) {
+          int q = nfreed_b;
+          while (q > 0 && sh_freed_b[q - 1] > bso) {
+            sh_freed_b[q] = sh_freed_b[q - 1];
+            q--;
+          }
+          sh_freed_b[q] = bso;
+        }
       }
+      __syncthreads();
+      if (sh_flag) { fail = ERR_UNSUP; break; }
+      if (nfreed_b < GC_TRACK) {
+        nfreed_b++;
+      } else {  // list full: scan the whole index range again
+        nfreed_b = 0;
+        b_scan = 0;
+      }
+      // a slot freed outside the limbo is free again for this call's scans
+      if (!listed) hs_scan = min(hs_scan, o - poff);
+      while (tail >= head && (gc[tail] < 0 || !(s.flags[gc[tail]] & DML_FL_GCMC))) tail--;
       b_nat--;
       nat--;
       if (listed) nlimbo++;
       n_del++;
-      __syncthreads();
-      if (sh_v[1]) { fail = ERR_UNSUP; break; }
+      idel++;
     }
   }
   __syncthreads();
+  // compact the gcmc and ref lists (stable), fix ref_pos
+  const int gnew = blk_compact(gc, glen, tmp, sh,
+                               [&](int, int v) { return v >= 0 && (s.flags[v] & DML_FL_GCMC); });
+  for (int q = tid; q < gnew; q += GC_BLOCK) gc[q] = tmp[q];
+  __syncthreads();  // tmp is the scratch of the next compaction
+  const int rnew = blk_compact(rord, rlen, tmp, sh, [&](int, int v) { return v >= 0; });
+  for (int q = tid; q < rnew; q += GC_BLOCK) {
+    const int v = tmp[q];
+    rord[q] = v;
+    s.ref_pos[v] = q;
+  }
+  __syncthreads();
   if (tid == 0) {
-    Pw->ngc = ngc;
+    Pw->ngc = gnew;
     Pw->nslots = nslots;
     Pw->nat = nat;
     Pw->nlimbo = nlimbo;
     Pw->b_amax = b_amax;
     Pw->b_nat = b_nat;
-    Pw->nref = nref;
+    Pw->nref = rnew;
     Pw->pid_next = pid_next;
     Pw->seq_next = seq_next;
     Pw->gcmc_calls = call + 1;
