@@ -236,6 +236,57 @@ def serving_leg(dml, duration, rate=20.0, step_scale=0.01, seed=104012 + 4):
     return out
 
 
+def c5_leg(dml, steps, warmup, trials=4096):
+    """C5: the 2M-particle dendrite-regime system (12-layer electrode with 30
+    hemispherical protrusions, ions at 0.01 A^-3), Ermak with FP32 pair
+    forces, the GCMC reservoir in the top quarter; every step also evaluates
+    4,096 insertion and 4,096 removal trial energies in one batched kernel
+    (FP64 sums).  Wall-clock timed (the trial batches go through the host)."""
+    import numpy as np
+    from dml import workloads as W
+    t0 = time.perf_counter()
+    sysd = W.system_c5()
+    log("built C5 (%d particles) in %.1f s" % (len(sysd["pos"]), time.perf_counter() - t0))
+    box = sysd["box"]
+    z0 = 0.75 * box[2]
+    n = len(sysd["pos"])
+    cfg = dml.dana_config(box, sysd["skin"], n + 65536, h=sysd["h"], z0=z0, zmax=box[2],
+                          integrator=dml.ERMAK, reservoir=dml.GCMC, precision=dml.FP32_FORCE,
+                          overlap_mode=dml.OVERLAP_ROUNDS, act=0.01, nadj=10, seed=104017)
+    c = dml.Context(cfg)
+    c.set_state(sysd["pos"], species=sysd["species"])
+    rng = np.random.default_rng(104017)
+
+    def trial_batch(gc):
+        ins = np.column_stack([rng.random(trials) * box[0], rng.random(trials) * box[1],
+                               z0 + rng.random(trials) * (box[2] - z0)])
+        rem = rng.choice(gc, trials).astype(np.int32)
+        a = c.gcmc_trial_energy(ins)
+        b = c.gcmc_trial_energy(None, exclude=rem)
+        return a, b
+
+    c.run(warmup, graph=True)
+    gc = c.gcmc_order()
+    c.sync()
+    t0 = time.perf_counter()
+    hard_ins = 0
+    for k in range(steps):
+        c.run(1, graph=True)
+        if k % 100 == 99:
+            gc = c.gcmc_order()
+        (du, hard), (dr, hr) = trial_batch(gc)
+        hard_ins += int((hard == 1).sum())
+    c.sync()
+    wall = time.perf_counter() - t0
+    cnt = c.counters()
+    st = c.replica_status(0)
+    ps = cnt["particle_steps"]
+    c.close()
+    return dict(n=n, steps=steps, wall=wall, rate=ps / wall if wall > 0 else None,
+                trials_per_s=2 * trials * steps / wall, hard_core_fraction=hard_ins / (trials * steps),
+                status=st, gcmc_accepted=cnt["ins"] + cnt["del_"])
+
+
 def cpu_baseline():
     """The CPU oracle (a bit-exact restatement of the reference) on the
     reference's own Ermak test: 1,204 particles, 10^4 steps, piston."""
@@ -270,7 +321,9 @@ def main():
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--replicas", type=int, default=256)
-    ap.add_argument("--legs", default="plain,batched,serving")
+    ap.add_argument("--legs", default="plain,batched,serving,c5")
+    ap.add_argument("--c5-steps", type=int, default=200,
+                    help="steps of the C5 leg (its configuration runs 2e3)")
     ap.add_argument("--serve-seconds", type=float, default=30.0,
                     help="arrival window of the serving leg")
     ap.add_argument("--no-cpu", action="store_true")
@@ -311,9 +364,17 @@ def main():
         serving = serving_leg(dml, args.serve_seconds)
         sync()
         log("serving", json.dumps(serving))
+    c5 = None
+    if "c5" in legs:
+        log("c5 leg: 2M dendrite-regime system, %d steps + 8192 trial energies per step" %
+            args.c5_steps)
+        sync()
+        c5 = c5_leg(dml, args.c5_steps, 5)
+        sync()
+        log("c5", json.dumps(c5))
     head = batched or plain
-    if head is None:  # serving only
-        print(json.dumps({"serving": serving}), flush=True)
+    if head is None:  # serving / c5 only
+        print(json.dumps({"serving": serving, "c5": c5}), flush=True)
         return
     res = {
         "metric": "particle-steps/s batched vs plain at force err<=1e-5 vs FP64; p99 run latency",
@@ -338,6 +399,8 @@ def main():
     if plain and batched:
         res["batched_over_plain"] = batched["rate"] / plain["rate"]
     res["p99_run_latency_s"] = serving["p99_latency_s"] if serving else None
+    if c5:
+        res["c5"] = c5
     if serving:
         res["serving"] = serving
     if batched:

@@ -1339,34 +1339,55 @@ int dml_get_slots(dml_ctx *c, int32_t rep, int32_t *nslots, int32_t *bkey, uint3
   return DML_OK;
 }
 
-int dml_gcmc_trial_energy(dml_ctx *c, int32_t rep, int32_t ntrial, const double *pos3, double *du,
-                          int32_t *hard) {
-  if (!c || rep < 0 || rep >= c->nrep || ntrial < 0 || !pos3 || !du || !hard) return DML_E_ARG;
+int dml_gcmc_trial_energy(dml_ctx *c, int32_t rep, int32_t ntrial, const double *pos3,
+                          const int32_t *exclude, double *du, int32_t *hard) {
+  if (!c || rep < 0 || rep >= c->nrep || ntrial < 0 || (!pos3 && !exclude) || !du || !hard)
+    return DML_E_ARG;
   if (ntrial == 0) return DML_OK;
-  {
-    const RepParams &p = c->reps[rep].p;
-    for (int t = 0; t < ntrial; t++) {
+  if (int rc0 = ensure_cells(c)) return rc0;
+  int rc = fetch_params(c);
+  if (rc) return rc;
+  const RepParams &p = c->reps[rep].p;
+  for (int t = 0; t < ntrial; t++) {
+    if (pos3) {
       const double *q = pos3 + 3 * (int64_t)t;
       if (!(q[0] >= 0 && q[0] < p.box[0] && q[1] >= 0 && q[1] < p.box[1] && q[2] >= 0 &&
             q[2] <= p.box[2]))
         return DML_E_ARG;
     }
+    if (exclude && (exclude[t] < (pos3 ? -1 : 0) || exclude[t] >= p.nslots)) return DML_E_ARG;
   }
-  double *dp = upload_tmp(c, pos3, 3 * (int64_t)ntrial);
-  double *de = nullptr;
-  int32_t *dh = nullptr;
-  if (dalloc(&de, ntrial) || dalloc(&dh, ntrial)) return DML_E_NOMEM;
-  int64_t threads = (int64_t)ntrial * 64;
-  hipLaunchKernelGGL(k_gcmc_energy, dim3((unsigned)((threads + DML_BLOCK - 1) / DML_BLOCK)),
-                     dim3(DML_BLOCK), 0, c->st, c->d, rep, ntrial, (const double *)dp, de, dh);
-  int rc = finish(c);
+  std::vector<int32_t> ex;
+  if (exclude) {
+    ex.resize(ntrial);
+    for (int t = 0; t < ntrial; t++) ex[t] = exclude[t] < 0 ? -1 : p.poff + exclude[t];
+  }
+  double *dp = nullptr, *de = nullptr;
+  int32_t *dx = nullptr, *dh = nullptr;
+  rc = DML_E_NOMEM;
+  if ((!pos3 || !dalloc(&dp, 3 * (int64_t)ntrial)) && !dalloc(&de, ntrial) &&
+      !dalloc(&dh, ntrial) && (!exclude || !dalloc(&dx, ntrial))) {
+    rc = DML_OK;
+    if ((pos3 && hipMemcpyAsync(dp, pos3, sizeof(double) * 3 * (size_t)ntrial,
+                                hipMemcpyHostToDevice, c->st) != hipSuccess) ||
+        (exclude && hipMemcpyAsync(dx, ex.data(), sizeof(int32_t) * (size_t)ntrial,
+                                   hipMemcpyHostToDevice, c->st) != hipSuccess))
+      rc = DML_E_HIP;
+  }
   if (!rc) {
-    HIPCHECK(hipMemcpy(du, de, sizeof(double) * ntrial, hipMemcpyDeviceToHost));
-    HIPCHECK(hipMemcpy(hard, dh, sizeof(int32_t) * ntrial, hipMemcpyDeviceToHost));
+    int64_t threads = (int64_t)ntrial * 64;
+    hipLaunchKernelGGL(k_gcmc_energy, dim3((unsigned)((threads + DML_BLOCK - 1) / DML_BLOCK)),
+                       dim3(DML_BLOCK), 0, c->st, c->d, rep, ntrial, (const double *)dp,
+                       (const int32_t *)dx, de, dh);
+    rc = finish(c);
   }
+  if (!rc && (hipMemcpy(du, de, sizeof(double) * ntrial, hipMemcpyDeviceToHost) != hipSuccess ||
+              hipMemcpy(hard, dh, sizeof(int32_t) * ntrial, hipMemcpyDeviceToHost) != hipSuccess))
+    rc = DML_E_HIP;
   hipFree(dp);
   hipFree(de);
   hipFree(dh);
+  hipFree(dx);
   return rc;
 }
 

@@ -43,5 +43,6 @@ __global__ void k_listbuild_wc(DevState s);
 __global__ void k_step_end(DevState s);
 __global__ void k_gcmc_run(DevState s, int rep_only, const double *u, const double *g,
                            double act_in, int nadj_in);
-__global__ void k_gcmc_energy(DevState s, int r, int ntrial, const double *pos3, double *du,
+__global__ void k_gcmc_energy(DevState s, int r, int ntrial, const double *pos3,
+                              const int32_t *excl, double *du,
                               int32_t *hard);

@@ -1588,55 +1588,96 @@ __global__ void k_mask_listed(DevState s, int on) {
 // ------------------------------------------------------------------------
 // GCMC (dana.F90:590-713)
 // ------------------------------------------------------------------------
-// batched insertion-trial energies: one wave per trial over the 27 cells
-// around the trial point (cells as sorted by the last test_update)
+// the cells within one cell step of (c0, c1, c2): x, y periodic, z not (its
+// ghost layers included), each cell once (ncells of 1 or 2 repeat a cell)
+__device__ inline int near_cells(const RepParams &P, int c0, int c1, int c2, int *out) {
+  int xs[3], ys[3], zs[3], nx = 0, ny = 0, nz = 0;
+  for (int d = -1; d <= 1; d++) {
+    int x = cell_wrap(c0 + d, P.ncells[0]);
+    bool dup = false;
+    for (int q = 0; q < nx; q++) dup |= xs[q] == x;
+    if (!dup) xs[nx++] = x;
+    int y = cell_wrap(c1 + d, P.ncells[1]);
+    dup = false;
+    for (int q = 0; q < ny; q++) dup |= ys[q] == y;
+    if (!dup) ys[ny++] = y;
+    int z = c2 + d;
+    if (z >= 0 && z <= P.ncells[2] + 1) zs[nz++] = z;
+  }
+  int n = 0;
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) out[n++] = cell_lin(P, xs[i], ys[j], zs[k]);
+  return n;
+}
+
+// batched trial energies (insertion at a point, or removal of an atom: the
+// same sum without the atom itself): one wave per trial over the cells
+// around the point.  The cells are those of the last sort; every atom has
+// moved less than the skin since, so one cell step (>= rcut + skin) still
+// covers the pair cutoff.
 __global__ void __launch_bounds__(DML_BLOCK) k_gcmc_energy(DevState s, int r, int ntrial,
-                                                            const double *pos3, double *du,
+                                                            const double *pos3,
+                                                            const int32_t *excl, double *du,
                                                             int32_t *hard) {
   int w = (blockIdx.x * DML_BLOCK + threadIdx.x) / WAVE, lane = threadIdx.x % WAVE;
   if (w >= ntrial) return;
   const RepParams &P = s.rp[r];
   if (P.halted) return;
-  double x = pos3[3 * w], y = pos3[3 * w + 1], z = pos3[3 * w + 2];
-  double e = 0.0;
-  int h = 0;
-  int nstencil = P.use_cells ? 27 : 1;
-  int c0 = (int)(x / P.cell[0]) + 1, c1 = (int)(y / P.cell[1]) + 1, c2 = (int)(z / P.cell[2]) + 1;
-  if (P.use_cells && (c0 < 0 || c1 < 0 || c2 < 0 || c0 > P.ncells[0] + 1 ||
-                      c1 > P.ncells[1] + 1 || c2 > P.ncells[2] + 1)) {
+  const int skip = excl ? excl[w] : -1;  // the atom itself, for a removal
+  if (!pos3 && (skip < 0 || !(s.flags[skip] & DML_FL_ALIVE))) {  // no such atom now
     if (lane == 0) {
       du[w] = 0.0;
       hard[w] = -1;
     }
     return;
   }
-  for (int k = 0; k < nstencil; k++) {
-    int cell = 0;
-    if (P.use_cells) {
-      int n0 = cell_wrap(c0 + kMap[k][0], P.ncells[0]);
-      int n1 = cell_wrap(c1 + kMap[k][1], P.ncells[1]);
-      int n2 = cell_wrap(c2 + kMap[k][2], P.ncells[2]);
-      cell = cell_lin(P, n0, n1, n2);
+  // the trial point, or the removed atom's own position
+  const double x = pos3 ? pos3[3 * w] : s.px[skip];
+  const double y = pos3 ? pos3[3 * w + 1] : s.py[skip];
+  const double z = pos3 ? pos3[3 * w + 2] : s.pz[skip];
+  double e = 0.0;
+  int h = 0;
+  int cells[27];
+  int ncell = 1;
+  cells[0] = 0;
+  if (P.use_cells) {
+    const int c0 = (int)(x / P.cell[0]) + 1, c1 = (int)(y / P.cell[1]) + 1;
+    const int c2 = (int)(z / P.cell[2]) + 1;
+    if (c0 < 0 || c1 < 0 || c2 < 0 || c0 > P.ncells[0] + 1 || c1 > P.ncells[1] + 1 ||
+        c2 > P.ncells[2] + 1) {
+      if (lane == 0) {
+        du[w] = 0.0;
+        hard[w] = -1;
+      }
+      return;
     }
+    // every atom within the pair cutoff lies in a neighbouring cell (x, y
+    // periodic, z with its ghost layers), each cell once
+    ncell = near_cells(P, c0, c1, c2, cells);
+  }
+  for (int k = 0; k < ncell; k++) {
+    const int cell = cells[k];
     int cnt = s.cell_cnt[P.coff + cell];
     if (cnt > P.bcap) cnt = P.bcap;
-    int64_t base = P.boff + (int64_t)cell * P.bcap;
+    const int64_t base = P.boff + (int64_t)cell * P.bcap;
     for (int q = lane; q < cnt; q += WAVE) {
-      int j = s.bucket[base + q];
+      const int j = s.bucket[base + q];
+      if (j == skip) continue;
       double dx, dy, dz;
       vdist_mic(P, x, y, z, s.px[j], s.py[j], s.pz[j], &dx, &dy, &dz);
-      double dr2 = dot3(dx, dy, dz);
+      const double dr2 = dot3(dx, dy, dz);
       if ((s.flags[j] & DML_FL_GCMC) && dr2 < P.rcut * P.rcut) h = 1;
-      int m = s.species[j];
-      double r0 = P.r0[m - 1], eps = P.eps[m - 1];
+      const int m = s.species[j];
+      const double r0 = P.r0[m - 1], eps = P.eps[m - 1];  // pair (Li, m)
       if (dr2 > r0 * r0) continue;
-      double dr = sqrt(dr2);
-      double b = pow6d(r0) / pow6d(dr);
+      const double dr = sqrt(dr2);
+      const double b = pow6d(r0) / pow6d(dr);
       e += eps * b * (b - 2) + eps;
     }
   }
   for (int off = WAVE / 2; off > 0; off >>= 1) e += __shfl_xor(e, off);
-  unsigned long long hm = __ballot(h);
+  const unsigned long long hm = __ballot(h);
   if (lane == 0) {
     du[w] = e;
     hard[w] = hm != 0;
@@ -1739,28 +1780,6 @@ __device__ int blk_compact(const int32_t *src, int len, int32_t *dst, int *sh, F
   return tot;
 }
 
-// the cells within one cell step of (c0, c1, c2): x, y periodic, z not (its
-// ghost layers included), each cell once (ncells of 1 or 2 repeat a cell)
-__device__ inline int near_cells(const RepParams &P, int c0, int c1, int c2, int *out) {
-  int xs[3], ys[3], zs[3], nx = 0, ny = 0, nz = 0;
-  for (int d = -1; d <= 1; d++) {
-    int x = cell_wrap(c0 + d, P.ncells[0]);
-    bool dup = false;
-    for (int q = 0; q < nx; q++) dup |= xs[q] == x;
-    if (!dup) xs[nx++] = x;
-    int y = cell_wrap(c1 + d, P.ncells[1]);
-    dup = false;
-    for (int q = 0; q < ny; q++) dup |= ys[q] == y;
-    if (!dup) ys[ny++] = y;
-    int z = c2 + d;
-    if (z >= 0 && z <= P.ncells[2] + 1) zs[nz++] = z;
-  }
-  int n = 0;
-  for (int k = 0; k < nz; k++)
-    for (int j = 0; j < ny; j++)
-      for (int i = 0; i < nx; i++) out[n++] = cell_lin(P, xs[i], ys[j], zs[k]);
-  return n;
-}
 
 __global__ void __launch_bounds__(GC_BLOCK) k_gcmc_run(DevState s, int rep_only, const double *u,
                                                          const double *g, double act_in,

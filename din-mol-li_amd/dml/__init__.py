@@ -101,7 +101,7 @@ def lib():
         L.dml_overlap_rounds.argtypes = [vp, vp, C.c_int32]
         L.dml_f_to_cg.argtypes = [vp]
         L.dml_gcmc.argtypes = [vp, C.c_int32, C.c_double, C.c_int32, vp, vp]
-        L.dml_gcmc_trial_energy.argtypes = [vp, C.c_int32, C.c_int32, vp, vp, vp]
+        L.dml_gcmc_trial_energy.argtypes = [vp, C.c_int32, C.c_int32, vp, vp, vp, vp]
         L.dml_run.argtypes = [vp, C.c_int32, C.c_int32]
         L.dml_get_counters.argtypes = [vp, C.c_int32, C.POINTER(Counters)]
         L.dml_list_stats.argtypes = [vp, C.c_int32, vp, vp, vp]
@@ -368,11 +368,14 @@ class Context:
     def end_step(self):
         _chk(self.L.dml_end_step(self.h), "dml_end_step")
 
-    def gcmc_trial_energy(self, pos3, rep=0):
-        p = np.ascontiguousarray(pos3, np.float64)
-        t = len(p)
+    def gcmc_trial_energy(self, pos3, rep=0, exclude=None):
+        """trial energies at pos3 (t, 3); exclude (t,) slots left out (-1 none);
+        pos3 None: removals of the exclude atoms at their current positions"""
+        p = None if pos3 is None else np.ascontiguousarray(pos3, np.float64)
+        t = len(p) if p is not None else len(exclude)
+        ex = None if exclude is None else np.ascontiguousarray(exclude, np.int32)
         du = np.zeros(max(t, 1)); hard = np.zeros(max(t, 1), np.int32)
-        _chk(self.L.dml_gcmc_trial_energy(self.h, rep, t, _p(p), _p(du), _p(hard)),
+        _chk(self.L.dml_gcmc_trial_energy(self.h, rep, t, _p(p), _p(ex), _p(du), _p(hard)),
              "dml_gcmc_trial_energy")
         return du[:t], hard[:t]
 

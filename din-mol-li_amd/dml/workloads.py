@@ -185,3 +185,94 @@ def system_chunks(n_total, seed, rho=RHO_CHUNK):
     z = np.ones(len(ions), np.int8)
     return dict(box=box, pos=ions, species=z, skin=10.0, h=1e-2, z0=z0, zmax=zmax, z1=z1,
                 dist=dist, chunk=chunk)
+
+
+def _slab_job(args):
+    seed, k, n, lo, hi, box, existing = args
+    return rsa_fill(np.random.default_rng([seed, k]), n, lo, hi, box, existing=existing,
+                    batch=100000)
+
+
+def rsa_fill_slabs(seed, n, lo, hi, box, existing=None, workers=16, slabs=32):
+    """rsa_fill in z slabs: the even slabs in parallel first, then the odd
+    ones against their neighbours (same hard-core rule, points spread over
+    the slabs by volume)."""
+    lo = np.asarray(lo, float); hi = np.asarray(hi, float)
+    slabs = max(2, min(slabs, int((hi[2] - lo[2]) // (4 * DMIN))))
+    edges = np.linspace(lo[2], hi[2], slabs + 1)
+    counts = np.diff(np.round(np.linspace(0, n, slabs + 1)).astype(int))
+    ex = existing if existing is not None else np.zeros((0, 3))
+    out = [None] * slabs
+
+    def jobs(parity):
+        js = []
+        for k in range(parity, slabs, 2):
+            near = [ex]
+            for q in (k - 1, k + 1):
+                if 0 <= q < slabs and out[q] is not None:
+                    near.append(out[q])
+            js.append((seed, k, int(counts[k]), [lo[0], lo[1], edges[k]],
+                       [hi[0], hi[1], edges[k + 1]], box, np.vstack(near)))
+        return js
+
+    for parity in (0, 1):
+        js = jobs(parity)
+        if workers > 1:
+            import multiprocessing as mp
+            with mp.get_context("fork").Pool(min(workers, len(js))) as pool:
+                res = pool.map(_slab_job, js)
+        else:
+            res = [_slab_job(j) for j in js]
+        for j, r in zip(js, res):
+            out[j[1]] = r
+    return np.vstack(out)
+
+
+def system_c5(n_total=2_000_000, seed=104012 + 5, L=600.0, n_prot=30, workers=16):
+    """The dendrite-regime system: a 12-layer bcc(100) electrode (a = 3.51,
+    species CG) on L x L, seeded hemispherical bcc protrusions of radius
+    10-40 A on its top surface, and Li+ ions by RSA at 0.01 A^-3 above it,
+    n_total particles in all.  Box L x L x (electrode + ion slab + 20 A)."""
+    rng = np.random.default_rng(seed)
+    a = A_BCC
+    slab = bcc_layers(L, L, 12, 0.0, rng=rng)
+    ztop = slab[:, 2].max()
+    nx = int(L // a)
+    prot = []
+    for _ in range(n_prot):
+        r = rng.uniform(10.0, 40.0)
+        cx, cy = rng.uniform(0, L, 2)
+        # lattice sites of the two bcc sublattices above the slab inside the sphere
+        for off in (0.0, 0.5):
+            i0 = np.arange(int((cx - r) // a) - 1, int((cx + r) // a) + 2)
+            j0 = np.arange(int((cy - r) // a) - 1, int((cy + r) // a) + 2)
+            k0 = np.arange(1, int(r // (a / 2)) + 2)
+            ii, jj, kk = np.meshgrid(i0, j0, k0, indexing="ij")
+            x = (ii + off) * a
+            y = (jj + off) * a
+            z = ztop + kk * a  # whole cells above the top layer
+            if off:
+                z = z - a / 2
+            p = np.column_stack([x.ravel(), y.ravel(), z.ravel()])
+            d = p - np.array([cx, cy, ztop])
+            d[:, :2] -= L * np.round(d[:, :2] / L)
+            p = p[(d * d).sum(1) <= r * r]
+            p[:, :2] %= L
+            prot.append(p)
+    prot = np.vstack(prot)
+    # one atom per lattice site
+    key = np.round(prot / (a / 2)).astype(np.int64)
+    _, first = np.unique(key[:, 0] * 10**12 + key[:, 1] * 10**6 + key[:, 2], return_index=True)
+    prot = prot[np.sort(first)]
+    prot[:, :2] = np.where(prot[:, :2] >= L - 1e-9, 0.0, prot[:, :2])
+    el = np.vstack([slab, prot])
+    etop = el[:, 2].max()
+    n_ion = n_total - len(el)
+    hgt = n_ion / (RHO_ION * L * L)
+    box = np.array([L, L, ztop + DMIN + hgt + 20.0])
+    ions = rsa_fill_slabs(seed, n_ion, [0, 0, ztop + DMIN], [L, L, ztop + DMIN + hgt], box,
+                          existing=el, workers=workers)
+    pos = np.vstack([el, ions])
+    z = np.array([2] * len(el) + [1] * len(ions), np.int8)
+    return dict(name="C5", box=box, pos=pos, species=z, skin=10.0, h=1e-2, z0=box[2],
+                zmax=box[2], electrode_top=etop, n_electrode=len(el))

@@ -202,11 +202,16 @@ int dml_get_gcmc_order(dml_ctx *ctx, int32_t rep, int32_t *ngc, int32_t *order);
  * hs%amax of igroup_attach_atom (Groups.F90:1083-1093).  Arrays may be NULL. */
 int dml_get_slots(dml_ctx *ctx, int32_t rep, int32_t *nslots, int32_t *bkey,
                   uint32_t *pid, int32_t *seq);
-/* batched trial energies: WCA sum (FP64) of an inserted Li at pos3[t] against
- * every atom of its 27 cells, and the hard-core indicator of
- * dana.F90:638-653 (gcmc atom closer than rcut). */
+/* batched trial energies: the WCA pair sum (FP64, fuerza's pair law,
+ * dana.F90:1110-1131) of a Li at pos3[t] against every atom within its pair
+ * cutoff, and the hard-core indicator of dana.F90:638-653 (a gcmc atom
+ * closer than rcut).  exclude[t] (may be NULL; -1 = none) leaves one slot
+ * out: the removal energy of atom exclude[t] at its own position.  With
+ * pos3 NULL the points are the current positions of the exclude[t] atoms
+ * (hard[t] = -1, du[t] = 0 for a slot that holds no atom). */
 int dml_gcmc_trial_energy(dml_ctx *ctx, int32_t rep, int32_t ntrial,
-                          const double *pos3, double *du, int32_t *hard);
+                          const double *pos3, const int32_t *exclude, double *du,
+                          int32_t *hard);
 
 /* --- whole steps -------------------------------------------------------
  * nsteps device-resident steps of every replica (the order of the main loop,

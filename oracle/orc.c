@@ -1259,7 +1259,7 @@ int orc_gcmc_order(const orc_sys *s, int32_t *out) {
 }
 
 void orc_gcmc_trial_energy(const orc_sys *s, int ntrial, const double *pos3,
-                           double *du, int32_t *hard) {
+                           const int32_t *exclude, double *du, int32_t *hard) {
   /* Trial energy of an inserted Li at pos3[t]: the reference's acceptance
    * uses only the hard-core test against the gcmc group
    * (dana.F90:638-653, rc = hs%rcut, '<'); the energy reported beside it is
@@ -1271,6 +1271,7 @@ void orc_gcmc_trial_energy(const orc_sys *s, int ntrial, const double *pos3,
     int h = 0;
     for (int j = 0; j < s->amax; j++) {
       if (s->st[j] != ORC_ALIVE) continue;
+      if (exclude && exclude[t] == j) continue;  /* the atom removed */
       double vd[3];
       vdist(s, r, s->pos[j], vd);
       double dr2 = dot3(vd);

@@ -195,7 +195,7 @@ int orc_gcmc_order(const orc_sys *s, int32_t *out);
 /* GCMC insertion trial energy: WCA sum over neighbours within 27 cells and
  * the hard-core indicator of dana.F90:638-653 (over the gcmc group). */
 void orc_gcmc_trial_energy(const orc_sys *s, int ntrial, const double *pos3,
-                           double *du, int32_t *hard);
+                           const int32_t *exclude, double *du, int32_t *hard);
 /* ranv export (ermak) */
 void orc_get_ranv(const orc_sys *s, double *ranv);
 

@@ -317,11 +317,12 @@ class System:
         r, keep = self._arr_rng(gcmc_u=u, gcmc_g=g)
         return self.L.orc_gcmc(self.s, C.byref(r), act, nadj, C.byref(self.c))
 
-    def gcmc_trial_energy(self, pos3):
+    def gcmc_trial_energy(self, pos3, exclude=None):
         pos3 = np.ascontiguousarray(pos3, np.float64)
         t = len(pos3)
+        ex = None if exclude is None else np.ascontiguousarray(exclude, np.int32)
         du = np.zeros(t); hard = np.zeros(t, np.int32)
-        self.L.orc_gcmc_trial_energy(C.c_void_p(self.s), t, _p(pos3), _p(du), _p(hard))
+        self.L.orc_gcmc_trial_energy(C.c_void_p(self.s), t, _p(pos3), _p(ex), _p(du), _p(hard))
         return du, hard
 
 

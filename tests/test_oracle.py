@@ -197,3 +197,33 @@ def test_nr_ran_is_uniform_and_reseeds(orc):
     assert np.array_equal(u[:50], orc.nr_stream(-104012, 50))
     g = orc.nr_stream(-7, 20000, "gauss")
     assert abs(g.mean()) < 0.03 and abs(g.std() - 1) < 0.03
+
+
+def test_gcmc_trial_energy_brute_force(orc):
+    """orc_gcmc_trial_energy against a direct numpy sum of fuerza's pair law
+    (Li at the trial point against every atom; r0, eps of the (Li, m) pair),
+    for insertion points and for removals (the atom itself left out)."""
+    rng = np.random.default_rng(61)
+    box = np.array([40.0, 40.0, 40.0])
+    pos = rng.random((500, 3)) * box
+    z = np.where(rng.random(500) < 0.3, 2, 1).astype(np.int8)
+    z[::17] = 3
+    s = orc.System(orc.dana_params(box, 2.0, ermak=False, piston=False), pos, z)
+    s.gcmc_init()
+    pts = np.vstack([rng.random((40, 3)) * box, pos[:40]])
+    ex = np.concatenate([-np.ones(40, np.int32), np.arange(40, dtype=np.int32)])
+    du, hard = s.gcmc_trial_energy(pts, exclude=ex)
+    d = pts[:, None, :] - pos[None, :, :]
+    d[..., :2] -= box[:2] * np.round(d[..., :2] / box[:2])
+    r2 = (d * d).sum(-1)
+    keep = np.ones_like(r2, bool)
+    keep[np.arange(40, 80), np.arange(40)] = False
+    r0 = np.array([3.2, 3.5, 1.564])[z - 1][None, :]
+    eps = np.array([2313.6, 0.0, 529.1])[z - 1][None, :]
+    inside = (r2 <= r0 ** 2) & keep
+    rr = np.sqrt(np.where(inside, r2, 1.0))
+    b = (r0 / rr) ** 6
+    e = np.where(inside, eps * b * (b - 2) + eps, 0.0).sum(1)
+    np.testing.assert_allclose(du, e, rtol=1e-12, atol=1e-9)
+    hard_o = ((r2 < 3.2 ** 2) & keep & (z == 1)[None, :]).any(1)
+    np.testing.assert_array_equal(hard.astype(bool), hard_o)
