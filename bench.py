@@ -63,7 +63,32 @@ def force_rel_err(f32, f64):
     return float(np.max(na / np.maximum(nb, floor)))
 
 
-def plain_leg(dml, W, steps, warmup):
+def energy_drift(dml, mk, snap, steps, every=100):
+    """<sum epot> of FP32-force vs FP64 Ermak runs from the same state with
+    the same Philox streams: |<E>_32 - <E>_64| / |<E>_64| over the run, and
+    each run's raw slope dE/dt (least squares over samples every `every`
+    steps; energia of salida, computed on the device)."""
+    import numpy as np
+    out = {}
+    for prec in (dml.FP64, dml.FP32_FORCE):
+        c = dml.Context(mk(prec))
+        c.set_state(snap["pos"], species=snap["species"], vel=snap["vel"], acel=snap["acel"])
+        e = []
+        for _ in range(steps // every):
+            c.run(every, graph=True)
+            e.append(c.observables()[0][0])
+        out[prec] = np.array(e)
+        c.close()
+    h = mk(dml.FP64).h
+    t = (np.arange(len(out[dml.FP64])) + 1) * every * h
+    m64, m32 = out[dml.FP64].mean(), out[dml.FP32_FORCE].mean()
+    return dict(steps=steps, samples=len(t), mean_epot_fp64=float(m64), mean_epot_fp32=float(m32),
+                rel=float(abs(m32 - m64) / abs(m64)),
+                slope_fp64_per_ps=float(np.polyfit(t, out[dml.FP64], 1)[0]),
+                slope_fp32_per_ps=float(np.polyfit(t, out[dml.FP32_FORCE], 1)[0]))
+
+
+def plain_leg(dml, W, steps, warmup, drift_steps=10000):
     import numpy as np
     c2 = W.system_c2()
     n = len(c2["pos"])
@@ -97,8 +122,10 @@ def plain_leg(dml, W, steps, warmup):
     f64 = out[dml.FP64][ions]
     hot = np.linalg.norm(f64, axis=1) > 0
     err = force_rel_err(out[dml.FP32_FORCE][ions][hot], f64[hot]) if hot.any() else None
+    drift = energy_drift(dml, mk, snap, drift_steps) if drift_steps else None
     return dict(n=n, ms=ms, wall=wall, rate=n * steps / (ms * 1e-3), mean_nn=mean_nn,
-                rebuilds=reb, steps_done=st, force_err=err, interacting=int(hot.sum()))
+                rebuilds=reb, steps_done=st, force_err=err, interacting=int(hot.sum()),
+                energy_drift=drift)
 
 
 def batched_leg(dml, nrep, steps, warmup):
@@ -322,6 +349,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--replicas", type=int, default=256)
     ap.add_argument("--legs", default="plain,batched,serving,c5")
+    ap.add_argument("--drift-steps", type=int, default=10000,
+                    help="Ermak steps of each FP32/FP64 energy-drift run on C2 (0 = skip)")
     ap.add_argument("--c5-steps", type=int, default=200,
                     help="steps of the C5 leg (its configuration runs 2e3)")
     ap.add_argument("--serve-seconds", type=float, default=30.0,
@@ -348,7 +377,7 @@ def main():
     if "plain" in legs:
         log("plain leg: C2 100k FP64")
         sync()
-        plain = plain_leg(dml, W, args.steps, args.warmup)
+        plain = plain_leg(dml, W, args.steps, args.warmup, args.drift_steps)
         sync()
         log("plain", json.dumps(plain))
     if "batched" in legs:
@@ -396,6 +425,7 @@ def main():
     if plain:
         res["plain_value"] = plain["rate"]
         res["force_rel_err_fp32_vs_fp64"] = plain["force_err"]
+        res["energy_drift_fp32_vs_fp64"] = plain["energy_drift"]
     if plain and batched:
         res["batched_over_plain"] = batched["rate"] / plain["rate"]
     res["p99_run_latency_s"] = serving["p99_latency_s"] if serving else None

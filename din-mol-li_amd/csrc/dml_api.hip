@@ -671,6 +671,7 @@ int dml_create_ensemble(int32_t nrep, const dml_config *cfgs, dml_ctx **out) {
   e |= dalloc(&d.ov_rev, P);
   e |= dalloc(&d.ov_st, P);
   e |= dalloc(&d.part_d, 2 * (P / DML_BLOCK));
+  e |= dalloc(&d.obs, 4 * (P / DML_BLOCK) + 4 * (int64_t)nrep);
   e |= dalloc(&d.part_i, P / DML_BLOCK);
   e |= dalloc(&d.blk2rep, P / DML_BLOCK);
   e |= dalloc(&d.rp, nrep);
@@ -746,7 +747,7 @@ int dml_destroy(dml_ctx *c) {
                   d.ov_st, d.part_d, d.part_i, d.blk2rep, d.rp, d.cnt, d.err, d.wl[0],
                   d.wl[1], d.wl_cnt, d.wl_stamp, d.ov_more, d.pf4, d.bf4, d.gc_order,
                   d.b2slot, d.seq, d.chx, d.chy, d.chz, d.chox, d.choy, d.choz,
-                  d.cm_rep, d.cm_cell, d.tm_rep, d.tm_cell, d.wm_blk};
+                  d.cm_rep, d.cm_cell, d.tm_rep, d.tm_cell, d.wm_blk, d.obs};
   for (void *q : ptrs)
     if (q) hipFree(q);
   if (c->e0) hipEventDestroy(c->e0);
@@ -1200,6 +1201,29 @@ int dml_calc_rho(dml_ctx *c, double *rho) {
     const RepParams &p = c->reps[r].p;
     const double top = p.reservoir == DML_CHUNKS ? p.z1 : p.zmax;
     rho[r] = (double)p.rho_count / (p.box[0] * p.box[1] * (top - p.z0));
+  }
+  return DML_OK;
+}
+
+int dml_observables(dml_ctx *c, double *out) {
+  if (!c || !out) return DML_E_ARG;
+  hipLaunchKernelGGL(k_observe_part, dim3(c->d.nblocks), dim3(DML_BLOCK), 0, c->st, c->d);
+  hipLaunchKernelGGL(k_observe_final, dim3(c->nrep), dim3(DML_BLOCK), 0, c->st, c->d);
+  std::vector<double> o(4 * (size_t)c->nrep);
+  if (hipMemcpyAsync(o.data(), c->d.obs + 4 * (int64_t)c->d.nblocks, o.size() * sizeof(double),
+                     hipMemcpyDeviceToHost, c->st) != hipSuccess)
+    return DML_E_HIP;
+  int rc = finish(c);
+  if (rc) return rc;
+  // kB_ui of dana.F90:22-24, the constant kion divides by
+  const double kB_eVK = 8.617330350e-5, kJm_ui = 100.0, eV_kJm = 96.485;
+  const double kB_ui = kB_eVK * (eV_kJm * kJm_ui);
+  for (int r = 0; r < c->nrep; r++) {
+    const double *q = &o[4 * (size_t)r];
+    out[4 * r + 0] = q[0];
+    out[4 * r + 1] = q[1] / (q[2] * 3.0 * kB_ui);  // kion: vdac/(j*3._dp*kB_ui)
+    out[4 * r + 2] = q[2];
+    out[4 * r + 3] = q[3];
   }
   return DML_OK;
 }

@@ -274,6 +274,7 @@ struct DevState {
   float4 *bf4;            // bucket-ordered
   // reductions
   double *part_d;         // 2 per particle block
+  double *obs;            // 4 per particle block, then 4 per replica
   int32_t *part_i;        // per particle block
   // tables
   int32_t *blk2rep;

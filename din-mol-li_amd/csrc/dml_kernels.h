@@ -34,6 +34,8 @@ __global__ void k_f2cg_scatter(DevState s);
 __global__ void k_f2cg_copy(DevState s);
 __global__ void k_f2cg_end(DevState s);
 __global__ void k_calc_rho(DevState s);
+__global__ void k_observe_part(DevState s);
+__global__ void k_observe_final(DevState s);
 __global__ void k_maxz(DevState s);
 __global__ void k_rep_op(DevState s, int op);
 __global__ void k_mask_listed(DevState s, int on);

@@ -1504,6 +1504,79 @@ __global__ void __launch_bounds__(DML_BLOCK) k_calc_rho(DevState s) {
   if (threadIdx.x == 0 && cnt) atomicAdd(&s.rp[r].rho_count, cnt);
 }
 
+// ------------------------------------------------------------------------
+// salida's observables (dana.F90:1141-1183): energia = sum of epot over sys,
+// and the sum of m |v|^2 over the atoms that are not CG for kion
+// (dana.F90:1342-1376).  Per-block partials, then one block per replica adds
+// them in a fixed order, so the result does not depend on scheduling.
+// ------------------------------------------------------------------------
+__device__ inline double block_sum(double v, double *sh) {
+  for (int off = WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+  __syncthreads();  // earlier readers of sh are done
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int q = 0; q < (int)(blockDim.x / WAVE); q++) t += sh[q];
+  return t;
+}
+
+__global__ void __launch_bounds__(DML_BLOCK) k_observe_part(DevState s) {
+  __shared__ double sh[DML_BLOCK / WAVE];
+  int r;
+  const RepParams &P = rep_of_block(s, blockIdx.x, &r);
+  const int slot = blockIdx.x * DML_BLOCK + threadIdx.x;
+  const int local = slot - P.poff;
+  double e = 0.0, k = 0.0, ni = 0.0, na = 0.0;
+  if (local < P.nslots && (s.flags[slot] & DML_FL_ALIVE)) {
+    e = s.epot[slot];
+    na = 1.0;
+    const int z = s.species[slot];
+    if (z != SP_CG) {
+      const double vx = s.vx[slot], vy = s.vy[slot], vz = s.vz[slot];
+      k = (vx * vx + vy * vy + vz * vz) * P.mass[z - 1];  // dot_product(vel,vel)*mass
+      ni = 1.0;
+    }
+  }
+  e = block_sum(e, sh);
+  k = block_sum(k, sh);
+  ni = block_sum(ni, sh);
+  na = block_sum(na, sh);
+  if (threadIdx.x == 0) {
+    double *o = s.obs + 4 * (int64_t)blockIdx.x;
+    o[0] = e;
+    o[1] = k;
+    o[2] = ni;
+    o[3] = na;
+  }
+}
+
+__global__ void __launch_bounds__(DML_BLOCK) k_observe_final(DevState s) {
+  __shared__ double sh[DML_BLOCK / WAVE];
+  const int r = blockIdx.x;
+  const RepParams &P = s.rp[r];
+  const int b0 = P.poff / DML_BLOCK, nb = P.pcap / DML_BLOCK;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  for (int q = threadIdx.x; q < nb; q += blockDim.x) {
+    const double *o = s.obs + 4 * (int64_t)(b0 + q);
+    a0 += o[0];
+    a1 += o[1];
+    a2 += o[2];
+    a3 += o[3];
+  }
+  a0 = block_sum(a0, sh);
+  a1 = block_sum(a1, sh);
+  a2 = block_sum(a2, sh);
+  a3 = block_sum(a3, sh);
+  if (threadIdx.x == 0) {
+    double *o = s.obs + 4 * (int64_t)s.nblocks + 4 * r;
+    o[0] = a0;
+    o[1] = a1;
+    o[2] = a2;
+    o[3] = a3;
+  }
+}
+
 // the piston factor of maxz from the count of calc_rho (dana.F90:785)
 __device__ inline double piston_lohi(const RepParams &P) {
   double vol = P.box[0] * P.box[1] * (P.zmax - P.z0);

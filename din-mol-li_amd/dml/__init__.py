@@ -34,7 +34,7 @@ EXPORTS = ["dml_create", "dml_create_ensemble", "dml_destroy", "dml_sync", "dml_
            "dml_gcmc_trial_energy", "dml_run", "dml_get_counters", "dml_list_stats",
            "dml_last_run_ms", "dml_time_kernel", "dml_set_tuning",
            "dml_replica_status", "dml_get_gcmc_order", "dml_get_slots",
-           "dml_calc_rho", "dml_maxz", "dml_end_step",
+           "dml_calc_rho", "dml_observables", "dml_maxz", "dml_end_step",
            "dml_get_tuning", "dml_reconfigure", "dml_set_step_limit", "dml_set_active",
            "dml_progress", "dml_set_chunk", "dml_bloques"]
 K_LISTBUILD, K_FORCE, K_CELLS, K_DISPMAX = 0, 1, 2, 3
@@ -349,6 +349,14 @@ class Context:
         rho = np.zeros(self.nrep)
         _chk(self.L.dml_calc_rho(self.h, _p(rho)), "dml_calc_rho")
         return rho
+
+    def observables(self):
+        """salida's per-frame observables of every replica, shape (nrep, 4):
+        energia (sum of epot over sys), the kion temperature, the count of
+        non-CG atoms, the atom count (dana.F90:1141-1183, 1342-1376)"""
+        out = np.zeros((self.nrep, 4))
+        _chk(self.L.dml_observables(self.h, _p(out)), "dml_observables")
+        return out
 
     def maxz(self):
         """maxz (dana.F90:776-794) of the piston replicas"""

@@ -1,22 +1,26 @@
 ! dana_dml -- the main loop of dana (dana.F90:101-263) driving libdml through
 ! the dml_f interfaces: same input files (entrada.ini, movedor.ini,
-! pos_inic.xyz), same call order, same Li.xyz frames, with every hot routine
-! on the GPU and the state resident there between calls.
+! pos_inic.xyz, chunk.xyz for the chunks reservoir), same call order, same
+! output (Li.xyz frames and the E, T, rho, try and depo series of salida),
+! with every hot routine on the GPU and the state resident there between
+! calls.
 !
-! Usage: dana_dml [nst]   (run in a directory holding the three input files;
+! Usage: dana_dml [nst]   (run in a directory holding the input files;
 ! the optional argument overrides nst of entrada.ini)
 program dana_dml
   use, intrinsic :: iso_c_binding
   use dml_f
   implicit none
 
-  integer(c_int32_t) :: idum, nst, nwr, nadj, n, i, k, nslots, rep
-  real(c_double) :: prob, h, xi, yi, dist, z0, zmax, dif_sc, dif_sei, nb_dcut, act
-  logical :: integrador, s_piston, s_gcmc
+  integer(c_int32_t) :: idum, nst, nwr, nadj, n, i, k, nslots, rep, nchunk
+  real(c_double) :: prob, h, xi, yi, dist, z0, zmax, dif_sc, dif_sei, nb_dcut, act, z1, t
+  real(c_double), target :: obs(4)
+  integer(c_int64_t) :: try_last, depo_last
+  logical :: integrador, s_piston, s_gcmc, s_chunk
   character(len=100) :: a
   character(len=10) :: sym
   character(len=32) :: arg
-  real(c_double), allocatable, target :: pos(:,:), rho(:)
+  real(c_double), allocatable, target :: pos(:,:), rho(:), chunk(:,:)
   integer(c_int8_t), allocatable, target :: species(:)
   integer(c_int8_t), allocatable, target :: flags(:)
   integer(c_int32_t), allocatable, target :: seq(:)
@@ -50,6 +54,7 @@ program dana_dml
   ! config_run (dana.F90:399-427)
   s_piston = .false.
   s_gcmc = .false.
+  s_chunk = .false.
   act = 0._c_double
   nadj = 0
   open(15, file='movedor.ini', status='old')
@@ -64,15 +69,20 @@ program dana_dml
     reservoir = DML_RES_GCMC
     read(15,*) act, nadj
   case('chunks')
-    write(0,*) 'dana_dml: the chunks reservoir is not on the device'
-    stop 1
+    s_chunk = .true.
+    reservoir = DML_RES_CHUNKS
   case default
     write(0,*) 'Unknown reservoir type'
     stop 1
   end select
   close(15)
 
-  ! config_inic (dana.F90:430-517)
+  ! config_inic (dana.F90:430-517): the chunks reservoir sizes (437-442)
+  if (s_chunk) then
+    dist = dist + 3.2_c_double
+    z1 = z0 + dist
+    zmax = z1 + dist
+  endif
   open(11, file='pos_inic.xyz', status='old')
   read(11,*) n
   read(11,*)
@@ -90,8 +100,23 @@ program dana_dml
   enddo
   close(11)
 
+  ! config_chunk (dana.F90:552-587) reads chunk.xyz; the template keeps its
+  ! file coordinates here, dml_set_chunk places it at zmax + z
+  nchunk = 0
+  if (s_chunk) then
+    open(11, file='chunk.xyz', status='old')
+    read(11,*) nchunk
+    read(11,*)
+    allocate(chunk(nchunk,3))
+    do i = 1, nchunk
+      read(11,*) sym, chunk(i,:)
+    enddo
+    close(11)
+  endif
+
   capacity = n
   if (s_gcmc) capacity = n + max(1024, 2 * nadj * nst)
+  if (s_chunk) capacity = n + 64 * nchunk
   call dml_dana_config(cfg, xi, yi, zmax, z0, nb_dcut, h, prob, dif_sc, dif_sei, integrador, &
                        reservoir, capacity, int(abs(idum), c_int64_t))
   cfg%act = act
@@ -106,8 +131,17 @@ program dana_dml
   if (integrador) call dml_check(dml_forces(ctx), 'fuerza')
   allocate(rho(1))
   call dml_check(dml_calc_rho(ctx, c_loc(rho)), 'calc_rho')
+  if (s_chunk) call dml_check(dml_set_chunk(ctx, rep, nchunk, c_loc(chunk), dist, z1), 'config_chunk')
 
   open(11, file='Li.xyz')
+  open(12, file='E.dat')
+  open(13, file='T.dat')
+  open(14, file='rho.dat')
+  open(15, file='try.dat')
+  open(16, file='depo.dat')
+  t = 0._c_double
+  try_last = 0
+  depo_last = 0
   call salida()
 
   ! the main loop (dana.F90:172-262)
@@ -125,11 +159,18 @@ program dana_dml
     call dml_check(dml_f_to_cg(ctx), 'F to CG')
     if (s_gcmc) call dml_check(dml_gcmc(ctx, rep, act, nadj, c_null_ptr, c_null_ptr), 'gcmc_run')
     call dml_check(dml_calc_rho(ctx, c_loc(rho)), 'calc_rho')
+    if (s_chunk) call dml_check(dml_bloques(ctx), 'bloques')
     if (mod(i, nwr) == 0) call salida()
     if (s_piston) call dml_check(dml_maxz(ctx), 'maxz')
     call dml_check(dml_end_step(ctx), 'end of step')
+    t = t + h
   enddo
   close(11)
+  close(12)
+  close(13)
+  close(14)
+  close(15)
+  close(16)
 
   call dml_check(dml_get_counters(ctx, rep, cnt), 'counters')
   write(*,'(a,i0,a,i0,a,i0,a,i0,a,i0)') 'steps ', nst, ' nupd ', cnt%nupd, ' depo ', cnt%depo, &
@@ -187,6 +228,21 @@ contains
       end select
       write(11,*) s, p(m,:), int(species(m))
     enddo
+    ! energia and kion on the device; try and depo since the last frame
+    call dml_check(dml_observables(ctx, c_loc(obs)), 'observables')
+    call dml_check(dml_get_counters(ctx, rep, cnt), 'counters')
+    write(12,*) t, obs(1)
+    write(13,*) t, obs(2)
+    write(14,*) t, rho(1)
+    write(15,*) t, cnt%try_ - try_last
+    write(16,*) t, cnt%depo - depo_last
+    try_last = cnt%try_
+    depo_last = cnt%depo
+    flush(16)
+    flush(15)
+    flush(14)
+    flush(13)
+    flush(12)
     flush(11)
   end subroutine salida
 

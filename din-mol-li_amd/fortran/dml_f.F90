@@ -9,6 +9,8 @@
 !   F -> CG (dana.F90:228-236)           -> dml_f_to_cg
 !   gcmc_run (dana.F90:590-713)          -> dml_gcmc
 !   calc_rho / maxz (dana.F90:521-549, 776-794) -> dml_calc_rho / dml_maxz
+!   config_chunk / bloques (dana.F90:552-587, 716-773) -> dml_set_chunk / dml_bloques
+!   salida's energia and kion (dana.F90:1141-1183, 1342-1376) -> dml_observables
 ! A Fortran array real(c_double) :: pos(n,3) is the structure-of-arrays
 ! layout the ABI takes (element (i,k) at [k*n + i]).
 module dml_f
@@ -143,6 +145,21 @@ module dml_f
     integer(c_int) function dml_calc_rho(ctx, rho) bind(C, name='dml_calc_rho')
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx, rho
+    end function
+    integer(c_int) function dml_observables(ctx, out) bind(C, name='dml_observables')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, out
+    end function
+    integer(c_int) function dml_set_chunk(ctx, rep, n, pos, dist, z1) bind(C, name='dml_set_chunk')
+      import :: c_int, c_int32_t, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: rep, n
+      type(c_ptr), value :: pos
+      real(c_double), value :: dist, z1
+    end function
+    integer(c_int) function dml_bloques(ctx) bind(C, name='dml_bloques')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
     end function
     integer(c_int) function dml_maxz(ctx) bind(C, name='dml_maxz')
       import :: c_int, c_ptr

@@ -169,6 +169,15 @@ int dml_f_to_cg(dml_ctx *ctx);
  * chunks replica) per replica, rho[rep] = count / (Lx Ly (top - z0)) (may be
  * NULL); the count stays for dml_maxz and dml_bloques. */
 int dml_calc_rho(dml_ctx *ctx, double *rho);
+
+/* The per-frame observables of salida (dana.F90:1141-1183) for every
+ * replica, out[4*rep + k]: k = 0 energia, the sum of epot over sys
+ * (dana.F90:1150-1158); 1 the kinetic temperature of kion
+ * (dana.F90:1342-1376): sum of m |v|^2 over the atoms that are not CG,
+ * divided by (j * 3 * kB_ui), j their count (NaN when j = 0, as in the
+ * reference); 2 that count j; 3 the atom count of sys.  FP64 sums in a fixed
+ * order (per particle block, then per replica); synchronous. */
+int dml_observables(dml_ctx *ctx, double *out);
 /* maxz (dana.F90:776-794) of the piston replicas, with the count of the last
  * dml_calc_rho */
 int dml_maxz(dml_ctx *ctx);

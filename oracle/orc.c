@@ -1311,6 +1311,32 @@ static double calc_rho(orc_sys *s, int chunks) {
   return (double)g / vol;
 }
 
+void orc_observables(const orc_sys *s, double out[4]) {
+  /* salida (dana.F90:1147-1158): energia = energia + pa%epot over sys%alist */
+  double energia = 0.0, vdac = 0.0;
+  int j = 0;
+  for (int q = 0; q < s->sysl.n; q++) {
+    int i = s->sysl.v[q];
+    energia = energia + s->epot[i];
+  }
+  /* kion (dana.F90:1342-1376): CG skipped, vd = dot(vel,vel)*mass */
+  for (int q = 0; q < s->sysl.n; q++) {
+    int i = s->sysl.v[q];
+    if (s->zs[i] == 2) continue;
+    j = j + 1;
+    double vd = s->vel[i][0] * s->vel[i][0] + s->vel[i][1] * s->vel[i][1] +
+                s->vel[i][2] * s->vel[i][2];
+    vd = vd * s->p.mass[s->zs[i] - 1];
+    vdac = vdac + vd;
+  }
+  const double kB_eVK = 8.617330350e-5, kJm_ui = 100.0, eV_kJm = 96.485;
+  const double kB_ui = kB_eVK * (eV_kJm * kJm_ui); /* dana.F90:22-24 */
+  out[0] = energia;
+  out[1] = vdac / (j * 3.0 * kB_ui);
+  out[2] = j;
+  out[3] = s->sysl.n;
+}
+
 double orc_calc_rho(orc_sys *s, int chunks) { return calc_rho(s, chunks); }
 
 void orc_set_z1(orc_sys *s, double z1) { s->z1 = z1; }

@@ -184,6 +184,9 @@ void orc_chunk_free(orc_chunk *ch);
 int orc_bloques(orc_sys *s, orc_chunk *ch, double rho, int *fired, orc_counters *c);
 /* calc_rho (dana.F90:521-549): chunks = 1 counts up to z1 */
 double orc_calc_rho(orc_sys *s, int chunks);
+/* salida's observables (dana.F90:1141-1183) in sys order: out = energia,
+ * kion temperature (dana.F90:1342-1376), j (non-CG count), sys%nat */
+void orc_observables(const orc_sys *s, double out[4]);
 void orc_set_z1(orc_sys *s, double z1);
 /* z0, z1, zmax, box(3) */
 void orc_get_reservoir(const orc_sys *s, double out[4]);

@@ -288,6 +288,12 @@ class System:
                                 C.byref(fired), C.byref(self.c))
         return st, fired.value
 
+    def observables(self):
+        """energia, kion temperature, non-CG count, sys%nat (salida)"""
+        out = np.zeros(4)
+        self.L.orc_observables(C.c_void_p(self.s), _p(out))
+        return out
+
     def calc_rho(self, chunks=False):
         return self.L.orc_calc_rho(C.c_void_p(self.s), int(chunks))
 

@@ -242,6 +242,11 @@ def test_ermak_step_matches_oracle(orc, dml):
     ref = so["species"] != 2
     assert rel_err(sd["vel"][ref], so["vel"][ref]) <= 1e-12
     assert rel_err(sd["acel"][ref], so["acel"][ref]) <= 1e-12
+    # salida's energia and kion temperature on these states
+    ob_d, ob_o = c.observables()[0], s.observables()
+    assert ob_o[0] > 0
+    np.testing.assert_allclose(ob_d[:2], ob_o[:2], rtol=1e-12)
+    assert ob_d[2] == ob_o[2] and ob_d[3] == ob_o[3]
     c.close()
 
 
@@ -410,6 +415,7 @@ def test_ensemble_replicas_match_single_runs(dml):
     for k, (b, p) in enumerate(systems):
         ens.set_state(p, rep=k)
     ens.run(15, graph=True)
+    ob_ens = ens.observables()
     for k, (b, p) in enumerate(systems):
         one = dml.Context(cfgs[k])
         one.set_state(p)
@@ -417,6 +423,7 @@ def test_ensemble_replicas_match_single_runs(dml):
         a, o = ens.state(rep=k), one.state()
         np.testing.assert_array_equal(a["pos"], o["pos"])
         np.testing.assert_array_equal(a["species"], o["species"])
+        np.testing.assert_array_equal(ob_ens[k], one.observables()[0])
         one.close()
     ens.close()
 

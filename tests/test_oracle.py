@@ -227,3 +227,47 @@ def test_gcmc_trial_energy_brute_force(orc):
     np.testing.assert_allclose(du, e, rtol=1e-12, atol=1e-9)
     hard_o = ((r2 < 3.2 ** 2) & keep & (z == 1)[None, :]).any(1)
     np.testing.assert_array_equal(hard.astype(bool), hard_o)
+
+
+def test_observables_closed_forms(orc):
+    """salida's energia and kion (dana.F90:1141-1183, 1342-1376) in the
+    oracle: energia after fuerza equals twice the WCA pair energy summed over
+    unordered ion pairs (the full list visits each ion pair twice, adding half
+    of u to both atoms each time; pairs with CG have eps 0); the temperature
+    is T exactly when every ion has m |v|^2 = 3 kB_ui T (equipartition), and
+    CG velocities do not enter it."""
+    rng = np.random.default_rng(71)
+    box = np.array([30.0, 30.0, 30.0])
+    pos = rng.random((300, 3)) * box
+    z = np.where(rng.random(300) < 0.25, 2, 1).astype(np.int8)
+    z[::13] = 3
+    s = orc.System(orc.dana_params(box, 2.0, ermak=True, piston=False), pos, z)
+    s.test_update()
+    s.fuerza()
+    ions = z != 2
+    d = pos[:, None, :] - pos[None, :, :]
+    d[..., :2] -= box[:2] * np.round(d[..., :2] / box[:2])
+    r2 = (d * d).sum(-1)
+    eps_t = np.array([[2313.6, 0, 529.1], [0, 0, 0], [529.1, 0, 121.0]])
+    r0_t = np.array([[3.2, 3.5, 1.564], [3.5, 0, 0], [1.564, 0, 3.61]])
+    eps = eps_t[z[:, None] - 1, z[None, :] - 1]
+    r0 = r0_t[z[:, None] - 1, z[None, :] - 1]
+    iu = np.triu_indices(len(pos), 1)
+    inside = (r2[iu] <= r0[iu] ** 2) & ions[iu[0]] & ions[iu[1]]
+    rr = np.sqrt(np.where(inside, r2[iu], 1.0))
+    b = (r0[iu] / np.where(inside, rr, 1.0)) ** 6
+    e_pairs = np.where(inside, eps[iu] * b * (b - 2) + eps[iu], 0.0).sum()
+    assert inside.sum() > 5
+    ob = s.observables()
+    np.testing.assert_allclose(ob[0], 2.0 * e_pairs, rtol=1e-12)
+    # equipartition
+    kB_ui = 8.617330350e-5 * 96.485 * 100.0
+    m = 6.94
+    u = rng.normal(size=(len(pos), 3))
+    u /= np.linalg.norm(u, axis=1)[:, None]
+    vel = u * np.sqrt(3.0 * kB_ui * 300.0 / m)
+    vel[~ions] *= 50.0  # CG atoms are skipped by kion
+    s.set_state(vel=vel)
+    ob = s.observables()
+    assert ob[2] == ions.sum() and ob[3] == len(pos)
+    np.testing.assert_allclose(ob[1], 300.0, rtol=1e-12)
