@@ -87,6 +87,20 @@ int tessellate(RepParams &p, int first) {
 int bucket_cap(const RepParams &p);
 void set_ermak_consts(RepParams &p);
 
+// The FP32 prefilter's band around a squared cutoff R^2 (R <= rl).  With
+// coordinates below B = DML_MAX_BOX each float carries at most B 2^-25 of
+// rounding; a minimum-image component (two coordinates, the subtraction, the
+// rounded box and the shift) then errs by at most 2.5 B 2^-24 = delta, and
+// the squared distance of a pair near R by at most 2 sqrt(3) R delta plus
+// 3 R^2 2^-24 from the squares and sums.  The band is twice that, and at
+// least 0.02 A^2.
+static float pf_margin_for(double rl) {
+  const double u = ldexp(1.0, -24);
+  const double delta = 2.5 * DML_MAX_BOX * u;
+  const double bound = 2.0 * sqrt(3.0) * rl * delta + 3.0 * rl * rl * u;
+  return (float)std::max(0.02, 2.0 * bound);
+}
+
 // the static parameters of one replica from its config (everything but its
 // slot range and cell layout; p.pcap must be set: a Verlet bucket holds it)
 int setup_rep(RepParams &p, const dml_config &g) {
@@ -121,6 +135,9 @@ int setup_rep(RepParams &p, const dml_config &g) {
   p.overlap_mode = g.overlap_mode;
   p.seed = g.seed;
   p.use_cells = 0;
+  for (int k = 0; k < 3; k++)
+    if (p.box[k] > DML_MAX_BOX) return DML_E_UNSUPPORTED;
+  p.pf_margin = pf_margin_for(rl);
   if (tessellate(p, 1)) return DML_E_UNSUPPORTED;
   p.bcap = bucket_cap(p);
   return DML_OK;
@@ -984,6 +1001,9 @@ int dml_get_zmax(dml_ctx *c, int32_t rep, double *zmax) {
 
 int dml_set_box(dml_ctx *c, int32_t rep, const double box[3]) {
   if (!c || rep < 0 || rep >= c->nrep || !box) return DML_E_ARG;
+  for (int k = 0; k < 3; k++)
+    if (!(box[k] > 0)) return DML_E_ARG;
+    else if (box[k] > DML_MAX_BOX) return DML_E_UNSUPPORTED;
   int rc = fetch_params(c);
   if (rc) return rc;
   RepParams &p = c->reps[rep].p;

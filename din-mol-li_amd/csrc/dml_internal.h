@@ -32,6 +32,10 @@
 #define ERR_BUCKET 32
 #define ERR_UNSUP 64
 
+// largest box edge: the FP32 position mirror used as a distance prefilter is
+// given a margin that holds for coordinates up to this size
+#define DML_MAX_BOX 4096.0
+
 // RepParams::halted bits
 #define HALT_FATAL 1   // a fatal condition of the reference (err_bits says which)
 #define HALT_PAUSED 2  // not running: free serving slot, or step_limit reached
@@ -40,6 +44,7 @@
 struct RepParams {
   double box[3], one_box[3];
   double rcut, rcut2, rl2, skin;
+  float pf_margin;   // band (A^2) around a squared cutoff decided in FP64
   double zmax, z0, prob, h, z_switch, dif_sc, dif_sei;
   double cc0, cc1, cc2, sdr, sdv, crv1, crv2, skt;
   double eps[9], r0[9], mass[3];

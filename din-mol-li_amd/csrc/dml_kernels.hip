@@ -20,10 +20,11 @@ __device__ inline void put_pf4(const DevState &s, int i, double x, double y, dou
 __device__ inline void put_pf4_sp(const DevState &s, int i, int sp) {
   reinterpret_cast<float *>(&s.pf4[i])[3] = __int_as_float(sp);
 }
-// squared minimum-image distance in FP32 and its safe margin: coordinates
-// below 2^10 A carry at most 2^-14 A of rounding, so for pairs within 16 A
-// |d2_f32 - d2_f64| < 2e-3 A^2; the margin keeps a factor 10.
-#define PF_MARGIN 0.02f
+// squared minimum-image distance in FP32.  A pair is decided in FP32 only
+// outside the band cutoff^2 +- RepParams::pf_margin (set by the host from
+// the largest box and the list radius, see pf_margin_for); inside it the FP64
+// test of the reference decides.  The image chosen near half a box may
+// differ from idnint's, which changes d2 only by O(box * 2^-24).
 __device__ inline float d2_f32(const RepParams &p, float4 a, float4 b, float bx, float by,
                                float ibx, float iby) {
   float vx = a.x - b.x, vy = a.y - b.y, vz = a.z - b.z;
@@ -390,7 +391,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_listbuild_cells(DevState s) {
     if (P.halted || !P.rebuild || P.lb_kind != 0 || c >= ncell_total(P)) continue;
     const double rl2 = P.rl2;
     const int strict = P.use_cells;
-    const float rl2lo = (float)rl2 - PF_MARGIN, rl2hi = (float)rl2 + PF_MARGIN;
+    const float rl2lo = (float)rl2 - P.pf_margin, rl2hi = (float)rl2 + P.pf_margin;
     const float bxf = (float)P.box[0], byf = (float)P.box[1];
     const float ibx = (float)P.one_box[0], iby = (float)P.one_box[1];
     if (tid == 0) {
@@ -613,7 +614,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_listbuild_wc(DevState s) {
     const int total = __shfl(incl, 26);
     const double rl2 = P.rl2;
     const int strict = P.use_cells;
-    const float rl2lo = (float)rl2 - PF_MARGIN, rl2hi = (float)rl2 + PF_MARGIN;
+    const float rl2lo = (float)rl2 - P.pf_margin, rl2hi = (float)rl2 + P.pf_margin;
     const float bxf = (float)P.box[0], byf = (float)P.box[1];
     const float ibx = (float)P.one_box[0], iby = (float)P.one_box[1];
     for (int a0 = 0; a0 < ccnt; a0 += WAVE) {
@@ -825,7 +826,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_force(DevState s, int only_ermak)
       const float4 fj4 = s.pf4[j];
       {  // surely beyond the pair's r0: no force, no energy
         float r0f = (float)P.r0[(k - 1) * 3 + (__float_as_int(fj4.w) - 1)];
-        if (d2_f32(P, fi, fj4, bxf, byf, ibx, iby) > r0f * r0f + PF_MARGIN) continue;
+        if (d2_f32(P, fi, fj4, bxf, byf, ibx, iby) > r0f * r0f + P.pf_margin) continue;
       }
       uint8_t fj = s.flags[j];
       if (fj & DML_FL_LIMBO) continue;
@@ -1230,7 +1231,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_ov_scan_l(DevState s, const doubl
     const float4 fi = s.pf4[i];
     const float bxf = (float)P.box[0], byf = (float)P.box[1];
     const float ibx = (float)P.one_box[0], iby = (float)P.one_box[1];
-    const float rcf = (float)P.rcut2 + PF_MARGIN;
+    const float rcf = (float)P.rcut2 + P.pf_margin;
     const int nn = s.nn[i];
     const int32_t *row = s.nbr + (int64_t)i * s.maxnn;
     const int NH = 4;
@@ -2323,6 +2324,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_chunk_run(DevState s) {
   Q.zmax = P.zmax + P.dist;
   Q.box[2] = Q.zmax;
   Q.one_box[2] = 1.0 / Q.box[2];
+  if (Q.box[2] > DML_MAX_BOX) fail = ERR_UNSUP;  // beyond the FP32 mirror's range
   if (tessellate_rep(Q, 0)) fail = ERR_UNSUP;
   Q.ccap = Q.use_cells ? (Q.ncells[0] + 2) * (Q.ncells[1] + 2) * (Q.ncells[2] + 2) : 1;
   if (Q.ccap > P.creg) fail = ERR_BUCKET;

@@ -40,6 +40,43 @@ def bcc_layers(lx, ly, nl, z_bottom=0.0, vac=0.0, rng=None, a=A_BCC):
     return np.vstack(pts)
 
 
+class _Near:
+    """points of `existing` with zlo <= z < zhi, queried for neighbours
+    closer than dmin under the x, y periodic minimum image"""
+
+    def __init__(self, existing, box, zlo, zhi, dmin):
+        self.tree = None
+        self.dmin = dmin
+        if existing is None or not len(existing):
+            return
+        e = existing[(existing[:, 2] >= zlo) & (existing[:, 2] < zhi)]
+        if not len(e):
+            return
+        from scipy.spatial import cKDTree
+        self.box = np.asarray(box[:2], float)
+        self.zoff = zlo - dmin
+        bz = 3.0 * (zhi + dmin - self.zoff)  # no z image within dmin
+        self.tree = cKDTree(self._map(e), boxsize=[self.box[0], self.box[1], bz])
+
+    def _map(self, p):
+        xy = np.mod(p[:, :2], self.box)
+        xy = np.where(xy >= self.box, xy - self.box, xy)
+        return np.column_stack([xy, p[:, 2] - self.zoff])
+
+    def blocked(self, cand):
+        if self.tree is None:
+            return np.zeros(len(cand), bool)
+        d, _ = self.tree.query(self._map(cand), k=1, distance_upper_bound=self.dmin)
+        return d < self.dmin
+
+    def free_fraction(self, rng, lo, hi, samples=8192):
+        """fraction of [lo, hi) farther than dmin from every point"""
+        if self.tree is None:
+            return 1.0
+        p = np.asarray(lo) + rng.random((samples, 3)) * (np.asarray(hi) - np.asarray(lo))
+        return float(1.0 - self.blocked(p).mean())
+
+
 def rsa_fill(rng, n, lo, hi, box, existing=None, dmin=DMIN, batch=None):
     """n points uniform in [lo, hi) (x, y periodic in box) at least dmin from
     each other and from `existing`, by batched random sequential addition."""
@@ -49,16 +86,12 @@ def rsa_fill(rng, n, lo, hi, box, existing=None, dmin=DMIN, batch=None):
     gx, gy = box[0] / nx, box[1] / ny
     zlo = lo[2] - 2 * dmin
     nz = int(math.ceil((hi[2] + 2 * dmin - zlo) / g)) + 1
-    grid = -np.ones((nx, ny, nz), np.int64)
+    grid = -np.ones((nx, ny, nz), np.int64)  # the new points, at most one per cell
     pts = np.zeros((0, 3))
-    if existing is not None and len(existing):
-        e = existing[(existing[:, 2] >= zlo) & (existing[:, 2] < hi[2] + 2 * dmin)]
-        pts = e.copy()
-        cx = np.floor(e[:, 0] / gx).astype(int) % nx
-        cy = np.floor(e[:, 1] / gy).astype(int) % ny
-        cz = np.clip(np.floor((e[:, 2] - zlo) / g).astype(int), 0, nz - 1)
-        grid[cx, cy, cz] = np.arange(len(e))
-    n0 = len(pts)
+    # existing points (an electrode lattice may put two in one grid cell) are
+    # checked with a k-d tree, periodic in x and y
+    near = _Near(existing, box, zlo, hi[2] + 2 * dmin, dmin)
+    n0 = 0
     offs = np.array([(a, b, c) for a in range(-2, 3) for b in range(-2, 3) for c in range(-2, 3)])
     batch = batch or max(256, min(20000, n // 4 + 256))
     stall = 0
@@ -67,7 +100,7 @@ def rsa_fill(rng, n, lo, hi, box, existing=None, dmin=DMIN, batch=None):
         cx = np.floor(cand[:, 0] / gx).astype(int) % nx
         cy = np.floor(cand[:, 1] / gy).astype(int) % ny
         cz = np.floor((cand[:, 2] - zlo) / g).astype(int)
-        bad = np.zeros(batch, bool)
+        bad = near.blocked(cand)
         # earliest candidate of each cell (a cell holds at most one point)
         tmp = np.full((nx, ny, nz), batch, np.int64)
         np.minimum.at(tmp, (cx, cy, cz), np.arange(batch))
@@ -190,18 +223,23 @@ def system_chunks(n_total, seed, rho=RHO_CHUNK):
 def _slab_job(args):
     seed, k, n, lo, hi, box, existing = args
     return rsa_fill(np.random.default_rng([seed, k]), n, lo, hi, box, existing=existing,
-                    batch=100000)
+                    batch=min(100000, max(2048, n // 2)))
 
 
 def rsa_fill_slabs(seed, n, lo, hi, box, existing=None, workers=16, slabs=32):
     """rsa_fill in z slabs: the even slabs in parallel first, then the odd
     ones against their neighbours (same hard-core rule, points spread over
-    the slabs by volume)."""
+    the slabs by the volume `existing` leaves free)."""
     lo = np.asarray(lo, float); hi = np.asarray(hi, float)
     slabs = max(2, min(slabs, int((hi[2] - lo[2]) // (4 * DMIN))))
     edges = np.linspace(lo[2], hi[2], slabs + 1)
-    counts = np.diff(np.round(np.linspace(0, n, slabs + 1)).astype(int))
     ex = existing if existing is not None else np.zeros((0, 3))
+    near = _Near(ex, box, lo[2] - 2 * DMIN, hi[2] + 2 * DMIN, DMIN)
+    frng = np.random.default_rng([seed, 1 << 20])
+    free = np.array([near.free_fraction(frng, [lo[0], lo[1], edges[k]], [hi[0], hi[1], edges[k + 1]])
+                     for k in range(slabs)])
+    cum = np.concatenate([[0.0], np.cumsum(free)])
+    counts = np.diff(np.round(n * cum / cum[-1]).astype(int))
     out = [None] * slabs
 
     def jobs(parity):
@@ -268,7 +306,12 @@ def system_c5(n_total=2_000_000, seed=104012 + 5, L=600.0, n_prot=30, workers=16
     el = np.vstack([slab, prot])
     etop = el[:, 2].max()
     n_ion = n_total - len(el)
-    hgt = n_ion / (RHO_ION * L * L)
+    # the ion slab starts DMIN above the flat top; the protrusions take part
+    # of its volume, so it is made taller by the volume they exclude
+    z_lo = ztop + DMIN
+    excl = 1.0 - _Near(el, [L, L], z_lo - 2 * DMIN, etop + 2 * DMIN, DMIN).free_fraction(
+        np.random.default_rng([seed, 7]), [0, 0, z_lo], [L, L, etop + DMIN], samples=200_000)
+    hgt = n_ion / (RHO_ION * L * L) + excl * (etop + DMIN - z_lo)
     box = np.array([L, L, ztop + DMIN + hgt + 20.0])
     ions = rsa_fill_slabs(seed, n_ion, [0, 0, ztop + DMIN], [L, L, ztop + DMIN + hgt], box,
                           existing=el, workers=workers)
