@@ -70,22 +70,30 @@ def energy_drift(dml, mk, snap, steps, every=100):
     steps; energia of salida, computed on the device)."""
     import numpy as np
     out = {}
-    for prec in (dml.FP64, dml.FP32_FORCE):
+    # the third run is FP64 from positions moved by 1e-9 A: two FP64
+    # trajectories that part like the FP32 one does (the dynamics is
+    # chaotic), the scale against which the FP32 difference is read
+    rng = np.random.default_rng(5)
+    kick = 1e-9 * rng.standard_normal(snap["pos"].shape)
+    for key, prec, dp in (("fp64", dml.FP64, 0.0), ("fp32", dml.FP32_FORCE, 0.0),
+                          ("fp64_kicked", dml.FP64, kick)):
         c = dml.Context(mk(prec))
-        c.set_state(snap["pos"], species=snap["species"], vel=snap["vel"], acel=snap["acel"])
+        c.set_state(snap["pos"] + dp, species=snap["species"], vel=snap["vel"], acel=snap["acel"])
         e = []
         for _ in range(steps // every):
             c.run(every, graph=True)
             e.append(c.observables()[0][0])
-        out[prec] = np.array(e)
+        out[key] = np.array(e)
         c.close()
     h = mk(dml.FP64).h
-    t = (np.arange(len(out[dml.FP64])) + 1) * every * h
-    m64, m32 = out[dml.FP64].mean(), out[dml.FP32_FORCE].mean()
-    return dict(steps=steps, samples=len(t), mean_epot_fp64=float(m64), mean_epot_fp32=float(m32),
-                rel=float(abs(m32 - m64) / abs(m64)),
-                slope_fp64_per_ps=float(np.polyfit(t, out[dml.FP64], 1)[0]),
-                slope_fp32_per_ps=float(np.polyfit(t, out[dml.FP32_FORCE], 1)[0]))
+    t = (np.arange(len(out["fp64"])) + 1) * every * h
+    m = {k: float(v.mean()) for k, v in out.items()}
+    return dict(steps=steps, samples=len(t), mean_epot_fp64=m["fp64"], mean_epot_fp32=m["fp32"],
+                rel=abs(m["fp32"] - m["fp64"]) / abs(m["fp64"]),
+                rel_fp64_kicked=abs(m["fp64_kicked"] - m["fp64"]) / abs(m["fp64"]),
+                slope_fp64_per_ps=float(np.polyfit(t, out["fp64"], 1)[0]),
+                slope_fp32_per_ps=float(np.polyfit(t, out["fp32"], 1)[0]),
+                slope_fp64_kicked_per_ps=float(np.polyfit(t, out["fp64_kicked"], 1)[0]))
 
 
 def plain_leg(dml, W, steps, warmup, drift_steps=10000):

@@ -363,20 +363,22 @@ void assign_list_kernels(dml_ctx *c) {
   }
 }
 
-void enq_cells_lists(dml_ctx *c) {
+// gcs = 1: also re-sort the cells of the GCMC replicas (their trials read
+// them before the next test_update)
+void enq_cells_lists(dml_ctx *c, int gcs) {
   dim3 gc(clamp_grid((c->d.n_cm + DML_BLOCK - 1) / DML_BLOCK));
   dim3 gw(clamp_grid((c->d.n_cm + 3) / 4));
   int nb = c->d.nblocks;
-  hipLaunchKernelGGL(k_zero_cells, gc, dim3(DML_BLOCK), 0, c->st, c->d);
-  hipLaunchKernelGGL(k_cell_key, dim3(nb), dim3(DML_BLOCK), 0, c->st, c->d);
-  hipLaunchKernelGGL(k_cell_sort_w, gw, dim3(DML_BLOCK), 0, c->st, c->d);
+  hipLaunchKernelGGL(k_zero_cells, gc, dim3(DML_BLOCK), 0, c->st, c->d, gcs);
+  hipLaunchKernelGGL(k_cell_key, dim3(nb), dim3(DML_BLOCK), 0, c->st, c->d, gcs);
+  hipLaunchKernelGGL(k_cell_sort_w, gw, dim3(DML_BLOCK), 0, c->st, c->d, gcs);
   enq_listbuild(c);
 }
 
-void enq_test_update(dml_ctx *c) {
+void enq_test_update(dml_ctx *c, int gcs) {
   hipLaunchKernelGGL(k_pbc_dispmax, dim3(c->d.nblocks), dim3(DML_BLOCK), 0, c->st, c->d);
   hipLaunchKernelGGL(k_dispmax_final, dim3(c->nrep), dim3(DML_BLOCK), 0, c->st, c->d);
-  enq_cells_lists(c);
+  enq_cells_lists(c, gcs);
 }
 
 template <int FUSEB>
@@ -443,6 +445,29 @@ void enq_overlap(dml_ctx *c) {
   if (c->any_rounds) enq_overlap_rounds(c, nullptr, 0, c->rounds, 0);
 }
 
+// gcmc_run (dana.F90:590-713) of one replica (rep_only >= 0, with act and
+// nadj given) or of every GCMC replica (their own act, nadj): the grid-wide
+// list staging, the sequential trials (a workgroup per replica), the
+// grid-wide compaction of the gcmc and ref lists
+static void enq_gcmc(dml_ctx *c, int rep_only, const double *u, const double *g, double act,
+                     int nadj) {
+  const int nb = c->d.nblocks;
+  for (int stage = 0; stage < 2; stage++) {
+    if (stage == 1) {
+      if (rep_only >= 0)
+        hipLaunchKernelGGL(k_gcmc_run, dim3(1), dim3(DML_GCMC_BLOCK), 0, c->st, c->d, rep_only,
+                           u, g, act, nadj);
+      else
+        hipLaunchKernelGGL(k_gcmc_run, dim3(c->nrep), dim3(DML_GCMC_BLOCK), 0, c->st, c->d, -1,
+                           u, g, act, nadj);
+    }
+    hipLaunchKernelGGL(k_gc_count, dim3(nb), dim3(DML_BLOCK), 0, c->st, c->d, rep_only, stage);
+    hipLaunchKernelGGL(k_gc_scan, dim3(c->nrep), dim3(64), 0, c->st, c->d, rep_only, stage);
+    hipLaunchKernelGGL(k_gc_scatter, dim3(nb), dim3(DML_BLOCK), 0, c->st, c->d, rep_only, stage);
+  }
+  hipLaunchKernelGGL(k_gc_copy, dim3(nb), dim3(DML_BLOCK), 0, c->st, c->d, rep_only);
+}
+
 void enq_f2cg(dml_ctx *c) {
   int nb = c->d.nblocks;
   hipLaunchKernelGGL(k_f2cg_mark, dim3(nb), dim3(DML_BLOCK), 0, c->st, c->d);
@@ -457,7 +482,7 @@ void enq_bloques(dml_ctx *c) {
   const int rb = (c->nrep + 63) / 64;
   hipLaunchKernelGGL(k_chunk_run, dim3(c->nrep), dim3(DML_BLOCK), 0, c->st, c->d);
   hipLaunchKernelGGL(k_mask_fired, dim3(rb), dim3(64), 0, c->st, c->d, 1);
-  enq_test_update(c);
+  enq_test_update(c, 1);
   hipLaunchKernelGGL(k_mask_fired, dim3(rb), dim3(64), 0, c->st, c->d, 0);
 }
 
@@ -471,13 +496,14 @@ void enq_step(dml_ctx *c) {
   if (c->any_brown)
     hipLaunchKernelGGL(k_brownian, dim3(nb), dim3(DML_BLOCK), 0, c->st, c->d,
                        (const double *)nullptr, (const double *)nullptr);
-  enq_test_update(c);
+  // the cells of a GCMC replica are needed by its trials, after the second
+  // test_update; the first re-sorts them only when it rebuilds the lists
+  enq_test_update(c, 0);
   enq_overlap(c);
-  enq_test_update(c);
+  enq_test_update(c, 1);
   enq_f2cg(c);
   if (c->any_gcmc)  // gcmc_run after F -> CG (dana.F90:238-241)
-    hipLaunchKernelGGL(k_gcmc_run, dim3(c->nrep), dim3(DML_GCMC_BLOCK), 0, c->st, c->d, -1,
-                       (const double *)nullptr, (const double *)nullptr, 0.0, 0);
+    enq_gcmc(c, -1, nullptr, nullptr, 0.0, 0);
   // calc_rho (dana.F90:244), bloques (:247), maxz (:261)
   if (c->any_piston || c->any_chunks)
     hipLaunchKernelGGL(k_calc_rho, dim3(nb), dim3(DML_BLOCK), 0, c->st, c->d);
@@ -538,7 +564,7 @@ int handle_overflow(dml_ctx *c, int e, int resume) {
     }
     int z = 0;
     HIPCHECK(hipMemcpyAsync(c->d.err, &z, sizeof(int), hipMemcpyHostToDevice, c->st));
-    enq_cells_lists(c);
+    enq_cells_lists(c, 1);
     e = read_err(c, 0);
     if (e < 0) return DML_E_HIP;
   }
@@ -689,7 +715,7 @@ int dml_create_ensemble(int32_t nrep, const dml_config *cfgs, dml_ctx **out) {
   e |= dalloc(&d.ov_st, P);
   e |= dalloc(&d.part_d, 2 * (P / DML_BLOCK));
   e |= dalloc(&d.obs, 4 * (P / DML_BLOCK) + 4 * (int64_t)nrep);
-  e |= dalloc(&d.part_i, P / DML_BLOCK);
+  e |= dalloc(&d.part_i, 2 * (P / DML_BLOCK));
   e |= dalloc(&d.blk2rep, P / DML_BLOCK);
   e |= dalloc(&d.rp, nrep);
   e |= dalloc(&d.cnt, nrep);
@@ -874,6 +900,8 @@ int dml_set_state(dml_ctx *c, int32_t rep, int32_t n, const double *pos, const d
     p.seq_next = n;
     p.gcmc_calls = 0;
   }
+  p.free_lo = 0;
+  p.bfree_lo = 0;
   p.nslots = n;
   p.nref = (int32_t)rord.size();
   p.listed = 0;
@@ -933,6 +961,7 @@ int dml_set_order(dml_ctx *c, int32_t rep, int32_t nref, const int32_t *ref_orde
     HIPCHECK(hipMemcpy(c->d.bkey + o, bkey, sizeof(int32_t) * n, hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(c->d.b2slot + o, b2s.data(), sizeof(int32_t) * p.pcap, hipMemcpyHostToDevice));
     p.b_amax = amax;
+    p.bfree_lo = 0;
   }
   return upload_params(c);
 }
@@ -1034,7 +1063,7 @@ static int ensure_cells(dml_ctx *c);
 int dml_test_update(dml_ctx *c, int32_t *rebuilt) {
   if (!c) return DML_E_ARG;
   if (int rc0 = ensure_cells(c)) return rc0;
-  enq_test_update(c);
+  enq_test_update(c, 1);
   int rc = finish(c);
   if (rc) return rc;
   if (rebuilt) {
@@ -1048,7 +1077,7 @@ int dml_update(dml_ctx *c) {
   if (!c) return DML_E_ARG;
   if (int rc0 = ensure_cells(c)) return rc0;
   hipLaunchKernelGGL(k_force_rebuild, dim3((c->nrep + 63) / 64), dim3(64), 0, c->st, c->d);
-  enq_cells_lists(c);
+  enq_cells_lists(c, 1);
   return finish(c);
 }
 
@@ -1199,8 +1228,7 @@ int dml_gcmc(dml_ctx *c, int32_t rep, double act, int32_t nadj, const double *u,
     hipFree(dg);
     return DML_E_NOMEM;
   }
-  hipLaunchKernelGGL(k_gcmc_run, dim3(1), dim3(DML_GCMC_BLOCK), 0, c->st, c->d, rep,
-                     (const double *)du, (const double *)dg, act, nadj);
+  enq_gcmc(c, rep, du, dg, act, nadj);
   rc = finish(c);
   hipFree(du);
   hipFree(dg);
@@ -1448,7 +1476,7 @@ int dml_run(dml_ctx *c, int32_t nsteps, int32_t use_graph) {
     // replicas that have no lists yet; the others sit this phase out
     const int rb = (c->nrep + 63) / 64;
     hipLaunchKernelGGL(k_mask_listed, dim3(rb), dim3(64), 0, c->st, c->d, 1);
-    enq_test_update(c);
+    enq_test_update(c, 1);
     if (c->any_ermak) enq_forces(c, 0, 1);
     hipLaunchKernelGGL(k_mask_listed, dim3(rb), dim3(64), 0, c->st, c->d, 0);
     if ((rc = finish(c, 0))) return rc;
@@ -1512,7 +1540,7 @@ int dml_time_kernel(dml_ctx *c, int32_t which, int32_t reps, double *avg_ms) {
   if (which == DML_K_LISTBUILD || which == DML_K_CELLS) {
     // a rebuild at the current positions, then the phase alone, reps times
     hipLaunchKernelGGL(k_force_rebuild, dim3((c->nrep + 63) / 64), dim3(64), 0, c->st, c->d);
-    enq_cells_lists(c);
+    enq_cells_lists(c, 1);
   }
   HIPCHECK(hipEventRecord(c->e0, c->st));
   for (int q = 0; q < reps; q++) {
@@ -1525,10 +1553,10 @@ int dml_time_kernel(dml_ctx *c, int32_t which, int32_t reps, double *avg_ms) {
         break;
       case DML_K_CELLS: {
         dim3 gc(clamp_grid((c->d.n_cm + DML_BLOCK - 1) / DML_BLOCK));
-        hipLaunchKernelGGL(k_zero_cells, gc, dim3(DML_BLOCK), 0, c->st, c->d);
-        hipLaunchKernelGGL(k_cell_key, dim3(c->d.nblocks), dim3(DML_BLOCK), 0, c->st, c->d);
+        hipLaunchKernelGGL(k_zero_cells, gc, dim3(DML_BLOCK), 0, c->st, c->d, 1);
+        hipLaunchKernelGGL(k_cell_key, dim3(c->d.nblocks), dim3(DML_BLOCK), 0, c->st, c->d, 1);
         dim3 gw(clamp_grid((c->d.n_cm + 3) / 4));
-        hipLaunchKernelGGL(k_cell_sort_w, gw, dim3(DML_BLOCK), 0, c->st, c->d);
+        hipLaunchKernelGGL(k_cell_sort_w, gw, dim3(DML_BLOCK), 0, c->st, c->d, 1);
         break;
       }
       case DML_K_DISPMAX:

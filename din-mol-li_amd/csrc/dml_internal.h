@@ -80,6 +80,14 @@ struct RepParams {
   int32_t nat, nlimbo;     // hs members alive / in limbo
   int32_t b_amax, b_nat;   // hs%b index range and members
   int32_t ngc;             // members of the gcmc group (list order in gc_order)
+  // one gcmc_run call: lengths of the working gcmc list (wl[1]), of the
+  // in-volume list (ref_tmp) and of the ref list with deleted entries, and
+  // the stage the grid-wide list kernels reached (1 staged, 2 trials done,
+  // 3 counted; reset at the next call)
+  int32_t gc_len, gc_inv, ref_len, gc_pending;
+  // search hints: every slot below free_lo is alive or in limbo, every hs%b
+  // index below bfree_lo is taken (lowered wherever one is freed)
+  int32_t free_lo, bfree_lo;
   uint32_t pid_next;       // next particle id (Philox counter)
   int32_t seq_next;        // next creation number (sys list order)
   int32_t gcmc_calls;
@@ -280,7 +288,7 @@ struct DevState {
   // reductions
   double *part_d;         // 2 per particle block
   double *obs;            // 4 per particle block, then 4 per replica
-  int32_t *part_i;        // per particle block
+  int32_t *part_i;        // 2 per particle block
   // tables
   int32_t *blk2rep;
   RepParams *rp;

@@ -6,10 +6,10 @@ __global__ void k_pbc_dispmax(DevState s);
 __global__ void k_dispmax_final(DevState s);
 __global__ void k_init_pf4(DevState s, int64_t n);
 __global__ void k_force_rebuild(DevState s);
-__global__ void k_zero_cells(DevState s);
-__global__ void k_cell_key(DevState s);
-__global__ void k_cell_sort(DevState s);
-__global__ void k_cell_sort_w(DevState s);
+__global__ void k_zero_cells(DevState s, int gcs);
+__global__ void k_cell_key(DevState s, int gcs);
+__global__ void k_cell_sort(DevState s, int gcs);
+__global__ void k_cell_sort_w(DevState s, int gcs);
 __global__ void k_listbuild_cells(DevState s);
 template <int LPA>
 __global__ void k_ov_scan_l(DevState s, const double *u_round, int64_t u_stride, int round,
@@ -43,6 +43,10 @@ __global__ void k_chunk_run(DevState s);
 __global__ void k_mask_fired(DevState s, int on);
 __global__ void k_listbuild_wc(DevState s);
 __global__ void k_step_end(DevState s);
+__global__ void k_gc_count(DevState s, int rep_only, int stage);
+__global__ void k_gc_scan(DevState s, int rep_only, int stage);
+__global__ void k_gc_scatter(DevState s, int rep_only, int stage);
+__global__ void k_gc_copy(DevState s, int rep_only);
 __global__ void k_gcmc_run(DevState s, int rep_only, const double *u, const double *g,
                            double act_in, int nadj_in);
 __global__ void k_gcmc_energy(DevState s, int r, int ntrial, const double *pos3,

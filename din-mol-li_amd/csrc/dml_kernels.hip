@@ -159,32 +159,35 @@ __device__ inline int ncell_total(const RepParams &P) {
   return P.use_cells ? (P.ncells[0] + 2) * (P.ncells[1] + 2) * (P.ncells[2] + 2) : 1;
 }
 
-// resort: 1 = cells of this replica are rebuilt in this test_update
-__device__ inline int want_cells(const RepParams &P) {
-  return P.rebuild || P.reservoir == DML_GCMC;
+// resort: 1 = cells of this replica are rebuilt in this test_update: on a
+// list rebuild, and for a GCMC replica whose trials come before the next
+// test_update (gcs = 1; its trials read the cells at the current positions)
+__device__ inline int want_cells(const RepParams &P, int gcs) {
+  return P.rebuild || (gcs && P.reservoir == DML_GCMC);
 }
 
-__global__ void k_zero_cells(DevState s) {
+__global__ void k_zero_cells(DevState s, int gcs) {
   for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < s.n_cm;
        g += (int64_t)gridDim.x * blockDim.x) {
     const RepParams &P = s.rp[s.cm_rep[g]];
     const int c = s.cm_cell[g];
-    if (P.halted || !want_cells(P) || c >= ncell_total(P)) continue;
+    if (P.halted || !want_cells(P, gcs) || c >= ncell_total(P)) continue;
     s.cell_cnt[P.coff + c] = 0;
   }
 }
 
-__global__ void __launch_bounds__(DML_BLOCK) k_cell_key(DevState s) {
+__global__ void __launch_bounds__(DML_BLOCK) k_cell_key(DevState s, int gcs) {
   int r;
   const RepParams &P = rep_of_block(s, blockIdx.x, &r);
   if (P.halted) return;
-  if (!want_cells(P)) return;
+  if (!want_cells(P, gcs)) return;
   int slot = blockIdx.x * DML_BLOCK + threadIdx.x;
   int local = slot - P.poff;
   if (local >= P.nslots) return;
   uint8_t f = s.flags[slot];
   if (P.rebuild && (f & DML_FL_LIMBO)) {  // igroup_clean (Groups.F90:1036-1053)
     s.flags[slot] = f & (uint8_t)~DML_FL_LIMBO;
+    atomicMin(&s.rp[r].free_lo, local);
     return;
   }
   if (!(f & DML_FL_ALIVE)) return;
@@ -218,11 +221,11 @@ __global__ void __launch_bounds__(DML_BLOCK) k_cell_key(DevState s) {
 // order each bucket like the head/next chain: descending hs%b slot for
 // cells (push-front, Cells.F90:299-300), ascending for the Verlet scan
 // (Neighbor.F90:397); then copy positions into bucket order.
-__global__ void k_cell_sort(DevState s) {
+__global__ void k_cell_sort(DevState s, int gcs) {
   int r = blockIdx.y;
   const RepParams &P = s.rp[r];
   if (P.halted) return;
-  if (!want_cells(P)) return;
+  if (!want_cells(P, gcs)) return;
   int nc = ncell_total(P);
   int desc = P.use_cells;
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
@@ -705,35 +708,53 @@ __global__ void __launch_bounds__(DML_BLOCK) k_listbuild_wc(DevState s) {
   }
 }
 
-// bucket order by a rank sort inside one wave (cells of up to 64 atoms;
-// larger cells fall back to an insertion sort by lane 0)
-__global__ void __launch_bounds__(DML_BLOCK) k_cell_sort_w(DevState s) {
+// bucket order by a rank sort inside one wave (cells of up to 256 atoms,
+// four per lane: electrode cells hold ~100; larger cells fall back to an
+// insertion sort by lane 0)
+#define SORT_W 4
+__global__ void __launch_bounds__(DML_BLOCK) k_cell_sort_w(DevState s, int gcs) {
   const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
   for (int64_t g = (int64_t)blockIdx.x * (DML_BLOCK / WAVE) + w; g < s.n_cm;
        g += (int64_t)gridDim.x * (DML_BLOCK / WAVE)) {
     const RepParams &P = s.rp[s.cm_rep[g]];
     const int c = s.cm_cell[g];
-    if (P.halted || !want_cells(P) || c >= ncell_total(P)) continue;
+    if (P.halted || !want_cells(P, gcs) || c >= ncell_total(P)) continue;
     const int desc = P.use_cells;
     int cnt = s.cell_cnt[P.coff + c];
     if (cnt > P.bcap) cnt = P.bcap;
     if (cnt == 0) continue;
     const int64_t base = P.boff + (int64_t)c * P.bcap;
     int32_t *b = s.bucket + base;
-    if (cnt <= WAVE) {
-      int v = lane < cnt ? b[lane] : 0;
-      int k = lane < cnt ? s.bkey[v] : 0;
-      int rank = 0;
-      for (int q = 0; q < cnt; q++) {
-        int kq = __shfl(k, q);
-        rank += desc ? (kq > k) : (kq < k);
+    if (cnt <= SORT_W * WAVE) {
+      // up to SORT_W entries per lane; an entry's place is the number of
+      // keys before it (the hs%b keys of one bucket are distinct)
+      int v[SORT_W], k[SORT_W], rank[SORT_W];
+#pragma unroll
+      for (int j = 0; j < SORT_W; j++) {
+        const int q = j * WAVE + lane;
+        v[j] = q < cnt ? b[q] : 0;
+        k[j] = q < cnt ? s.bkey[v[j]] : 0;
+        rank[j] = 0;
       }
-      if (lane < cnt) {
-        b[rank] = v;
-        s.bx[base + rank] = s.px[v];
-        s.by[base + rank] = s.py[v];
-        s.bz[base + rank] = s.pz[v];
-        s.bf4[base + rank] = s.pf4[v];
+#pragma unroll
+      for (int j2 = 0; j2 < SORT_W; j2++) {
+        const int lim = min(WAVE, cnt - j2 * WAVE);
+        for (int q = 0; q < lim; q++) {
+          const int kq = __shfl(k[j2], q);
+#pragma unroll
+          for (int j = 0; j < SORT_W; j++) rank[j] += desc ? (kq > k[j]) : (kq < k[j]);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < SORT_W; j++) {
+        if (j * WAVE + lane < cnt) {
+          const int vj = v[j], rj = rank[j];
+          b[rj] = vj;
+          s.bx[base + rj] = s.px[vj];
+          s.by[base + rj] = s.py[vj];
+          s.bz[base + rj] = s.pz[vj];
+          s.bf4[base + rj] = s.pf4[vj];
+        }
       }
     } else {
       if (lane == 0) {
@@ -1815,8 +1836,156 @@ __device__ inline double gc_gauss(const RepParams &P, const double *g, int t, in
   return (k & 1) ? b : a;
 }
 
+// ------------------------------------------------------------------------
+// The O(n) list work around gcmc_run, spread over the grid (a block per
+// particle block of every GCMC replica) so that the sequential trial kernel
+// only touches what its trials touch.  Stage 0, before the trials: the
+// gcmc list drops the atoms that left the group (gcmc%detach at F -> CG,
+// dana.F90:235) into the working list wl[1], and the control volume's atoms
+// are listed in list order (dana.F90:609-615) into ref_tmp.  Stage 1, after
+// them: the working gcmc list and the ref list (deleted entries -1) are
+// compacted, ref_pos renumbered.  Each is a stable compaction: block counts,
+// a per-replica scan of the counts, a scatter.
+// ------------------------------------------------------------------------
+__device__ inline bool gc_sel(const RepParams &P, int r, int rep_only) {
+  return rep_only >= 0 ? r == rep_only : P.reservoir == DML_GCMC;
+}
+
+// the two keep flags of slot t's entries (stage 0: gcmc member, and in the
+// control volume; stage 1: gcmc entry kept, ref entry kept) and their values
+__device__ inline void gc_keep(const DevState &s, const RepParams &P, int t, int stage, int *v1,
+                               int *k1, int *v2, int *k2) {
+  const int local = t - P.poff;
+  if (stage == 0) {
+    const int v = local < P.ngc ? s.gc_order[t] : -1;
+    *v1 = *v2 = v;
+    *k1 = v >= 0 && (s.flags[v] & DML_FL_GCMC);
+    *k2 = 0;
+    if (*k1) {
+      const double z = s.pz[v];
+      *k2 = !(z < P.z0 || z > P.zmax);
+    }
+  } else {
+    *v1 = local < P.gc_len ? s.wl[1][t] : -1;
+    *k1 = *v1 >= 0 && (s.flags[*v1] & DML_FL_GCMC);
+    *v2 = local < P.ref_len ? s.ref_order[t] : -1;
+    *k2 = *v2 >= 0;
+  }
+}
+
+__global__ void __launch_bounds__(DML_BLOCK) k_gc_count(DevState s, int rep_only, int stage) {
+  __shared__ int w1[DML_BLOCK / WAVE], w2[DML_BLOCK / WAVE];
+  int r;
+  const RepParams &P = rep_of_block(s, blockIdx.x, &r);
+  if (!gc_sel(P, r, rep_only) || (stage == 0 ? P.halted != 0 : P.gc_pending != 2)) return;
+  int v1, k1, v2, k2;
+  gc_keep(s, P, blockIdx.x * DML_BLOCK + threadIdx.x, stage, &v1, &k1, &v2, &k2);
+  const unsigned long long m1 = __ballot(k1), m2 = __ballot(k2);
+  const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
+  if (lane == 0) {
+    w1[w] = __popcll(m1);
+    w2[w] = __popcll(m2);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int a = 0, b = 0;
+    for (int q = 0; q < DML_BLOCK / WAVE; q++) {
+      a += w1[q];
+      b += w2[q];
+    }
+    s.part_i[2 * blockIdx.x] = a;
+    s.part_i[2 * blockIdx.x + 1] = b;
+  }
+}
+
+// one wave per replica: exclusive scans of both block counts
+__global__ void k_gc_scan(DevState s, int rep_only, int stage) {
+  const int r = blockIdx.x;
+  RepParams *Pw = s.rp + r;
+  if (!gc_sel(*Pw, r, rep_only)) return;
+  const int lane = threadIdx.x;
+  if (stage == 0) {
+    if (Pw->halted) {  // nothing staged for this call
+      if (lane == 0) Pw->gc_pending = 0;
+      return;
+    }
+  } else if (Pw->gc_pending != 2) {
+    return;
+  }
+  const int b0 = Pw->poff / DML_BLOCK, nb = Pw->pcap / DML_BLOCK;
+  int c1 = 0, c2 = 0;
+  for (int q0 = 0; q0 < nb; q0 += WAVE) {
+    const int q = q0 + lane;
+    const int v1 = q < nb ? s.part_i[2 * (b0 + q)] : 0;
+    const int v2 = q < nb ? s.part_i[2 * (b0 + q) + 1] : 0;
+    int i1 = v1, i2 = v2;
+    for (int off = 1; off < WAVE; off <<= 1) {
+      const int x1 = __shfl_up(i1, off), x2 = __shfl_up(i2, off);
+      if (lane >= off) {
+        i1 += x1;
+        i2 += x2;
+      }
+    }
+    if (q < nb) {
+      s.part_i[2 * (b0 + q)] = c1 + i1 - v1;
+      s.part_i[2 * (b0 + q) + 1] = c2 + i2 - v2;
+    }
+    c1 += __shfl(i1, WAVE - 1);
+    c2 += __shfl(i2, WAVE - 1);
+  }
+  if (lane == 0) {
+    if (stage == 0) {
+      Pw->gc_len = c1;
+      Pw->gc_inv = c2;
+      Pw->gc_pending = 1;
+    } else {
+      Pw->ngc = c1;
+      Pw->nref = c2;
+      Pw->gc_pending = 3;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(DML_BLOCK) k_gc_scatter(DevState s, int rep_only, int stage) {
+  __shared__ int w1[DML_BLOCK / WAVE], w2[DML_BLOCK / WAVE];
+  int r;
+  const RepParams &P = rep_of_block(s, blockIdx.x, &r);
+  if (!gc_sel(P, r, rep_only) || P.gc_pending != (stage == 0 ? 1 : 3)) return;
+  int v1, k1, v2, k2;
+  gc_keep(s, P, blockIdx.x * DML_BLOCK + threadIdx.x, stage, &v1, &k1, &v2, &k2);
+  const unsigned long long m1 = __ballot(k1), m2 = __ballot(k2);
+  const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
+  if (lane == 0) {
+    w1[w] = __popcll(m1);
+    w2[w] = __popcll(m2);
+  }
+  __syncthreads();
+  int o1 = s.part_i[2 * blockIdx.x], o2 = s.part_i[2 * blockIdx.x + 1];
+  for (int q = 0; q < w; q++) {
+    o1 += w1[q];
+    o2 += w2[q];
+  }
+  const unsigned long long below = (1ull << lane) - 1ull;
+  int32_t *d1 = stage == 0 ? s.wl[1] : s.gc_order;
+  if (k1) d1[P.poff + o1 + __popcll(m1 & below)] = v1;
+  if (k2) s.ref_tmp[P.poff + o2 + __popcll(m2 & below)] = v2;
+}
+
+// stage 1: the compacted ref list back in place, ref_pos renumbered
+__global__ void __launch_bounds__(DML_BLOCK) k_gc_copy(DevState s, int rep_only) {
+  int r;
+  const RepParams &P = rep_of_block(s, blockIdx.x, &r);
+  if (!gc_sel(P, r, rep_only) || P.gc_pending != 3) return;
+  const int t = blockIdx.x * DML_BLOCK + threadIdx.x, local = t - P.poff;
+  if (local < P.nref) {
+    const int v = s.ref_tmp[t];
+    s.ref_order[t] = v;
+    s.ref_pos[v] = local;
+  }
+}
+
 #define GC_TRACK 256
-#define GC_BLOCK DML_GCMC_BLOCK  // 16 waves for the list passes
+#define GC_BLOCK DML_GCMC_BLOCK
 // stable block compaction of src[0, len) into dst (dst may not alias src):
 // keeps entries v with keep(q, v); each wave takes a contiguous quarter in
 // chunks of 64 (ballot counts, no barriers), one block scan of the wave
@@ -1868,7 +2037,7 @@ __global__ void __launch_bounds__(GC_BLOCK) k_gcmc_run(DevState s, int rep_only,
   if (r >= s.nrep) return;
   RepParams *Pw = s.rp + r;
   const RepParams &P = *Pw;
-  if (P.halted || (rep_only < 0 && P.reservoir != DML_GCMC)) return;
+  if (P.halted || (rep_only < 0 && P.reservoir != DML_GCMC) || P.gc_pending != 1) return;
   const int tid = threadIdx.x, lane = tid % WAVE, wv = tid / WAVE;
   const int poff = P.poff;
   const double act = rep_only >= 0 ? act_in : P.act;
@@ -1880,20 +2049,15 @@ __global__ void __launch_bounds__(GC_BLOCK) k_gcmc_run(DevState s, int rep_only,
   const double rl2 = P.rl2;
   const int maxnn = s.maxnn;
   const int listed = P.listed;
-  int32_t *gc = s.gc_order + poff;    // the gcmc list (tombstones -1 during the call)
-  int32_t *tmp = s.wl[1] + poff;      // scratch (pcap entries)
-  int32_t *invl = s.ref_tmp + poff;   // in-volume gcmc atoms, list order
+  // the lists staged by k_gc_scatter: the gcmc list without the atoms that
+  // left the group (tombstones -1 during the call) and the control volume's
+  // atoms in list order (dana.F90:609-615), n = their count
+  int32_t *gc = s.wl[1] + poff;
+  int32_t *invl = s.ref_tmp + poff;
+  int32_t *tmp = s.wl[0] + poff;      // scratch (pcap entries)
   int32_t *rord = s.ref_order + poff;
-  // gcmc%detach of the atoms turned CG at step end (dana.F90:235)
-  int glen = blk_compact(gc, P.ngc, tmp, sh,
-                         [&](int, int v) { return v >= 0 && (s.flags[v] & DML_FL_GCMC); });
-  for (int q = tid; q < glen; q += GC_BLOCK) gc[q] = tmp[q];
-  __syncthreads();
-  // the control volume's atoms in list order (dana.F90:609-615); n = their count
-  int ilen = blk_compact(gc, glen, invl, sh, [&](int, int v) {
-    double z = s.pz[v];
-    return !(z < z0 || z > zmax);
-  });
+  int glen = P.gc_len;
+  int ilen = P.gc_inv;
   int n = ilen;
   int idel = 0;  // in-volume entries deleted so far (tombstones in invl)
   int nslots = P.nslots, nat = P.nat, nlimbo = P.nlimbo, b_amax = P.b_amax, b_nat = P.b_nat;
@@ -1903,7 +2067,7 @@ __global__ void __launch_bounds__(GC_BLOCK) k_gcmc_run(DevState s, int rep_only,
   int seq_next = P.seq_next;
   unsigned long long n_ins = 0, n_del = 0;
   int nfreed_b = 0;
-  int hs_scan = 0, b_scan = 0;  // lowest slot / index not yet known to be taken
+  int hs_scan = P.free_lo, b_scan = P.bfree_lo;  // lowest slot / index not known to be taken
   int fail = 0;
   const double ui_ev = 1.0e2 * 1.0e2 * 1.6605402e-27 * (1.0 / 1.60219e-19);
   const double kB_ui = 8.617385e-05 * (1.0 / ui_ev);  // Constants.F90:100-167
@@ -1951,26 +2115,55 @@ __global__ void __launch_bounds__(GC_BLOCK) k_gcmc_run(DevState s, int rep_only,
       }
       __syncthreads();
       if (sh_hit) continue;
+      // appends below stay within the replica's pcap entries: drop the
+      // deleted entries of a list that is full (rare: needs as many
+      // deletions as free slots in one call)
+      if (glen == P.pcap) {
+        glen = blk_compact(gc, glen, tmp, sh,
+                           [&](int, int v) { return v >= 0 && (s.flags[v] & DML_FL_GCMC); });
+        for (int q = tid; q < glen; q += GC_BLOCK) gc[q] = tmp[q];
+        head = 0;
+        tail = glen - 1;
+        __syncthreads();
+      }
+      if (ilen == P.pcap) {
+        ilen = blk_compact(invl, ilen, tmp, sh, [&](int, int v) { return v >= 0; });
+        for (int q = tid; q < ilen; q += GC_BLOCK) invl[q] = tmp[q];
+        idel = 0;
+        __syncthreads();
+      }
+      if (rlen == P.pcap) {
+        rlen = blk_compact(rord, rlen, tmp, sh, [&](int, int v) { return v >= 0; });
+        for (int q = tid; q < rlen; q += GC_BLOCK) {
+          rord[q] = tmp[q];
+          s.ref_pos[tmp[q]] = q;
+        }
+        __syncthreads();
+      }
       n++;
       const int tail_atom = gc[tail];
       // hs slot: first free slot or append (Groups.F90:1083-1093)
       int hs;
       if (nslots >= nat + 1 + nlimbo) {
-        // 16 flags per load from the 16-aligned block holding hs_scan
+        // 16 flags per load from the 16-aligned block holding hs_scan; the
+        // block stops at the first window of 4096 slots that holds a free one
         int cand = 0x7fffffff;
         const uint8_t *fl = s.flags + poff;
-        for (int q16 = (hs_scan & ~15) + 16 * tid; q16 < nslots && cand == 0x7fffffff;
-             q16 += 16 * GC_BLOCK) {
-          const uint4 v = *reinterpret_cast<const uint4 *>(fl + q16);
-          for (int k = 0; k < 16; k++) {
-            const int q = q16 + k;
-            const uint32_t wd = k < 4 ? v.x : k < 8 ? v.y : k < 12 ? v.z : v.w;
-            const uint32_t f = (wd >> (8 * (k & 3))) & 0xffu;
-            if (q >= hs_scan && q < nslots && !(f & (DML_FL_ALIVE | DML_FL_LIMBO))) {
-              cand = q;
-              break;
+        for (int w0 = hs_scan & ~15; w0 < nslots; w0 += 16 * GC_BLOCK) {
+          const int q16 = w0 + 16 * tid;
+          if (q16 < nslots) {
+            const uint4 v = *reinterpret_cast<const uint4 *>(fl + q16);
+            for (int k = 0; k < 16; k++) {
+              const int q = q16 + k;
+              const uint32_t wd = k < 4 ? v.x : k < 8 ? v.y : k < 12 ? v.z : v.w;
+              const uint32_t f = (wd >> (8 * (k & 3))) & 0xffu;
+              if (q >= hs_scan && q < nslots && !(f & (DML_FL_ALIVE | DML_FL_LIMBO))) {
+                cand = q;
+                break;
+              }
             }
           }
+          if (__syncthreads_or(cand != 0x7fffffff)) break;
         }
         hs = blk_min(cand, sh);
         if (hs == 0x7fffffff) { fail = ERR_UNSUP; break; }
@@ -1983,8 +2176,11 @@ __global__ void __launch_bounds__(GC_BLOCK) k_gcmc_run(DevState s, int rep_only,
       int bs;
       if (b_amax >= b_nat + 1) {
         int cand = 0x7fffffff;
-        for (int q = b_scan + tid; q < b_amax && cand == 0x7fffffff; q += GC_BLOCK)
-          if (s.b2slot[poff + q] < 0) cand = q;
+        for (int w0 = b_scan; w0 < b_amax; w0 += GC_BLOCK) {
+          const int q = w0 + tid;
+          if (q < b_amax && s.b2slot[poff + q] < 0) cand = q;
+          if (__syncthreads_or(cand != 0x7fffffff)) break;
+        }
         bs = blk_min(cand, sh);
         if (nfreed_b && sh_freed_b[0] < bs) bs = sh_freed_b[0];
         if (bs == 0x7fffffff) { fail = ERR_UNSUP; break; }
@@ -2238,26 +2434,18 @@ __global__ void __launch_bounds__(GC_BLOCK) k_gcmc_run(DevState s, int rep_only,
     }
   }
   __syncthreads();
-  // compact the gcmc and ref lists (stable), fix ref_pos
-  const int gnew = blk_compact(gc, glen, tmp, sh,
-                               [&](int, int v) { return v >= 0 && (s.flags[v] & DML_FL_GCMC); });
-  for (int q = tid; q < gnew; q += GC_BLOCK) gc[q] = tmp[q];
-  __syncthreads();  // tmp is the scratch of the next compaction
-  const int rnew = blk_compact(rord, rlen, tmp, sh, [&](int, int v) { return v >= 0; });
-  for (int q = tid; q < rnew; q += GC_BLOCK) {
-    const int v = tmp[q];
-    rord[q] = v;
-    s.ref_pos[v] = q;
-  }
-  __syncthreads();
+  // the gcmc and ref lists are compacted by k_gc_count/scan/scatter/copy
   if (tid == 0) {
-    Pw->ngc = gnew;
+    Pw->gc_len = glen;
+    Pw->ref_len = rlen;
+    Pw->gc_pending = 2;
+    Pw->free_lo = hs_scan;
+    Pw->bfree_lo = nfreed_b ? min(b_scan, sh_freed_b[0]) : b_scan;
     Pw->nslots = nslots;
     Pw->nat = nat;
     Pw->nlimbo = nlimbo;
     Pw->b_amax = b_amax;
     Pw->b_nat = b_nat;
-    Pw->nref = rnew;
     Pw->pid_next = pid_next;
     Pw->seq_next = seq_next;
     Pw->gcmc_calls = call + 1;
