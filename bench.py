@@ -49,9 +49,7 @@ def build_replicas(nrep, seed0=104012):
     jobs = [(int(n), seed0 + 1000 + k, RESERVOIR_CYCLE[k % 3]) for k, n in enumerate(sizes)]
     workers = min(16, os.cpu_count() or 1)
     if workers > 1 and nrep > 4:
-        import multiprocessing as mp
-        with mp.get_context("fork").Pool(workers) as pool:
-            return pool.map(_replica, jobs, chunksize=2)
+        return W.pool_map(_replica, jobs, workers, chunksize=2)
     return [_replica(j) for j in jobs]
 
 
@@ -236,9 +234,7 @@ def serving_leg(dml, duration, rate=20.0, step_scale=0.01, seed=104012 + 4):
     t0 = time.perf_counter()
     jobs = [(int(n), seed + 17 * k, SERVING_CYCLE[k % 3]) for k, n in enumerate(sizes)]
     workers = min(16, os.cpu_count() or 1)
-    import multiprocessing as mp
-    with mp.get_context("fork").Pool(workers) as pool:
-        systems = pool.map(_request_system, jobs, chunksize=4)
+    systems = W.pool_map(_request_system, jobs, workers, chunksize=4)
     log("built %d requests in %.1f s" % (m, time.perf_counter() - t0))
     reqs = []
     for k, (sysd, st) in enumerate(zip(systems, steps)):
@@ -350,6 +346,33 @@ def calibrate_copy(nbytes=1 << 30):
     log("copy calibration: %d bytes read, %d bytes written" % (nbytes, nbytes))
 
 
+def pmc_summary(fetch_csv, write_csv, kernel="k_listbuild_wc", last=10, copy_bytes=1 << 30):
+    """HBM bytes per launch of `kernel` from two rocprofv3 --pmc runs of
+    `bench.py --legs batched --steps 1 --warmup 0 --no-cpu --calibrate-copy`
+    (FETCH_SIZE, then WRITE_SIZE; counter_collection.csv of each): the mean
+    over its last `last` dispatches (the timed launches of the row build over
+    every replica), each counter scaled by what it reads for the 1 GiB device
+    copy of the calibration (the largest copyBuffer dispatch)."""
+    import csv
+
+    def read(path, counter):
+        rows = [r for r in csv.DictReader(open(path)) if r["Counter_Name"] == counter]
+        mine = [float(r["Counter_Value"]) for r in rows if r["Kernel_Name"].startswith(kernel + "(")]
+        copy = max(float(r["Counter_Value"]) for r in rows
+                   if r["Kernel_Name"].startswith("__amd_rocclr_copyBuffer"))
+        return sum(mine[-last:]) / len(mine[-last:]), copy, len(mine)
+
+    f, fc, nf = read(fetch_csv, "FETCH_SIZE")
+    w, wc, nw = read(write_csv, "WRITE_SIZE")
+    fs, ws = copy_bytes / 1024.0 / fc, copy_bytes / 1024.0 / wc
+    rd, wr = f * 1024.0 * fs, w * 1024.0 * ws
+    return {"kernel": kernel, "dispatches_seen": [nf, nw], "averaged_last": last,
+            "fetch_size_kib_per_launch": f, "write_size_kib_per_launch": w,
+            "copy_calibration": {"bytes": copy_bytes, "fetch_size_kib": fc, "write_size_kib": wc},
+            "fetch_scale": fs, "write_scale": ws, "read_bytes": rd, "write_bytes": wr,
+            "hbm_bytes_per_launch": rd + wr}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -364,12 +387,15 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=30.0,
                     help="arrival window of the serving leg")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="print the per-launch HBM traffic of the row build from two "
+                         "rocprofv3 --pmc counter_collection.csv files and exit")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 1 GiB device copy (scale of the rocprofv3 byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU only (replicas shard across GPUs with no "
-              "exchange; see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py: only one GPU is measured here (replicas and requests shard across "
+              "GPUs with no exchange); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     import dml
     from dml import workloads as W
@@ -379,8 +405,9 @@ def main():
     except Exception:
         sync = lambda: None
     legs = args.legs.split(",")
-    if args.calibrate_copy:
-        calibrate_copy()
+    if args.pmc_summary:
+        print(json.dumps(pmc_summary(*args.pmc_summary), indent=1))
+        return
     plain = batched = None
     if "plain" in legs:
         log("plain leg: C2 100k FP64")
@@ -468,6 +495,8 @@ def main():
     if not args.no_cpu:
         log("cpu baseline: oracle on the reference Ermak test")
         res["cpu_baseline"] = cpu_baseline()
+    if args.calibrate_copy:
+        calibrate_copy()
     print(json.dumps(res), flush=True)
 
 

@@ -40,6 +40,22 @@ def bcc_layers(lx, ly, nl, z_bottom=0.0, vac=0.0, rng=None, a=A_BCC):
     return np.vstack(pts)
 
 
+def pool_map(fn, jobs, workers, chunksize=1):
+    """map over a forked process pool whose workers are let to finish and
+    exit on their own (close + join, no terminate signal: a profiler's
+    signal handler in a forked child can otherwise stall the parent)"""
+    import multiprocessing as mp
+    pool = mp.get_context("fork").Pool(workers)
+    try:
+        out = pool.map(fn, jobs, chunksize=chunksize)
+        pool.close()
+        pool.join()
+    except BaseException:
+        pool.terminate()
+        raise
+    return out
+
+
 class _Near:
     """points of `existing` with zlo <= z < zhi, queried for neighbours
     closer than dmin under the x, y periodic minimum image"""
@@ -256,9 +272,7 @@ def rsa_fill_slabs(seed, n, lo, hi, box, existing=None, workers=16, slabs=32):
     for parity in (0, 1):
         js = jobs(parity)
         if workers > 1:
-            import multiprocessing as mp
-            with mp.get_context("fork").Pool(min(workers, len(js))) as pool:
-                res = pool.map(_slab_job, js)
+            res = pool_map(_slab_job, js, min(workers, len(js)))
         else:
             res = [_slab_job(j) for j in js]
         for j, r in zip(js, res):
