@@ -566,14 +566,25 @@ __global__ void __launch_bounds__(DML_BLOCK) k_listbuild_wc(DevState s) {
   const int64_t g_end = min(s.n_cm, (wid + 1) * run);
   int cur_r = -1;
   unsigned long long acc_nn = 0, acc_rows = 0;
-  for (int64_t g = wid * run; g < g_end; g++) {
-    const int r = s.cm_rep[g];
-    const int c = s.cm_cell[g];
+  for (int64_t g0 = wid * run; g0 < g_end; g0 += WAVE) {
+    // 64 cells of the run at once, one per lane: most cells of sparse
+    // replicas are empty, and their checks need no serial chain of loads
+    int lr = 0, lc = 0, lcnt = 0;
+    if (g0 + lane < g_end) {
+      lr = s.cm_rep[g0 + lane];
+      lc = s.cm_cell[g0 + lane];
+      const RepParams &Q = s.rp[lr];
+      if (!Q.halted && Q.rebuild && Q.lb_kind == 2 && lc < ncell_total(Q))
+        lcnt = min(s.cell_cnt[Q.coff + lc], Q.bcap);
+    }
+    unsigned long long work = __ballot(lcnt > 0);
+    while (work) {
+    const int q = __builtin_ctzll(work);
+    work &= work - 1;
+    const int r = __builtin_amdgcn_readlane(lr, q);
+    const int c = __builtin_amdgcn_readlane(lc, q);
+    const int ccnt = __builtin_amdgcn_readlane(lcnt, q);
     const RepParams &P = s.rp[r];
-    if (P.halted || !P.rebuild || P.lb_kind != 2 || c >= ncell_total(P)) continue;
-    int ccnt = s.cell_cnt[P.coff + c];
-    if (ccnt > P.bcap) ccnt = P.bcap;
-    if (ccnt == 0) continue;
     if (r != cur_r) {
       if (cur_r >= 0 && lane == 0 && acc_rows) {
         atomicAdd(&s.cnt[cur_r].nn_sum, acc_nn);
@@ -701,6 +712,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_listbuild_wc(DevState s) {
       acc_nn += v;
       acc_rows += (unsigned long long)__popcll(refm);
     }
+    }
   }
   if (cur_r >= 0 && lane == 0 && acc_rows) {
     atomicAdd(&s.cnt[cur_r].nn_sum, acc_nn);
@@ -713,16 +725,30 @@ __global__ void __launch_bounds__(DML_BLOCK) k_listbuild_wc(DevState s) {
 // insertion sort by lane 0)
 #define SORT_W 4
 __global__ void __launch_bounds__(DML_BLOCK) k_cell_sort_w(DevState s, int gcs) {
-  const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-  for (int64_t g = (int64_t)blockIdx.x * (DML_BLOCK / WAVE) + w; g < s.n_cm;
-       g += (int64_t)gridDim.x * (DML_BLOCK / WAVE)) {
-    const RepParams &P = s.rp[s.cm_rep[g]];
-    const int c = s.cm_cell[g];
-    if (P.halted || !want_cells(P, gcs) || c >= ncell_total(P)) continue;
+  const int lane = threadIdx.x % WAVE;
+  // each wave one contiguous run of the cell list, its metadata read 64
+  // cells at a time (one per lane), then the non-empty cells one by one
+  const int64_t wid = (int64_t)blockIdx.x * (DML_BLOCK / WAVE) + threadIdx.x / WAVE;
+  const int64_t nwv = (int64_t)gridDim.x * (DML_BLOCK / WAVE);
+  const int64_t run = (s.n_cm + nwv - 1) / nwv;
+  const int64_t g_end = min(s.n_cm, (wid + 1) * run);
+  for (int64_t g0 = wid * run; g0 < g_end; g0 += WAVE) {
+    int lr = 0, lc = 0, lcnt = 0;
+    if (g0 + lane < g_end) {
+      lr = s.cm_rep[g0 + lane];
+      lc = s.cm_cell[g0 + lane];
+      const RepParams &Q = s.rp[lr];
+      if (!Q.halted && want_cells(Q, gcs) && lc < ncell_total(Q))
+        lcnt = min(s.cell_cnt[Q.coff + lc], Q.bcap);
+    }
+    unsigned long long work = __ballot(lcnt > 0);
+    while (work) {
+    const int q = __builtin_ctzll(work);
+    work &= work - 1;
+    const RepParams &P = s.rp[__builtin_amdgcn_readlane(lr, q)];
+    const int c = __builtin_amdgcn_readlane(lc, q);
+    const int cnt = __builtin_amdgcn_readlane(lcnt, q);
     const int desc = P.use_cells;
-    int cnt = s.cell_cnt[P.coff + c];
-    if (cnt > P.bcap) cnt = P.bcap;
-    if (cnt == 0) continue;
     const int64_t base = P.boff + (int64_t)c * P.bcap;
     int32_t *b = s.bucket + base;
     if (cnt <= SORT_W * WAVE) {
@@ -779,6 +805,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_cell_sort_w(DevState s, int gcs) 
         s.bz[base + q] = s.pz[v];
         s.bf4[base + q] = s.pf4[v];
       }
+    }
     }
   }
 }
