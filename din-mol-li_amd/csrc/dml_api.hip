@@ -411,8 +411,14 @@ void enq_overlap_rounds(dml_ctx *c, const double *u_round, int64_t stride, int r
   hipLaunchKernelGGL(k_ov_begin, dim3(1), dim3(64), 0, c->st, c->d);
   const int lpa = c->lpa_ov;
   for (int q = 0; q < rounds; q++) {
+    // round 0 covers every ref position; later rounds walk their work lists
+    // (grid-stride, a bounded grid)
     unsigned gs = (unsigned)(c->P * lpa / DML_BLOCK);
     unsigned ga = (unsigned)(c->P / DML_BLOCK);
+    if (q > 0) {
+      gs = std::min(gs, 4096u * (unsigned)lpa);
+      ga = std::min(ga, 4096u);
+    }
     switch (lpa) {
       case 1:
         hipLaunchKernelGGL(k_ov_scan_l<1>, dim3(gs), dim3(DML_BLOCK), 0, c->st, c->d, u_round,
@@ -432,6 +438,8 @@ void enq_overlap_rounds(dml_ctx *c, const double *u_round, int64_t stride, int r
         break;
     }
     hipLaunchKernelGGL(k_ov_apply, dim3(ga), dim3(DML_BLOCK), 0, c->st, c->d, q, c->P);
+    if (q > 0)
+      hipLaunchKernelGGL(k_ov_apply_r, dim3(ga), dim3(DML_BLOCK), 0, c->st, c->d, q);
   }
   hipLaunchKernelGGL(k_ov_end, dim3((c->nrep + 63) / 64), dim3(64), 0, c->st, c->d, rounds,
                      err_if_pending);
@@ -723,7 +731,10 @@ int dml_create_ensemble(int32_t nrep, const dml_config *cfgs, dml_ctx **out) {
   e |= dalloc(&d.wl[0], P);
   e |= dalloc(&d.wl[1], P);
   e |= dalloc(&d.wl_cnt, 4);
-  e |= dalloc(&d.wl_stamp, P);
+  e |= dalloc(&d.ovw[0], P);
+  e |= dalloc(&d.ovw[1], P);
+  e |= dalloc(&d.ovr[0], P);
+  e |= dalloc(&d.ovr[1], P);
   e |= dalloc(&d.ov_more, 3 * nrep);
   e |= dalloc(&d.pf4, P);
   e |= dalloc(&d.gc_order, P);
@@ -746,7 +757,6 @@ int dml_create_ensemble(int32_t nrep, const dml_config *cfgs, dml_ctx **out) {
   hipMemsetAsync(d.nn, 0, sizeof(int32_t) * P, c->st);
   hipMemsetAsync(d.ov_rev, 0, sizeof(int32_t) * P, c->st);
   hipMemsetAsync(d.ov_st, 0, P, c->st);
-  hipMemsetAsync(d.wl_stamp, 0, sizeof(int32_t) * P, c->st);
   hipMemsetAsync(d.wl_cnt, 0, sizeof(int32_t) * 4, c->st);
   hipMemsetAsync(d.pf4, 0, sizeof(float4) * P, c->st);
   hipMemsetAsync(d.ref_pos, 0xff, sizeof(int32_t) * P, c->st);
@@ -788,7 +798,7 @@ int dml_destroy(dml_ctx *c) {
                   d.species, d.flags, d.rowflag, d.bkey, d.pid, d.ref_order, d.ref_tmp,
                   d.ref_pos, d.nn, d.nbr, d.cell_cnt, d.bucket, d.bx, d.by, d.bz, d.ov_rev,
                   d.ov_st, d.part_d, d.part_i, d.blk2rep, d.rp, d.cnt, d.err, d.wl[0],
-                  d.wl[1], d.wl_cnt, d.wl_stamp, d.ov_more, d.pf4, d.bf4, d.gc_order,
+                  d.wl[1], d.wl_cnt, d.ovw[0], d.ovw[1], d.ovr[0], d.ovr[1], d.ov_more, d.pf4, d.bf4, d.gc_order,
                   d.b2slot, d.seq, d.chx, d.chy, d.chz, d.chox, d.choy, d.choz,
                   d.cm_rep, d.cm_cell, d.tm_rep, d.tm_cell, d.wm_blk, d.obs};
   for (void *q : ptrs)

@@ -269,8 +269,11 @@ struct DevState {
   int32_t *ov_rev;        // round stamp of a revert
   uint8_t *ov_st;         // bit0 active bit1 dominated bit2 deposit bit3 self-revert
   int32_t *wl[2];         // overlap worklists (global ref positions), by round parity
-  int32_t *wl_cnt;        // [nW0, nR0, nW1, nR1]: active and appended-revert counts
-  int32_t *wl_stamp;      // per slot: last round stamp it was listed with
+  // overlap rounds after the first visit only their work lists: ovw[q % 2]
+  // the ref positions still unresolved (skip = 0) for round q, ovr[q % 2]
+  // the atoms reverted in round q; wl_cnt = [|ovw0|, |ovw1|, |ovr0|, |ovr1|]
+  int32_t *wl_cnt;
+  int32_t *ovw[2], *ovr[2];
   int32_t *ov_more;       // [3][nrep]: the replica reverted or deferred in a round
   int32_t *gc_order;      // gcmc group in list order (global slots)
   int32_t *b2slot;        // hs%b index -> global slot, -1 when free

@@ -24,6 +24,7 @@ __global__ void k_brownian(DevState s, const double *noise, const double *udep);
 __global__ void k_ermak_a(DevState s, const double *noise, const double *udep);
 __global__ void k_ov_exact(DevState s, const double *u, int64_t n_u, int max_passes);
 __global__ void k_ov_begin(DevState s);
+__global__ void k_ov_apply_r(DevState s, int round);
 __global__ void k_ov_apply(DevState s, int round, int64_t ptotal);
 __global__ void k_ov_end(DevState s, int rounds, int err_if_pending);
 __global__ void k_ov_close(DevState s, int rounds);

@@ -1255,14 +1255,18 @@ __global__ void __launch_bounds__(DML_BLOCK) k_ov_scan_l(DevState s, const doubl
   // progress flags rotate over three buffers: this round writes round % 3,
   // is gated by (round - 1) % 3, and clears (round + 1) % 3 for the next one
   const int cur = round % 3, prv = (round + 2) % 3, nxt = (round + 1) % 3;
-  if (blockIdx.x == 0)
+  if (blockIdx.x == 0) {
     for (int q = threadIdx.x; q < s.nrep; q += DML_BLOCK) s.ov_more[nxt * s.nrep + q] = 0;
-  const int64_t nitems = ptotal;
+    if (threadIdx.x == 0) s.wl_cnt[(round + 1) % 2] = 0;  // the next round's list
+  }
+  // round 0: every ref position; later rounds: the positions left unresolved
+  const int64_t nitems = round == 0 ? ptotal : (int64_t)s.wl_cnt[round % 2];
+  const int32_t *wl = s.ovw[round % 2];
   const int gtid = blockIdx.x * DML_BLOCK + threadIdx.x;
   const int sub = gtid % LPA;
   const int64_t stride = (int64_t)gridDim.x * (DML_BLOCK / LPA);
   for (int64_t item = gtid / LPA; item < nitems; item += stride) {
-    const int t = (int)item;
+    const int t = round == 0 ? (int)item : wl[item];
     const int r = s.blk2rep[t / DML_BLOCK];
     const RepParams &P = s.rp[r];
     if (P.halted || P.overlap_mode != DML_OVERLAP_ROUNDS) continue;
@@ -1312,7 +1316,12 @@ __global__ void __launch_bounds__(DML_BLOCK) k_ov_scan_l(DevState s, const doubl
     const int stamp = ov_stamp(P, round);
     auto revert = [&](int j) {
       if (P.piston && same_pos(s, j) && same_pos(s, i)) { n_ch3++; return; }
-      s.ov_rev[j] = stamp;
+      if (round == 0) {
+        s.ov_rev[j] = stamp;
+      } else if (atomicExch(&s.ov_rev[j], stamp) != stamp) {
+        // first to revert j in this round: it is applied from the list
+        s.ovr[round % 2][atomicAdd(&s.wl_cnt[2 + round % 2], 1)] = j;
+      }
       n_ch++;
     };
     if (!dom) {
@@ -1363,45 +1372,75 @@ template __global__ void k_ov_scan_l<4>(DevState, const double *, int64_t, int, 
 template __global__ void k_ov_scan_l<8>(DevState, const double *, int64_t, int, int64_t);
 template __global__ void k_ov_scan_l<16>(DevState, const double *, int64_t, int, int64_t);
 
+// one atom's outcome of a round (its own verdict st, a revert by a
+// neighbour), then its entry in the next round's list when still unresolved
+__device__ inline void ov_apply_one(const DevState &s, int r, const RepParams &P, int i, int t,
+                                    int round) {
+  const uint8_t st = s.ov_st[i];
+  uint8_t f = s.flags[i];
+  int moved = 0;
+  if ((st & 1) && !(st & 2)) f |= DML_FL_SKIP;
+  if (st & 4) {
+    atomicAdd(&s.cnt[r].depo, 1ull);
+    s.species[i] = SP_F;
+    s.rp[r].any_f = 1;
+    moved = 1;
+    if (s.pz[i] > P.z0) rep_fail(s, r, ERR_DEPZ0);
+  }
+  if (st & 8) {
+    s.px[i] = s.cx[i]; s.py[i] = s.cy[i]; s.pz[i] = s.cz[i];
+    f &= (uint8_t)~DML_FL_SKIP;
+    moved = 1;
+  }
+  if (s.ov_rev[i] == ov_stamp(P, round)) {
+    s.ov_rev[i] = 0;  // applied (stamps are positive)
+    s.px[i] = s.cx[i]; s.py[i] = s.cy[i]; s.pz[i] = s.cz[i];
+    s.vx[i] = 0.0; s.vy[i] = 0.0; s.vz[i] = 0.0;
+    s.ax[i] = 0.0; s.ay[i] = 0.0; s.az[i] = 0.0;
+    f &= (uint8_t)~DML_FL_SKIP;
+    moved = 1;
+  }
+  if (st || moved) {
+    s.flags[i] = f;
+    s.ov_st[i] = 0;
+  }
+  if (moved) put_pf4(s, i, s.px[i], s.py[i], s.pz[i], s.species[i]);
+  // the scan of the next round visits this atom when it is unresolved and
+  // its replica goes on (its scan reverted or deferred something)
+  if (!(f & DML_FL_SKIP) && s.ov_more[(round % 3) * s.nrep + r])
+    s.ovw[(round + 1) % 2][atomicAdd(&s.wl_cnt[(round + 1) % 2], 1)] = t;
+}
+
+// round 0: every ref position; later rounds: the positions scanned in it
 __global__ void __launch_bounds__(DML_BLOCK) k_ov_apply(DevState s, int round, int64_t ptotal) {
   const int prv = (round + 2) % 3;
+  if (blockIdx.x == 0 && threadIdx.x == 0) s.wl_cnt[2 + (round + 1) % 2] = 0;  // next reverts
+  const int64_t nitems = round == 0 ? ptotal : (int64_t)s.wl_cnt[round % 2];
+  const int32_t *wl = s.ovw[round % 2];
   const int64_t stride = (int64_t)gridDim.x * DML_BLOCK;
-  for (int64_t item = blockIdx.x * DML_BLOCK + threadIdx.x; item < ptotal; item += stride) {
-    const int t = (int)item;
+  for (int64_t item = blockIdx.x * DML_BLOCK + threadIdx.x; item < nitems; item += stride) {
+    const int t = round == 0 ? (int)item : wl[item];
     const int r = s.blk2rep[t / DML_BLOCK];
     const RepParams &P = s.rp[r];
     if (P.halted || P.overlap_mode != DML_OVERLAP_ROUNDS) continue;
     if (round > 0 && !s.ov_more[prv * s.nrep + r]) continue;  // scan skipped it
     if (t - P.poff >= P.nref) continue;
-    const int i = s.ref_order[t];
-    const uint8_t st = s.ov_st[i];
-    uint8_t f = s.flags[i];
-    int moved = 0;
-    if ((st & 1) && !(st & 2)) f |= DML_FL_SKIP;
-    if (st & 4) {
-      atomicAdd(&s.cnt[r].depo, 1ull);
-      s.species[i] = SP_F;
-      s.rp[r].any_f = 1;
-      moved = 1;
-      if (s.pz[i] > P.z0) rep_fail(s, r, ERR_DEPZ0);
-    }
-    if (st & 8) {
-      s.px[i] = s.cx[i]; s.py[i] = s.cy[i]; s.pz[i] = s.cz[i];
-      f &= (uint8_t)~DML_FL_SKIP;
-      moved = 1;
-    }
-    if (s.ov_rev[i] == ov_stamp(P, round)) {
-      s.px[i] = s.cx[i]; s.py[i] = s.cy[i]; s.pz[i] = s.cz[i];
-      s.vx[i] = 0.0; s.vy[i] = 0.0; s.vz[i] = 0.0;
-      s.ax[i] = 0.0; s.ay[i] = 0.0; s.az[i] = 0.0;
-      f &= (uint8_t)~DML_FL_SKIP;
-      moved = 1;
-    }
-    if (st || moved) {
-      s.flags[i] = f;
-      s.ov_st[i] = 0;
-    }
-    if (moved) put_pf4(s, i, s.px[i], s.py[i], s.pz[i], s.species[i]);
+    ov_apply_one(s, r, P, s.ref_order[t], t, round);
+  }
+}
+
+// rounds > 0: the atoms reverted in the round that were not scanned in it
+// (those that were are applied, and their stamp cleared, by k_ov_apply)
+__global__ void __launch_bounds__(DML_BLOCK) k_ov_apply_r(DevState s, int round) {
+  const int64_t nitems = s.wl_cnt[2 + round % 2];
+  const int32_t *rl = s.ovr[round % 2];
+  const int64_t stride = (int64_t)gridDim.x * DML_BLOCK;
+  for (int64_t item = blockIdx.x * DML_BLOCK + threadIdx.x; item < nitems; item += stride) {
+    const int j = rl[item];
+    const int r = s.blk2rep[j / DML_BLOCK];
+    const RepParams &P = s.rp[r];
+    if (P.halted || s.ov_rev[j] != ov_stamp(P, round)) continue;
+    ov_apply_one(s, r, P, j, P.poff + s.ref_pos[j], round);  // ref_pos is replica-local
   }
 }
 
