@@ -539,7 +539,11 @@ __global__ void __launch_bounds__(DML_BLOCK) k_listbuild_cells(DevState s) {
 // compaction, so rows keep the order of ngroup_cells (Neighbor.F90:509-535).
 // No LDS and no block barriers: candidate loads are shared by all atoms of
 // the cell through registers, and dense and sparse cells cost in proportion
-// to their work.
+// to their work.  With at least 4 cells in x and y, a candidate's x, y are
+// moved once per chunk to their image nearest the centre cell's midpoint,
+// which is the minimum image for every centre atom whenever the pair can lie
+// within the list radius (a pair whose images disagree is more than 1.5
+// cells apart either way), so the pair test needs no image arithmetic.
 // ------------------------------------------------------------------------
 __device__ inline float rdlane_f(float v, int a) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), a));
@@ -631,6 +635,9 @@ __global__ void __launch_bounds__(DML_BLOCK) k_listbuild_wc(DevState s) {
     const float rl2lo = (float)rl2 - P.pf_margin, rl2hi = (float)rl2 + P.pf_margin;
     const float bxf = (float)P.box[0], byf = (float)P.box[1];
     const float ibx = (float)P.one_box[0], iby = (float)P.one_box[1];
+    const int img = __builtin_amdgcn_readfirstlane(
+        (P.use_cells && P.ncells[0] >= 4 && P.ncells[1] >= 4) ? 1 : 0);
+    const float mxf = (float)((ci0 - 0.5) * P.cell[0]), myf = (float)((ci1 - 0.5) * P.cell[1]);
     for (int a0 = 0; a0 < ccnt; a0 += WAVE) {
       // centre atoms a0 .. a0+63 of this cell, one per lane
       const int na = (ccnt - a0) < WAVE ? (ccnt - a0) : WAVE;
@@ -664,6 +671,41 @@ __global__ void __launch_bounds__(DML_BLOCK) k_listbuild_wc(DevState s) {
         const int cj = s.bucket[pos];
         const float4 cf = s.bf4[pos];
         const int valid = cidx < total;
+        if (img) {
+          // x, y at the candidate's image nearest the centre cell's midpoint
+          // (a lane past the stream's end sits infinitely far away)
+          const float cx = valid ? cf.x - bxf * rintf((cf.x - mxf) * ibx) : 3.0e38f;
+          const float cy = cf.y - byf * rintf((cf.y - myf) * iby);
+          unsigned long long m = refm;
+          while (m) {
+            const int a = __builtin_ctzll(m);
+            m &= m - 1;
+            const int i = __builtin_amdgcn_readlane(my_i, a);
+            const float dx = cx - rdlane_f(my_f.x, a), dy = cy - rdlane_f(my_f.y, a);
+            const float dz = cf.z - rdlane_f(my_f.z, a);
+            const float df = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+            bool hit = df < rl2lo;
+            if (__ballot(!hit && df <= rl2hi)) {
+              if (!hit && df <= rl2hi) {
+                double ex, ey, ez;  // near the list radius: decide in FP64
+                vdist_mic(P, s.px[cj], s.py[cj], s.pz[cj], rdlane_d(my_x, a), rdlane_d(my_y, a),
+                          rdlane_d(my_z, a), &ex, &ey, &ez);
+                const double rd = dot3(ex, ey, ez);
+                hit = strict ? (rd < rl2) : !(rd > rl2);
+              }
+            }
+            hit = hit && (cj != i);
+            const unsigned long long hm = __ballot(hit);
+            if (hm) {
+              const int nn0 = __builtin_amdgcn_readlane(my_nn, a);
+              const int pr = __builtin_amdgcn_mbcnt_hi((uint32_t)(hm >> 32),
+                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)hm, 0u));
+              if (hit && nn0 + pr < maxnn) s.nbr[(int64_t)i * maxnn + nn0 + pr] = cj;
+              if (lane == a) my_nn = nn0 + __popcll(hm);
+            }
+          }
+          continue;
+        }
         unsigned long long m = refm;
         while (m) {
           const int a = __builtin_ctzll(m);
