@@ -1328,8 +1328,9 @@ __global__ void __launch_bounds__(DML_BLOCK) k_ov_scan_l(DevState s, const doubl
     const float rcf = (float)P.rcut2 + P.pf_margin;
     const int nn = s.nn[i];
     const int32_t *row = s.nbr + (int64_t)i * s.maxnn;
-    const int NH = 4;
-    int hj[NH], he[NH];
+    // up to four non-CG hits per lane kept in registers (named, so that no
+    // private array is needed), more fall back to a second pass
+    int hj0 = 0, hj1 = 0, hj2 = 0, hj3 = 0, he0 = 0, he1 = 0, he2 = 0, he3 = 0;
     int nh = 0, over = 0, dom = 0, firstcg = 0x7fffffff;
     for (int e = sub; e < nn; e += LPA) {
       int j = row[e];
@@ -1342,8 +1343,14 @@ __global__ void __launch_bounds__(DML_BLOCK) k_ov_scan_l(DevState s, const doubl
       if ((fj & DML_FL_REF) && !(fj & DML_FL_SKIP) && s.ref_pos[j] < local) dom = 1;
       if (s.species[j] == SP_CG) {
         if (e < firstcg) firstcg = e;
-      } else if (nh < NH) {
-        hj[nh] = j; he[nh] = e; nh++;
+      } else if (nh == 0) {
+        hj0 = j; he0 = e; nh = 1;
+      } else if (nh == 1) {
+        hj1 = j; he1 = e; nh = 2;
+      } else if (nh == 2) {
+        hj2 = j; he2 = e; nh = 3;
+      } else if (nh == 3) {
+        hj3 = j; he3 = e; nh = 4;
       } else {
         over = 1;
       }
@@ -1356,20 +1363,25 @@ __global__ void __launch_bounds__(DML_BLOCK) k_ov_scan_l(DevState s, const doubl
     }
     unsigned long long n_ch = 0, n_ch3 = 0;
     const int stamp = ov_stamp(P, round);
-    auto revert = [&](int j) {
-      if (P.piston && same_pos(s, j) && same_pos(s, i)) { n_ch3++; return; }
+    // a revert of neighbour j (or, piston, the "unsolvable" count when
+    // both sit where they were, dana.F90:920-927); returns 1 for the latter.
+    // Counted by the callers with arithmetic, keeping the counters in registers
+    auto revert = [&](int j) -> int {
+      if (P.piston && same_pos(s, j) && same_pos(s, i)) return 1;
       if (round == 0) {
         s.ov_rev[j] = stamp;
       } else if (atomicExch(&s.ov_rev[j], stamp) != stamp) {
         // first to revert j in this round: it is applied from the list
         s.ovr[round % 2][atomicAdd(&s.wl_cnt[2 + round % 2], 1)] = j;
       }
-      n_ch++;
+      return 0;
     };
     if (!dom) {
       if (!over) {
-        for (int q = 0; q < nh; q++)
-          if (he[q] < firstcg) revert(hj[q]);
+        if (nh > 0 && he0 < firstcg) { const int c3 = revert(hj0); n_ch3 += c3; n_ch += 1 - c3; }
+        if (nh > 1 && he1 < firstcg) { const int c3 = revert(hj1); n_ch3 += c3; n_ch += 1 - c3; }
+        if (nh > 2 && he2 < firstcg) { const int c3 = revert(hj2); n_ch3 += c3; n_ch += 1 - c3; }
+        if (nh > 3 && he3 < firstcg) { const int c3 = revert(hj3); n_ch3 += c3; n_ch += 1 - c3; }
       } else {
         for (int e = sub; e < nn && e < firstcg; e += LPA) {
           int j = row[e];
@@ -1377,7 +1389,9 @@ __global__ void __launch_bounds__(DML_BLOCK) k_ov_scan_l(DevState s, const doubl
           double dx, dy, dz;
           vdist_mic(P, xi, yi, zi, s.px[j], s.py[j], s.pz[j], &dx, &dy, &dz);
           if (dot3(dx, dy, dz) > P.rcut2 || s.species[j] == SP_CG) continue;
-          revert(j);
+          const int c3 = revert(j);
+          n_ch3 += c3;
+          n_ch += 1 - c3;
         }
       }
     }
