@@ -808,7 +808,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_cell_sort_w(DevState s, int gcs) 
       for (int j2 = 0; j2 < SORT_W; j2++) {
         const int lim = min(WAVE, cnt - j2 * WAVE);
         for (int q = 0; q < lim; q++) {
-          const int kq = __shfl(k[j2], q);
+          const int kq = __builtin_amdgcn_readlane(k[j2], q);  // q is wave-uniform
 #pragma unroll
           for (int j = 0; j < SORT_W; j++) rank[j] += desc ? (kq > k[j]) : (kq < k[j]);
         }
