@@ -382,8 +382,8 @@ def main():
     ap.add_argument("--legs", default="plain,batched,serving,c5")
     ap.add_argument("--drift-steps", type=int, default=10000,
                     help="Ermak steps of each FP32/FP64 energy-drift run on C2 (0 = skip)")
-    ap.add_argument("--c5-steps", type=int, default=200,
-                    help="steps of the C5 leg (its configuration runs 2e3)")
+    ap.add_argument("--c5-steps", type=int, default=2000,
+                    help="steps of the C5 leg (its configuration: 2e3)")
     ap.add_argument("--serve-seconds", type=float, default=30.0,
                     help="arrival window of the serving leg")
     ap.add_argument("--no-cpu", action="store_true")
