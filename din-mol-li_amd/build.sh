@@ -14,6 +14,11 @@ hipcc=${HIPCC:-/opt/rocm/bin/hipcc}
 mv "$out.tmp" "$out"
 echo "built $out"
 
+# host-side known-answer check of the Philox generator the kernels use
+"$hipcc" -O2 -std=c++17 "$src/philox_kat.cpp" -o "$here/dml/philox_kat" 2>&1 \
+  | grep -v "packed-fp32-ops" || true
+[[ -x "$here/dml/philox_kat" ]] || { echo "philox_kat build failed" >&2; exit 1; }
+
 # the Fortran side: ISO_C_BINDING interfaces and the dana driver on libdml
 fc=${FC_AMD:-/opt/rocm/bin/amdflang}
 if [[ -x "$fc" ]]; then
