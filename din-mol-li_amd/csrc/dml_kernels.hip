@@ -656,10 +656,11 @@ __global__ void __launch_bounds__(DML_BLOCK) k_listbuild_wc(DevState s) {
       const unsigned long long refm = __ballot(my_ref);
       if (!refm) continue;
       int my_nn = 0;
-      for (int q0 = 0; q0 < total; q0 += WAVE) {
-        const int cidx = q0 + lane;
+      // candidate cidx of the stream: its bucket entry (the stencil cell
+      // holding it by a binary search over the cell starts)
+      auto fetch = [&](int cidx, int *cj, float4 *cf) {
         const int cc = cidx < total ? cidx : total - 1;
-        int lo = 0, hi = 26;  // the stencil cell holding candidate cc
+        int lo = 0, hi = 26;
         for (int it = 0; it < 5; it++) {
           int mid = (lo + hi + 1) >> 1;
           int pm = __shfl(kpre, mid);
@@ -668,8 +669,18 @@ __global__ void __launch_bounds__(DML_BLOCK) k_listbuild_wc(DevState s) {
           }
         }
         const int64_t pos = __shfl(kbase, lo) + (cc - __shfl(kpre, lo));
-        const int cj = s.bucket[pos];
-        const float4 cf = s.bf4[pos];
+        *cj = s.bucket[pos];
+        *cf = s.bf4[pos];
+      };
+      // the next chunk's loads are issued before this chunk's tests
+      int nj;
+      float4 nf;
+      fetch(lane, &nj, &nf);
+      for (int q0 = 0; q0 < total; q0 += WAVE) {
+        const int cidx = q0 + lane;
+        const int cj = nj;
+        const float4 cf = nf;
+        if (q0 + WAVE < total) fetch(q0 + WAVE + lane, &nj, &nf);
         const int valid = cidx < total;
         if (img) {
           // x, y at the candidate's image nearest the centre cell's midpoint
