@@ -804,7 +804,34 @@ __global__ void __launch_bounds__(DML_BLOCK) k_cell_sort_w(DevState s, int gcs) 
     const int desc = P.use_cells;
     const int64_t base = P.boff + (int64_t)c * P.bcap;
     int32_t *b = s.bucket + base;
-    if (cnt <= SORT_W * WAVE) {
+    if (cnt <= WAVE) {
+      // one entry per lane; its key and position loads are issued together,
+      // before the ranks are known
+      const bool in = lane < cnt;
+      const int v = in ? b[lane] : 0;
+      int k = 0;
+      double x = 0.0, y = 0.0, z = 0.0;
+      float4 f = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (in) {
+        k = s.bkey[v];
+        x = s.px[v];
+        y = s.py[v];
+        z = s.pz[v];
+        f = s.pf4[v];
+      }
+      int rank = 0;
+      for (int q = 0; q < cnt; q++) {
+        const int kq = __builtin_amdgcn_readlane(k, q);  // q is wave-uniform
+        rank += desc ? (kq > k) : (kq < k);
+      }
+      if (in) {
+        b[rank] = v;
+        s.bx[base + rank] = x;
+        s.by[base + rank] = y;
+        s.bz[base + rank] = z;
+        s.bf4[base + rank] = f;
+      }
+    } else if (cnt <= SORT_W * WAVE) {
       // up to SORT_W entries per lane; an entry's place is the number of
       // keys before it (the hs%b keys of one bucket are distinct)
       int v[SORT_W], k[SORT_W], rank[SORT_W];
