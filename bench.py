@@ -102,12 +102,15 @@ def plain_leg(dml, W, steps, warmup, drift_steps=10000):
                                       zmax=c2["zmax"], integrator=dml.ERMAK,
                                       reservoir=dml.NO_RESERVOIR, precision=prec,
                                       overlap_mode=dml.OVERLAP_ROUNDS, seed=104012)
+    from dml import autotune
+    choice = autotune.tune([mk(dml.FP64)], [(c2["pos"], c2["species"])], log=log)
     ctx = dml.Context(mk(dml.FP64))
+    autotune.apply(ctx, choice)
     ctx.set_state(c2["pos"], species=c2["species"])
-    ctx.run(warmup, graph=False)
+    ctx.run(warmup, graph=True)
     ctx.sync()
     t0 = time.perf_counter()
-    ms = ctx.run(steps, graph=False)
+    ms = ctx.run(steps, graph=True)
     wall = time.perf_counter() - t0
     mean_nn, reb, st = ctx.list_stats()
     snap = ctx.state()
