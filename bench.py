@@ -188,6 +188,9 @@ def batched_leg(dml, nrep, steps, warmup):
     after = [ens.counters(k) for k in range(nrep)]
     psteps = sum(a["particle_steps"] - b["particle_steps"] for a, b in zip(after, before))
     trials = sum(a["ins"] + a["del_"] - b["ins"] - b["del_"] for a, b in zip(after, before))
+    # overlap calls of the timed steps that ran out of rounds with atoms
+    # still overlapping (the state carries into the next step)
+    ov_left = sum(a["ov_unfinished"] - b["ov_unfinished"] for a, b in zip(after, before))
     halted = sum(1 for k in range(nrep) if ens.replica_status(k) != 0)
     # dominant kernel: the neighbour-row build (see profiles/), timed on the
     # context's own stream over every replica; its rows are those of the ref
@@ -204,7 +207,7 @@ def batched_leg(dml, nrep, steps, warmup):
     return dict(n=ntot, ms=ms, wall=wall, rate=psteps / (ms * 1e-3), mean_nn=mean_nn,
                 rows_at_timing=rows_now, mean_nn_rows=mean_nn_w,
                 rebuilds_per_rep_step=reb / max(stepsum, 1), list_ms=kl_ms, force_ms=kf_ms,
-                halted=halted, gcmc_accepted=trials,
+                halted=halted, gcmc_accepted=trials, overlap_unfinished=ov_left,
                 tuning={k: choice[k] for k in ("lanes_per_atom", "list_kernel", "overlap_lanes")},
                 reservoirs={x: sum(1 for k in range(nrep) if RESERVOIR_CYCLE[k % 3] == x)
                             for x in RESERVOIR_CYCLE})
@@ -495,6 +498,7 @@ def main():
         res["reservoir_mix"] = batched["reservoirs"]
         res["tuning"] = batched["tuning"]
         res["gcmc_moves_accepted"] = batched["gcmc_accepted"]
+        res["overlap_calls_unfinished"] = batched["overlap_unfinished"]
     if not args.no_cpu:
         log("cpu baseline: oracle on the reference Ermak test")
         res["cpu_baseline"] = cpu_baseline()

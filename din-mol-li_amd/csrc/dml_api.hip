@@ -56,7 +56,7 @@ struct dml_ctx {
   // (0 or 1), 3 wave per cell
   int lb_mode = 3;
   int any_lb[3] = {0, 0, 1};  // some replica uses list kernel 0 / 1 / 2 (wave per cell)
-  int rounds = 8;
+  int rounds = 16;  // overlap rounds per step (the C3 ensemble resolves in < 16)
   int any_ermak = 0, any_brown = 0, any_rounds = 0, any_exact = 0, any_piston = 0, any_gcmc = 0;
   int any_chunks = 0;
   int64_t chunk_cap = 0;    // template atoms the device chunk pool holds
@@ -1723,6 +1723,7 @@ int dml_get_counters(dml_ctx *c, int32_t rep, dml_counters *out) {
   out->particle_steps = (int64_t)h.particle_steps;
   out->row_entries = (int64_t)h.nn_sum;
   out->rows = (int64_t)h.nn_rows;
+  out->ov_unfinished = (int64_t)h.ov_unfinished;
   return DML_OK;
 }
 

@@ -61,7 +61,7 @@ class Config(C.Structure):
 class Counters(C.Structure):
     _fields_ = [(n, C.c_int64) for n in ("try_", "depo", "choques", "choques3", "nupd",
                                          "ins", "del_", "passes", "particle_steps",
-                                         "row_entries", "rows")]
+                                         "row_entries", "rows", "ov_unfinished")]
 
 
 _lib = None

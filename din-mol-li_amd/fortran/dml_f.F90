@@ -45,7 +45,7 @@ module dml_f
   ! struct dml_counters
   type, bind(C) :: dml_counters
     integer(c_int64_t) :: try_, depo, choques, choques3, nupd, ins, del, passes
-    integer(c_int64_t) :: particle_steps, row_entries, rows
+    integer(c_int64_t) :: particle_steps, row_entries, rows, ov_unfinished
   end type dml_counters
 
   interface
