@@ -290,7 +290,12 @@ def c5_leg(dml, steps, warmup, trials=4096):
     cfg = dml.dana_config(box, sysd["skin"], n + 65536, h=sysd["h"], z0=z0, zmax=box[2],
                           integrator=dml.ERMAK, reservoir=dml.GCMC, precision=dml.FP32_FORCE,
                           overlap_mode=dml.OVERLAP_ROUNDS, act=0.01, nadj=10, seed=104017)
+    from dml import autotune
+    t0 = time.perf_counter()
+    choice = autotune.tune([cfg], [(sysd["pos"], sysd["species"])], log=log)
+    log("c5 autotune %.1f s" % (time.perf_counter() - t0))
     c = dml.Context(cfg)
+    autotune.apply(c, choice)
     c.set_state(sysd["pos"], species=sysd["species"])
     rng = np.random.default_rng(104017)
 
