@@ -208,7 +208,8 @@ def batched_leg(dml, nrep, steps, warmup):
                 rows_at_timing=rows_now, mean_nn_rows=mean_nn_w,
                 rebuilds_per_rep_step=reb / max(stepsum, 1), list_ms=kl_ms, force_ms=kf_ms,
                 halted=halted, gcmc_accepted=trials, overlap_unfinished=ov_left,
-                tuning={k: choice[k] for k in ("lanes_per_atom", "list_kernel", "overlap_lanes")},
+                tuning={k: choice[k] for k in ("lanes_per_atom", "list_kernel", "overlap_lanes",
+                                               "overlap_rounds")},
                 reservoirs={x: sum(1 for k in range(nrep) if RESERVOIR_CYCLE[k % 3] == x)
                             for x in RESERVOIR_CYCLE})
 

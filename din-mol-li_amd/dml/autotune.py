@@ -6,7 +6,9 @@ states), the launch shapes that make a step fastest:
 * lanes per ref atom of the force kernel (2..32),
 * the list kernel: cell tiles staged through LDS (1), one wave per atom (2),
   each replica by its atoms per cell (3) or one wave per cell (4),
-* lanes per ref atom of the overlap round scan (4, 8, 16).
+* lanes per ref atom of the overlap round scan (4, 8, 16),
+* overlap rounds per step: the fewest of 8, 16, 32 with which no overlap
+  call of the tuning steps ends with atoms still overlapping.
 
 Everything runs on a scratch context holding the same replicas, so the
 context being tuned keeps its exact trajectory.  The first two are timed per
@@ -27,6 +29,7 @@ from . import K_FORCE, K_LISTBUILD, Context
 FORCE_LANES = (2, 4, 8, 16, 32)
 LIST_KERNELS = (1, 2, 3, 4)
 OVERLAP_LANES = (4, 8, 16)
+OVERLAP_ROUNDS = (8, 16, 32)
 
 
 def signature(configs, sizes):
@@ -98,10 +101,22 @@ def tune(configs, states, steps=4, reps=10, cache_path=None, use_cache=True, log
                 ctx.run(1, graph=True)  # capture with this shape
                 t_ov[lanes].append(ctx.run(steps, graph=True) / steps)
         best_ov = min(t_ov, key=lambda x: min(t_ov[x]))
+        ctx.set_tuning(overlap_lanes=best_ov)
+        nrep = len(configs)
+        # from the most rounds down, stopping at the first count that leaves
+        # an overlap unresolved (its leftovers would burden the next trial)
+        best_rounds = OVERLAP_ROUNDS[-1]
+        for rounds in sorted(OVERLAP_ROUNDS, reverse=True):
+            ctx.set_tuning(overlap_rounds=rounds)
+            before = sum(ctx.counters(k)["ov_unfinished"] for k in range(nrep))
+            ctx.run(steps, graph=True)
+            if sum(ctx.counters(k)["ov_unfinished"] for k in range(nrep)) != before:
+                break
+            best_rounds = rounds
     finally:
         ctx.close()
     out = dict(lanes_per_atom=int(best_lanes), list_kernel=int(best_list),
-               overlap_lanes=int(best_ov),
+               overlap_lanes=int(best_ov), overlap_rounds=int(best_rounds),
                timings=dict(force_ms={str(k): v for k, v in t_force.items()},
                             list_ms={str(k): v for k, v in t_list.items()},
                             step_ms_by_overlap_lanes={str(k): min(v) for k, v in t_ov.items()}))
@@ -114,4 +129,5 @@ def tune(configs, states, steps=4, reps=10, cache_path=None, use_cache=True, log
 
 def apply(ctx, choice):
     ctx.set_tuning(lanes_per_atom=choice["lanes_per_atom"], list_kernel=choice["list_kernel"],
-                   overlap_lanes=choice["overlap_lanes"])
+                   overlap_lanes=choice["overlap_lanes"],
+                   overlap_rounds=choice.get("overlap_rounds", 16))

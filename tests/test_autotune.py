@@ -41,12 +41,15 @@ def test_autotune_choices_and_same_trajectory(dml, tmp_path):
     assert choice["lanes_per_atom"] in autotune.FORCE_LANES
     assert choice["list_kernel"] in autotune.LIST_KERNELS
     assert choice["overlap_lanes"] in autotune.OVERLAP_LANES
+    assert choice["overlap_rounds"] in autotune.OVERLAP_ROUNDS
     assert autotune.tune(cfgs, states, cache_path=cache)["cached"]
     out = []
     for tuned in (False, True):
         c = dml.Context(cfgs)
         if tuned:
             autotune.apply(c, choice)
+        else:  # the round count is part of the schedule, not a launch shape
+            c.set_tuning(overlap_rounds=choice["overlap_rounds"])
         for k, (p, s) in enumerate(states):
             c.set_state(p, species=s, rep=k)
         c.run(6, graph=True)
