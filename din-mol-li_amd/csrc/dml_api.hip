@@ -470,7 +470,7 @@ static void enq_gcmc(dml_ctx *c, int rep_only, const double *u, const double *g,
                            u, g, act, nadj);
     }
     hipLaunchKernelGGL(k_gc_count, dim3(nb), dim3(DML_BLOCK), 0, c->st, c->d, rep_only, stage);
-    hipLaunchKernelGGL(k_gc_scan, dim3(c->nrep), dim3(64), 0, c->st, c->d, rep_only, stage);
+    hipLaunchKernelGGL(k_gc_scan, dim3(c->nrep), dim3(1024), 0, c->st, c->d, rep_only, stage);
     hipLaunchKernelGGL(k_gc_scatter, dim3(nb), dim3(DML_BLOCK), 0, c->st, c->d, rep_only, stage);
   }
   hipLaunchKernelGGL(k_gc_copy, dim3(nb), dim3(DML_BLOCK), 0, c->st, c->d, rep_only);
@@ -479,7 +479,7 @@ static void enq_gcmc(dml_ctx *c, int rep_only, const double *u, const double *g,
 void enq_f2cg(dml_ctx *c) {
   int nb = c->d.nblocks;
   hipLaunchKernelGGL(k_f2cg_mark, dim3(nb), dim3(DML_BLOCK), 0, c->st, c->d);
-  hipLaunchKernelGGL(k_f2cg_scan, dim3(c->nrep), dim3(64), 0, c->st, c->d);
+  hipLaunchKernelGGL(k_f2cg_scan, dim3(c->nrep), dim3(1024), 0, c->st, c->d);
   hipLaunchKernelGGL(k_f2cg_scatter, dim3(nb), dim3(DML_BLOCK), 0, c->st, c->d);
   hipLaunchKernelGGL(k_f2cg_copy, dim3(nb), dim3(DML_BLOCK), 0, c->st, c->d);
   hipLaunchKernelGGL(k_f2cg_end, dim3((c->nrep + 63) / 64), dim3(64), 0, c->st, c->d);

@@ -1570,6 +1570,52 @@ __global__ void k_zero_i32(int32_t *a, int64_t n) {
   if (q < n) a[q] = 0;
 }
 
+// exclusive prefix sums of two per-block counters over one replica's
+// particle blocks, in chunks of SCAN_BLOCK by a whole workgroup (wave scans,
+// then the wave totals); returns the two totals to every thread
+#define SCAN_BLOCK 1024
+__device__ inline void rep_exscan2(int32_t *part, int stride, int b0, int nb, int *t1, int *t2) {
+  __shared__ int w1[SCAN_BLOCK / WAVE], w2[SCAN_BLOCK / WAVE];
+  const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
+  int c1 = 0, c2 = 0;
+  for (int q0 = 0; q0 < nb; q0 += SCAN_BLOCK) {
+    const int q = q0 + threadIdx.x;
+    const int v1 = q < nb ? part[stride * (b0 + q)] : 0;
+    const int v2 = (stride == 2 && q < nb) ? part[2 * (b0 + q) + 1] : 0;
+    int i1 = v1, i2 = v2;
+    for (int off = 1; off < WAVE; off <<= 1) {
+      const int x1 = __shfl_up(i1, off), x2 = __shfl_up(i2, off);
+      if (lane >= off) {
+        i1 += x1;
+        i2 += x2;
+      }
+    }
+    if (lane == WAVE - 1) {
+      w1[w] = i1;
+      w2[w] = i2;
+    }
+    __syncthreads();
+    int o1 = 0, o2 = 0, s1 = 0, s2 = 0;
+    for (int k = 0; k < SCAN_BLOCK / WAVE; k++) {
+      if (k < w) {
+        o1 += w1[k];
+        o2 += w2[k];
+      }
+      s1 += w1[k];
+      s2 += w2[k];
+    }
+    if (q < nb) {
+      part[stride * (b0 + q)] = c1 + o1 + i1 - v1;
+      if (stride == 2) part[2 * (b0 + q) + 1] = c2 + o2 + i2 - v2;
+    }
+    c1 += s1;
+    c2 += s2;
+    __syncthreads();  // w1, w2 are rewritten by the next chunk
+  }
+  *t1 = c1;
+  *t2 = c2;
+}
+
 // ------------------------------------------------------------------------
 // F -> CG at step end and removal from hs%ref (dana.F90:228-236): stable
 // compaction of each replica's ref order
@@ -1598,27 +1644,15 @@ __global__ void __launch_bounds__(DML_BLOCK) k_f2cg_mark(DevState s) {
   if (threadIdx.x == 0) s.part_i[blockIdx.x] = cnt;
 }
 
-// one wave per replica: exclusive scan of the block counts
-__global__ void k_f2cg_scan(DevState s) {
+// one workgroup per replica: exclusive scan of the block counts
+__global__ void __launch_bounds__(SCAN_BLOCK) k_f2cg_scan(DevState s) {
   int r = blockIdx.x;
   RepParams *Pw = s.rp + r;
   if (Pw->halted) return;
   if (!Pw->any_f) return;
-  int lane = threadIdx.x;
-  int b0 = Pw->poff / DML_BLOCK, nb = Pw->pcap / DML_BLOCK;
-  int carry = 0;
-  for (int q0 = 0; q0 < nb; q0 += WAVE) {
-    int q = q0 + lane;
-    int v = q < nb ? s.part_i[b0 + q] : 0;
-    int incl = v;
-    for (int off = 1; off < WAVE; off <<= 1) {
-      int x = __shfl_up(incl, off);
-      if (lane >= off) incl += x;
-    }
-    if (q < nb) s.part_i[b0 + q] = carry + incl - v;
-    carry += __shfl(incl, WAVE - 1);
-  }
-  if (lane == 0) Pw->nref_new = carry;
+  int t1, t2;
+  rep_exscan2(s.part_i, 1, Pw->poff / DML_BLOCK, Pw->pcap / DML_BLOCK, &t1, &t2);
+  if (threadIdx.x == 0) Pw->nref_new = t1;
 }
 
 __global__ void __launch_bounds__(DML_BLOCK) k_f2cg_scatter(DevState s) {
@@ -2058,42 +2092,22 @@ __global__ void __launch_bounds__(DML_BLOCK) k_gc_count(DevState s, int rep_only
   }
 }
 
-// one wave per replica: exclusive scans of both block counts
-__global__ void k_gc_scan(DevState s, int rep_only, int stage) {
+// one workgroup per replica: exclusive scans of both block counts
+__global__ void __launch_bounds__(SCAN_BLOCK) k_gc_scan(DevState s, int rep_only, int stage) {
   const int r = blockIdx.x;
   RepParams *Pw = s.rp + r;
   if (!gc_sel(*Pw, r, rep_only)) return;
-  const int lane = threadIdx.x;
   if (stage == 0) {
     if (Pw->halted) {  // nothing staged for this call
-      if (lane == 0) Pw->gc_pending = 0;
+      if (threadIdx.x == 0) Pw->gc_pending = 0;
       return;
     }
   } else if (Pw->gc_pending != 2) {
     return;
   }
-  const int b0 = Pw->poff / DML_BLOCK, nb = Pw->pcap / DML_BLOCK;
-  int c1 = 0, c2 = 0;
-  for (int q0 = 0; q0 < nb; q0 += WAVE) {
-    const int q = q0 + lane;
-    const int v1 = q < nb ? s.part_i[2 * (b0 + q)] : 0;
-    const int v2 = q < nb ? s.part_i[2 * (b0 + q) + 1] : 0;
-    int i1 = v1, i2 = v2;
-    for (int off = 1; off < WAVE; off <<= 1) {
-      const int x1 = __shfl_up(i1, off), x2 = __shfl_up(i2, off);
-      if (lane >= off) {
-        i1 += x1;
-        i2 += x2;
-      }
-    }
-    if (q < nb) {
-      s.part_i[2 * (b0 + q)] = c1 + i1 - v1;
-      s.part_i[2 * (b0 + q) + 1] = c2 + i2 - v2;
-    }
-    c1 += __shfl(i1, WAVE - 1);
-    c2 += __shfl(i2, WAVE - 1);
-  }
-  if (lane == 0) {
+  int c1, c2;
+  rep_exscan2(s.part_i, 2, Pw->poff / DML_BLOCK, Pw->pcap / DML_BLOCK, &c1, &c2);
+  if (threadIdx.x == 0) {
     if (stage == 0) {
       Pw->gc_len = c1;
       Pw->gc_inv = c2;
