@@ -1977,7 +1977,10 @@ __global__ void __launch_bounds__(DML_BLOCK) k_gcmc_energy(DevState s, int r, in
 // GCMC reservoir: gcmc_run (dana.F90:590-713) with the group bookkeeping it
 // drives (Groups.F90:730-766, 1058-1135; Cells.F90:105-175, 281-352;
 // Neighbor.F90:173-318).  Trials are sequential, as in the reference: one
-// workgroup per replica, each trial's scans spread over the block.
+// workgroup per replica, each trial's scans spread over the block.  The
+// cells' bucket-ordered positions are current here (a GCMC replica's cells
+// are sorted by the test_update before its trials, and insertions and
+// deletions keep them), so the scans read them instead of gathering.
 // ------------------------------------------------------------------------
 __device__ inline int blk_sum(int v, int *sh) {
   __syncthreads();
@@ -2283,7 +2286,8 @@ __global__ void __launch_bounds__(GC_BLOCK) k_gcmc_run(DevState s, int rep_only,
           const int o = s.bucket[base + q];
           if (!(s.flags[o] & DML_FL_GCMC)) continue;
           double dx, dy, dz;  // distance(o%pos, r): r - o%pos
-          vdist_mic(P, rx, ry, rz, s.px[o], s.py[o], s.pz[o], &dx, &dy, &dz);
+          vdist_mic(P, rx, ry, rz, s.bx[base + q], s.by[base + q], s.bz[base + q], &dx, &dy,
+                    &dz);
           if (dot3(dx, dy, dz) < rc2) sh_hit = 1;
         }
       }
@@ -2454,7 +2458,8 @@ __global__ void __launch_bounds__(GC_BLOCK) k_gcmc_run(DevState s, int rep_only,
                 j = s.bucket[base + q];
                 if (j != i) {
                   double dx, dy, dz;
-                  vdist_mic(P, s.px[j], s.py[j], s.pz[j], rx, ry, rz, &dx, &dy, &dz);
+                  vdist_mic(P, s.bx[base + q], s.by[base + q], s.bz[base + q], rx, ry, rz, &dx,
+                            &dy, &dz);
                   const double rd = dot3(dx, dy, dz);
                   hitq = P.use_cells ? (rd < rl2) : !(rd > rl2);
                 }
@@ -2483,7 +2488,8 @@ __global__ void __launch_bounds__(GC_BLOCK) k_gcmc_run(DevState s, int rep_only,
               const int aj = s.bucket[base + q];
               if (aj == i || !(s.flags[aj] & DML_FL_REF)) continue;
               double dx, dy, dz;
-              vdist_mic(P, rx, ry, rz, s.px[aj], s.py[aj], s.pz[aj], &dx, &dy, &dz);
+              vdist_mic(P, rx, ry, rz, s.bx[base + q], s.by[base + q], s.bz[base + q], &dx, &dy,
+                        &dz);
               if (dot3(dx, dy, dz) > rl2) continue;
               const int m = s.nn[aj];
               if (m < maxnn) {
