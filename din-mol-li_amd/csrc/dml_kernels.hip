@@ -1501,7 +1501,7 @@ __device__ inline void ov_apply_one(const DevState &s, int r, const RepParams &P
   if (moved) put_pf4(s, i, s.px[i], s.py[i], s.pz[i], s.species[i]);
   // the scan of the next round visits this atom when it is unresolved and
   // its replica goes on (its scan reverted or deferred something)
-  if (!(f & DML_FL_SKIP) && s.ov_more[(round % 3) * s.nrep + r])
+  if (!(f & DML_FL_SKIP) && t >= P.poff && s.ov_more[(round % 3) * s.nrep + r])
     s.ovw[(round + 1) % 2][atomicAdd(&s.wl_cnt[(round + 1) % 2], 1)] = t;
 }
 
