@@ -721,6 +721,7 @@ int dml_create_ensemble(int32_t nrep, const dml_config *cfgs, dml_ctx **out) {
   e |= dalloc(&d.nbr, P * (int64_t)maxnn);
   e |= dalloc(&d.ov_rev, P);
   e |= dalloc(&d.ov_st, P);
+  e |= dalloc(&d.ov_near, P);
   e |= dalloc(&d.part_d, 2 * (P / DML_BLOCK));
   e |= dalloc(&d.obs, 4 * (P / DML_BLOCK) + 4 * (int64_t)nrep);
   e |= dalloc(&d.part_i, 2 * (P / DML_BLOCK));
@@ -797,7 +798,7 @@ int dml_destroy(dml_ctx *c) {
                   d.epot, d.ox, d.oy, d.oz, d.cx, d.cy, d.cz, d.rvx, d.rvy, d.rvz,
                   d.species, d.flags, d.rowflag, d.bkey, d.pid, d.ref_order, d.ref_tmp,
                   d.ref_pos, d.nn, d.nbr, d.cell_cnt, d.bucket, d.bx, d.by, d.bz, d.ov_rev,
-                  d.ov_st, d.part_d, d.part_i, d.blk2rep, d.rp, d.cnt, d.err, d.wl[0],
+                  d.ov_st, d.ov_near, d.part_d, d.part_i, d.blk2rep, d.rp, d.cnt, d.err, d.wl[0],
                   d.wl[1], d.wl_cnt, d.ovw[0], d.ovw[1], d.ovr[0], d.ovr[1], d.ov_more, d.pf4, d.bf4, d.gc_order,
                   d.b2slot, d.seq, d.chx, d.chy, d.chz, d.chox, d.choy, d.choz,
                   d.cm_rep, d.cm_cell, d.tm_rep, d.tm_cell, d.wm_blk, d.obs};
@@ -912,6 +913,7 @@ int dml_set_state(dml_ctx *c, int32_t rep, int32_t n, const double *pos, const d
   }
   p.free_lo = 0;
   p.bfree_lo = 0;
+  p.ovn_stamp = 0;
   p.nslots = n;
   p.nref = (int32_t)rord.size();
   p.listed = 0;
@@ -1050,6 +1052,7 @@ int dml_set_box(dml_ctx *c, int32_t rep, const double box[3]) {
     p.box[k] = box[k];
     p.one_box[k] = 1.0 / box[k];
   }
+  p.ovn_stamp = 0;  // distances change with the box
   int oldcap = p.ccap;
   if ((rc = tessellate(p, 0))) return rc;
   int nb = bucket_cap(p);

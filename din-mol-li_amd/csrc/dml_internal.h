@@ -85,6 +85,10 @@ struct RepParams {
   // the stage the grid-wide list kernels reached (1 staged, 2 trials done,
   // 3 counted; reset at the next call)
   int32_t gc_len, gc_inv, ref_len, gc_pending;
+  // step + 1 of the fuerza call whose ov_near flags describe the current
+  // positions and rows (0: none; cleared by every kernel that moves or adds
+  // atoms)
+  int32_t ovn_stamp;
   // search hints: every slot below free_lo is alive or in limbo, every hs%b
   // index below bfree_lo is taken (lowered wherever one is freed)
   int32_t free_lo, bfree_lo;
@@ -268,6 +272,7 @@ struct DevState {
   // overlap rounds
   int32_t *ov_rev;        // round stamp of a revert
   uint8_t *ov_st;         // bit0 active bit1 dominated bit2 deposit bit3 self-revert
+  uint8_t *ov_near;       // fuerza saw a row entry within rcut (FP32, with margin)
   int32_t *wl[2];         // overlap worklists (global ref positions), by round parity
   // overlap rounds after the first visit only their work lists: ovw[q % 2]
   // the ref positions still unresolved (skip = 0) for round q, ovr[q % 2]

@@ -939,8 +939,11 @@ __global__ void __launch_bounds__(DML_BLOCK) k_force(DevState s, int only_ermak)
   int active = local < P.nref;
   int i = active ? s.ref_order[t] : 0;
   double fx = 0.0, fy = 0.0, fz = 0.0, ea = 0.0;
+  int near = 0;  // an entry within rcut (FP32 with margin): overlap_moveback will look
+  if (local == 0 && sub == 0) s.rp[r].ovn_stamp = P.step + 1;
   if (active) {
     int k = s.species[i];
+    const float rcf = (float)P.rcut2 + P.pf_margin;
     double xi = s.px[i], yi = s.py[i], zi = s.pz[i];
     double hbx = P.box[0] * .5, hby = P.box[1] * .5;
     int ghost = s.rowflag[i] & 1;
@@ -954,7 +957,9 @@ __global__ void __launch_bounds__(DML_BLOCK) k_force(DevState s, int only_ermak)
       const float4 fj4 = s.pf4[j];
       {  // surely beyond the pair's r0: no force, no energy
         float r0f = (float)P.r0[(k - 1) * 3 + (__float_as_int(fj4.w) - 1)];
-        if (d2_f32(P, fi, fj4, bxf, byf, ibx, iby) > r0f * r0f + P.pf_margin) continue;
+        const float d2f = d2_f32(P, fi, fj4, bxf, byf, ibx, iby);
+        near |= d2f <= rcf;
+        if (d2f > r0f * r0f + P.pf_margin) continue;
       }
       uint8_t fj = s.flags[j];
       if (fj & DML_FL_LIMBO) continue;
@@ -1007,8 +1012,10 @@ __global__ void __launch_bounds__(DML_BLOCK) k_force(DevState s, int only_ermak)
     fy += __shfl_xor(fy, off);
     fz += __shfl_xor(fz, off);
     ea += __shfl_xor(ea, off);
+    near |= __shfl_xor(near, off);
   }
   if (active && sub == 0) {
+    s.ov_near[i] = (uint8_t)near;
     s.fx[i] = fx;
     s.fy[i] = fy;
     s.fz[i] = fz;
@@ -1136,6 +1143,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_brownian(DevState s, const double
   const RepParams &P = rep_of_block(s, blockIdx.x, &r);
   if (P.halted) return;
   if (P.integrator != DML_BROWNIAN) return;
+  if (t == P.poff) s.rp[r].ovn_stamp = 0;  // positions move
   if (t - P.poff >= P.nref) return;
   int i = s.ref_order[t];
   double x = s.px[i], y = s.py[i], z = s.pz[i];
@@ -1167,6 +1175,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_ermak_a(DevState s, const double 
   const RepParams &P = rep_of_block(s, blockIdx.x, &r);
   if (P.halted) return;
   if (P.integrator != DML_ERMAK) return;
+  if (t == P.poff) s.rp[r].ovn_stamp = 0;  // positions move
   if (t - P.poff >= P.nref) return;
   int i = s.ref_order[t];
   double x = s.px[i], y = s.py[i], z = s.pz[i];
@@ -1359,6 +1368,13 @@ __global__ void __launch_bounds__(DML_BLOCK) k_ov_scan_l(DevState s, const doubl
       if (sub == 0) s.ov_st[i] = 0;
       continue;
     }
+    // fuerza of this step scanned this same row at these same positions
+    // (no rebuild since) and met nothing within rcut: the scan would find no
+    // contact, no neighbour to revert and no earlier atom to wait for
+    if (round == 0 && !P.rebuild && P.ovn_stamp == P.step + 1 && !s.ov_near[i]) {
+      if (sub == 0) s.ov_st[i] = 1;
+      continue;
+    }
     const double xi = s.px[i], yi = s.py[i], zi = s.pz[i];
     const float4 fi = s.pf4[i];
     const float bxf = (float)P.box[0], byf = (float)P.box[1];
@@ -1517,6 +1533,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_ov_apply(DevState s, int round, i
     const int r = s.blk2rep[t / DML_BLOCK];
     const RepParams &P = s.rp[r];
     if (P.halted || P.overlap_mode != DML_OVERLAP_ROUNDS) continue;
+    if (round == 0 && t == P.poff) s.rp[r].ovn_stamp = 0;  // reverts move atoms
     if (round > 0 && !s.ov_more[prv * s.nrep + r]) continue;  // scan skipped it
     if (t - P.poff >= P.nref) continue;
     ov_apply_one(s, r, P, s.ref_order[t], t, round);
@@ -1807,6 +1824,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_maxz(DevState s) {
   const RepParams &P = rep_of_block(s, blockIdx.x, &r);
   if (P.halted) return;
   if (!P.piston) return;
+  if (slot == P.poff) s.rp[r].ovn_stamp = 0;  // positions move
   int local = slot - P.poff;
   if (local < P.nslots && (s.flags[slot] & DML_FL_ALIVE)) {
     double z = s.pz[slot];
@@ -2619,6 +2637,7 @@ __global__ void __launch_bounds__(GC_BLOCK) k_gcmc_run(DevState s, int rep_only,
     Pw->gc_len = glen;
     Pw->ref_len = rlen;
     Pw->gc_pending = 2;
+    if (n_ins || n_del) Pw->ovn_stamp = 0;  // atoms added or removed
     Pw->free_lo = hs_scan;
     Pw->bfree_lo = nfreed_b ? min(b_scan, sh_freed_b[0]) : b_scan;
     Pw->nslots = nslots;
@@ -2753,6 +2772,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_chunk_run(DevState s) {
     Pw->seq_next = P.seq_next + nch;
     Pw->listed = 0;  // hs%listed = .false.: the next test_update rebuilds
     Pw->chunk_fired = 1;
+    Pw->ovn_stamp = 0;  // atoms added
   }
 }
 
