@@ -414,10 +414,13 @@ void enq_overlap_rounds(dml_ctx *c, const double *u_round, int64_t stride, int r
     // round 0 covers every ref position; later rounds walk their work lists
     // (grid-stride, a bounded grid)
     unsigned gs = (unsigned)(c->P * lpa / DML_BLOCK);
-    unsigned ga = (unsigned)(c->P / DML_BLOCK);
+    const unsigned chunk = DML_BLOCK * OVA_PER;
+    unsigned ga = (unsigned)((c->P + chunk - 1) / chunk);
+    unsigned gr = (unsigned)(c->P / DML_BLOCK);
     if (q > 0) {
       gs = std::min(gs, 4096u * (unsigned)lpa);
-      ga = std::min(ga, 4096u);
+      ga = std::min(ga, 2048u);
+      gr = std::min(gr, 4096u);
     }
     switch (lpa) {
       case 1:
@@ -439,7 +442,7 @@ void enq_overlap_rounds(dml_ctx *c, const double *u_round, int64_t stride, int r
     }
     hipLaunchKernelGGL(k_ov_apply, dim3(ga), dim3(DML_BLOCK), 0, c->st, c->d, q, c->P);
     if (q > 0)
-      hipLaunchKernelGGL(k_ov_apply_r, dim3(ga), dim3(DML_BLOCK), 0, c->st, c->d, q);
+      hipLaunchKernelGGL(k_ov_apply_r, dim3(gr), dim3(DML_BLOCK), 0, c->st, c->d, q);
   }
   hipLaunchKernelGGL(k_ov_end, dim3((c->nrep + 63) / 64), dim3(64), 0, c->st, c->d, rounds,
                      err_if_pending);

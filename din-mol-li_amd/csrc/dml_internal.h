@@ -16,6 +16,7 @@
 #include "../../include/dml.h"
 
 #define DML_BLOCK 256
+#define OVA_PER 8  // items per thread of one k_ov_apply block pass
 #define DML_GCMC_BLOCK 256  // one workgroup per GCMC replica
 
 // species codes (dana.F90:81-84)

@@ -1483,9 +1483,9 @@ template __global__ void k_ov_scan_l<8>(DevState, const double *, int64_t, int, 
 template __global__ void k_ov_scan_l<16>(DevState, const double *, int64_t, int, int64_t);
 
 // one atom's outcome of a round (its own verdict st, a revert by a
-// neighbour), then its entry in the next round's list when still unresolved
-__device__ inline void ov_apply_one(const DevState &s, int r, const RepParams &P, int i, int t,
-                                    int round) {
+// neighbour); returns 1 when the atom belongs in the next round's list
+__device__ inline int ov_apply_one(const DevState &s, int r, const RepParams &P, int i, int t,
+                                   int round) {
   const uint8_t st = s.ov_st[i];
   uint8_t f = s.flags[i];
   int moved = 0;
@@ -1517,26 +1517,59 @@ __device__ inline void ov_apply_one(const DevState &s, int r, const RepParams &P
   if (moved) put_pf4(s, i, s.px[i], s.py[i], s.pz[i], s.species[i]);
   // the scan of the next round visits this atom when it is unresolved and
   // its replica goes on (its scan reverted or deferred something)
-  if (!(f & DML_FL_SKIP) && t >= P.poff && s.ov_more[(round % 3) * s.nrep + r])
-    s.ovw[(round + 1) % 2][atomicAdd(&s.wl_cnt[(round + 1) % 2], 1)] = t;
+  return !(f & DML_FL_SKIP) && t >= P.poff && s.ov_more[(round % 3) * s.nrep + r];
 }
 
-// round 0: every ref position; later rounds: the positions scanned in it
+// round 0: every ref position; later rounds: the positions scanned in it.
+// A block takes OVA_PER * DML_BLOCK consecutive items at a time and appends
+// the unresolved ones to the next round's list with one atomic (a per-item
+// or per-wave atomic on the single list counter serialises across the chip);
+// the list order is free, every round's outcome depends on its start state only
 __global__ void __launch_bounds__(DML_BLOCK) k_ov_apply(DevState s, int round, int64_t ptotal) {
+  __shared__ int32_t buf[DML_BLOCK * OVA_PER];
+  __shared__ int wcnt[DML_BLOCK / WAVE];
+  __shared__ int64_t base_sh;
   const int prv = (round + 2) % 3;
+  const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
   if (blockIdx.x == 0 && threadIdx.x == 0) s.wl_cnt[2 + (round + 1) % 2] = 0;  // next reverts
   const int64_t nitems = round == 0 ? ptotal : (int64_t)s.wl_cnt[round % 2];
   const int32_t *wl = s.ovw[round % 2];
-  const int64_t stride = (int64_t)gridDim.x * DML_BLOCK;
-  for (int64_t item = blockIdx.x * DML_BLOCK + threadIdx.x; item < nitems; item += stride) {
-    const int t = round == 0 ? (int)item : wl[item];
-    const int r = s.blk2rep[t / DML_BLOCK];
-    const RepParams &P = s.rp[r];
-    if (P.halted || P.overlap_mode != DML_OVERLAP_ROUNDS) continue;
-    if (round == 0 && t == P.poff) s.rp[r].ovn_stamp = 0;  // reverts move atoms
-    if (round > 0 && !s.ov_more[prv * s.nrep + r]) continue;  // scan skipped it
-    if (t - P.poff >= P.nref) continue;
-    ov_apply_one(s, r, P, s.ref_order[t], t, round);
+  int32_t *next = s.ovw[(round + 1) % 2];
+  int32_t *wb = buf + w * WAVE * OVA_PER;
+  const int64_t chunk = (int64_t)DML_BLOCK * OVA_PER;
+  for (int64_t b0 = blockIdx.x * chunk; b0 < nitems; b0 += (int64_t)gridDim.x * chunk) {
+    int n = 0;  // this wave's appends so far (wave-uniform)
+    for (int k = 0; k < OVA_PER; k++) {
+      const int64_t item = b0 + (int64_t)k * DML_BLOCK + threadIdx.x;
+      int push = 0, t = 0;
+      if (item < nitems) {
+        t = round == 0 ? (int)item : wl[item];
+        const int r = s.blk2rep[t / DML_BLOCK];
+        const RepParams &P = s.rp[r];
+        if (!P.halted && P.overlap_mode == DML_OVERLAP_ROUNDS) {
+          if (round == 0 && t == P.poff) s.rp[r].ovn_stamp = 0;  // reverts move atoms
+          // round > 0 and a replica whose scan stopped: nothing to apply
+          if ((round == 0 || s.ov_more[prv * s.nrep + r]) && t - P.poff < P.nref)
+            push = ov_apply_one(s, r, P, s.ref_order[t], t, round);
+        }
+      }
+      const unsigned long long m = __ballot(push);
+      if (push) wb[n + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                      __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u))] = t;
+      n += __popcll(m);
+    }
+    if (lane == 0) wcnt[w] = n;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int tot = 0;
+      for (int q = 0; q < DML_BLOCK / WAVE; q++) tot += wcnt[q];
+      base_sh = tot ? (int64_t)atomicAdd(&s.wl_cnt[(round + 1) % 2], tot) : 0;
+    }
+    __syncthreads();
+    int64_t off = base_sh;
+    for (int q = 0; q < w; q++) off += wcnt[q];
+    for (int q = lane; q < n; q += WAVE) next[off + q] = wb[q];
+    __syncthreads();  // buf, wcnt and base_sh are reused
   }
 }
 
@@ -1551,7 +1584,9 @@ __global__ void __launch_bounds__(DML_BLOCK) k_ov_apply_r(DevState s, int round)
     const int r = s.blk2rep[j / DML_BLOCK];
     const RepParams &P = s.rp[r];
     if (P.halted || s.ov_rev[j] != ov_stamp(P, round)) continue;
-    ov_apply_one(s, r, P, j, P.poff + s.ref_pos[j], round);  // ref_pos is replica-local
+    const int t = P.poff + s.ref_pos[j];  // ref_pos is replica-local
+    if (ov_apply_one(s, r, P, j, t, round))
+      s.ovw[(round + 1) % 2][atomicAdd(&s.wl_cnt[(round + 1) % 2], 1)] = t;
   }
 }
 

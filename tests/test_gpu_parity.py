@@ -250,7 +250,7 @@ def test_ermak_step_matches_oracle(orc, dml):
     c.close()
 
 
-def _step_pair(orc, dml, s, c, rng, mode, nsteps, brown=True, gcmc=False):
+def _step_pair(orc, dml, s, c, rng, mode, nsteps, brown=True, gcmc=False, r1s=None):
     """integrator + test_update + overlap + test_update + F->CG, on both"""
     for k in range(nsteps):
         nref = s.nref
@@ -270,6 +270,8 @@ def _step_pair(orc, dml, s, c, rng, mode, nsteps, brown=True, gcmc=False):
         assert st == 0
         r1d = c.test_update()[0]
         assert r1 == r1d, k
+        if r1s is not None:
+            r1s.append(r1)
         if mode == "exact":
             u = rng.random(4096)
             st, used = s.overlap(u)
@@ -326,6 +328,58 @@ def test_ermak_piston_steps_match_oracle(orc, dml):
     s.fuerza(); c.forces()
     so, sd = _step_pair(orc, dml, s, c, rng, "exact", 6, brown=False)
     assert rel_err(sd["vel"][so["species"] != 2], so["vel"][so["species"] != 2]) <= 1e-12
+    c.close()
+
+
+def test_ermak_steps_with_round_overlap_match_oracle(orc, dml):
+    """Ermak steps with the round schedule: fuerza runs between the move and
+    the overlap pass, so on steps whose first skin test does not rebuild,
+    round 0 takes the atoms whose force row held nothing within rcut as
+    resolved without scanning them; the oracle scans every atom."""
+    rng = np.random.default_rng(31)
+    box = np.array([40.0, 40.0, 60.0])
+    el = electrode(3.51, 11, 11, 2, 3.51)
+    ions = rsa(rng, 700, np.array([0, 0, 6.0]), np.array([40, 40, 59.0]), 3.2, box)
+    pos = np.vstack([el, ions])
+    z = np.array([2] * len(el) + [1] * len(ions), np.int8)
+    say("ermak round steps", len(pos))
+    s, c = pair(orc, dml, box, 10.0, pos, z, zmax=60.0, z0=30.0, h=1e-2,
+                overlap_mode=dml.OVERLAP_ROUNDS)
+    s.test_update(); c.test_update()
+    s.fuerza(); c.forces()
+    r1s = []
+    so, sd = _step_pair(orc, dml, s, c, rng, "rounds", 10, brown=False, r1s=r1s)
+    assert rel_err(sd["vel"][so["species"] != 2], so["vel"][so["species"] != 2]) <= 1e-12
+    assert 0 in r1s  # some step kept its rows, so the shortcut was open
+    cnt_o, cnt_d = s.c, c.counters()
+    assert cnt_o.choques == cnt_d["choques"]
+    assert cnt_o.choques > 0
+    c.close()
+
+
+@pytest.mark.parametrize("lb_mode", ["3", "1"])
+def test_brownian_round_overlap_after_rebuilds_matches_oracle(orc, dml, lb_mode, monkeypatch):
+    """Brownian steps (h 3e-3: some first skin tests rebuild, some do not)
+    with the round schedule, on rows from the wave-per-cell (lb_mode 3) and
+    the wave-per-atom (lb_mode 1) row builds."""
+    monkeypatch.setenv("DML_LISTBUILD", lb_mode)
+    rng = np.random.default_rng(31)
+    box = np.array([40.0, 40.0, 60.0])
+    el = electrode(3.51, 11, 11, 2, 3.51)
+    ions = rsa(rng, 700, np.array([0, 0, 6.0]), np.array([40, 40, 59.0]), 3.2, box)
+    pos = np.vstack([el, ions])
+    z = np.array([2] * len(el) + [1] * len(ions), np.int8)
+    say("brownian round steps", len(pos), "lb_mode", lb_mode)
+    s, c = pair(orc, dml, box, 10.0, pos, z, ermak=False, zmax=60.0, z0=30.0, h=3e-3,
+                overlap_mode=dml.OVERLAP_ROUNDS)
+    s.test_update(); c.test_update()
+    r1s = []
+    _step_pair(orc, dml, s, c, rng, "rounds", 10, r1s=r1s)
+    assert 0 in r1s and 1 in r1s
+    cnt_o, cnt_d = s.c, c.counters()
+    assert cnt_o.choques == cnt_d["choques"]
+    assert cnt_o.depo == cnt_d["depo"]
+    assert cnt_o.choques > 0
     c.close()
 
 
