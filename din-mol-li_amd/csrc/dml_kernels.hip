@@ -695,24 +695,40 @@ __global__ void __launch_bounds__(DML_BLOCK) k_listbuild_wc(DevState s) {
             const float dx = cx - rdlane_f(my_f.x, a), dy = cy - rdlane_f(my_f.y, a);
             const float dz = cf.z - rdlane_f(my_f.z, a);
             const float df = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-            bool hit = df < rl2lo;
-            if (__ballot(!hit && df <= rl2hi)) {
-              if (!hit && df <= rl2hi) {
-                double ex, ey, ez;  // near the list radius: decide in FP64
+            // hits append in candidate order; the masks stay in scalar
+            // registers (no per-lane 0/1 round trip through a ballot)
+            auto append = [&](bool hit, unsigned long long hm) {
+              if (!hm) return;
+              const int nn0 = __builtin_amdgcn_readlane(my_nn, a);
+              const int nh = __popcll(hm);
+              const int pr = __builtin_amdgcn_mbcnt_hi((uint32_t)(hm >> 32),
+                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)hm, 0u));
+              int32_t *row = s.nbr + (int64_t)i * maxnn + nn0;
+              if (nn0 + nh <= maxnn) {
+                if (hit) row[pr] = cj;
+              } else if (hit && nn0 + pr < maxnn) {
+                row[pr] = cj;
+              }
+              my_nn = lane == a ? nn0 + nh : my_nn;
+            };
+            const unsigned long long gem = __builtin_amdgcn_ballot_w64(df >= rl2lo);
+            if (gem & __builtin_amdgcn_ballot_w64(df <= rl2hi)) {
+              // near the list radius: decide in FP64 (rare)
+              bool hit = df < rl2lo;
+              if (df >= rl2lo && df <= rl2hi) {
+                double ex, ey, ez;
                 vdist_mic(P, s.px[cj], s.py[cj], s.pz[cj], rdlane_d(my_x, a), rdlane_d(my_y, a),
                           rdlane_d(my_z, a), &ex, &ey, &ez);
                 const double rd = dot3(ex, ey, ez);
                 hit = strict ? (rd < rl2) : !(rd > rl2);
               }
-            }
-            hit = hit && (cj != i);
-            const unsigned long long hm = __ballot(hit);
-            if (hm) {
-              const int nn0 = __builtin_amdgcn_readlane(my_nn, a);
-              const int pr = __builtin_amdgcn_mbcnt_hi((uint32_t)(hm >> 32),
-                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)hm, 0u));
-              if (hit && nn0 + pr < maxnn) s.nbr[(int64_t)i * maxnn + nn0 + pr] = cj;
-              if (lane == a) my_nn = nn0 + __popcll(hm);
+              const bool h = hit && cj != i;
+              append(h, __builtin_amdgcn_ballot_w64(h));
+            } else {
+              // a ballot of a plain compare is that compare's own mask; the
+              // hits are the lanes below rl2lo (not in gem) that are not i
+              const bool other = cj != i;
+              append(!(df >= rl2lo) && other, ~gem & __builtin_amdgcn_ballot_w64(other));
             }
           }
           continue;
