@@ -725,6 +725,7 @@ int dml_create_ensemble(int32_t nrep, const dml_config *cfgs, dml_ctx **out) {
   e |= dalloc(&d.ov_rev, P);
   e |= dalloc(&d.ov_st, P);
   e |= dalloc(&d.ov_near, P);
+  e |= dalloc(&d.nearl, P * (int64_t)NEAR_MAX);
   e |= dalloc(&d.part_d, 2 * (P / DML_BLOCK));
   e |= dalloc(&d.obs, 4 * (P / DML_BLOCK) + 4 * (int64_t)nrep);
   e |= dalloc(&d.part_i, 2 * (P / DML_BLOCK));
@@ -801,7 +802,7 @@ int dml_destroy(dml_ctx *c) {
                   d.epot, d.ox, d.oy, d.oz, d.cx, d.cy, d.cz, d.rvx, d.rvy, d.rvz,
                   d.species, d.flags, d.rowflag, d.bkey, d.pid, d.ref_order, d.ref_tmp,
                   d.ref_pos, d.nn, d.nbr, d.cell_cnt, d.bucket, d.bx, d.by, d.bz, d.ov_rev,
-                  d.ov_st, d.ov_near, d.part_d, d.part_i, d.blk2rep, d.rp, d.cnt, d.err, d.wl[0],
+                  d.ov_st, d.ov_near, d.nearl, d.part_d, d.part_i, d.blk2rep, d.rp, d.cnt, d.err, d.wl[0],
                   d.wl[1], d.wl_cnt, d.ovw[0], d.ovw[1], d.ovr[0], d.ovr[1], d.ov_more, d.pf4, d.bf4, d.gc_order,
                   d.b2slot, d.seq, d.chx, d.chy, d.chz, d.chox, d.choy, d.choz,
                   d.cm_rep, d.cm_cell, d.tm_rep, d.tm_cell, d.wm_blk, d.obs};

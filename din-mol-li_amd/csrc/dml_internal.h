@@ -17,6 +17,7 @@
 
 #define DML_BLOCK 256
 #define OVA_PER 8  // items per thread of one k_ov_apply block pass
+#define NEAR_MAX 4  // row positions within rcut listed per atom by the row build
 #define DML_GCMC_BLOCK 256  // one workgroup per GCMC replica
 
 // species codes (dana.F90:81-84)
@@ -86,10 +87,12 @@ struct RepParams {
   // the stage the grid-wide list kernels reached (1 staged, 2 trials done,
   // 3 counted; reset at the next call)
   int32_t gc_len, gc_inv, ref_len, gc_pending;
-  // step + 1 of the fuerza call whose ov_near flags describe the current
-  // positions and rows (0: none; cleared by every kernel that moves or adds
-  // atoms)
-  int32_t ovn_stamp;
+  // step + 1 of the fuerza call (ovn_kind 1: ov_near flags) or row build
+  // (ovn_kind 2: ov_near counts and nearl lists) whose record of the entries
+  // within rcut describes the current positions and rows (0: none; cleared
+  // by every kernel that moves or adds atoms, and by a rebuild whose row
+  // kernel keeps no record)
+  int32_t ovn_stamp, ovn_kind;
   // search hints: every slot below free_lo is alive or in limbo, every hs%b
   // index below bfree_lo is taken (lowered wherever one is freed)
   int32_t free_lo, bfree_lo;
@@ -273,7 +276,9 @@ struct DevState {
   // overlap rounds
   int32_t *ov_rev;        // round stamp of a revert
   uint8_t *ov_st;         // bit0 active bit1 dominated bit2 deposit bit3 self-revert
-  uint8_t *ov_near;       // fuerza saw a row entry within rcut (FP32, with margin)
+  uint8_t *ov_near;       // row entries within rcut (FP32, with margin): a flag from
+                          // fuerza, or the count listed in nearl by the row build
+  int32_t *nearl;         // NEAR_MAX row positions per atom
   int32_t *wl[2];         // overlap worklists (global ref positions), by round parity
   // overlap rounds after the first visit only their work lists: ovw[q % 2]
   // the ref positions still unresolved (skip = 0) for round q, ovr[q % 2]

@@ -135,6 +135,9 @@ __global__ void __launch_bounds__(DML_BLOCK) k_dispmax_final(DevState s) {
       Pw->listed = 1;
       Pw->any_ghost = 0;
       Pw->nlimbo = 0;  // the rebuild cleans the limbo (Groups.F90:1036-1053)
+      // new rows: the wave-per-cell row build lists their entries within rcut
+      Pw->ovn_stamp = Pw->lb_kind == 2 ? Pw->step + 1 : 0;
+      Pw->ovn_kind = 2;
       atomicAdd(&s.cnt[r].nupd, 1ull);
     }
   }
@@ -149,6 +152,8 @@ __global__ void k_force_rebuild(DevState s) {
   Pw->listed = 1;
   Pw->any_ghost = 0;
   Pw->nlimbo = 0;
+  Pw->ovn_stamp = Pw->lb_kind == 2 ? Pw->step + 1 : 0;
+  Pw->ovn_kind = 2;
   atomicAdd(&s.cnt[r].nupd, 1ull);
 }
 
@@ -537,6 +542,10 @@ __global__ void __launch_bounds__(DML_BLOCK) k_listbuild_cells(DevState s) {
 // time, one per lane; every ref centre atom in turn tests the whole chunk
 // (FP32 prefilter, FP64 inside the band) and appends its hits by ballot
 // compaction, so rows keep the order of ngroup_cells (Neighbor.F90:509-535).
+// With the image shift the same tests also list, per ref atom, the row
+// positions of its entries within rcut (FP32 with margin; at most NEAR_MAX,
+// count in ov_near, 255 when not listed) for the overlap pass at these
+// positions.
 // No LDS and no block barriers: candidate loads are shared by all atoms of
 // the cell through registers, and dense and sparse cells cost in proportion
 // to their work.  With at least 4 cells in x and y, a candidate's x, y are
@@ -637,6 +646,7 @@ k_listbuild_wc(DevState s) {
     const double rl2 = P.rl2;
     const int strict = P.use_cells;
     const float rl2lo = (float)rl2 - P.pf_margin, rl2hi = (float)rl2 + P.pf_margin;
+    const float rcf = (float)P.rcut2 + P.pf_margin;
     const float bxf = (float)P.box[0], byf = (float)P.box[1];
     const float ibx = (float)P.one_box[0], iby = (float)P.one_box[1];
     const int img = __builtin_amdgcn_readfirstlane(
@@ -657,7 +667,7 @@ k_listbuild_wc(DevState s) {
       }
       const unsigned long long refm = __ballot(my_ref);
       if (!refm) continue;
-      int my_nn = 0;
+      int my_nn = 0, my_nc = 0;  // row length, entries within rcut
       // candidate cidx of the stream: its bucket entry (the stencil cell
       // holding it by a binary search over the cell starts)
       auto fetch = [&](int cidx, int *cj, float4 *cf) {
@@ -712,6 +722,15 @@ k_listbuild_wc(DevState s) {
                 row[pr] = cj;
               }
               my_nn = lane == a ? nn0 + nh : my_nn;
+              // the hits within rcut: their row positions
+              const unsigned long long nm = __builtin_amdgcn_ballot_w64(df <= rcf) & hm;
+              if (nm) {
+                const int nc0 = __builtin_amdgcn_readlane(my_nc, a);
+                const int q = nc0 + (int)__builtin_amdgcn_mbcnt_hi(
+                                        (uint32_t)(nm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)nm, 0u));
+                if (hit && df <= rcf && q < NEAR_MAX) s.nearl[(int64_t)i * NEAR_MAX + q] = nn0 + pr;
+                my_nc = lane == a ? nc0 + __popcll(nm) : my_nc;
+              }
             };
             const unsigned long long gem = __builtin_amdgcn_ballot_w64(df >= rl2lo);
             if (gem & __builtin_amdgcn_ballot_w64(df <= rl2hi)) {
@@ -777,6 +796,7 @@ k_listbuild_wc(DevState s) {
         }
         s.nn[my_i] = nn;
         s.rowflag[my_i] = (uint8_t)ghost;
+        s.ov_near[my_i] = (uint8_t)(img && my_nc <= NEAR_MAX ? my_nc : 255);
         if (ghost) s.rp[r].any_ghost = 1;
       }
       // row statistics of this group
@@ -960,7 +980,10 @@ __global__ void __launch_bounds__(DML_BLOCK) k_force(DevState s, int only_ermak)
   int i = active ? s.ref_order[t] : 0;
   double fx = 0.0, fy = 0.0, fz = 0.0, ea = 0.0;
   int near = 0;  // an entry within rcut (FP32 with margin): overlap_moveback will look
-  if (local == 0 && sub == 0) s.rp[r].ovn_stamp = P.step + 1;
+  if (local == 0 && sub == 0) {
+    s.rp[r].ovn_stamp = P.step + 1;
+    s.rp[r].ovn_kind = 1;
+  }
   if (active) {
     int k = s.species[i];
     const float rcf = (float)P.rcut2 + P.pf_margin;
@@ -1391,7 +1414,8 @@ __global__ void __launch_bounds__(DML_BLOCK) k_ov_scan_l(DevState s, const doubl
     // fuerza of this step scanned this same row at these same positions
     // (no rebuild since) and met nothing within rcut: the scan would find no
     // contact, no neighbour to revert and no earlier atom to wait for
-    if (round == 0 && !P.rebuild && P.ovn_stamp == P.step + 1 && !s.ov_near[i]) {
+    const int fresh = round == 0 && P.ovn_stamp == P.step + 1;
+    if (fresh && P.ovn_kind == 1 && !s.ov_near[i]) {
       if (sub == 0) s.ov_st[i] = 1;
       continue;
     }
@@ -1406,14 +1430,13 @@ __global__ void __launch_bounds__(DML_BLOCK) k_ov_scan_l(DevState s, const doubl
     // private array is needed), more fall back to a second pass
     int hj0 = 0, hj1 = 0, hj2 = 0, hj3 = 0, he0 = 0, he1 = 0, he2 = 0, he3 = 0;
     int nh = 0, over = 0, dom = 0, firstcg = 0x7fffffff;
-    for (int e = sub; e < nn; e += LPA) {
-      int j = row[e];
-      if (d2_f32(P, fi, s.pf4[j], bxf, byf, ibx, iby) > rcf) continue;  // surely no hit
+    // row entry e (neighbour j) within rcut in FP64
+    auto visit = [&](int e, int j) {
       uint8_t fj = s.flags[j];
-      if (fj & DML_FL_LIMBO) continue;
+      if (fj & DML_FL_LIMBO) return;
       double dx, dy, dz;
       vdist_mic(P, xi, yi, zi, s.px[j], s.py[j], s.pz[j], &dx, &dy, &dz);
-      if (dot3(dx, dy, dz) > P.rcut2) continue;
+      if (dot3(dx, dy, dz) > P.rcut2) return;
       if ((fj & DML_FL_REF) && !(fj & DML_FL_SKIP) && s.ref_pos[j] < local) dom = 1;
       if (s.species[j] == SP_CG) {
         if (e < firstcg) firstcg = e;
@@ -1427,6 +1450,21 @@ __global__ void __launch_bounds__(DML_BLOCK) k_ov_scan_l(DevState s, const doubl
         hj3 = j; he3 = e; nh = 4;
       } else {
         over = 1;
+      }
+    };
+    // the row build of this step listed the row positions of the entries
+    // within rcut (FP32 with margin): every other entry lies beyond rcut
+    const int nlist = fresh && P.ovn_kind == 2 ? s.ov_near[i] : 255;
+    if (nlist <= NEAR_MAX) {
+      for (int k = sub; k < nlist; k += LPA) {
+        const int e = s.nearl[(int64_t)i * NEAR_MAX + k];
+        visit(e, row[e]);
+      }
+    } else {
+      for (int e = sub; e < nn; e += LPA) {
+        int j = row[e];
+        if (d2_f32(P, fi, s.pf4[j], bxf, byf, ibx, iby) > rcf) continue;  // surely no hit
+        visit(e, j);
       }
     }
     for (int off = LPA / 2; off > 0; off >>= 1) {
