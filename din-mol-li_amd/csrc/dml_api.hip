@@ -427,6 +427,10 @@ void enq_overlap_rounds(dml_ctx *c, const double *u_round, int64_t stride, int r
         hipLaunchKernelGGL(k_ov_scan_l<1>, dim3(gs), dim3(DML_BLOCK), 0, c->st, c->d, u_round,
                            stride, q, c->P);
         break;
+      case 2:
+        hipLaunchKernelGGL(k_ov_scan_l<2>, dim3(gs), dim3(DML_BLOCK), 0, c->st, c->d, u_round,
+                           stride, q, c->P);
+        break;
       case 4:
         hipLaunchKernelGGL(k_ov_scan_l<4>, dim3(gs), dim3(DML_BLOCK), 0, c->st, c->d, u_round,
                            stride, q, c->P);
@@ -782,7 +786,10 @@ int dml_create_ensemble(int32_t nrep, const dml_config *cfgs, dml_ctx **out) {
   hipEventCreate(&c->e0);
   hipEventCreate(&c->e1);
   if (const char *ev = getenv("DML_LISTBUILD")) c->lb_mode = atoi(ev);
-  if (const char *ev = getenv("DML_OVERLAP_LPA")) c->lpa_ov = atoi(ev);
+  if (const char *ev = getenv("DML_OVERLAP_LPA")) {
+    const int v = atoi(ev);
+    if (v == 1 || v == 2 || v == 4 || v == 8 || v == 16) c->lpa_ov = v;
+  }
   refresh_modes(c);
   if (hipStreamSynchronize(c->st) != hipSuccess) {
     dml_destroy(c);
@@ -1596,7 +1603,7 @@ int dml_set_tuning(dml_ctx *c, int32_t lanes_per_atom, int32_t overlap_rounds,
   if (!c) return DML_E_ARG;
   auto pow2_le64 = [](int v) { return v > 0 && v <= 64 && !(v & (v - 1)); };
   // the overlap scan is instantiated for 1, 4, 8 and 16 lanes
-  auto ov_ok = [](int v) { return v == 1 || v == 4 || v == 8 || v == 16; };
+  auto ov_ok = [](int v) { return v == 1 || v == 2 || v == 4 || v == 8 || v == 16; };
   if ((lanes_per_atom && !pow2_le64(lanes_per_atom)) || (overlap_lanes && !ov_ok(overlap_lanes)) ||
       overlap_rounds < 0 || overlap_rounds > kMaxRounds || list_kernel < 0 || list_kernel > 4)
     return DML_E_ARG;

@@ -87,12 +87,11 @@ struct RepParams {
   // the stage the grid-wide list kernels reached (1 staged, 2 trials done,
   // 3 counted; reset at the next call)
   int32_t gc_len, gc_inv, ref_len, gc_pending;
-  // step + 1 of the fuerza call (ovn_kind 1: ov_near flags) or row build
-  // (ovn_kind 2: ov_near counts and nearl lists) whose record of the entries
-  // within rcut describes the current positions and rows (0: none; cleared
-  // by every kernel that moves or adds atoms, and by a rebuild whose row
-  // kernel keeps no record)
-  int32_t ovn_stamp, ovn_kind;
+  // step + 1 of the fuerza call or row build whose record of the entries
+  // within rcut (ov_near counts, nearl lists) describes the current
+  // positions and rows (0: none; cleared by every kernel that moves or adds
+  // atoms, and by a rebuild whose row kernel keeps no record)
+  int32_t ovn_stamp;
   // search hints: every slot below free_lo is alive or in limbo, every hs%b
   // index below bfree_lo is taken (lowered wherever one is freed)
   int32_t free_lo, bfree_lo;
@@ -276,8 +275,8 @@ struct DevState {
   // overlap rounds
   int32_t *ov_rev;        // round stamp of a revert
   uint8_t *ov_st;         // bit0 active bit1 dominated bit2 deposit bit3 self-revert
-  uint8_t *ov_near;       // row entries within rcut (FP32, with margin): a flag from
-                          // fuerza, or the count listed in nearl by the row build
+  uint8_t *ov_near;       // row entries within rcut (FP32, with margin) listed in
+                          // nearl by fuerza or the row build (255: not listed)
   int32_t *nearl;         // NEAR_MAX row positions per atom
   int32_t *wl[2];         // overlap worklists (global ref positions), by round parity
   // overlap rounds after the first visit only their work lists: ovw[q % 2]

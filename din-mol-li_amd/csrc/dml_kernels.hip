@@ -135,9 +135,9 @@ __global__ void __launch_bounds__(DML_BLOCK) k_dispmax_final(DevState s) {
       Pw->listed = 1;
       Pw->any_ghost = 0;
       Pw->nlimbo = 0;  // the rebuild cleans the limbo (Groups.F90:1036-1053)
-      // new rows: the wave-per-cell row build lists their entries within rcut
-      Pw->ovn_stamp = Pw->lb_kind == 2 ? Pw->step + 1 : 0;
-      Pw->ovn_kind = 2;
+      // new rows: the cell-tile and wave-per-cell row builds list their
+      // entries within rcut
+      Pw->ovn_stamp = Pw->lb_kind != 1 ? Pw->step + 1 : 0;
       atomicAdd(&s.cnt[r].nupd, 1ull);
     }
   }
@@ -152,8 +152,7 @@ __global__ void k_force_rebuild(DevState s) {
   Pw->listed = 1;
   Pw->any_ghost = 0;
   Pw->nlimbo = 0;
-  Pw->ovn_stamp = Pw->lb_kind == 2 ? Pw->step + 1 : 0;
-  Pw->ovn_kind = 2;
+  Pw->ovn_stamp = Pw->lb_kind != 1 ? Pw->step + 1 : 0;
   atomicAdd(&s.cnt[r].nupd, 1ull);
 }
 
@@ -373,7 +372,8 @@ template __global__ void k_listbuild<4>(DevState s);
 // cell; each wave owns some centre atoms and appends to their rows by
 // ballot compaction, so every row keeps the order of ngroup_cells
 // (Neighbor.F90:509-535).  Same rows as k_listbuild, ~cell-occupancy times
-// fewer L2/HBM reads.
+// fewer L2/HBM reads.  Like k_listbuild_wc it lists each ref atom's entries
+// within rcut (nearl, count in ov_near) for the overlap pass.
 // ------------------------------------------------------------------------
 #define LB_TILE 1024
 #define LB_CMAX 256
@@ -382,7 +382,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_listbuild_cells(DevState s) {
   __shared__ float4 t_f[LB_TILE];
   __shared__ int32_t c_slot[LB_CMAX];
   __shared__ double c_x[LB_CMAX], c_y[LB_CMAX], c_z[LB_CMAX];
-  __shared__ int32_t c_nn[LB_CMAX];
+  __shared__ int32_t c_nn[LB_CMAX], c_nc[LB_CMAX];
   __shared__ int64_t st_base[27];
   __shared__ int32_t st_pre[28];
   __shared__ int32_t sh_A, sh_nrefc;
@@ -400,6 +400,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_listbuild_cells(DevState s) {
     const double rl2 = P.rl2;
     const int strict = P.use_cells;
     const float rl2lo = (float)rl2 - P.pf_margin, rl2hi = (float)rl2 + P.pf_margin;
+    const float rcf = (float)P.rcut2 + P.pf_margin;
     const float bxf = (float)P.box[0], byf = (float)P.box[1];
     const float ibx = (float)P.one_box[0], iby = (float)P.one_box[1];
     if (tid == 0) {
@@ -430,6 +431,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_listbuild_cells(DevState s) {
         c_y[q] = s.by[cbase + a0 + q];
         c_z[q] = s.bz[cbase + a0 + q];
         c_nn[q] = 0;
+        c_nc[q] = 0;
         if (isref) atomicAdd(&sh_nrefc, 1);
       }
       if (tid < 27) {
@@ -480,14 +482,15 @@ __global__ void __launch_bounds__(DML_BLOCK) k_listbuild_cells(DevState s) {
           if (i < 0) continue;
           const double xi = c_x[a], yi = c_y[a], zi = c_z[a];
           const float4 fi = make_float4((float)xi, (float)yi, (float)zi, 0.f);
-          int nn = c_nn[a];
+          int nn = c_nn[a], nc = c_nc[a];
           int32_t *row = s.nbr + (int64_t)i * maxnn;
           for (int q0 = 0; q0 < tn; q0 += WAVE) {
             int q = q0 + lane;
             int hit = 0, j = -1;
+            float df = 3.0e38f;
             if (q < tn) {
               j = t_slot[q];
-              float df = d2_f32(P, t_f[q], fi, bxf, byf, ibx, iby);
+              df = d2_f32(P, t_f[q], fi, bxf, byf, ibx, iby);
               if (df < rl2lo) {
                 hit = 1;  // inside by more than the FP32 error
               } else if (df <= rl2hi) {
@@ -501,9 +504,20 @@ __global__ void __launch_bounds__(DML_BLOCK) k_listbuild_cells(DevState s) {
             unsigned long long m = __ballot(hit);
             int pr = __popcll(m & ((1ull << lane) - 1ull));
             if (hit && nn + pr < maxnn) row[nn + pr] = j;
+            // the hits within rcut: their row positions
+            const int nr = hit && df <= rcf;
+            const unsigned long long nm = __ballot(nr);
+            if (nm) {
+              const int qn = nc + __popcll(nm & ((1ull << lane) - 1ull));
+              if (nr && qn < NEAR_MAX) s.nearl[(int64_t)i * NEAR_MAX + qn] = nn + pr;
+              nc += __popcll(nm);
+            }
             nn += __popcll(m);
           }
-          if (lane == 0) c_nn[a] = nn;
+          if (lane == 0) {
+            c_nn[a] = nn;
+            c_nc[a] = nc;
+          }
         }
         __syncthreads();
       }
@@ -519,6 +533,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_listbuild_cells(DevState s) {
         }
         s.nn[i] = nn;
         s.rowflag[i] = (uint8_t)ghost;
+        s.ov_near[i] = (uint8_t)(c_nc[q] <= NEAR_MAX ? c_nc[q] : 255);
         if (ghost) s.rp[r].any_ghost = 1;
         atomicAdd(&sh_nnsum, (unsigned long long)nn);
         atomicAdd(&sh_rows, 1ull);
@@ -979,11 +994,10 @@ __global__ void __launch_bounds__(DML_BLOCK) k_force(DevState s, int only_ermak)
   int active = local < P.nref;
   int i = active ? s.ref_order[t] : 0;
   double fx = 0.0, fy = 0.0, fz = 0.0, ea = 0.0;
-  int near = 0;  // an entry within rcut (FP32 with margin): overlap_moveback will look
-  if (local == 0 && sub == 0) {
-    s.rp[r].ovn_stamp = P.step + 1;
-    s.rp[r].ovn_kind = 1;
-  }
+  // entries within rcut (FP32 with margin), the ones overlap_moveback looks
+  // at: their number on this lane and the row position of the last one
+  int ncnt = 0, ne = 0;
+  if (local == 0 && sub == 0) s.rp[r].ovn_stamp = P.step + 1;
   if (active) {
     int k = s.species[i];
     const float rcf = (float)P.rcut2 + P.pf_margin;
@@ -1001,7 +1015,10 @@ __global__ void __launch_bounds__(DML_BLOCK) k_force(DevState s, int only_ermak)
       {  // surely beyond the pair's r0: no force, no energy
         float r0f = (float)P.r0[(k - 1) * 3 + (__float_as_int(fj4.w) - 1)];
         const float d2f = d2_f32(P, fi, fj4, bxf, byf, ibx, iby);
-        near |= d2f <= rcf;
+        if (d2f <= rcf) {
+          ncnt++;
+          ne = e;
+        }
         if (d2f > r0f * r0f + P.pf_margin) continue;
       }
       uint8_t fj = s.flags[j];
@@ -1050,15 +1067,26 @@ __global__ void __launch_bounds__(DML_BLOCK) k_force(DevState s, int only_ermak)
       }
     }
   }
+  // the atom's list: one row position per lane that met exactly one entry
+  // (any order; none listed when a lane met more, or more than NEAR_MAX)
+  const unsigned long long one = __ballot(ncnt == 1);
+  int tot = ncnt, ovf = ncnt > 1;
   for (int off = LPA / 2; off > 0; off >>= 1) {
     fx += __shfl_xor(fx, off);
     fy += __shfl_xor(fy, off);
     fz += __shfl_xor(fz, off);
     ea += __shfl_xor(ea, off);
-    near |= __shfl_xor(near, off);
+    tot += __shfl_xor(tot, off);
+    ovf |= __shfl_xor(ovf, off);
+  }
+  const int listed = !ovf && tot <= NEAR_MAX;
+  if (active && listed && ncnt == 1) {
+    const int g0 = (threadIdx.x % WAVE) & ~(LPA - 1);  // first lane of this atom
+    const unsigned long long gm = LPA == WAVE ? one : (one >> g0) & ((1ull << (LPA & 63)) - 1ull);
+    s.nearl[(int64_t)i * NEAR_MAX + __popcll(gm & ((1ull << sub) - 1ull))] = ne;
   }
   if (active && sub == 0) {
-    s.ov_near[i] = (uint8_t)near;
+    s.ov_near[i] = (uint8_t)(listed ? tot : 255);
     s.fx[i] = fx;
     s.fy[i] = fy;
     s.fz[i] = fz;
@@ -1411,11 +1439,12 @@ __global__ void __launch_bounds__(DML_BLOCK) k_ov_scan_l(DevState s, const doubl
       if (sub == 0) s.ov_st[i] = 0;
       continue;
     }
-    // fuerza of this step scanned this same row at these same positions
-    // (no rebuild since) and met nothing within rcut: the scan would find no
-    // contact, no neighbour to revert and no earlier atom to wait for
-    const int fresh = round == 0 && P.ovn_stamp == P.step + 1;
-    if (fresh && P.ovn_kind == 1 && !s.ov_near[i]) {
+    // fuerza or the row build of this step, at these same positions and on
+    // this same row, listed the row positions of the entries within rcut
+    // (FP32 with margin): every other entry lies beyond rcut.  None listed:
+    // no contact, no neighbour to revert and no earlier atom to wait for
+    const int nlist = round == 0 && P.ovn_stamp == P.step + 1 ? s.ov_near[i] : 255;
+    if (nlist == 0) {
       if (sub == 0) s.ov_st[i] = 1;
       continue;
     }
@@ -1452,9 +1481,6 @@ __global__ void __launch_bounds__(DML_BLOCK) k_ov_scan_l(DevState s, const doubl
         over = 1;
       }
     };
-    // the row build of this step listed the row positions of the entries
-    // within rcut (FP32 with margin): every other entry lies beyond rcut
-    const int nlist = fresh && P.ovn_kind == 2 ? s.ov_near[i] : 255;
     if (nlist <= NEAR_MAX) {
       for (int k = sub; k < nlist; k += LPA) {
         const int e = s.nearl[(int64_t)i * NEAR_MAX + k];
@@ -1536,6 +1562,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_ov_scan_l(DevState s, const doubl
   }
 }
 template __global__ void k_ov_scan_l<1>(DevState, const double *, int64_t, int, int64_t);
+template __global__ void k_ov_scan_l<2>(DevState, const double *, int64_t, int, int64_t);
 template __global__ void k_ov_scan_l<4>(DevState, const double *, int64_t, int, int64_t);
 template __global__ void k_ov_scan_l<8>(DevState, const double *, int64_t, int, int64_t);
 template __global__ void k_ov_scan_l<16>(DevState, const double *, int64_t, int, int64_t);

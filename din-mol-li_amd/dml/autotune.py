@@ -28,7 +28,7 @@ from . import K_FORCE, K_LISTBUILD, Context
 
 FORCE_LANES = (2, 4, 8, 16, 32)
 LIST_KERNELS = (1, 2, 3, 4)
-OVERLAP_LANES = (4, 8, 16)
+OVERLAP_LANES = (1, 2, 4, 8, 16)
 OVERLAP_ROUNDS = (8, 16, 32)
 
 

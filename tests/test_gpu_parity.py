@@ -357,15 +357,20 @@ def test_ermak_steps_with_round_overlap_match_oracle(orc, dml):
     c.close()
 
 
-@pytest.mark.parametrize("lb_mode,wide", [("3", False), ("1", False), ("3", True), ("1", True)])
-def test_brownian_round_overlap_after_rebuilds_matches_oracle(orc, dml, lb_mode, wide,
+@pytest.mark.parametrize("lb_mode,wide,lpa", [("3", False, "8"), ("1", False, "8"),
+                                              ("0", False, "8"), ("3", True, "8"),
+                                              ("1", True, "8"), ("0", True, "8"),
+                                              ("3", True, "1"), ("0", True, "2")])
+def test_brownian_round_overlap_after_rebuilds_matches_oracle(orc, dml, lb_mode, wide, lpa,
                                                               monkeypatch):
     """Brownian steps (h 3e-3: some first skin tests rebuild, some do not)
-    with the round schedule, on rows from the wave-per-cell (lb_mode 3) and
-    the wave-per-atom (lb_mode 1) row builds.  In the wide box (4 cells of
-    13.2 A across x and y) the wave-per-cell build lists each atom's entries
-    within rcut and round 0 after a rebuild visits only those."""
+    with the round schedule, on rows from the wave-per-cell (lb_mode 3), the
+    wave-per-atom (lb_mode 1) and the cell-tile (lb_mode 0) row builds.  The
+    cell-tile build, and the wave-per-cell build in the wide box (4 cells of
+    13.2 A across x and y), list each atom's entries within rcut, and round 0
+    after a rebuild visits only those."""
     monkeypatch.setenv("DML_LISTBUILD", lb_mode)
+    monkeypatch.setenv("DML_OVERLAP_LPA", lpa)
     rng = np.random.default_rng(41 if wide else 31)
     L, nel, nion = (60.0, 17, 1600) if wide else (40.0, 11, 700)
     box = np.array([L, L, 60.0])
@@ -374,7 +379,7 @@ def test_brownian_round_overlap_after_rebuilds_matches_oracle(orc, dml, lb_mode,
     ions = rsa(rng, nion, np.array([0, 0, 6.0]), np.array([L, L, 59.0]), 3.2, box)
     pos = np.vstack([el, ions])
     z = np.array([2] * len(el) + [1] * len(ions), np.int8)
-    say("brownian round steps", len(pos), "lb_mode", lb_mode, "wide", wide)
+    say("brownian round steps", len(pos), "lb_mode", lb_mode, "wide", wide, "lpa", lpa)
     s, c = pair(orc, dml, box, 10.0, pos, z, ermak=False, zmax=60.0, z0=30.0, h=3e-3,
                 overlap_mode=dml.OVERLAP_ROUNDS)
     s.test_update(); c.test_update()
