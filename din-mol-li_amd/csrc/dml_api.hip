@@ -766,6 +766,7 @@ int dml_create_ensemble(int32_t nrep, const dml_config *cfgs, dml_ctx **out) {
   hipMemsetAsync(d.nn, 0, sizeof(int32_t) * P, c->st);
   hipMemsetAsync(d.ov_rev, 0, sizeof(int32_t) * P, c->st);
   hipMemsetAsync(d.ov_st, 0, P, c->st);
+  hipMemsetAsync(d.ov_near, 0xff, P, c->st);  // 255: no entries listed
   hipMemsetAsync(d.wl_cnt, 0, sizeof(int32_t) * 4, c->st);
   hipMemsetAsync(d.pf4, 0, sizeof(float4) * P, c->st);
   hipMemsetAsync(d.ref_pos, 0xff, sizeof(int32_t) * P, c->st);
@@ -925,6 +926,9 @@ int dml_set_state(dml_ctx *c, int32_t rep, int32_t n, const double *pos, const d
   p.free_lo = 0;
   p.bfree_lo = 0;
   p.ovn_stamp = 0;
+  // no entries-within-rcut lists for the new atoms until fuerza or a row
+  // build writes them
+  HIPCHECK(hipMemsetAsync(c->d.ov_near + o, 0xff, (size_t)p.pcap, c->st));
   p.nslots = n;
   p.nref = (int32_t)rord.size();
   p.listed = 0;
@@ -986,6 +990,7 @@ int dml_set_order(dml_ctx *c, int32_t rep, int32_t nref, const int32_t *ref_orde
     p.b_amax = amax;
     p.bfree_lo = 0;
   }
+  p.ovn_stamp = 0;  // another set of ref atoms: no lists for the new ones
   return upload_params(c);
 }
 
@@ -1509,7 +1514,7 @@ int dml_run(dml_ctx *c, int32_t nsteps, int32_t use_graph) {
   // shapes (rounds <= 32, lanes <= 64)
   const int64_t modes = (int64_t)c->any_ermak | c->any_brown << 1 | c->any_rounds << 2 |
                         c->any_exact << 3 | c->any_piston << 4 | c->any_gcmc << 5 |
-                        (int64_t)c->any_chunks << 27 |
+                        (int64_t)c->any_chunks << 31 |
                         (int64_t)c->rounds << 6 | (int64_t)c->lpa << 12 |
                         (int64_t)c->lpa_ov << 19 | (int64_t)c->lb_mode << 26 |
                         (int64_t)c->any_lb[0] << 28 | (int64_t)c->any_lb[1] << 29 |
@@ -1602,7 +1607,7 @@ int dml_set_tuning(dml_ctx *c, int32_t lanes_per_atom, int32_t overlap_rounds,
                    int32_t overlap_lanes, int32_t list_kernel) {
   if (!c) return DML_E_ARG;
   auto pow2_le64 = [](int v) { return v > 0 && v <= 64 && !(v & (v - 1)); };
-  // the overlap scan is instantiated for 1, 4, 8 and 16 lanes
+  // the overlap scan is instantiated for 1, 2, 4, 8 and 16 lanes
   auto ov_ok = [](int v) { return v == 1 || v == 2 || v == 4 || v == 8 || v == 16; };
   if ((lanes_per_atom && !pow2_le64(lanes_per_atom)) || (overlap_lanes && !ov_ok(overlap_lanes)) ||
       overlap_rounds < 0 || overlap_rounds > kMaxRounds || list_kernel < 0 || list_kernel > 4)
@@ -1712,6 +1717,7 @@ int dml_reconfigure(dml_ctx *c, int32_t rep, const dml_config *cfg) {
   }
   hipMemsetAsync(c->d.cell_cnt + p.coff, 0, sizeof(int32_t) * p.ccap, c->st);
   hipMemsetAsync(c->d.cnt + rep, 0, sizeof(DevCounters), c->st);
+  hipMemsetAsync(c->d.ov_near + p.poff, 0xff, (size_t)p.pcap, c->st);
   refresh_modes(c);
   return upload_params(c);
 }

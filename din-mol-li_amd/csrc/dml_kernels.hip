@@ -1443,7 +1443,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_ov_scan_l(DevState s, const doubl
     // this same row, listed the row positions of the entries within rcut
     // (FP32 with margin): every other entry lies beyond rcut.  None listed:
     // no contact, no neighbour to revert and no earlier atom to wait for
-    const int nlist = round == 0 && P.ovn_stamp == P.step + 1 ? s.ov_near[i] : 255;
+    int nlist = round == 0 && P.ovn_stamp == P.step + 1 ? s.ov_near[i] : 255;
     if (nlist == 0) {
       if (sub == 0) s.ov_st[i] = 1;
       continue;
@@ -1481,9 +1481,18 @@ __global__ void __launch_bounds__(DML_BLOCK) k_ov_scan_l(DevState s, const doubl
         over = 1;
       }
     };
+    // a listed position outside this row (a list the row no longer matches)
+    // sends the atom through the whole row instead
+    const int32_t *nl = s.nearl + (int64_t)i * NEAR_MAX;
+    if (nlist <= NEAR_MAX) {
+      int bad = 0;
+      for (int k = sub; k < nlist; k += LPA) bad |= (unsigned)nl[k] >= (unsigned)nn;
+      for (int off = LPA / 2; off > 0; off >>= 1) bad |= __shfl_xor(bad, off, LPA);
+      if (bad) nlist = 255;
+    }
     if (nlist <= NEAR_MAX) {
       for (int k = sub; k < nlist; k += LPA) {
-        const int e = s.nearl[(int64_t)i * NEAR_MAX + k];
+        const int e = nl[k];
         visit(e, row[e]);
       }
     } else {
