@@ -586,13 +586,18 @@ __global__ void __launch_bounds__(DML_BLOCK) __attribute__((amdgpu_waves_per_eu(
 k_listbuild_wc(DevState s) {
   const int lane = threadIdx.x % WAVE;
   const int maxnn = s.maxnn;
+  // this lane's stencil offset (lanes 0-26), three fields of 4 bits, biased by 1
+  const int kofs = lane < 27 ? (kMap[lane][0] + 1) | (kMap[lane][1] + 1) << 4 |
+                                   (kMap[lane][2] + 1) << 8
+                             : 0x111;
   // XCD-aware: workgroups are dealt round-robin to the 8 XCDs, so
   // workgroup b works on slice (b % 8) of the cell list and, inside it,
   // every wave on one contiguous run of cells; the waves of one XCD then
   // sweep neighbouring cells together and share their candidates in its L2
   const int nb = gridDim.x;  // a multiple of 8
   const int64_t vb = (int64_t)(blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8;
-  const int64_t wid = vb * (DML_BLOCK / WAVE) + threadIdx.x / WAVE;
+  const int64_t wid =
+      vb * (DML_BLOCK / WAVE) + __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
   const int64_t nwv = (int64_t)nb * (DML_BLOCK / WAVE);
   const int64_t run = (s.n_cm + nwv - 1) / nwv;
   const int64_t g_end = min(s.n_cm, (wid + 1) * run);
@@ -639,9 +644,9 @@ k_listbuild_wc(DevState s) {
     int64_t kbase = 0;
     if (P.use_cells) {
       if (lane < 27) {
-        int n0 = cell_wrap(ci0 + kMap[lane][0], P.ncells[0]);
-        int n1 = cell_wrap(ci1 + kMap[lane][1], P.ncells[1]);
-        int n2 = cell_wrap(ci2 + kMap[lane][2], P.ncells[2]);
+        int n0 = cell_wrap(ci0 + (kofs & 15) - 1, P.ncells[0]);
+        int n1 = cell_wrap(ci1 + (kofs >> 4 & 15) - 1, P.ncells[1]);
+        int n2 = cell_wrap(ci2 + (kofs >> 8 & 15) - 1, P.ncells[2]);
         int cell = cell_lin(P, n0, n1, n2);
         kcnt = s.cell_cnt[P.coff + cell];
         if (kcnt > P.bcap) kcnt = P.bcap;
@@ -682,7 +687,9 @@ k_listbuild_wc(DevState s) {
       }
       const unsigned long long refm = __ballot(my_ref);
       if (!refm) continue;
-      int my_nn = 0, my_nc = 0;  // row length, entries within rcut
+      // row length (bits 0-23) and entries within rcut (bits 24-31,
+      // saturating) in one register
+      int my_nn = 0;
       // candidate cidx of the stream: its bucket entry (the stencil cell
       // holding it by a binary search over the cell starts)
       auto fetch = [&](int cidx, int *cj, float4 *cf) {
@@ -726,7 +733,8 @@ k_listbuild_wc(DevState s) {
             // registers (no per-lane 0/1 round trip through a ballot)
             auto append = [&](bool hit, unsigned long long hm) {
               if (!hm) return;
-              const int nn0 = __builtin_amdgcn_readlane(my_nn, a);
+              const int pk = __builtin_amdgcn_readlane(my_nn, a);
+              const int nn0 = pk & 0xffffff;
               const int nh = __popcll(hm);
               const int pr = __builtin_amdgcn_mbcnt_hi((uint32_t)(hm >> 32),
                                                        __builtin_amdgcn_mbcnt_lo((uint32_t)hm, 0u));
@@ -736,16 +744,18 @@ k_listbuild_wc(DevState s) {
               } else if (hit && nn0 + pr < maxnn) {
                 row[pr] = cj;
               }
-              my_nn = lane == a ? nn0 + nh : my_nn;
+              int npk = pk + nh;
               // the hits within rcut: their row positions
               const unsigned long long nm = __builtin_amdgcn_ballot_w64(df <= rcf) & hm;
               if (nm) {
-                const int nc0 = __builtin_amdgcn_readlane(my_nc, a);
+                const int nc0 = (int)((uint32_t)pk >> 24);
                 const int q = nc0 + (int)__builtin_amdgcn_mbcnt_hi(
                                         (uint32_t)(nm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)nm, 0u));
                 if (hit && df <= rcf && q < NEAR_MAX) s.nearl[(int64_t)i * NEAR_MAX + q] = nn0 + pr;
-                my_nc = lane == a ? nc0 + __popcll(nm) : my_nc;
+                const int add = __popcll(nm);
+                npk += (add < 255 - nc0 ? add : 255 - nc0) << 24;
               }
+              my_nn = lane == a ? npk : my_nn;
             };
             const unsigned long long gem = __builtin_amdgcn_ballot_w64(df >= rl2lo);
             if (gem & __builtin_amdgcn_ballot_w64(df <= rl2hi)) {
@@ -794,13 +804,16 @@ k_listbuild_wc(DevState s) {
           }
           const unsigned long long hm = __ballot(hit);
           if (hm) {
-            const int nn0 = __builtin_amdgcn_readlane(my_nn, a);
+            const int pk = __builtin_amdgcn_readlane(my_nn, a);
+            const int nn0 = pk & 0xffffff;
             const int pr = __popcll(hm & ((1ull << lane) - 1ull));
             if (hit && nn0 + pr < maxnn) s.nbr[(int64_t)i * maxnn + nn0 + pr] = cj;
-            if (lane == a) my_nn = nn0 + __popcll(hm);
+            if (lane == a) my_nn = pk + __popcll(hm);
           }
         }
       }
+      const int my_nc = (int)((uint32_t)my_nn >> 24);
+      my_nn &= 0xffffff;
       if (my_ref) {
         int nn = my_nn;
         if (nn > maxnn) {
@@ -814,10 +827,10 @@ k_listbuild_wc(DevState s) {
         s.ov_near[my_i] = (uint8_t)(img && my_nc <= NEAR_MAX ? my_nc : 255);
         if (ghost) s.rp[r].any_ghost = 1;
       }
-      // row statistics of this group
-      unsigned long long v = my_ref ? (unsigned long long)my_nn : 0ull;
+      // row statistics of this group (wave-uniform: kept in scalar registers)
+      int v = my_ref ? my_nn : 0;
       for (int off = WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
-      acc_nn += v;
+      acc_nn += (unsigned long long)(uint32_t)__builtin_amdgcn_readfirstlane(v);
       acc_rows += (unsigned long long)__popcll(refm);
     }
     }
