@@ -1433,13 +1433,16 @@ __global__ void __launch_bounds__(DML_BLOCK) k_ov_scan_l(DevState s, const doubl
     if (threadIdx.x == 0) s.wl_cnt[(round + 1) % 2] = 0;  // the next round's list
   }
   // round 0: every ref position; later rounds: the positions left unresolved
-  const int64_t nitems = round == 0 ? ptotal : (int64_t)s.wl_cnt[round % 2];
+  // (lists hold at most ptotal entries, each a position below ptotal)
+  const int64_t nitems =
+      round == 0 ? ptotal : min((int64_t)s.wl_cnt[round % 2], ptotal);
   const int32_t *wl = s.ovw[round % 2];
   const int gtid = blockIdx.x * DML_BLOCK + threadIdx.x;
   const int sub = gtid % LPA;
   const int64_t stride = (int64_t)gridDim.x * (DML_BLOCK / LPA);
   for (int64_t item = gtid / LPA; item < nitems; item += stride) {
     const int t = round == 0 ? (int)item : wl[item];
+    if ((uint32_t)t >= (uint64_t)ptotal) continue;
     const int r = s.blk2rep[t / DML_BLOCK];
     const RepParams &P = s.rp[r];
     if (P.halted || P.overlap_mode != DML_OVERLAP_ROUNDS) continue;
@@ -1532,7 +1535,8 @@ __global__ void __launch_bounds__(DML_BLOCK) k_ov_scan_l(DevState s, const doubl
         s.ov_rev[j] = stamp;
       } else if (atomicExch(&s.ov_rev[j], stamp) != stamp) {
         // first to revert j in this round: it is applied from the list
-        s.ovr[round % 2][atomicAdd(&s.wl_cnt[2 + round % 2], 1)] = j;
+        const int q = atomicAdd(&s.wl_cnt[2 + round % 2], 1);
+        if (q < ptotal) s.ovr[round % 2][q] = j;
       }
       return 0;
     };
@@ -1639,7 +1643,8 @@ __global__ void __launch_bounds__(DML_BLOCK) k_ov_apply(DevState s, int round, i
   const int prv = (round + 2) % 3;
   const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
   if (blockIdx.x == 0 && threadIdx.x == 0) s.wl_cnt[2 + (round + 1) % 2] = 0;  // next reverts
-  const int64_t nitems = round == 0 ? ptotal : (int64_t)s.wl_cnt[round % 2];
+  const int64_t nitems =
+      round == 0 ? ptotal : min((int64_t)s.wl_cnt[round % 2], ptotal);
   const int32_t *wl = s.ovw[round % 2];
   int32_t *next = s.ovw[(round + 1) % 2];
   int32_t *wb = buf + w * WAVE * OVA_PER;
@@ -1649,8 +1654,8 @@ __global__ void __launch_bounds__(DML_BLOCK) k_ov_apply(DevState s, int round, i
     for (int k = 0; k < OVA_PER; k++) {
       const int64_t item = b0 + (int64_t)k * DML_BLOCK + threadIdx.x;
       int push = 0, t = 0;
-      if (item < nitems) {
-        t = round == 0 ? (int)item : wl[item];
+      if (item < nitems) t = round == 0 ? (int)item : wl[item];
+      if (item < nitems && (uint32_t)t < (uint64_t)ptotal) {
         const int r = s.blk2rep[t / DML_BLOCK];
         const RepParams &P = s.rp[r];
         if (!P.halted && P.overlap_mode == DML_OVERLAP_ROUNDS) {
@@ -1675,7 +1680,8 @@ __global__ void __launch_bounds__(DML_BLOCK) k_ov_apply(DevState s, int round, i
     __syncthreads();
     int64_t off = base_sh;
     for (int q = 0; q < w; q++) off += wcnt[q];
-    for (int q = lane; q < n; q += WAVE) next[off + q] = wb[q];
+    for (int q = lane; q < n; q += WAVE)
+      if (off + q < ptotal) next[off + q] = wb[q];
     __syncthreads();  // buf, wcnt and base_sh are reused
   }
 }
@@ -1683,17 +1689,21 @@ __global__ void __launch_bounds__(DML_BLOCK) k_ov_apply(DevState s, int round, i
 // rounds > 0: the atoms reverted in the round that were not scanned in it
 // (those that were are applied, and their stamp cleared, by k_ov_apply)
 __global__ void __launch_bounds__(DML_BLOCK) k_ov_apply_r(DevState s, int round) {
-  const int64_t nitems = s.wl_cnt[2 + round % 2];
+  const int64_t cap = (int64_t)s.nblocks * DML_BLOCK;
+  const int64_t nitems = min((int64_t)s.wl_cnt[2 + round % 2], cap);
   const int32_t *rl = s.ovr[round % 2];
   const int64_t stride = (int64_t)gridDim.x * DML_BLOCK;
   for (int64_t item = blockIdx.x * DML_BLOCK + threadIdx.x; item < nitems; item += stride) {
     const int j = rl[item];
+    if ((uint32_t)j >= (uint64_t)cap) continue;
     const int r = s.blk2rep[j / DML_BLOCK];
     const RepParams &P = s.rp[r];
     if (P.halted || s.ov_rev[j] != ov_stamp(P, round)) continue;
     const int t = P.poff + s.ref_pos[j];  // ref_pos is replica-local
-    if (ov_apply_one(s, r, P, j, t, round))
-      s.ovw[(round + 1) % 2][atomicAdd(&s.wl_cnt[(round + 1) % 2], 1)] = t;
+    if (ov_apply_one(s, r, P, j, t, round)) {
+      const int q = atomicAdd(&s.wl_cnt[(round + 1) % 2], 1);
+      if (q < cap) s.ovw[(round + 1) % 2][q] = t;
+    }
   }
 }
 
