@@ -1422,7 +1422,7 @@ __global__ void k_ov_begin(DevState s) {
 }
 
 template <int LPA>
-__global__ void __launch_bounds__(DML_BLOCK) k_ov_scan_l(DevState s, const double *u_round,
+__global__ void __launch_bounds__(DML_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) k_ov_scan_l(DevState s, const double *u_round,
                                                           int64_t u_stride, int round,
                                                           int64_t ptotal) {
   // progress flags rotate over three buffers: this round writes round % 3,
@@ -1440,20 +1440,23 @@ __global__ void __launch_bounds__(DML_BLOCK) k_ov_scan_l(DevState s, const doubl
   const int gtid = blockIdx.x * DML_BLOCK + threadIdx.x;
   const int sub = gtid % LPA;
   const int64_t stride = (int64_t)gridDim.x * (DML_BLOCK / LPA);
-  for (int64_t item = gtid / LPA; item < nitems; item += stride) {
+  // one ref position: its verdict, and what it adds to its replica's
+  // counters (k_r = the replica, -1 when the position is not scanned)
+  auto scan_one = [&](int64_t item, unsigned long long &k_try, unsigned long long &k_ch,
+                      unsigned long long &k_ch3, int &k_more, int &k_r) {
     const int t = round == 0 ? (int)item : wl[item];
-    if ((uint32_t)t >= (uint64_t)ptotal) continue;
+    if ((uint32_t)t >= (uint64_t)ptotal) return;
     const int r = s.blk2rep[t / DML_BLOCK];
     const RepParams &P = s.rp[r];
-    if (P.halted || P.overlap_mode != DML_OVERLAP_ROUNDS) continue;
+    if (P.halted || P.overlap_mode != DML_OVERLAP_ROUNDS) return;
     // the sequential pass recurses only after a revert or a deferred atom
-    if (round > 0 && !s.ov_more[prv * s.nrep + r]) continue;
+    if (round > 0 && !s.ov_more[prv * s.nrep + r]) return;
     const int local = t - P.poff;
-    if (local >= P.nref) continue;
+    if (local >= P.nref) return;
     const int i = s.ref_order[t];
     if (s.flags[i] & DML_FL_SKIP) {
       if (sub == 0) s.ov_st[i] = 0;
-      continue;
+      return;
     }
     // fuerza or the row build of this step, at these same positions and on
     // this same row, listed the row positions of the entries within rcut
@@ -1462,7 +1465,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_ov_scan_l(DevState s, const doubl
     int nlist = round == 0 && P.ovn_stamp == P.step + 1 ? s.ov_near[i] : 255;
     if (nlist == 0) {
       if (sub == 0) s.ov_st[i] = 1;
-      continue;
+      return;
     }
     const double xi = s.px[i], yi = s.py[i], zi = s.pz[i];
     const float4 fi = s.pf4[i];
@@ -1568,7 +1571,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_ov_scan_l(DevState s, const doubl
       if (dom) {
         st |= 2;
       } else if (firstcg != 0x7fffffff) {
-        atomicAdd(&s.cnt[r].try_, 1ull);
+        k_try = 1;
         double ne;
         if (u_round) {
           ne = u_round[(int64_t)round * u_stride + i];
@@ -1580,10 +1583,49 @@ __global__ void __launch_bounds__(DML_BLOCK) k_ov_scan_l(DevState s, const doubl
         st |= (ne < P.prob) ? 4 : 8;
       }
       s.ov_st[i] = st;
-      if (n_ch) atomicAdd(&s.cnt[r].choques, n_ch);
-      if (n_ch3) atomicAdd(&s.cnt[r].choques3, n_ch3);
+      k_ch = n_ch;
+      k_ch3 = n_ch3;
       // a revert or a dominated atom: the sequential pass would recurse
-      if (dom || n_ch) s.ov_more[cur * s.nrep + r] = 1;
+      k_more = dom || n_ch;
+    }
+    k_r = r;
+  };
+  // a wave-uniform loop (every lane takes every pass), so that the counters
+  // can be summed over the wave: one atomic per wave and counter when the
+  // wave's items belong to one replica (always in round 0, whose items are
+  // consecutive positions of one 256-slot block; mostly later, the work
+  // lists being appended block by block), one per atom otherwise.  The
+  // counters of a replica are single addresses, which serialise across the
+  // chip when every atom adds to them.
+  for (int64_t it0 = (int64_t)blockIdx.x * (DML_BLOCK / LPA); it0 < nitems; it0 += stride) {
+    const int64_t item = it0 + threadIdx.x / LPA;
+    unsigned long long k_try = 0, k_ch = 0, k_ch3 = 0;
+    int k_more = 0, k_r = -1;
+    if (item < nitems) scan_one(item, k_try, k_ch, k_ch3, k_more, k_r);
+    int r_hi = k_r, r_lo = k_r < 0 ? 0x7fffffff : k_r;
+    for (int off = WAVE / 2; off > 0; off >>= 1) {
+      r_hi = max(r_hi, __shfl_xor(r_hi, off));
+      r_lo = min(r_lo, __shfl_xor(r_lo, off));
+    }
+    if (r_hi < 0) continue;  // nothing scanned in this wave
+    if (r_lo == r_hi) {
+      for (int off = WAVE / 2; off > 0; off >>= 1) {
+        k_try += __shfl_xor(k_try, off);
+        k_ch += __shfl_xor(k_ch, off);
+        k_ch3 += __shfl_xor(k_ch3, off);
+        k_more |= __shfl_xor(k_more, off);
+      }
+      if (threadIdx.x % WAVE == 0) {
+        if (k_try) atomicAdd(&s.cnt[r_hi].try_, k_try);
+        if (k_ch) atomicAdd(&s.cnt[r_hi].choques, k_ch);
+        if (k_ch3) atomicAdd(&s.cnt[r_hi].choques3, k_ch3);
+        if (k_more) s.ov_more[cur * s.nrep + r_hi] = 1;
+      }
+    } else if (k_r >= 0) {
+      if (k_try) atomicAdd(&s.cnt[k_r].try_, k_try);
+      if (k_ch) atomicAdd(&s.cnt[k_r].choques, k_ch);
+      if (k_ch3) atomicAdd(&s.cnt[k_r].choques3, k_ch3);
+      if (k_more) s.ov_more[cur * s.nrep + k_r] = 1;
     }
   }
 }
