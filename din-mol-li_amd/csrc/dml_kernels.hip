@@ -1742,9 +1742,17 @@ __global__ void __launch_bounds__(DML_BLOCK) k_ov_apply_r(DevState s, int round)
     const RepParams &P = s.rp[r];
     if (P.halted || s.ov_rev[j] != ov_stamp(P, round)) continue;
     const int t = P.poff + s.ref_pos[j];  // ref_pos is replica-local
-    if (ov_apply_one(s, r, P, j, t, round)) {
-      const int q = atomicAdd(&s.wl_cnt[(round + 1) % 2], 1);
-      if (q < cap) s.ovw[(round + 1) % 2][q] = t;
+    const int push = ov_apply_one(s, r, P, j, t, round);
+    // one atomic per wave for its appends (the active lanes of this pass)
+    const unsigned long long m = __ballot(push);
+    if (m) {
+      const int lead = __builtin_ctzll(m);
+      int base = 0;
+      if ((int)(threadIdx.x % WAVE) == lead) base = atomicAdd(&s.wl_cnt[(round + 1) % 2], __popcll(m));
+      base = __shfl(base, lead);
+      const int q = base + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                          __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+      if (push && q < cap) s.ovw[(round + 1) % 2][q] = t;
     }
   }
 }
