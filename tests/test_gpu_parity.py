@@ -1,7 +1,7 @@
 """Parity of the HIP path (through the C ABI) with the oracle and with the
 reference-built cell/list fixtures, on identical inputs with injected noise.
 
-Bars (per north_star):
+Bars:
 * cells and neighbour lists: bit-exact, entry for entry (list order too);
 * FP64 forces, energies and one-step positions: relative error <= 1e-12
   (norm with a floor of 1e-3 * rms so near-cutoff zeros do not divide by 0);
