@@ -866,9 +866,52 @@ __global__ void __launch_bounds__(DML_BLOCK) k_cell_sort_w(DevState s, int gcs) 
     while (work) {
     const int q = __builtin_ctzll(work);
     work &= work - 1;
+    const int cnt = __builtin_amdgcn_readlane(lcnt, q);
+    if (cnt <= WAVE / 2 && work &&
+        __builtin_amdgcn_readlane(lcnt, __builtin_ctzll(work)) <= WAVE / 2) {
+      // two cells of up to 32 entries at once, one per half-wave: their
+      // loads are in flight together (a cell costs a round trip to memory)
+      const int q2 = __builtin_ctzll(work);
+      work &= work - 1;
+      const int half = lane >> 5, sub = lane & 31;
+      const int cnt2 = __builtin_amdgcn_readlane(lcnt, q2);
+      const int rq = half ? __builtin_amdgcn_readlane(lr, q2) : __builtin_amdgcn_readlane(lr, q);
+      const int cq = half ? __builtin_amdgcn_readlane(lc, q2) : __builtin_amdgcn_readlane(lc, q);
+      const int mycnt = half ? cnt2 : cnt;
+      const RepParams &H = s.rp[rq];
+      const int hdesc = H.use_cells;
+      const int64_t hbase = H.boff + (int64_t)cq * H.bcap;
+      const bool in = sub < mycnt;
+      const int v = in ? s.bucket[hbase + sub] : 0;
+      int k = 0;
+      double x = 0.0, y = 0.0, z = 0.0;
+      float4 f = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (in) {
+        k = s.bkey[v];
+        x = s.px[v];
+        y = s.py[v];
+        z = s.pz[v];
+        f = s.pf4[v];
+      }
+      int rank = 0;
+      const int cmax = cnt > cnt2 ? cnt : cnt2;
+      for (int j = 0; j < cmax; j++) {  // j is wave-uniform
+        const int ka = __builtin_amdgcn_readlane(k, j);
+        const int kb = __builtin_amdgcn_readlane(k, j + WAVE / 2);
+        const int kq = half ? kb : ka;
+        rank += (j < mycnt) & (hdesc ? (kq > k) : (kq < k));
+      }
+      if (in) {
+        s.bucket[hbase + rank] = v;
+        s.bx[hbase + rank] = x;
+        s.by[hbase + rank] = y;
+        s.bz[hbase + rank] = z;
+        s.bf4[hbase + rank] = f;
+      }
+      continue;
+    }
     const RepParams &P = s.rp[__builtin_amdgcn_readlane(lr, q)];
     const int c = __builtin_amdgcn_readlane(lc, q);
-    const int cnt = __builtin_amdgcn_readlane(lcnt, q);
     const int desc = P.use_cells;
     const int64_t base = P.boff + (int64_t)c * P.bcap;
     int32_t *b = s.bucket + base;
