@@ -409,8 +409,10 @@ void enq_forces(dml_ctx *c, int fuse_b, int only_ermak) {
 void enq_overlap_rounds(dml_ctx *c, const double *u_round, int64_t stride, int rounds,
                         int err_if_pending) {
   hipLaunchKernelGGL(k_ov_begin, dim3(1), dim3(64), 0, c->st, c->d);
-  const int lpa = c->lpa_ov;
   for (int q = 0; q < rounds; q++) {
+    // round 0 mostly visits the few listed entries within rcut (the tuned
+    // lanes, often 1); later rounds scan whole rows, with at least 2 lanes
+    const int lpa = q == 0 ? c->lpa_ov : std::max(c->lpa_ov, 2);
     // round 0 covers every ref position; later rounds walk their work lists
     // (grid-stride, a bounded grid)
     unsigned gs = (unsigned)(c->P * lpa / DML_BLOCK);
