@@ -1065,17 +1065,23 @@ __global__ void __launch_bounds__(DML_BLOCK) k_force(DevState s, int only_ermak)
     const float4 fi = s.pf4[i];
     const float bxf = (float)P.box[0], byf = (float)P.box[1];
     const float ibx = (float)P.one_box[0], iby = (float)P.one_box[1];
+    // this atom's FP32 prefilter bound r0^2 + margin against each of the
+    // three species (held in registers: no table load per entry)
+    const float th1 = (float)P.r0[(k - 1) * 3 + 0] * (float)P.r0[(k - 1) * 3 + 0] + P.pf_margin;
+    const float th2 = (float)P.r0[(k - 1) * 3 + 1] * (float)P.r0[(k - 1) * 3 + 1] + P.pf_margin;
+    const float th3 = (float)P.r0[(k - 1) * 3 + 2] * (float)P.r0[(k - 1) * 3 + 2] + P.pf_margin;
     for (int e = sub; e < nn; e += LPA) {
       int j = row[e];
       const float4 fj4 = s.pf4[j];
       {  // surely beyond the pair's r0: no force, no energy
-        float r0f = (float)P.r0[(k - 1) * 3 + (__float_as_int(fj4.w) - 1)];
+        const int sj = __float_as_int(fj4.w);
+        const float th = sj == 1 ? th1 : (sj == 2 ? th2 : th3);
         const float d2f = d2_f32(P, fi, fj4, bxf, byf, ibx, iby);
         if (d2f <= rcf) {
           ncnt++;
           ne = e;
         }
-        if (d2f > r0f * r0f + P.pf_margin) continue;
+        if (d2f > th) continue;
       }
       uint8_t fj = s.flags[j];
       if (fj & DML_FL_LIMBO) continue;
