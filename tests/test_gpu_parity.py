@@ -393,6 +393,39 @@ def test_brownian_round_overlap_after_rebuilds_matches_oracle(orc, dml, lb_mode,
     c.close()
 
 
+def test_set_order_after_forces_matches_recorded_overlap_pass(dml):
+    """dml_set_order after fuerza drops the force kernel's entries-within-rcut
+    records, so round 0 of the next overlap pass scans every ref atom; with the
+    same order handed back, that pass must end exactly where the pass that
+    used the records ends."""
+    rng = np.random.default_rng(51)
+    box = np.array([60.0, 60.0, 60.0])
+    el = electrode(3.51, 17, 17, 2, 3.51)
+    # ions 3.0 A apart at least: shallow overlaps (r0 3.2 A) that rounds resolve
+    ions = rsa(rng, 1600, np.array([0, 0, 6.0]), np.array([60, 60, 59.0]), 3.0, box)
+    pos = np.vstack([el, ions])
+    z = np.array([2] * len(el) + [1] * len(ions), np.int8)
+    say("set_order after forces", len(pos))
+    cfg = dml.dana_config(box, 10.0, len(pos) + 64, h=1e-2, z0=30.0, zmax=60.0,
+                          integrator=dml.ERMAK, overlap_mode=dml.OVERLAP_ROUNDS)
+    a, b = dml.Context(cfg), dml.Context(cfg)
+    for c in (a, b):
+        c.set_state(pos, species=z)
+        c.test_update()
+        c.forces()
+    a.set_order(ref_order=a.ref_order())
+    u = rng.random((32, a.total))
+    for c in (a, b):
+        c.overlap_rounds(u, 32)
+    sa, sb = a.state(), b.state()
+    for k in ("pos", "species", "flags"):
+        np.testing.assert_array_equal(sa[k], sb[k], err_msg=k)
+    np.testing.assert_array_equal(a.ref_order(), b.ref_order())
+    assert a.counters()["choques"] == b.counters()["choques"] > 0
+    a.close()
+    b.close()
+
+
 def test_gcmc_trial_energy_matches_oracle(orc, dml):
     rng = np.random.default_rng(31)
     box = np.array([60.0, 60.0, 60.0])
