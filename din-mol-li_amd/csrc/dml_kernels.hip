@@ -1397,8 +1397,14 @@ __global__ void __launch_bounds__(WAVE) k_ov_exact(DevState s, const double *u,
             int jj = __shfl(j, q2);
             double dx, dy, dz;
             vdist_mic(P, xi, yi, zi, s.px[jj], s.py[jj], s.pz[jj], &dx, &dy, &dz);
-            if (dot3(dx, dy, dz) > P.rcut2) continue;
-            if (s.species[jj] == SP_CG) {
+            // every lane takes the same decisions from the same reads, then
+            // writes the same values: the reads come before any write
+            const bool near = !(dot3(dx, dy, dz) > P.rcut2);
+            const bool cg = near && s.species[jj] == SP_CG;
+            const bool stay = near && !cg && P.piston && same_pos(s, jj) && same_pos(s, ii);
+            __builtin_amdgcn_wave_barrier();
+            if (!near) continue;
+            if (cg) {
               n_try++;
               double ne;
               if (ur) {
@@ -1425,7 +1431,7 @@ __global__ void __launch_bounds__(WAVE) k_ov_exact(DevState s, const double *u,
               done = 1;
               break;
             }
-            if (P.piston && same_pos(s, jj) && same_pos(s, ii)) {
+            if (stay) {
               n_ch3++;
               continue;
             }
@@ -2802,6 +2808,7 @@ __global__ void __launch_bounds__(GC_BLOCK) k_gcmc_run(DevState s, int rep_only,
       }
       if (p < 0 || p >= ilen) { fail = ERR_NOPART; break; }
       const int o = invl[p];
+      if (o < poff || o >= poff + P.pcap) { fail = ERR_UNSUP; break; }
       n--;
       // cgroup_unsort_atom (Cells.F90:304-352)
       int cell = 0;
@@ -2814,6 +2821,10 @@ __global__ void __launch_bounds__(GC_BLOCK) k_gcmc_run(DevState s, int rep_only,
         cell = cell_lin(P, c0, c1, c2);
       }
       const int bso = s.bkey[o];
+      // every thread has read the tombstones, invl[p] and the atom: only now
+      // may thread 0 change them (a wave still reading would otherwise see
+      // the tombstone it is about to step over, or the shifted list)
+      __syncthreads();
       if (tid == 0) {
         invl[p] = -1;
         {
@@ -3001,6 +3012,9 @@ __global__ void __launch_bounds__(DML_BLOCK) k_chunk_run(DevState s) {
     s.chz[c] = s.chz[c] + P.dist;
     s.choz[c] = s.choz[c] + P.dist;
   }
+  // every thread has copied the replica's parameters (without reusable
+  // slots no barrier above separates that copy from the writes below)
+  __syncthreads();
   if (tid == 0) {
     Pw->z0 = Q.z0;
     Pw->z1 = Q.z1;

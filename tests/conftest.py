@@ -13,6 +13,13 @@ for p in (ROOT, PKG, os.path.join(ROOT, "oracle")):
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (run with -m gpu)")
     config.addinivalue_line("markers", "slow: longer CPU checks")
+    if os.environ.get("DML_SIMT"):
+        # the GPU tests against the CPU SIMT build of the same sources
+        # (tests/simt: fibers per lane, AddressSanitizer); a debugging aid,
+        # never the default
+        sys.path.insert(0, os.path.join(ROOT, "tests", "simt"))
+        import sim
+        sim.use()
 
 
 @pytest.fixture(scope="session")
