@@ -14,10 +14,8 @@ Also reported: the dominant kernel against the HBM roofline and the CPU
 oracle on the reference's Ermak test (1,204 particles, 10^4 steps) on one
 host core.
 
-The plain leg (with its FP32-vs-FP64 force error and energy drift) and the
-serving leg are switched off: one full run of all four legs faulted the GPU
-in one of them, and no cause has been found.  Asking for either exits with
-an error; the default is the batched and c5 legs.
+  serving  C4: Poisson arrivals at 20/s, continuous batching over a slot pool;
+           p50/p99 completion latency.
 """
 import argparse
 import json
@@ -390,22 +388,13 @@ def pmc_summary(fetch_csv, write_csv, kernel="k_listbuild_wc", last=10, copy_byt
             "hbm_bytes_per_launch": rd + wr}
 
 
-# legs that stay off until the GPU fault seen in one full run is explained
-LEGS_OFF = {
-    "plain": "a full run of all legs faulted the GPU in the plain leg (C2 timing and "
-             "10^4-step energy-drift runs) or at the start of the serving leg; no cause found",
-    "serving": "a full run of all legs faulted the GPU at the start of the serving leg or in "
-               "the plain leg before it; no cause found",
-}
-
-
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--replicas", type=int, default=256)
-    ap.add_argument("--legs", default="batched,c5")
+    ap.add_argument("--legs", default="plain,batched,serving,c5")
     ap.add_argument("--drift-steps", type=int, default=10000,
                     help="Ermak steps of each FP32/FP64 energy-drift run on C2 (0 = skip)")
     ap.add_argument("--c5-steps", type=int, default=2000,
@@ -431,11 +420,6 @@ def main():
     except Exception:
         sync = lambda: None
     legs = args.legs.split(",")
-    off = sorted(set(legs) & set(LEGS_OFF))
-    if off:
-        print("bench.py: the %s leg(s) are switched off: %s" % (",".join(off), LEGS_OFF[off[0]]),
-              file=sys.stderr)
-        sys.exit(2)
     if args.pmc_summary:
         print(json.dumps(pmc_summary(*args.pmc_summary), indent=1))
         return
@@ -495,7 +479,6 @@ def main():
     if plain and batched:
         res["batched_over_plain"] = batched["rate"] / plain["rate"]
     res["p99_run_latency_s"] = serving["p99_latency_s"] if serving else None
-    res["legs_not_run"] = {k: v for k, v in LEGS_OFF.items() if k not in legs}
     if c5:
         res["c5"] = c5
     if serving:

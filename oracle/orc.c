@@ -1416,6 +1416,9 @@ int orc_bloques(orc_sys *s, orc_chunk *ch, double rho, int *fired, orc_counters 
 }
 
 int orc_dana_run(const orc_run_cfg *cfg, orc_run_out *out) {
+  double *u_round = NULL; /* round-schedule uniforms (overlap_rounds > 0) */
+  size_t u_cap = 0;
+  uint64_t u_state = 0;
   orc_params p;
   memset(&p, 0, sizeof(p));
   /* eps and r0 (dana.F90:86-100); index [(k-1)*3 + (m-1)] */
@@ -1509,6 +1512,7 @@ int orc_dana_run(const orc_run_cfg *cfg, orc_run_out *out) {
   /* config_chunk (dana.F90:552-587) */
   if (chunks && (status = orc_chunk_init(s, &ch, cfg->nchunk, cfg->chunk_pos, dist))) goto done;
   const double tau = 0.1;
+  u_state = cfg->rounds_seed;
   double t0 = now_s();
   int64_t psteps = 0;
   for (int step = 1; step <= cfg->nst; step++) {
@@ -1521,7 +1525,28 @@ int orc_dana_run(const orc_run_cfg *cfg, orc_run_out *out) {
       if ((status = orc_brownian(s, &rng, &cnt))) goto done;
     }
     if ((status = orc_test_update(s, NULL, &cnt))) goto done;
-    if ((status = orc_overlap(s, &rng, &cnt, 100000))) goto done;
+    if (cfg->overlap_rounds > 0) {
+      /* the round schedule with its own uniforms (splitmix64 per call) */
+      size_t need = (size_t)cfg->overlap_rounds * (size_t)s->amax;
+      if (need > u_cap) {
+        free(u_round);
+        u_cap = need;
+        u_round = (double *)malloc(sizeof(double) * u_cap);
+        if (!u_round) { status = ORC_E_ALLOC; goto done; }
+      }
+      for (size_t q = 0; q < need; q++) {
+        uint64_t z = (u_state += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        u_round[q] = ((double)(z >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+      }
+      int st = orc_overlap_rounds(s, u_round, cfg->overlap_rounds, &cnt);
+      if (st == ORC_E_OVERLAP_LOOP) cnt.ov_unfinished++;
+      else if ((status = st)) goto done;
+    } else if ((status = orc_overlap(s, &rng, &cnt, 100000))) {
+      goto done;
+    }
     if ((status = orc_test_update(s, NULL, &cnt))) goto done;
     orc_f_to_cg(s, gcmc);
     if (gcmc)
@@ -1566,6 +1591,7 @@ int orc_dana_run(const orc_run_cfg *cfg, orc_run_out *out) {
   }
   out->c = cnt;
 done:
+  free(u_round);
   orc_chunk_free(&ch);
   free(r);
   free(zinit);

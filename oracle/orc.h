@@ -106,6 +106,7 @@ typedef struct orc_params {
 typedef struct orc_counters {
   int64_t try_, depo, choques, choques2, choques3, nupd, ins, del;
   int64_t passes;
+  int64_t ov_unfinished; /* round-schedule calls that ran out of rounds */
 } orc_counters;
 
 /* ---- system construction and access (hs-slot indexed) ---- */
@@ -218,6 +219,12 @@ typedef struct orc_run_cfg {
   int32_t ninit;
   const double *init_pos;  /* ninit x 3 row-major */
   const int8_t *init_z;    /* species, NULL = all Li */
+  /* 0: overlap_moveback as the reference (dana.F90:849-943); n > 0: the
+     round schedule (orc_overlap_rounds) with at most n rounds per call and
+     its uniforms from a stream of its own, a call that runs out of rounds
+     counted and the run continued (statistics of the schedule only) */
+  int32_t overlap_rounds;
+  uint64_t rounds_seed;
 } orc_run_cfg;
 
 typedef struct orc_run_out {

@@ -40,7 +40,7 @@ class Params(C.Structure):
 
 class Counters(C.Structure):
     _fields_ = [(n, C.c_int64) for n in ("try_", "depo", "choques", "choques2", "choques3",
-                                         "nupd", "ins", "del_", "passes")]
+                                         "nupd", "ins", "del_", "passes", "ov_unfinished")]
 
 
 class Rng(C.Structure):
@@ -62,7 +62,7 @@ class RunCfg(C.Structure):
                 ("nb_dcut", C.c_double), ("ermak", C.c_int32), ("reservoir", C.c_int32),
                 ("act", C.c_double), ("nadj", C.c_int32), ("nchunk", C.c_int32),
                 ("chunk_pos", C.c_void_p), ("ninit", C.c_int32), ("init_pos", C.c_void_p),
-                ("init_z", C.c_void_p)]
+                ("init_z", C.c_void_p), ("overlap_rounds", C.c_int32), ("rounds_seed", C.c_uint64)]
 
 
 class RunOut(C.Structure):
@@ -342,8 +342,11 @@ def nr_stream(idum, n, kind="ran"):
 
 
 def dana_run(idum, prob, h, nst, nwr, xi, yi, dist, z0, zmax, dif_sc, dif_sei, nb_dcut,
-             ermak, reservoir, act=0.0, nadj=0, chunk=None, init_pos=None, init_z=None, cap=1 << 20):
-    """Whole dana run; returns (nat, zmax, pos[nat,3], z[nat], counters, loop_s, psteps)."""
+             ermak, reservoir, act=0.0, nadj=0, chunk=None, init_pos=None, init_z=None, cap=1 << 20,
+             overlap_rounds=0, rounds_seed=0):
+    """Whole dana run; returns (nat, zmax, pos[nat,3], z[nat], counters, loop_s, psteps).
+    overlap_rounds > 0 replaces overlap_moveback by the round schedule (at
+    most that many rounds per call, uniforms from rounds_seed)."""
     L = lib()
     c = RunCfg()
     c.idum, c.prob, c.h, c.nst, c.nwr = idum, prob, h, nst, nwr
@@ -352,6 +355,7 @@ def dana_run(idum, prob, h, nst, nwr, xi, yi, dist, z0, zmax, dif_sc, dif_sei, n
     c.ermak = int(ermak)
     c.reservoir = {"piston": 0, "chunks": 1, "gcmc": 2}[reservoir] if isinstance(reservoir, str) else reservoir
     c.act, c.nadj = act, nadj
+    c.overlap_rounds, c.rounds_seed = int(overlap_rounds), int(rounds_seed)
     keep = []
     if chunk is not None:
         ch = np.ascontiguousarray(chunk, np.float64)
