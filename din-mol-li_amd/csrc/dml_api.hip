@@ -447,8 +447,8 @@ void enq_overlap_rounds(dml_ctx *c, const double *u_round, int64_t stride, int r
         break;
     }
     hipLaunchKernelGGL(k_ov_apply, dim3(ga), dim3(DML_BLOCK), 0, c->st, c->d, q, c->P);
-    if (q > 0)
-      hipLaunchKernelGGL(k_ov_apply_r, dim3(gr), dim3(DML_BLOCK), 0, c->st, c->d, q);
+    hipLaunchKernelGGL(k_ov_apply_r, dim3(q == 0 ? std::min(gr, 64u) : gr), dim3(DML_BLOCK), 0,
+                       c->st, c->d, q);
   }
   hipLaunchKernelGGL(k_ov_end, dim3((c->nrep + 63) / 64), dim3(64), 0, c->st, c->d, rounds,
                      err_if_pending);
@@ -967,11 +967,20 @@ int dml_set_order(dml_ctx *c, int32_t rep, int32_t nref, const int32_t *ref_orde
   if (ref_order) {
     if (nref < 0 || nref > n) return DML_E_ARG;
     std::vector<int32_t> g(std::max(nref, 1)), rpos(std::max(n, 1), -1);
+    std::vector<uint8_t> fl(std::max(n, 1));
+    HIPCHECK(hipMemcpy(fl.data(), c->d.flags + o, n, hipMemcpyDeviceToHost));
     for (int t = 0; t < nref; t++) {
-      if (ref_order[t] < 0 || ref_order[t] >= n) return DML_E_ARG;
-      g[t] = (int32_t)(o + ref_order[t]);
-      rpos[ref_order[t]] = t;
+      const int i = ref_order[t];
+      // each listed atom once, and alive
+      if (i < 0 || i >= n || rpos[i] >= 0 || !(fl[i] & DML_FL_ALIVE)) return DML_E_ARG;
+      g[t] = (int32_t)(o + i);
+      rpos[i] = t;
     }
+    // hs%ref membership is exactly the list: the force and overlap kernels
+    // read it from the flags
+    for (int i = 0; i < n; i++)
+      fl[i] = (uint8_t)(rpos[i] >= 0 ? (fl[i] | DML_FL_REF) : (fl[i] & ~DML_FL_REF));
+    HIPCHECK(hipMemcpy(c->d.flags + o, fl.data(), n, hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(c->d.ref_order + o, g.data(), sizeof(int32_t) * nref, hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(c->d.ref_pos + o, rpos.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
     p.nref = nref;

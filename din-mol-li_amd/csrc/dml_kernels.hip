@@ -1474,6 +1474,7 @@ __device__ inline int ov_stamp(const RepParams &P, int round) { return P.ov_roun
 __global__ void k_ov_begin(DevState s) {
   int q = blockIdx.x * blockDim.x + threadIdx.x;
   for (int k = q; k < 3 * s.nrep; k += gridDim.x * blockDim.x) s.ov_more[k] = 0;
+  if (q == 0) s.wl_cnt[2] = 0;  // round 0's list of reverted atoms outside ref
 }
 
 template <int LPA>
@@ -1540,7 +1541,10 @@ __global__ void __launch_bounds__(DML_BLOCK) __attribute__((amdgpu_waves_per_eu(
       double dx, dy, dz;
       vdist_mic(P, xi, yi, zi, s.px[j], s.py[j], s.pz[j], &dx, &dy, &dz);
       if (dot3(dx, dy, dz) > P.rcut2) return;
-      if ((fj & DML_FL_REF) && !(fj & DML_FL_SKIP) && s.ref_pos[j] < local) dom = 1;
+      if ((fj & DML_FL_REF) && !(fj & DML_FL_SKIP)) {
+        const int pj = s.ref_pos[j];
+        if (pj >= 0 && pj < local) dom = 1;
+      }
       if (s.species[j] == SP_CG) {
         if (e < firstcg) firstcg = e;
       } else if (nh == 0) {
@@ -1589,8 +1593,8 @@ __global__ void __launch_bounds__(DML_BLOCK) __attribute__((amdgpu_waves_per_eu(
     // Counted by the callers with arithmetic, keeping the counters in registers
     auto revert = [&](int j) -> int {
       if (P.piston && same_pos(s, j) && same_pos(s, i)) return 1;
-      if (round == 0) {
-        s.ov_rev[j] = stamp;
+      if (round == 0 && (s.flags[j] & DML_FL_REF) && s.ref_pos[j] >= 0) {
+        s.ov_rev[j] = stamp;  // k_ov_apply walks every ref position in round 0
       } else if (atomicExch(&s.ov_rev[j], stamp) != stamp) {
         // first to revert j in this round: it is applied from the list
         const int q = atomicAdd(&s.wl_cnt[2 + round % 2], 1);
@@ -1783,8 +1787,10 @@ __global__ void __launch_bounds__(DML_BLOCK) k_ov_apply(DevState s, int round, i
   }
 }
 
-// rounds > 0: the atoms reverted in the round that were not scanned in it
-// (those that were are applied, and their stamp cleared, by k_ov_apply)
+// the atoms reverted in a round that k_ov_apply does not reach: in round 0
+// those outside the ref list (it walks every ref position), later the ones
+// the round did not scan (those that were are applied, and their stamp
+// cleared, by k_ov_apply)
 __global__ void __launch_bounds__(DML_BLOCK) k_ov_apply_r(DevState s, int round) {
   const int64_t cap = (int64_t)s.nblocks * DML_BLOCK;
   const int64_t nitems = min((int64_t)s.wl_cnt[2 + round % 2], cap);

@@ -729,3 +729,59 @@ def test_chunks_reservoir_matches_oracle(orc, dml, band):
         _assert_same_groups(s, c, "bloques %d" % it)
     assert fired == (3 if band == "fires" else 0)
     c.close()
+
+
+def test_round_overlap_with_subset_ref_order_matches_restatement(dml):
+    """dml_set_order with a shuffled ref list that leaves some ions out: the
+    flags follow the list (REF on exactly the listed atoms), an ion outside
+    the list is reverted like any non-CG neighbour (dana.F90:928-934) and
+    waits for nobody; the device's round schedule equals the restatement of
+    oracle/ovl.py (itself equal to orc_overlap_rounds) on the device's own
+    state before the call, with the same uniforms.  A repeated entry is
+    refused."""
+    import ovl
+    rng = np.random.default_rng(61)
+    box = np.array([40.0, 40.0, 30.0])
+    el = electrode(3.51, 11, 11, 2, 1.755)
+    ions = rsa(rng, 220, np.array([0, 0, 4.0]), np.array([40, 40, 29.0]), 3.3, box)
+    pos = np.vstack([el, ions])
+    z = np.array([2] * len(el) + [1] * len(ions), np.int8)
+    say("subset ref order", len(pos))
+    c = dml.Context(dml.dana_config(box, 2.0, len(pos) + 64, h=1e-2, zmax=30.0, prob=0.6,
+                                    integrator=dml.BROWNIAN, reservoir=dml.NO_RESERVOIR,
+                                    overlap_mode=dml.OVERLAP_ROUNDS))
+    c.set_state(pos, species=z)
+    c.test_update()
+    n = len(pos)
+    ion_idx = np.arange(len(el), n)
+    c.brownian(rng.standard_normal(3 * n), np.ones(n))
+    c.test_update()
+    keep = ion_idx[rng.random(len(ion_idx)) > 0.2]
+    order = rng.permutation(keep).astype(np.int32)
+    with pytest.raises(dml.DmlError):
+        c.set_order(ref_order=np.concatenate([order[:5], order[:1]]))
+    c.set_order(ref_order=order)
+    st = c.state()
+    np.testing.assert_array_equal((st["flags"] & 4) != 0, np.isin(np.arange(n), order))
+    nn, rp, idx = c.lists()
+    m = dict(pos=st["pos"].copy(), old=st["old_cg"].copy(), vel=st["vel"].copy(),
+             acel=st["acel"].copy(), zs=st["species"].astype(int), skip=(st["flags"] & 8) != 0,
+             rows=[list(idx[rp[i]:rp[i + 1]]) for i in range(n)], order=list(order),
+             limbo=(st["flags"] & 2) != 0, box=box, rcut=3.2, prob=0.6, z0=100.0, piston=False)
+    R = 32
+    ur = rng.random((R, c.total))
+    want, wc, wst = ovl.round_overlap(m, lambda i, k, r: ur[r, i], R)
+    assert wst == 0
+    c0 = c.counters()
+    c.overlap_rounds(ur, R)
+    got = c.state()
+    outside = np.setdiff1d(ion_idx, order)
+    moved_out = [i for i in outside if not np.array_equal(want["pos"][i], m["pos"][i])]
+    assert moved_out, "no ion outside the ref list was reverted: the case does not test it"
+    np.testing.assert_array_equal(got["pos"], want["pos"])
+    np.testing.assert_array_equal(got["species"].astype(int), want["zs"])
+    np.testing.assert_array_equal((got["flags"] & 8) != 0, want["skip"])
+    np.testing.assert_array_equal(got["vel"], want["vel"])
+    c1 = c.counters()
+    assert (c1["try_"] - c0["try_"], c1["choques"] - c0["choques"]) == (wc["try_"], wc["choques"])
+    c.close()
