@@ -1415,6 +1415,19 @@ int orc_bloques(orc_sys *s, orc_chunk *ch, double rho, int *fired, orc_counters 
   return orc_test_update(s, NULL, c);
 }
 
+/* maxz (dana.F90:776-794): the piston factor lohi from rho (calc_rho of
+   this step) and rho0 (the initial one), applied to every sys atom above z0
+   and to zmax */
+void orc_maxz(orc_sys *s, double rho, double rho0) {
+  const double tau = 0.1;
+  double lohi = ((s->p.h / tau) * ((rho0 - rho) / rho));
+  for (int q = 0; q < s->sys.nat; q++) {
+    int i = s->sys.slot[q];
+    if (s->pos[i][2] > s->p.z0) s->pos[i][2] = s->pos[i][2] - lohi * (s->pos[i][2] - s->p.z0);
+  }
+  s->p.zmax = s->p.zmax - lohi * (s->p.zmax - s->p.z0);
+}
+
 int orc_dana_run(const orc_run_cfg *cfg, orc_run_out *out) {
   double *u_round = NULL; /* round-schedule uniforms (overlap_rounds > 0) */
   size_t u_cap = 0;
@@ -1511,7 +1524,6 @@ int orc_dana_run(const orc_run_cfg *cfg, orc_run_out *out) {
   double rho0 = rho;
   /* config_chunk (dana.F90:552-587) */
   if (chunks && (status = orc_chunk_init(s, &ch, cfg->nchunk, cfg->chunk_pos, dist))) goto done;
-  const double tau = 0.1;
   u_state = cfg->rounds_seed;
   double t0 = now_s();
   int64_t psteps = 0;
@@ -1566,16 +1578,7 @@ int orc_dana_run(const orc_run_cfg *cfg, orc_run_out *out) {
         out->z[q] = s->zs[i];
       }
     }
-    if (s->p.piston) {
-      /* maxz (dana.F90:776-794) over sys slots 1..nat */
-      double lohi = ((s->p.h / tau) * ((rho0 - rho) / rho));
-      for (int q = 0; q < s->sys.nat; q++) {
-        int i = s->sys.slot[q];
-        if (s->pos[i][2] > s->p.z0)
-          s->pos[i][2] = s->pos[i][2] - lohi * (s->pos[i][2] - s->p.z0);
-      }
-      s->p.zmax = s->p.zmax - lohi * (s->p.zmax - s->p.z0);
-    }
+    if (s->p.piston) orc_maxz(s, rho, rho0);
   }
   out->loop_seconds = now_s() - t0;
   out->particle_steps = psteps;

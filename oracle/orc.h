@@ -185,6 +185,8 @@ void orc_chunk_free(orc_chunk *ch);
 int orc_bloques(orc_sys *s, orc_chunk *ch, double rho, int *fired, orc_counters *c);
 /* calc_rho (dana.F90:521-549): chunks = 1 counts up to z1 */
 double orc_calc_rho(orc_sys *s, int chunks);
+/* maxz (dana.F90:776-794) with this step's rho and the initial rho0 */
+void orc_maxz(orc_sys *s, double rho, double rho0);
 /* salida's observables (dana.F90:1141-1183) in sys order: out = energia,
  * kion temperature (dana.F90:1342-1376), j (non-CG count), sys%nat */
 void orc_observables(const orc_sys *s, double out[4]);

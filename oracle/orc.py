@@ -297,6 +297,10 @@ class System:
     def calc_rho(self, chunks=False):
         return self.L.orc_calc_rho(C.c_void_p(self.s), int(chunks))
 
+    def maxz(self, rho, rho0):
+        """maxz (dana.F90:776-794): the piston move with this step's rho"""
+        self.L.orc_maxz(C.c_void_p(self.s), C.c_double(rho), C.c_double(rho0))
+
     def set_z1(self, z1):
         self.L.orc_set_z1(C.c_void_p(self.s), C.c_double(z1))
 

@@ -326,7 +326,21 @@ def test_ermak_piston_steps_match_oracle(orc, dml):
     s, c = pair(orc, dml, box, skin, pos, z, zmax=100.0, z0=50.0, piston=True, prob=0.8)
     s.test_update(); c.test_update()
     s.fuerza(); c.forces()
-    so, sd = _step_pair(orc, dml, s, c, rng, "exact", 6, brown=False)
+    rho0 = s.calc_rho()
+    for k in range(6):
+        so, sd = _step_pair(orc, dml, s, c, rng, "exact", 1, brown=False)
+        # the piston reservoir of the step (dana.F90:244, 261): calc_rho
+        # (dana.F90:521-549) counts the same atoms, maxz (dana.F90:776-794)
+        # moves the atoms above z0 and zmax by the same factor
+        rho_o = s.calc_rho()
+        rho_d = c.calc_rho()[0]
+        assert rho_d == rho_o, (k, rho_d, rho_o)
+        s.maxz(rho_o, rho0)
+        c.maxz()
+        assert c.zmax() == s.reservoir()[2], k
+        so, sd = s.state(), c.state()
+        assert rel_err(sd["pos"], so["pos"]) <= 1e-12, k
+        np.testing.assert_array_equal(sd["pos"][:, 2] > 50.0, so["pos"][:, 2] > 50.0)
     assert rel_err(sd["vel"][so["species"] != 2], so["vel"][so["species"] != 2]) <= 1e-12
     c.close()
 
@@ -784,4 +798,39 @@ def test_round_overlap_with_subset_ref_order_matches_restatement(dml):
     np.testing.assert_array_equal(got["vel"], want["vel"])
     c1 = c.counters()
     assert (c1["try_"] - c0["try_"], c1["choques"] - c0["choques"]) == (wc["try_"], wc["choques"])
+    c.close()
+
+
+def test_device_brownian_noise_is_the_philox_stream(dml):
+    """dml_brownian without injected noise draws, for the atom of particle id
+    p at step s, gauss2(philox(seed, p, s, RS_BROWN, 0)) and the first of
+    gauss2(... draw 1) (cbrownian_hs's three gasdev, dana.F90:823-833): the
+    normals recovered from one step of well-separated atoms, D small enough
+    that nothing touches, equal the host restatement of tests/philox_host.py
+    (pinned by the SC11 vectors) within 1e-12."""
+    import philox_host as ph
+    rng = np.random.default_rng(71)
+    box = np.array([60.0, 60.0, 60.0])
+    pos = rsa(rng, 300, np.array([5.0, 5.0, 10.0]), np.array([55.0, 55.0, 50.0]), 4.0, box)
+    z = np.ones(len(pos), np.int8)
+    seed = 0x1234ABCD5678
+    say("philox stream", len(pos))
+    cfg = dml.dana_config(box, 2.0, len(pos) + 64, h=1e-2, zmax=60.0, dif_sc=1.0, dif_sei=1.0,
+                          integrator=dml.BROWNIAN, reservoir=dml.NO_RESERVOIR, seed=seed,
+                          overlap_mode=dml.OVERLAP_ROUNDS)
+    c = dml.Context(cfg)
+    c.set_state(pos, species=z)
+    c.test_update()
+    for step in range(3):
+        before = c.state()["pos"]
+        c.brownian()
+        after = c.state()["pos"]
+        xi = (after - before) / np.sqrt(2.0 * 1.0 * 1e-2)
+        want = np.zeros_like(xi)
+        for i in range(len(pos)):
+            g0, g1 = ph.gauss2(ph.draw(seed, i, step, ph.RS_BROWN, 0))
+            g2, _ = ph.gauss2(ph.draw(seed, i, step, ph.RS_BROWN, 1))
+            want[i] = (g0, g1, g2)
+        assert np.max(np.abs(xi - want)) <= 1e-12, step
+        c.end_step()
     c.close()

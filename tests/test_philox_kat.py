@@ -17,3 +17,13 @@ def test_philox4x32_10_known_answers():
     out = subprocess.run([KAT], capture_output=True, text=True, timeout=30)
     assert out.returncode == 0, out.stderr
     assert out.stdout.split("\n")[:3] == EXPECTED
+
+
+def test_host_restatement_known_answers():
+    """tests/philox_host.py (the checker of the device streams) on the same vectors"""
+    import philox_host as ph
+    vec = [((0, 0, 0, 0), (0, 0)),
+           ((0xffffffff,) * 4, (0xffffffff, 0xffffffff)),
+           ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xa4093822, 0x299f31d0))]
+    got = [" ".join("%08x" % w for w in ph.philox4x32_10(c, k)) for c, k in vec]
+    assert got == EXPECTED
