@@ -140,10 +140,10 @@ def plain_leg(dml, W, steps, warmup, drift_steps=10000):
                 energy_drift=drift)
 
 
-def batched_leg(dml, nrep, steps, warmup):
-    t0 = time.perf_counter()
-    reps = build_replicas(nrep)
-    log("built %d replicas in %.1f s" % (nrep, time.perf_counter() - t0))
+def c3_configs(dml, reps, precision=None):
+    """the configs of the C3 ensemble: replica k has reservoir
+    RESERVOIR_CYCLE[k % 3], the Brownian (k even) or Ermak (k odd)
+    integrator, FP32 pair forces and the round schedule"""
     cfgs = []
     for k, r in enumerate(reps):
         n = len(r["pos"])
@@ -159,8 +159,16 @@ def batched_leg(dml, nrep, steps, warmup):
             n += 4 * len(r["chunk"])  # room for four blocks (the sensor holds after ~3)
         cfgs.append(dml.dana_config(r["box"], r["skin"], n, h=r["h"],
                                     integrator=dml.ERMAK if k % 2 else dml.BROWNIAN,
-                                    precision=dml.FP32_FORCE, overlap_mode=dml.OVERLAP_ROUNDS,
-                                    seed=104012 + 7919 * k, **kw))
+                                    precision=dml.FP32_FORCE if precision is None else precision,
+                                    overlap_mode=dml.OVERLAP_ROUNDS, seed=104012 + 7919 * k, **kw))
+    return cfgs
+
+
+def batched_leg(dml, nrep, steps, warmup, full_steps=0):
+    t0 = time.perf_counter()
+    reps = build_replicas(nrep)
+    log("built %d replicas in %.1f s" % (nrep, time.perf_counter() - t0))
+    cfgs = c3_configs(dml, reps)
     from dml import autotune
     t0 = time.perf_counter()
     choice = autotune.tune(cfgs, [(r["pos"], r["species"],
@@ -206,8 +214,27 @@ def batched_leg(dml, nrep, steps, warmup):
     entries_now = sum(b["row_entries"] - a["row_entries"] for a, b in zip(c0, c1)) / 11.0
     mean_nn_w = entries_now / max(rows_now, 1.0)
     kf_ms = ens.time_kernel(dml.K_FORCE, 10)
+    full = None
+    if full_steps > warmup + steps:
+        # the configuration's own length: the same ensemble run on to
+        # full_steps steps in all (deposit fronts, chunk blocks and GCMC
+        # moves change the work as it goes), timed in segments
+        done_steps, f_ms = warmup + steps, 0.0
+        p0 = sum(ens.counters(k)["particle_steps"] for k in range(nrep))
+        while done_steps < full_steps:
+            seg = min(500, full_steps - done_steps)
+            f_ms += ens.run(seg, graph=True)
+            done_steps += seg
+            log("c3 full run: %d / %d steps, %.1f s" % (done_steps, full_steps, f_ms * 1e-3))
+        p1 = sum(ens.counters(k)["particle_steps"] for k in range(nrep))
+        full = dict(steps=full_steps, steps_timed=full_steps - warmup - steps, seconds=f_ms * 1e-3,
+                    rate=(p1 - p0) / (f_ms * 1e-3),
+                    mean_particles=(p1 - p0) / (full_steps - warmup - steps),
+                    halted=sum(1 for k in range(nrep) if ens.replica_status(k) != 0),
+                    overlap_unfinished=sum(ens.counters(k)["ov_unfinished"] for k in range(nrep)))
     ens.close()
     return dict(n=ntot, ms=ms, wall=wall, rate=psteps / (ms * 1e-3), mean_nn=mean_nn,
+                particles_stepped=psteps / steps, full_run=full,
                 rows_at_timing=rows_now, mean_nn_rows=mean_nn_w,
                 rebuilds_per_rep_step=reb / max(stepsum, 1), list_ms=kl_ms, force_ms=kf_ms,
                 halted=halted, gcmc_accepted=trials, overlap_unfinished=ov_left,
@@ -334,19 +361,53 @@ def c5_leg(dml, steps, warmup, trials=4096):
 
 
 def cpu_baseline():
-    """The CPU oracle (a bit-exact restatement of the reference) on the
-    reference's own Ermak test: 1,204 particles, 10^4 steps, piston."""
+    """The CPU oracle (a bit-exact C restatement of the reference's routines,
+    one host core) on C1: 1,000 particles (400 CG electrode atoms, 600 Li+)
+    in 50 x 50 x 100 A, 10^4 Ermak steps, no reservoir, skin 10; also on the reference's own ermak test (1,204 particles, 10^4
+    steps), on which the reference Fortran built at -O2 takes 10.3 s on one
+    core of the build container's host (1.17e6 particle-steps/s)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import orc
+    from dml import workloads as W
+    c1 = W.system_c1()
+    nst = 10000
+    out = orc.dana_run(-104012, 1.0, c1["h"], nst, nst, c1["box"][0], c1["box"][1], 50.0,
+                       c1["z0"], c1["zmax"], 250.0, 250.0, c1["skin"], 1, "none",
+                       init_pos=c1["pos"], init_z=c1["species"])
+    n = len(c1["pos"])
+    rate = n * nst / out["loop_seconds"]
     cfg = json.load(open(os.path.join(ROOT, "tests", "golden", "ref_runs.json")))["ermak"]
-    out = orc.dana_run(cfg["idum"], cfg["prob"], cfg["h"], cfg["nst"], cfg["nwr"], cfg["xi"],
-                       cfg["yi"], cfg["dist"], cfg["z0"], cfg["zmax"], cfg["dif_sc"],
-                       cfg["dif_sei"], cfg["nb_dcut"], cfg["ermak"], cfg["reservoir"])
-    n = out["nat"]
-    return dict(value=n * cfg["nst"] / out["loop_seconds"], unit="particle-steps/s", cores=1,
-                kind="port",
-                sample="reference tests/ermak workload: %d particles x %d Ermak steps, piston, "
-                       "skin 10, FP64 oracle on one host core, %.2f s" % (n, cfg["nst"], out["loop_seconds"]))
+    e = orc.dana_run(cfg["idum"], cfg["prob"], cfg["h"], cfg["nst"], cfg["nwr"], cfg["xi"],
+                     cfg["yi"], cfg["dist"], cfg["z0"], cfg["zmax"], cfg["dif_sc"],
+                     cfg["dif_sei"], cfg["nb_dcut"], cfg["ermak"], cfg["reservoir"])
+    return dict(value=rate, unit="particle-steps/s", cores=1, kind="port",
+                sample="C1: %d particles (400 CG) x %d Ermak steps, no reservoir, skin 10, the C "
+                       "restatement of the reference's routines on one host core, %.2f s; "
+                       "on the reference's ermak test (%d particles x %d steps) it does %.3g "
+                       "particle-steps/s, where the reference Fortran at -O2 does 1.17e6 on "
+                       "one core of the build host" % (n, nst, out["loop_seconds"], e["nat"],
+                                                       cfg["nst"], e["nat"] * cfg["nst"] /
+                                                       e["loop_seconds"]))
+
+
+def copy_ceiling(nbytes=1 << 30, reps=10):
+    """HBM bandwidth a plain device-to-device copy reaches (bytes read +
+    written per second, HIP events around `reps` copies of nbytes)"""
+    import torch
+    src = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del src, dst
+    torch.cuda.empty_cache()
+    return 2.0 * nbytes / (ms * 1e-3) / 1e9
 
 
 def calibrate_copy(nbytes=1 << 30):
@@ -399,6 +460,9 @@ def main():
                     help="Ermak steps of each FP32/FP64 energy-drift run on C2 (0 = skip)")
     ap.add_argument("--c5-steps", type=int, default=2000,
                     help="steps of the C5 leg (its configuration: 2e3)")
+    ap.add_argument("--c3-full-steps", type=int, default=5000,
+                    help="run the C3 ensemble on to this many steps (its configuration: 5e3; "
+                         "0 = only the timed steps)")
     ap.add_argument("--serve-seconds", type=float, default=30.0,
                     help="arrival window of the serving leg")
     ap.add_argument("--no-cpu", action="store_true")
@@ -433,7 +497,7 @@ def main():
     if "batched" in legs:
         log("batched leg: C3 %d replicas FP32 forces" % args.replicas)
         sync()
-        batched = batched_leg(dml, args.replicas, args.steps, args.warmup)
+        batched = batched_leg(dml, args.replicas, args.steps, args.warmup, args.c3_full_steps)
         sync()
         log("batched", json.dumps(batched))
     serving = None
@@ -501,6 +565,8 @@ def main():
                            "traffic": measured, "avg_launch_ms": batched["list_ms"],
                            "algorithmic_bytes_per_launch": traffic}
         res["kernel_ms"] = {"k_listbuild_wc": batched["list_ms"], "k_force": batched["force_ms"]}
+        res["particles_stepped_mean"] = batched["particles_stepped"]
+        res["c3_full_run"] = batched["full_run"]
         res["mean_list_length"] = batched["mean_nn"]
         res["rebuilds_per_step"] = batched["rebuilds_per_rep_step"]
         res["replicas_halted"] = batched["halted"]
@@ -508,8 +574,15 @@ def main():
         res["tuning"] = batched["tuning"]
         res["gcmc_moves_accepted"] = batched["gcmc_accepted"]
         res["overlap_calls_unfinished"] = batched["overlap_unfinished"]
+    try:
+        res["hbm_copy_ceiling_gbs"] = copy_ceiling()
+    except Exception as e:  # torch without a device: the line still prints
+        log("copy ceiling not measured: %s" % e)
+    fl = os.path.join(ROOT, "profiles", "flops_c3_256rep.json")
+    if batched and args.replicas == 256 and os.path.exists(fl):
+        res["flop_rates"] = json.load(open(fl))["rates"]
     if not args.no_cpu:
-        log("cpu baseline: oracle on the reference Ermak test")
+        log("cpu baseline: oracle on C1")
         res["cpu_baseline"] = cpu_baseline()
     if args.calibrate_copy:
         calibrate_copy()

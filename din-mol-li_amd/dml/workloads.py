@@ -6,8 +6,11 @@ place ions by random sequential addition on a hash grid (O(N)) with the same
 3.2 A hard-core rule, and add bcc(100) Li0 electrode layers (species CG).
 
 Decisions recorded where the configuration text is open:
-* C1: 2 bcc layers of 14 x 14 sites (a = 3.51) plus 8 adatoms -> 400 CG; the
-  piston reservoir with z0 = 50, zmax = 100.
+* C1: 2 bcc layers of 14 x 14 sites (a = 3.51) plus 8 adatoms -> 400 CG; no
+  reservoir (the configuration names none), z0 = zmax = 100 (z0 then only
+  feeds the deposit-above-z0 stop); with a piston band the deposit front
+  reaches any z0 up to 90 A before 10^4 steps and stops the run
+  (dana.F90:905-908).
 * C2: 4 bcc layers of 71 x 71 sites with 20% vacancies; N = 100,000 kept by
   filling ions at 0.01 A^-3 in a slab above the electrode (0.01 x the whole
   box would be 250k).
@@ -159,7 +162,7 @@ def system_c1(seed=104012):
     ions = rsa_fill(rng, 600, [0, 0, A_BCC + DMIN], [50, 50, 100.0], box, existing=el)
     pos = np.vstack([el, ions])
     z = np.array([2] * len(el) + [1] * len(ions), np.int8)
-    return dict(name="C1", box=box, pos=pos, species=z, skin=10.0, h=1e-2, z0=50.0, zmax=100.0)
+    return dict(name="C1", box=box, pos=pos, species=z, skin=10.0, h=1e-2, z0=100.0, zmax=100.0)
 
 
 def system_c2(seed=104012 + 2, n_total=100_000):

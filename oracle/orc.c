@@ -1449,7 +1449,7 @@ int orc_dana_run(const orc_run_cfg *cfg, orc_run_out *out) {
   p.ermak = cfg->ermak;
   p.piston = cfg->reservoir == 0;
   p.box[0] = cfg->xi; p.box[1] = cfg->yi; p.box[2] = cfg->zmax;
-  if (cfg->reservoir < 0 || cfg->reservoir > 2) return ORC_E_RESERVOIR;
+  if (cfg->reservoir < 0 || cfg->reservoir > 3) return ORC_E_RESERVOIR;
   int chunks = cfg->reservoir == 1, gcmc = cfg->reservoir == 2;
 
   orc_rng rng;

@@ -212,7 +212,7 @@ typedef struct orc_run_cfg {
   int32_t nst, nwr;
   double xi, yi, dist, z0, zmax, dif_sc, dif_sei, nb_dcut;
   int32_t ermak;      /* movedor.ini line 1 */
-  int32_t reservoir;  /* 0 piston, 1 chunks, 2 gcmc */
+  int32_t reservoir;  /* 0 piston, 1 chunks, 2 gcmc, 3 none (z0 = zmax) */
   double act;
   int32_t nadj;
   int32_t nchunk;

@@ -357,7 +357,7 @@ def dana_run(idum, prob, h, nst, nwr, xi, yi, dist, z0, zmax, dif_sc, dif_sei, n
     c.xi, c.yi, c.dist, c.z0, c.zmax = xi, yi, dist, z0, zmax
     c.dif_sc, c.dif_sei, c.nb_dcut = dif_sc, dif_sei, nb_dcut
     c.ermak = int(ermak)
-    c.reservoir = {"piston": 0, "chunks": 1, "gcmc": 2}[reservoir] if isinstance(reservoir, str) else reservoir
+    c.reservoir = {"piston": 0, "chunks": 1, "gcmc": 2, "none": 3}[reservoir] if isinstance(reservoir, str) else reservoir
     c.act, c.nadj = act, nadj
     c.overlap_rounds, c.rounds_seed = int(overlap_rounds), int(rounds_seed)
     keep = []
