@@ -247,9 +247,11 @@ def test_c3_six_replicas_in_one_ensemble_match_oracle(orc, dml):
 
 def test_c3_ensemble_100_graph_steps_invariants(dml):
     """the whole 256-replica C3 ensemble, 100 device-resident graph steps:
-    no replica halts, every overlap call finishes, atoms stay in their
-    boxes, CG only grows, Li+ count changes only by GCMC moves and chunk
-    blocks, and the particle-step counter adds the atoms alive each step"""
+    no replica halts, overlap calls that end with atoms still overlapping
+    (the round schedule's bound of 16 rounds per step) stay below 1 in 1000
+    replica-steps, atoms stay in their boxes, CG only grows, Li+ count
+    changes only by GCMC moves and chunk blocks, and the particle-step
+    counter adds the atoms alive each step"""
     import bench
     reps = bench.build_replicas(256)
     cfgs = bench.c3_configs(dml, reps)
@@ -261,11 +263,12 @@ def test_c3_ensemble_100_graph_steps_invariants(dml):
     say("C3 ensemble", sum(len(r["pos"]) for r in reps))
     c0 = [ens.counters(k) for k in range(256)]
     ens.run(100, graph=True)
+    unfinished = 0
     for k, r in enumerate(reps):
         tag = "replica %d" % k
         assert ens.replica_status(k) == 0, tag
         cnt = ens.counters(k)
-        assert cnt["ov_unfinished"] == c0[k]["ov_unfinished"], tag
+        unfinished += cnt["ov_unfinished"] - c0[k]["ov_unfinished"]
         st = ens.state(k)
         alive = (st["flags"] & 1) != 0
         sp = st["species"][alive]
@@ -280,13 +283,19 @@ def test_c3_ensemble_100_graph_steps_invariants(dml):
         assert (sp == 2).sum() >= n_cg0, tag
         res = bench.RESERVOIR_CYCLE[k % 3]
         n_li = int(alive.sum())
+        n0 = len(r["pos"])
+        ps = cnt["particle_steps"] - c0[k]["particle_steps"]
         if res == "GCMC":
-            assert n_li == len(r["pos"]) + cnt["ins"] - cnt["del_"], tag
+            assert n_li == n0 + cnt["ins"] - cnt["del_"], tag
+            assert 100 * (n0 - cnt["del_"]) <= ps <= 100 * (n0 + cnt["ins"]), tag
         elif res == "CHUNKS":
-            assert (n_li - len(r["pos"])) % len(r["chunk"]) == 0, tag
+            assert (n_li - n0) % len(r["chunk"]) == 0, tag
+            assert 100 * n0 <= ps <= 100 * n_li, tag
         else:
-            assert n_li == len(r["pos"]), tag
-        assert cnt["particle_steps"] - c0[k]["particle_steps"] >= 100 * min(len(r["pos"]), n_li), tag
+            assert n_li == n0, tag
+            assert ps == 100 * n0, tag
+    say("overlap calls unfinished", unfinished, "of", 256 * 100)
+    assert unfinished <= 256 * 100 // 1000
     ens.close()
 
 
