@@ -10,6 +10,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "dml_kernels.h"
@@ -23,7 +24,38 @@
     }                                                                 \
   } while (0)
 
+// Staging of dml_gcmc_trial_energy_noalloc: trial points (3 x double), du
+// (double), hard and exclude (int32) for at most DML_NOALLOC_MAX_TRIAL trials.
+// Every GPU holds its own copy until the process ends.  Host code never uses
+// the name `scratch` as a device address: dml_create_ensemble resolves the
+// copy of the current device once (scratch_setup) and the entry point passes
+// that address.
+constexpr size_t kScratchBytes = (size_t)DML_NOALLOC_MAX_TRIAL * (4 * sizeof(double) +
+                                                                  2 * sizeof(int32_t));
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch[kMaxDevices] = {};  // device address of `scratch` per GPU
+
+// with device dev current: the address of that GPU's copy of `scratch`
+int scratch_setup(int dev, unsigned char **out) {
+  if (dev < 0 || dev >= kMaxDevices) return DML_E_UNSUPPORTED;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      fprintf(stderr, "libdml: hipGetSymbolAddress(scratch) failed: %s\n", hipGetErrorString(e));
+      return DML_E_HIP;
+    }
+    g_scratch[dev] = (unsigned char *)ptr;
+  }
+  *out = g_scratch[dev];
+  return DML_OK;
+}
 
 constexpr int kMaxRounds = 32;
 
@@ -67,6 +99,8 @@ struct dml_ctx {
   int64_t cap_cm = 0, cap_tm = 0, cap_wm = 0;
   int64_t graph_modes = -1;  // mode signature the captured step was built for
   uint64_t graph_dsig = 0;   // device-state bytes the captured step was built with
+  int dev = 0;               // the GPU current at creation
+  unsigned char *scratch = nullptr;  // that GPU's copy of `scratch`
 };
 
 namespace {
@@ -704,6 +738,14 @@ int dml_create_ensemble(int32_t nrep, const dml_config *cfgs, dml_ctx **out) {
   }
   c->P = P;
   if (lpa) c->lpa = lpa;
+  if (hipGetDevice(&c->dev) != hipSuccess) {
+    delete c;
+    return DML_E_HIP;
+  }
+  if (int rc = scratch_setup(c->dev, &c->scratch)) {
+    delete c;
+    return rc;
+  }
   if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return DML_E_HIP;
@@ -1469,11 +1511,6 @@ int dml_gcmc_trial_energy(dml_ctx *c, int32_t rep, int32_t ntrial, const double 
     }
     if (exclude && (exclude[t] < (pos3 ? -1 : 0) || exclude[t] >= p.nslots)) return DML_E_ARG;
   }
-  std::vector<int32_t> ex;
-  if (exclude) {
-    ex.resize(ntrial);
-    for (int t = 0; t < ntrial; t++) ex[t] = exclude[t] < 0 ? -1 : p.poff + exclude[t];
-  }
   double *dp = nullptr, *de = nullptr;
   int32_t *dx = nullptr, *dh = nullptr;
   rc = DML_E_NOMEM;
@@ -1482,7 +1519,7 @@ int dml_gcmc_trial_energy(dml_ctx *c, int32_t rep, int32_t ntrial, const double 
     rc = DML_OK;
     if ((pos3 && hipMemcpyAsync(dp, pos3, sizeof(double) * 3 * (size_t)ntrial,
                                 hipMemcpyHostToDevice, c->st) != hipSuccess) ||
-        (exclude && hipMemcpyAsync(dx, ex.data(), sizeof(int32_t) * (size_t)ntrial,
+        (exclude && hipMemcpyAsync(dx, exclude, sizeof(int32_t) * (size_t)ntrial,
                                    hipMemcpyHostToDevice, c->st) != hipSuccess))
       rc = DML_E_HIP;
   }
@@ -1501,6 +1538,55 @@ int dml_gcmc_trial_energy(dml_ctx *c, int32_t rep, int32_t ntrial, const double 
   hipFree(dh);
   hipFree(dx);
   return rc;
+}
+
+int dml_gcmc_trial_energy_noalloc(dml_ctx *c, int32_t rep, int32_t ntrial, const double *pos3,
+                                  const int32_t *exclude, double *du, int32_t *hard) {
+  if (!c || rep < 0 || rep >= c->nrep || ntrial < 0 || (!pos3 && !exclude) || !du || !hard)
+    return DML_E_ARG;
+  if (ntrial > DML_NOALLOC_MAX_TRIAL) {
+    fprintf(stderr, "libdml: dml_gcmc_trial_energy_noalloc: %d trials, at most %d\n", ntrial,
+            DML_NOALLOC_MAX_TRIAL);
+    return DML_E_NOMEM;
+  }
+  if (ntrial == 0) return DML_OK;
+  // the cell regions must be in place: growing them allocates
+  if (c->cells_dirty) return DML_E_STATE;
+  RepParams p;
+  HIPCHECK(hipMemcpyAsync(&p, c->d.rp + rep, sizeof(RepParams), hipMemcpyDeviceToHost, c->st));
+  HIPCHECK(hipStreamSynchronize(c->st));
+  for (int t = 0; t < ntrial; t++) {
+    if (pos3) {
+      const double *q = pos3 + 3 * (int64_t)t;
+      if (!(q[0] >= 0 && q[0] < p.box[0] && q[1] >= 0 && q[1] < p.box[1] && q[2] >= 0 &&
+            q[2] <= p.box[2]))
+        return DML_E_ARG;
+    }
+    if (exclude && (exclude[t] < (pos3 ? -1 : 0) || exclude[t] >= p.nslots)) return DML_E_ARG;
+  }
+  // the array's layout: points | du | hard | exclude, each sized for the maximum
+  double *dp = (double *)c->scratch;
+  double *de = dp + 3 * (size_t)DML_NOALLOC_MAX_TRIAL;
+  int32_t *dh = (int32_t *)(de + DML_NOALLOC_MAX_TRIAL);
+  int32_t *dx = dh + DML_NOALLOC_MAX_TRIAL;
+  if (pos3)
+    HIPCHECK(hipMemcpyAsync(dp, pos3, sizeof(double) * 3 * (size_t)ntrial, hipMemcpyHostToDevice,
+                            c->st));
+  if (exclude)
+    HIPCHECK(hipMemcpyAsync(dx, exclude, sizeof(int32_t) * (size_t)ntrial,
+                            hipMemcpyHostToDevice, c->st));
+  // every trial's du and hard are written by the kernel: nothing to clear
+  const int64_t threads = (int64_t)ntrial * 64;
+  hipLaunchKernelGGL(k_gcmc_energy, dim3((unsigned)((threads + DML_BLOCK - 1) / DML_BLOCK)),
+                     dim3(DML_BLOCK), 0, c->st, c->d, rep, ntrial,
+                     (const double *)(pos3 ? dp : nullptr),
+                     (const int32_t *)(exclude ? dx : nullptr), de, dh);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipMemcpyAsync(du, de, sizeof(double) * (size_t)ntrial, hipMemcpyDeviceToHost, c->st));
+  HIPCHECK(hipMemcpyAsync(hard, dh, sizeof(int32_t) * (size_t)ntrial, hipMemcpyDeviceToHost,
+                          c->st));
+  HIPCHECK(hipStreamSynchronize(c->st));
+  return DML_OK;
 }
 
 int dml_run(dml_ctx *c, int32_t nsteps, int32_t use_graph) {

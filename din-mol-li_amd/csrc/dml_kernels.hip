@@ -2185,7 +2185,8 @@ __device__ inline int near_cells(const RepParams &P, int c0, int c1, int c2, int
 // same sum without the atom itself): one wave per trial over the cells
 // around the point.  The cells are those of the last sort; every atom has
 // moved less than the skin since, so one cell step (>= rcut + skin) still
-// covers the pair cutoff.
+// covers the pair cutoff.  excl holds slots of replica r (-1 = none); a
+// halted replica gets du = 0, hard = -1 for every trial.
 __global__ void __launch_bounds__(DML_BLOCK) k_gcmc_energy(DevState s, int r, int ntrial,
                                                             const double *pos3,
                                                             const int32_t *excl, double *du,
@@ -2193,9 +2194,9 @@ __global__ void __launch_bounds__(DML_BLOCK) k_gcmc_energy(DevState s, int r, in
   int w = (blockIdx.x * DML_BLOCK + threadIdx.x) / WAVE, lane = threadIdx.x % WAVE;
   if (w >= ntrial) return;
   const RepParams &P = s.rp[r];
-  if (P.halted) return;
-  const int skip = excl ? excl[w] : -1;  // the atom itself, for a removal
-  if (!pos3 && (skip < 0 || !(s.flags[skip] & DML_FL_ALIVE))) {  // no such atom now
+  const int ex = excl ? excl[w] : -1;
+  const int skip = ex < 0 ? -1 : P.poff + ex;  // the atom itself, for a removal
+  if (P.halted || (!pos3 && (skip < 0 || !(s.flags[skip] & DML_FL_ALIVE)))) {  // no such atom now
     if (lane == 0) {
       du[w] = 0.0;
       hard[w] = -1;

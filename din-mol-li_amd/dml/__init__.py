@@ -31,13 +31,14 @@ EXPORTS = ["dml_create", "dml_create_ensemble", "dml_destroy", "dml_sync", "dml_
            "dml_test_update", "dml_update", "dml_cells", "dml_neighbors", "dml_forces",
            "dml_brownian", "dml_ermak_a", "dml_ermak_b", "dml_ref_offset",
            "dml_overlap_moveback", "dml_overlap_rounds", "dml_f_to_cg", "dml_gcmc",
-           "dml_gcmc_trial_energy", "dml_run", "dml_get_counters", "dml_list_stats",
+           "dml_gcmc_trial_energy", "dml_gcmc_trial_energy_noalloc", "dml_run", "dml_get_counters", "dml_list_stats",
            "dml_last_run_ms", "dml_time_kernel", "dml_set_tuning",
            "dml_replica_status", "dml_get_gcmc_order", "dml_get_slots",
            "dml_calc_rho", "dml_observables", "dml_maxz", "dml_end_step",
            "dml_get_tuning", "dml_reconfigure", "dml_set_step_limit", "dml_set_active",
            "dml_progress", "dml_set_chunk", "dml_bloques"]
 K_LISTBUILD, K_FORCE, K_CELLS, K_DISPMAX = 0, 1, 2, 3
+NOALLOC_MAX_TRIAL = 8192  # DML_NOALLOC_MAX_TRIAL
 
 
 class DmlError(RuntimeError):
@@ -102,6 +103,7 @@ def lib():
         L.dml_f_to_cg.argtypes = [vp]
         L.dml_gcmc.argtypes = [vp, C.c_int32, C.c_double, C.c_int32, vp, vp]
         L.dml_gcmc_trial_energy.argtypes = [vp, C.c_int32, C.c_int32, vp, vp, vp, vp]
+        L.dml_gcmc_trial_energy_noalloc.argtypes = [vp, C.c_int32, C.c_int32, vp, vp, vp, vp]
         L.dml_run.argtypes = [vp, C.c_int32, C.c_int32]
         L.dml_get_counters.argtypes = [vp, C.c_int32, C.POINTER(Counters)]
         L.dml_list_stats.argtypes = [vp, C.c_int32, vp, vp, vp]
@@ -376,15 +378,17 @@ class Context:
     def end_step(self):
         _chk(self.L.dml_end_step(self.h), "dml_end_step")
 
-    def gcmc_trial_energy(self, pos3, rep=0, exclude=None):
+    def gcmc_trial_energy(self, pos3, rep=0, exclude=None, noalloc=False):
         """trial energies at pos3 (t, 3); exclude (t,) slots left out (-1 none);
-        pos3 None: removals of the exclude atoms at their current positions"""
+        pos3 None: removals of the exclude atoms at their current positions.
+        noalloc: dml_gcmc_trial_energy_noalloc (at most DML_NOALLOC_MAX_TRIAL
+        trials, staged in the library's fixed device array)"""
         p = None if pos3 is None else np.ascontiguousarray(pos3, np.float64)
         t = len(p) if p is not None else len(exclude)
         ex = None if exclude is None else np.ascontiguousarray(exclude, np.int32)
         du = np.zeros(max(t, 1)); hard = np.zeros(max(t, 1), np.int32)
-        _chk(self.L.dml_gcmc_trial_energy(self.h, rep, t, _p(p), _p(ex), _p(du), _p(hard)),
-             "dml_gcmc_trial_energy")
+        fn = "dml_gcmc_trial_energy_noalloc" if noalloc else "dml_gcmc_trial_energy"
+        _chk(getattr(self.L, fn)(self.h, rep, t, _p(p), _p(ex), _p(du), _p(hard)), fn)
         return du[:t], hard[:t]
 
     def run(self, nsteps, graph=True):

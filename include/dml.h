@@ -224,6 +224,21 @@ int dml_gcmc_trial_energy(dml_ctx *ctx, int32_t rep, int32_t ntrial,
                           const double *pos3, const int32_t *exclude, double *du,
                           int32_t *hard);
 
+/* Largest ntrial of dml_gcmc_trial_energy_noalloc. */
+#define DML_NOALLOC_MAX_TRIAL 8192
+/* dml_gcmc_trial_energy without any device allocation or free: has no
+ * counterpart in the reference.  It is for callers that cannot allocate
+ * (code under stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, kernel and results; inputs and outputs are
+ * staged in one fixed 320 KiB device array per GPU that the library holds
+ * until the process ends.  ntrial > DML_NOALLOC_MAX_TRIAL: DML_E_NOMEM;
+ * cell memory still to be grown after dml_set_chunk: DML_E_STATE (any
+ * dml_run or dml_gcmc_trial_energy call grows it).  All contexts on one GPU
+ * share that array: calls that overlap in time on one GPU race. */
+int dml_gcmc_trial_energy_noalloc(dml_ctx *ctx, int32_t rep, int32_t ntrial,
+                                  const double *pos3, const int32_t *exclude,
+                                  double *du, int32_t *hard);
+
 /* --- whole steps -------------------------------------------------------
  * nsteps device-resident steps of every replica (the order of the main loop,
  * dana.F90:173-263) with Philox noise; use_graph != 0 captures one step in a

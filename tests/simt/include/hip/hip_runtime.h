@@ -31,6 +31,8 @@
 #define __shared__ static
 #define __launch_bounds__(...)
 #define __forceinline__ inline
+#define __align__(n) alignas(n)
+#define HIP_SYMBOL(x) ((const void *)&(x))
 
 struct dim3 {
   unsigned x, y, z;
@@ -209,6 +211,10 @@ hipError_t hipEventRecord(hipEvent_t e, hipStream_t st = nullptr);
 hipError_t hipEventSynchronize(hipEvent_t e);
 hipError_t hipEventElapsedTime(float *ms, hipEvent_t a, hipEvent_t b);
 const char *hipGetErrorString(hipError_t e);
+hipError_t hipGetLastError();
+hipError_t hipGetDevice(int *dev);
+// a __device__ variable is a host object here: its address is the symbol
+hipError_t hipGetSymbolAddress(void **p, const void *sym);
 
 namespace sim {
 template <class... P, class... A>
