@@ -307,10 +307,11 @@ def serving_leg(dml, duration, rate=20.0, step_scale=0.01, seed=104012 + 4):
 def c5_leg(dml, steps, warmup, trials=4096):
     """C5: the 2M-particle dendrite-regime system (12-layer electrode with 30
     hemispherical protrusions, ions at 0.01 A^-3), Ermak with FP32 pair
-    forces, the GCMC reservoir in the top quarter; every step also evaluates
-    4,096 insertion and 4,096 removal trial energies in one batched kernel
-    (FP64 sums).  Wall-clock timed (the trial batches go through the host)."""
-    import numpy as np
+    forces, the GCMC reservoir in the top quarter with 4,096 insertion and
+    4,096 deletion trials every step (alternating kinds): evaluated in the
+    batch kernels (points, hard-core neighbours, FP64 trial energies) and
+    committed in trial order, with the sequential gcmc_run's result, inside
+    the captured step."""
     from dml import workloads as W
     t0 = time.perf_counter()
     sysd = W.system_c5()
@@ -318,9 +319,11 @@ def c5_leg(dml, steps, warmup, trials=4096):
     box = sysd["box"]
     z0 = 0.75 * box[2]
     n = len(sysd["pos"])
+    nadj = 2 * trials
     cfg = dml.dana_config(box, sysd["skin"], n + 65536, h=sysd["h"], z0=z0, zmax=box[2],
                           integrator=dml.ERMAK, reservoir=dml.GCMC, precision=dml.FP32_FORCE,
-                          overlap_mode=dml.OVERLAP_ROUNDS, act=0.01, nadj=10, seed=104017)
+                          overlap_mode=dml.OVERLAP_ROUNDS, act=0.01, nadj=nadj, seed=104017,
+                          gcmc_mode=dml.GCMC_BATCHED | dml.GCMC_ALTERNATE)
     from dml import autotune
     t0 = time.perf_counter()
     choice = autotune.tune([cfg], [(sysd["pos"], sysd["species"])], log=log)
@@ -328,36 +331,36 @@ def c5_leg(dml, steps, warmup, trials=4096):
     c = dml.Context(cfg)
     autotune.apply(c, choice)
     c.set_state(sysd["pos"], species=sysd["species"])
-    rng = np.random.default_rng(104017)
-
-    def trial_batch(gc):
-        ins = np.column_stack([rng.random(trials) * box[0], rng.random(trials) * box[1],
-                               z0 + rng.random(trials) * (box[2] - z0)])
-        rem = rng.choice(gc, trials).astype(np.int32)
-        a = c.gcmc_trial_energy(ins)
-        b = c.gcmc_trial_energy(None, exclude=rem)
-        return a, b
-
     c.run(warmup, graph=True)
-    gc = c.gcmc_order()
     c.sync()
+    cnt0 = c.counters()
     t0 = time.perf_counter()
-    hard_ins = 0
-    for k in range(steps):
-        c.run(1, graph=True)
-        if k % 100 == 99:
-            gc = c.gcmc_order()
-        (du, hard), (dr, hr) = trial_batch(gc)
-        hard_ins += int((hard == 1).sum())
+    done = 0
+    while done < steps:
+        seg = min(500, steps - done)
+        c.run(seg, graph=True)
+        done += seg
+        log("c5: %d / %d steps, %.1f s" % (done, steps, time.perf_counter() - t0))
     c.sync()
     wall = time.perf_counter() - t0
     cnt = c.counters()
+    out = c.gcmc_outcomes(nadj)
     st = c.replica_status(0)
-    ps = cnt["particle_steps"]
+    ps = cnt["particle_steps"] - cnt0["particle_steps"]
+    moves = (cnt["ins"] - cnt0["ins"]) + (cnt["del_"] - cnt0["del_"])
+    last = None
+    if out is not None:
+        ins = (out["kind"] == 0)
+        acc = out["accepted"] == 1
+        last = dict(insertions_accepted=int((ins & acc).sum()),
+                    deletions_accepted=int((~ins & acc).sum()),
+                    mean_du_accepted_insertion=float(out["du"][ins & acc].mean()) if (ins & acc).any() else None,
+                    hard_core_or_rejected_insertions=int((ins & ~acc).sum()))
     c.close()
     return dict(n=n, steps=steps, wall=wall, rate=ps / wall if wall > 0 else None,
-                trials_per_s=2 * trials * steps / wall, hard_core_fraction=hard_ins / (trials * steps),
-                status=st, gcmc_accepted=cnt["ins"] + cnt["del_"])
+                trials_per_step=nadj, trials_per_s=nadj * steps / wall,
+                moves_committed_per_s=moves / wall, batched=out is not None, last_call=last,
+                status=st)
 
 
 def cpu_baseline():

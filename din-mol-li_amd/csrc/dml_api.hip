@@ -99,6 +99,7 @@ struct dml_ctx {
   int64_t cap_cm = 0, cap_tm = 0, cap_wm = 0;
   int64_t graph_modes = -1;  // mode signature the captured step was built for
   uint64_t graph_dsig = 0;   // device-state bytes the captured step was built with
+  int64_t gb_total = 0;      // batched-GCMC trial pool (sum of the replicas' capacities)
   int dev = 0;               // the GPU current at creation
   unsigned char *scratch = nullptr;  // that GPU's copy of `scratch`
 };
@@ -168,6 +169,7 @@ int setup_rep(RepParams &p, const dml_config &g) {
   p.precision = g.precision;
   p.overlap_mode = g.overlap_mode;
   p.seed = g.seed;
+  p.gcmc_mode = g.gcmc_mode;
   p.use_cells = 0;
   for (int k = 0; k < 3; k++)
     if (p.box[k] > DML_MAX_BOX) return DML_E_UNSUPPORTED;
@@ -342,6 +344,11 @@ int alloc_cells(dml_ctx *c) {
   hipFree(c->d.by);
   hipFree(c->d.bz);
   hipFree(c->d.bf4);
+  hipFree(c->d.gb_chead);
+  hipFree(c->d.gb_ccnt);
+  hipFree(c->d.gb_claim);
+  c->d.gb_chead = c->d.gb_ccnt = nullptr;
+  c->d.gb_claim = nullptr;
   c->d.cell_cnt = nullptr;
   c->d.bucket = nullptr;
   c->d.bx = c->d.by = c->d.bz = nullptr;
@@ -353,8 +360,16 @@ int alloc_cells(dml_ctx *c) {
   e |= dalloc(&c->d.bx, B);
   e |= dalloc(&c->d.by, B);
   e |= dalloc(&c->d.bz, B);
+  // batched GCMC, per cell: stamped list heads and counts, rebuild claims
+  const int64_t CS = c->gb_total ? C : 1;
+  e |= dalloc(&c->d.gb_chead, CS);
+  e |= dalloc(&c->d.gb_ccnt, CS);
+  e |= dalloc(&c->d.gb_claim, CS);
   if (e) return DML_E_NOMEM;
   HIPCHECK(hipMemsetAsync(c->d.cell_cnt, 0, sizeof(int32_t) * C, c->st));
+  HIPCHECK(hipMemsetAsync(c->d.gb_chead, 0, sizeof(unsigned long long) * CS, c->st));
+  HIPCHECK(hipMemsetAsync(c->d.gb_ccnt, 0, sizeof(unsigned long long) * CS, c->st));
+  HIPCHECK(hipMemsetAsync(c->d.gb_claim, 0, sizeof(int32_t) * CS, c->st));
   c->C = C;
   c->B = B;
   return DML_OK;
@@ -500,11 +515,46 @@ void enq_overlap(dml_ctx *c) {
 // nadj given) or of every GCMC replica (their own act, nadj): the grid-wide
 // list staging, the sequential trials (a workgroup per replica), the
 // grid-wide compaction of the gcmc and ref lists
+// the batch kernels of one replica (their grids sized by its pool)
+static void enq_gcmc_batch(dml_ctx *c, int r, const double *u, const double *g) {
+  const RepParams &p = c->reps[r].p;
+  const unsigned nbp = (unsigned)(p.pcap / DML_BLOCK);
+  const unsigned cap = (unsigned)p.gb_cap;
+  const unsigned wb = (cap * 64 + DML_BLOCK - 1) / DML_BLOCK;  // a wave per trial
+  const unsigned tb = (cap + DML_BLOCK - 1) / DML_BLOCK;         // a thread per trial
+  hipStream_t st = c->st;
+  const DevState &d = c->d;
+  hipLaunchKernelGGL(k_gb_count, dim3(nbp), dim3(DML_BLOCK), 0, st, d, r);
+  hipLaunchKernelGGL(k_gb_scan, dim3(1), dim3(1024), 0, st, d, r);
+  hipLaunchKernelGGL(k_gb_scatter, dim3(nbp), dim3(DML_BLOCK), 0, st, d, r);
+  hipLaunchKernelGGL(k_gb_trials, dim3(wb), dim3(DML_BLOCK), 0, st, d, r, u, g);
+  hipLaunchKernelGGL(k_gb_conflicts, dim3(wb), dim3(DML_BLOCK), 0, st, d, r);
+  hipLaunchKernelGGL(k_gb_decide, dim3(1), dim3(64), 0, st, d, r);
+  hipLaunchKernelGGL(k_gb_lists, dim3(tb), dim3(DML_BLOCK), 0, st, d, r);
+  hipLaunchKernelGGL(k_gb_check, dim3(wb), dim3(DML_BLOCK), 0, st, d, r);
+  hipLaunchKernelGGL(k_gb_dels, dim3(wb), dim3(DML_BLOCK), 0, st, d, r);
+  hipLaunchKernelGGL(k_gb_brank, dim3(2 * tb), dim3(DML_BLOCK), 0, st, d, r);
+  hipLaunchKernelGGL(k_gb_commit, dim3(1), dim3(64), 0, st, d, r, g);
+  hipLaunchKernelGGL(k_gb_fields, dim3(tb), dim3(DML_BLOCK), 0, st, d, r);
+  hipLaunchKernelGGL(k_gb_ownrow, dim3(wb), dim3(DML_BLOCK), 0, st, d, r);
+  hipLaunchKernelGGL(k_gb_append, dim3(wb), dim3(DML_BLOCK), 0, st, d, r);
+  hipLaunchKernelGGL(k_gb_append_fix, dim3(256), dim3(DML_BLOCK), 0, st, d, r);
+  hipLaunchKernelGGL(k_gb_cells, dim3(2 * cap), dim3(64), 0, st, d, r);
+}
+
 static void enq_gcmc(dml_ctx *c, int rep_only, const double *u, const double *g, double act,
                      int nadj) {
   const int nb = c->d.nblocks;
   for (int stage = 0; stage < 2; stage++) {
     if (stage == 1) {
+      hipLaunchKernelGGL(k_gb_begin, dim3((c->nrep + 63) / 64), dim3(64), 0, c->st, c->d,
+                         rep_only, act, nadj);
+      for (int r = 0; r < c->nrep; r++) {
+        const RepParams &p = c->reps[r].p;
+        if ((rep_only < 0 || r == rep_only) && p.gb_cap > 0 && (p.gcmc_mode & DML_GCMC_BATCHED) &&
+            p.reservoir == DML_GCMC)
+          enq_gcmc_batch(c, r, u, g);
+      }
       if (rep_only >= 0)
         hipLaunchKernelGGL(k_gcmc_run, dim3(1), dim3(DML_GCMC_BLOCK), 0, c->st, c->d, rep_only,
                            u, g, act, nadj);
@@ -729,6 +779,11 @@ int dml_create_ensemble(int32_t nrep, const dml_config *cfgs, dml_ctx **out) {
     }
     p.poff = (int32_t)P;
     P += p.pcap;
+    if ((g.gcmc_mode & DML_GCMC_BATCHED) && g.reservoir == DML_GCMC) {
+      p.gb_off = (int32_t)c->gb_total;
+      p.gb_cap = DML_GCMC_BATCH_MAX;
+      c->gb_total += DML_GCMC_BATCH_MAX;
+    }
     maxnn = std::max(maxnn, rows_for(g));
     if (g.lanes_per_atom > 0) lpa = g.lanes_per_atom;
   }
@@ -793,6 +848,32 @@ int dml_create_ensemble(int32_t nrep, const dml_config *cfgs, dml_ctx **out) {
   e |= dalloc(&d.gc_order, P);
   e |= dalloc(&d.b2slot, P);
   e |= dalloc(&d.seq, P);
+  {
+    // batched GCMC: per trial of the pool, per slot only with a batched replica
+    const int64_t T = std::max<int64_t>(c->gb_total, 1), PS = c->gb_total ? P : 1;
+    e |= dalloc(&d.gb_pt, 3 * T); e |= dalloc(&d.gb_u, 2 * T); e |= dalloc(&d.gb_du, T);
+    e |= dalloc(&d.gb_g, 3 * T);
+    e |= dalloc(&d.gb_kind, T); e |= dalloc(&d.gb_cell, T); e |= dalloc(&d.gb_nblk, T);
+    e |= dalloc(&d.gb_blk, 16 * T); e |= dalloc(&d.gb_ncf, T); e |= dalloc(&d.gb_cf, 16 * T);
+    e |= dalloc(&d.gb_res, T); e |= dalloc(&d.gb_slot, T);
+    e |= dalloc(&d.gi_trial, T); e |= dalloc(&d.gi_next, T); e |= dalloc(&d.gi_del, T);
+    e |= dalloc(&d.gi_tmpl, T);
+    e |= dalloc(&d.gd_trial, T); e |= dalloc(&d.gd_vpos, T); e |= dalloc(&d.gd_slot, T);
+    e |= dalloc(&d.gd_bso, T); e |= dalloc(&d.gd_brank, T); e |= dalloc(&d.gd_rpos, T);
+    e |= dalloc(&d.gd_oflags, T); e |= dalloc(&d.gd_cell, T);
+    e |= dalloc(&d.gb_free, T); e |= dalloc(&d.gb_freeb, T); e |= dalloc(&d.gb_frank, T);
+    e |= dalloc(&d.gb_U, 3 * T);
+    e |= dalloc(&d.gb_part, 2 * (P / DML_BLOCK));
+    e |= dalloc(&d.gb_vpos, PS); e |= dalloc(&d.gb_delt, PS); e |= dalloc(&d.gb_alist, PS);
+    e |= dalloc(&d.gb_app, PS);
+    if (!e) {
+      // the stamped counters start empty; the records are validated on use
+      hipMemsetAsync(d.gb_app, 0, sizeof(unsigned long long) * PS, c->st);
+      hipMemsetAsync(d.gb_delt, 0xff, sizeof(int32_t) * PS, c->st);
+      hipMemsetAsync(d.gb_vpos, 0xff, sizeof(int32_t) * PS, c->st);
+      hipMemsetAsync(d.gb_slot, 0xff, sizeof(int32_t) * T, c->st);
+    }
+  }
   if (e) {
     dml_destroy(c);
     return DML_E_NOMEM;
@@ -857,7 +938,13 @@ int dml_destroy(dml_ctx *c) {
                   d.ov_st, d.ov_near, d.nearl, d.part_d, d.part_i, d.blk2rep, d.rp, d.cnt, d.err, d.wl[0],
                   d.wl[1], d.wl_cnt, d.ovw[0], d.ovw[1], d.ovr[0], d.ovr[1], d.ov_more, d.pf4, d.bf4, d.gc_order,
                   d.b2slot, d.seq, d.chx, d.chy, d.chz, d.chox, d.choy, d.choz,
-                  d.cm_rep, d.cm_cell, d.tm_rep, d.tm_cell, d.wm_blk, d.obs};
+                  d.cm_rep, d.cm_cell, d.tm_rep, d.tm_cell, d.wm_blk, d.obs,
+                  d.gb_pt, d.gb_u, d.gb_du, d.gb_g, d.gb_kind, d.gb_cell, d.gb_nblk, d.gb_blk,
+                  d.gb_ncf, d.gb_cf, d.gb_res, d.gb_slot, d.gi_trial, d.gi_next, d.gi_del,
+                  d.gi_tmpl, d.gd_trial, d.gd_vpos, d.gd_slot, d.gd_bso, d.gd_brank,
+                  d.gd_rpos, d.gd_oflags, d.gd_cell, d.gb_free, d.gb_freeb, d.gb_frank,
+                  d.gb_U, d.gb_part, d.gb_vpos, d.gb_delt, d.gb_alist, d.gb_app,
+                  d.gb_chead, d.gb_ccnt, d.gb_claim};
   for (void *q : ptrs)
     if (q) hipFree(q);
   if (c->e0) hipEventDestroy(c->e0);
@@ -1477,6 +1564,33 @@ int dml_get_gcmc_order(dml_ctx *c, int32_t rep, int32_t *ngc, int32_t *order) {
   return DML_OK;
 }
 
+int dml_gcmc_outcomes(dml_ctx *c, int32_t rep, int32_t ntrial, int32_t *kind, int32_t *accepted,
+                      int32_t *slot, double *du, int32_t *batched) {
+  if (!c || rep < 0 || rep >= c->nrep || ntrial < 0 || !batched) return DML_E_ARG;
+  int rc = fetch_params(c);
+  if (rc) return rc;
+  const RepParams &p = c->reps[rep].p;
+  *batched = p.gb_state == GB_ON && p.gb_cap > 0;
+  if (!*batched) return DML_OK;
+  if (ntrial > p.gb_t) return DML_E_ARG;
+  if (ntrial == 0) return DML_OK;
+  std::vector<int32_t> k(ntrial), rs(ntrial), sl(ntrial);
+  HIPCHECK(hipMemcpy(k.data(), c->d.gb_kind + p.gb_off, sizeof(int32_t) * ntrial,
+                     hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(rs.data(), c->d.gb_res + p.gb_off, sizeof(int32_t) * ntrial,
+                     hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(sl.data(), c->d.gb_slot + p.gb_off, sizeof(int32_t) * ntrial,
+                     hipMemcpyDeviceToHost));
+  if (du)
+    HIPCHECK(hipMemcpy(du, c->d.gb_du + p.gb_off, sizeof(double) * ntrial, hipMemcpyDeviceToHost));
+  for (int t = 0; t < ntrial; t++) {
+    if (kind) kind[t] = k[t];
+    if (accepted) accepted[t] = rs[t] >= 0;
+    if (slot) slot[t] = rs[t] >= 0 ? sl[t] - p.poff : -1;
+  }
+  return DML_OK;
+}
+
 int dml_get_slots(dml_ctx *c, int32_t rep, int32_t *nslots, int32_t *bkey, uint32_t *pid,
                   int32_t *seq) {
   if (!c || rep < 0 || rep >= c->nrep || !nslots) return DML_E_ARG;
@@ -1782,7 +1896,9 @@ int dml_reconfigure(dml_ctx *c, int32_t rep, const dml_config *cfg) {
   memset(&p, 0, sizeof(p));
   p.pcap = h.p.pcap;
   if (setup_rep(p, g)) return DML_E_UNSUPPORTED;
-  // keep the slot range and the reserved cell regions
+  // keep the slot range, the reserved cell regions and the trial pool
+  p.gb_off = h.p.gb_off;
+  p.gb_cap = h.p.gb_cap;
   p.poff = h.p.poff;
   p.coff = h.p.coff;
   p.boff = h.p.boff;

@@ -39,6 +39,10 @@
 #define DML_MAX_BOX 4096.0
 
 // RepParams::halted bits
+// batched GCMC state of a call (RepParams::gb_state)
+#define GB_OFF 0       // sequential trials (k_gcmc_run)
+#define GB_ON 1        // the batch kernels commit this call
+#define GB_FALLBACK 2  // a batched replica's call the batch cannot take: k_gcmc_run
 #define HALT_FATAL 1   // a fatal condition of the reference (err_bits says which)
 #define HALT_PAUSED 2  // not running: free serving slot, or step_limit reached
 #define HALT_MASK 4    // held out of one phase (dml_run's first rebuild)
@@ -107,6 +111,16 @@ struct RepParams {
   int32_t chunk_fired;     // bloques added a block in this step
   int64_t creg, breg;      // reserved cell counts / bucket entries (grow room)
   int32_t lb_kind;         // list kernel of this replica: 0 cell tiles, 1 wave per atom
+  // batched GCMC (DML_GCMC_BATCHED): trial pool range, this call's state
+  // (GB_OFF / GB_ON / GB_FALLBACK) and the group sizes at its start
+  int32_t gcmc_mode;
+  int32_t gb_off, gb_cap;
+  int32_t gb_state, gb_t;
+  int32_t gb_ilen, gb_glen, gb_rlen, gb_nslots, gb_bamax;
+  int32_t gb_nF, gb_nFB;   // free slots below nslots / free hs%b indices below b_amax
+  int32_t gb_nins, gb_ndel, gb_nalist;
+  uint32_t gb_stamp;       // gcmc_calls + 1 of this call (stamped per-cell / per-slot marks)
+  double gb_act;
 };
 
 // cgroup_tessellate (Cells.F90:180-265) on the replica's box: the fast path
@@ -307,6 +321,29 @@ struct DevState {
   RepParams *rp;
   DevCounters *cnt;
   int32_t *err;
+  // batched GCMC.  Per trial (pool of every batched replica, gb_off + t):
+  // point, (u1, u2), energy, kind (0 insertion, 1 deletion, 2 point outside
+  // the cells), cell, hard-core neighbours (in-volume list positions, -1 in
+  // nblk: a neighbour that cannot leave), earlier insertion points within
+  // rcut, decision (insertion rank / in-volume list position, -1 rejected)
+  // and the slot inserted or deleted.
+  double *gb_pt, *gb_u, *gb_du;
+  double *gb_g;           // an insertion's three velocity normals (3 per trial)
+  int32_t *gb_kind, *gb_cell, *gb_nblk, *gb_blk, *gb_ncf, *gb_cf, *gb_res, *gb_slot;
+  // per accepted insertion (rank k): trial, next in its cell's list, trial
+  // of its deletion (INT_MAX: none), template
+  int32_t *gi_trial, *gi_next, *gi_del, *gi_tmpl;
+  // per accepted deletion (index d): trial, in-volume position, slot, and
+  // for atoms present at the call's start: hs%b index and its rank, ref
+  // list position, flags, cell
+  int32_t *gd_trial, *gd_vpos, *gd_slot, *gd_bso, *gd_brank, *gd_rpos, *gd_oflags, *gd_cell;
+  int32_t *gb_free, *gb_freeb, *gb_frank;  // first free slots / hs%b indices, ranks of the latter
+  int32_t *gb_U;          // 3 per trial: the free hs%b indices in ascending order
+  int32_t *gb_part;       // 2 per particle block
+  int32_t *gb_vpos, *gb_delt, *gb_alist;  // per slot
+  unsigned long long *gb_app;             // per slot: stamped counts
+  unsigned long long *gb_chead, *gb_ccnt; // per cell: stamped list heads and counts
+  int32_t *gb_claim;                      // per cell: stamp of the call that rebuilt it
   int32_t nblocks;        // particle blocks
   int32_t nrep;
 };

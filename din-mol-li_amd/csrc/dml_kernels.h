@@ -50,6 +50,23 @@ __global__ void k_gc_scatter(DevState s, int rep_only, int stage);
 __global__ void k_gc_copy(DevState s, int rep_only);
 __global__ void k_gcmc_run(DevState s, int rep_only, const double *u, const double *g,
                            double act_in, int nadj_in);
+__global__ void k_gb_begin(DevState s, int rep_only, double act_in, int nadj_in);
+__global__ void k_gb_count(DevState s, int r);
+__global__ void k_gb_scan(DevState s, int r);
+__global__ void k_gb_scatter(DevState s, int r);
+__global__ void k_gb_trials(DevState s, int r, const double *u, const double *g);
+__global__ void k_gb_conflicts(DevState s, int r);
+__global__ void k_gb_decide(DevState s, int r);
+__global__ void k_gb_lists(DevState s, int r);
+__global__ void k_gb_check(DevState s, int r);
+__global__ void k_gb_dels(DevState s, int r);
+__global__ void k_gb_brank(DevState s, int r);
+__global__ void k_gb_commit(DevState s, int r, const double *g);
+__global__ void k_gb_fields(DevState s, int r);
+__global__ void k_gb_ownrow(DevState s, int r);
+__global__ void k_gb_append(DevState s, int r);
+__global__ void k_gb_append_fix(DevState s, int r);
+__global__ void k_gb_cells(DevState s, int r);
 __global__ void k_gcmc_energy(DevState s, int r, int ntrial, const double *pos3,
                               const int32_t *excl, double *du,
                               int32_t *hard);

@@ -2497,6 +2497,7 @@ __global__ void __launch_bounds__(GC_BLOCK) k_gcmc_run(DevState s, int rep_only,
   RepParams *Pw = s.rp + r;
   const RepParams &P = *Pw;
   if (P.halted || (rep_only < 0 && P.reservoir != DML_GCMC) || P.gc_pending != 1) return;
+  if (P.gb_state == GB_ON) return;  // this call's trials were committed in a batch
   const int tid = threadIdx.x, lane = tid % WAVE, wv = tid / WAVE;
   const int poff = P.poff;
   const double act = rep_only >= 0 ? act_in : P.act;
@@ -2536,7 +2537,9 @@ __global__ void __launch_bounds__(GC_BLOCK) k_gcmc_run(DevState s, int rep_only,
     if (head > tail) { fail = ERR_NOPART; break; }
     const int tmpl = gc[head];
     const double beta = sqrt(kB_ui * 300.0 / P.mass[s.species[tmpl] - 1]);
-    if (gc_unif(P, u, t, 0, call) < 0.5) {
+    const bool insert = (P.gcmc_mode & DML_GCMC_ALTERNATE) ? !(t & 1)
+                                                          : gc_unif(P, u, t, 0, call) < 0.5;
+    if (insert) {
       // ---- creation attempt (dana.F90:627-677)
       if (act * vol / (double)(n + 1) < gc_unif(P, u, t, 1, call)) continue;
       const double rx = gc_unif(P, u, t, 2, call) * P.box[0];
@@ -2921,6 +2924,1272 @@ __global__ void __launch_bounds__(GC_BLOCK) k_gcmc_run(DevState s, int rep_only,
     if (n_del) atomicAdd(&s.cnt[r].del, n_del);
     if (fail) rep_fail(s, r, fail);
   }
+}
+
+// ------------------------------------------------------------------------
+// GCMC in batches (DML_GCMC_BATCHED): the nadj trials of one gcmc_run call
+// (dana.F90:590-713) are evaluated in parallel and committed in trial order
+// with the result of the sequential k_gcmc_run, bit for bit.
+//
+// Within a call the positions do not move; only the set of atoms changes.
+// So every trial's point, its hard-core neighbours among the atoms present
+// at the call's start and the earlier trial points within rcut can be found
+// in parallel (k_gb_trials, k_gb_conflicts).  What stays sequential is the
+// count n of the control volume and which atoms are present: one wave walks
+// the trials (k_gb_decide) with the in-volume list in LDS as a bitmap
+// (deletions take the m-th present entry, insertions append), and a second
+// wave walks the accepted moves (k_gb_commit) for the hs%b indices (lowest
+// free first, indices freed in this call included), the template and the
+// list tail that the reference's insertion reads, and the ordered stores of
+// flags, lists and index maps.  Slots (first free slot; a deleted atom goes
+// to the limbo, so deletions free none in this call), rows and cell chains
+// then follow in parallel: an inserted atom's own row is the stencil scan
+// at its insertion time (the cell chains then hold the atoms inserted
+// before it, newest first, then the atoms of the call's start not yet
+// deleted), the rows it joins get it appended in insertion order, and each
+// touched cell's chain is rebuilt once.
+//
+// A call the batch cannot take exactly (Verlet mode, no rows, more than
+// DML_GCMC_BATCH_MAX trials, an in-volume list beyond the LDS bitmap, a
+// gcmc list that could empty, more hard-core neighbours or nearby earlier
+// points than kept, a bucket or row that could overflow, an atom outside its
+// cells) is left to k_gcmc_run, which gives the same result one trial at a
+// time; the decision is taken before anything is written.
+// ------------------------------------------------------------------------
+#define GB_NB 16                     // hard-core neighbours kept per point
+#define GB_NC 16                     // earlier points within rcut kept per point
+#define GB_VWORDS (96 * 1024 / 4)    // in-volume bitmap words in LDS (786,432 entries)
+#define GB_CHUNKS 768                // its 1024-entry chunks
+#define GB_UW 12                     // hs%b universe bitmap words per lane (24,576 entries)
+#define GB_CELLMAX 512               // chain length a rebuilt cell may reach
+
+__device__ inline bool gb_on(const RepParams &P) { return P.gb_state == GB_ON && !P.halted; }
+__device__ inline void gb_fallback(RepParams *Pw) { Pw->gb_state = GB_FALLBACK; }
+
+// the call's sizes held in registers by the ordered walks (RepParams lives
+// in global memory: read through a reference after the walk's own stores,
+// every field access would wait for those stores)
+struct GbView {
+  int poff, off, nF, nslots, cap, T;
+};
+__device__ inline GbView gb_view(const RepParams &P) {
+  GbView v;
+  v.poff = P.poff;
+  v.off = P.gb_off;
+  v.nF = P.gb_nF;
+  v.nslots = P.gb_nslots;
+  v.cap = P.gb_cap;
+  v.T = P.gb_t;
+  return v;
+}
+__device__ inline int gb_ins_slot_v(const DevState &s, const GbView &v, int k) {
+  return v.poff + (k < v.nF ? s.gb_free[v.off + k] : v.nslots + (k - v.nF));
+}
+__device__ inline int gb_del_time_v(const DevState &s, const GbView &v, int o) {
+  const int t = s.gb_delt[o];
+  if (t < 0 || t >= v.T || s.gb_slot[v.off + t] != o || s.gb_kind[v.off + t] != 1)
+    return 0x7fffffff;
+  return t;
+}
+
+// slot (global) of the k-th insertion of the call: the free slots in
+// ascending order, then appended slots (Groups.F90:1083-1093)
+__device__ inline int gb_ins_slot(const DevState &s, const RepParams &P, int k) {
+  return P.poff + (k < P.gb_nF ? s.gb_free[P.gb_off + k] : P.gb_nslots + (k - P.gb_nF));
+}
+// insertion rank of an inserted slot (inverse of gb_ins_slot)
+__device__ inline int gb_ins_rank(const DevState &s, const RepParams &P, int i) {
+  const int loc = i - P.poff;
+  if (loc >= P.gb_nslots) return P.gb_nF + (loc - P.gb_nslots);
+  int lo = 0, hi = min(P.gb_nF, P.gb_cap) - 1;
+  const int32_t *f = s.gb_free + P.gb_off;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (f[mid] < loc) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+// trial at which atom o (present at the call's start) was deleted, or
+// INT_MAX; gb_delt is validated against the trial's own record
+__device__ inline int gb_del_time(const DevState &s, const RepParams &P, int o) {
+  const int t = s.gb_delt[o];
+  if (t < 0 || t >= P.gb_t || s.gb_slot[P.gb_off + t] != o || s.gb_kind[P.gb_off + t] != 1)
+    return 0x7fffffff;
+  return t;
+}
+// in-volume list position of atom o, or -1 (validated against the list)
+__device__ inline int gb_vpos_of(const DevState &s, const RepParams &P, int o) {
+  const int v = s.gb_vpos[o];
+  if (v < 0 || v >= P.gb_ilen || s.ref_tmp[P.poff + v] != o) return -1;
+  return v;
+}
+// counters and list heads tagged with the call's stamp: a value of another
+// call reads as zero / empty, so nothing needs clearing between calls
+__device__ inline int gb_stamped_inc(unsigned long long *a, uint32_t stamp) {
+  unsigned long long old = __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  while (true) {
+    const uint32_t cnt = (uint32_t)(old >> 32) == stamp ? (uint32_t)old : 0u;
+    const unsigned long long nw = ((unsigned long long)stamp << 32) | (cnt + 1u);
+    const unsigned long long got = atomicCAS(a, old, nw);
+    if (got == old) return (int)cnt;
+    old = got;
+  }
+}
+__device__ inline int gb_stamped_get(const unsigned long long *a, uint32_t stamp) {
+  const unsigned long long v = *a;
+  return (uint32_t)(v >> 32) == stamp ? (int)(uint32_t)v : 0;
+}
+__device__ inline int gb_list_head(const DevState &s, const RepParams &P, int cell) {
+  const unsigned long long v = s.gb_chead[P.coff + cell];
+  return (uint32_t)(v >> 32) == P.gb_stamp ? (int)(uint32_t)v : -1;
+}
+
+// every GCMC replica of the call: batched or not, and the sizes at its start
+__global__ void k_gb_begin(DevState s, int rep_only, double act_in, int nadj_in) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= s.nrep || (rep_only >= 0 && r != rep_only)) return;
+  RepParams &P = s.rp[r];
+  if (rep_only < 0 && P.reservoir != DML_GCMC) return;
+  P.gb_state = GB_OFF;
+  if (!(P.gcmc_mode & DML_GCMC_BATCHED) || P.halted || P.gc_pending != 1) return;
+  const int nadj = rep_only >= 0 ? nadj_in : P.nadj;
+  P.gb_t = nadj;
+  P.gb_act = rep_only >= 0 ? act_in : P.act;
+  P.gb_ilen = P.gc_inv;
+  P.gb_glen = P.gc_len;
+  P.gb_rlen = P.nref;
+  P.gb_nslots = P.nslots;
+  P.gb_bamax = P.b_amax;
+  P.gb_nF = P.gb_nFB = 0;
+  P.gb_nins = P.gb_ndel = P.gb_nalist = 0;
+  P.gb_stamp = (uint32_t)P.gcmc_calls + 1u;
+  const bool ok = P.use_cells && P.listed && nadj > 0 && nadj <= P.gb_cap &&
+                  (int64_t)P.gc_inv + nadj <= (int64_t)GB_VWORDS * 32 && P.gc_len >= nadj + 2;
+  P.gb_state = ok ? GB_ON : GB_FALLBACK;
+}
+
+// per particle block of the replica: the in-volume positions of its atoms,
+// and counts of free slots (below nslots) and free hs%b indices (below b_amax)
+__device__ inline void gb_free_flags(const DevState &s, const RepParams &P, int q, int *kf,
+                                     int *kb) {
+  const int gq = P.poff + q;
+  *kf = q < P.gb_nslots && !(s.flags[gq] & (DML_FL_ALIVE | DML_FL_LIMBO));
+  *kb = q < P.gb_bamax && s.b2slot[gq] < 0;
+}
+__global__ void __launch_bounds__(DML_BLOCK) k_gb_count(DevState s, int r) {
+  __shared__ int w1[DML_BLOCK / WAVE], w2[DML_BLOCK / WAVE];
+  const RepParams &P = s.rp[r];
+  if (!gb_on(P)) return;
+  const int q = blockIdx.x * DML_BLOCK + threadIdx.x;
+  if (q < P.gb_ilen) s.gb_vpos[s.ref_tmp[P.poff + q]] = q;
+  int kf, kb;
+  gb_free_flags(s, P, q, &kf, &kb);
+  const unsigned long long m1 = __ballot(kf), m2 = __ballot(kb);
+  const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
+  if (lane == 0) {
+    w1[w] = __popcll(m1);
+    w2[w] = __popcll(m2);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int a = 0, b = 0;
+    for (int k = 0; k < DML_BLOCK / WAVE; k++) {
+      a += w1[k];
+      b += w2[k];
+    }
+    const int gb = P.poff / DML_BLOCK + blockIdx.x;
+    s.gb_part[2 * gb] = a;
+    s.gb_part[2 * gb + 1] = b;
+  }
+}
+__global__ void __launch_bounds__(SCAN_BLOCK) k_gb_scan(DevState s, int r) {
+  RepParams *Pw = s.rp + r;
+  if (!gb_on(*Pw)) return;
+  int c1, c2;
+  rep_exscan2(s.gb_part, 2, Pw->poff / DML_BLOCK, Pw->pcap / DML_BLOCK, &c1, &c2);
+  if (threadIdx.x == 0) {
+    Pw->gb_nF = c1;
+    Pw->gb_nFB = c2;
+  }
+}
+__global__ void __launch_bounds__(DML_BLOCK) k_gb_scatter(DevState s, int r) {
+  __shared__ int w1[DML_BLOCK / WAVE], w2[DML_BLOCK / WAVE];
+  const RepParams &P = s.rp[r];
+  if (!gb_on(P)) return;
+  const int q = blockIdx.x * DML_BLOCK + threadIdx.x;
+  int kf, kb;
+  gb_free_flags(s, P, q, &kf, &kb);
+  const unsigned long long m1 = __ballot(kf), m2 = __ballot(kb);
+  const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
+  if (lane == 0) {
+    w1[w] = __popcll(m1);
+    w2[w] = __popcll(m2);
+  }
+  __syncthreads();
+  const int gb = P.poff / DML_BLOCK + blockIdx.x;
+  int o1 = s.gb_part[2 * gb], o2 = s.gb_part[2 * gb + 1];
+  for (int k = 0; k < w; k++) {
+    o1 += w1[k];
+    o2 += w2[k];
+  }
+  const unsigned long long below = (1ull << lane) - 1ull;
+  const int a = o1 + __popcll(m1 & below), b = o2 + __popcll(m2 & below);
+  if (kf && a < P.gb_cap) s.gb_free[P.gb_off + a] = q;
+  if (kb && b < P.gb_cap) s.gb_freeb[P.gb_off + b] = q;
+}
+
+// one wave per trial: kind, uniforms, and for an insertion its point, cell,
+// hard-core neighbours (gcmc atoms closer than rcut, dana.F90:638-653) and
+// FP64 energy (the same sum as k_gcmc_energy)
+__global__ void __launch_bounds__(DML_BLOCK) k_gb_trials(DevState s, int r, const double *u,
+                                                          const double *g) {
+  const int w = (blockIdx.x * DML_BLOCK + threadIdx.x) / WAVE, lane = threadIdx.x % WAVE;
+  const RepParams &P = s.rp[r];
+  if (!gb_on(P) || w >= P.gb_t) return;
+  const int call = P.gcmc_calls;
+  const int o = P.gb_off + w;
+  const int kind = (P.gcmc_mode & DML_GCMC_ALTERNATE) ? (w & 1)
+                                                      : !(gc_unif(P, u, w, 0, call) < 0.5);
+  const double u1 = gc_unif(P, u, w, 1, call), u2 = gc_unif(P, u, w, 2, call);
+  if (lane == 0) {
+    s.gb_u[2 * o] = u1;
+    s.gb_u[2 * o + 1] = u2;
+    s.gb_nblk[o] = 0;
+    s.gb_du[o] = 0.0;
+  }
+  if (kind) {
+    if (lane == 0) s.gb_kind[o] = 1;
+    return;
+  }
+  // the normals the velocity of an accepted insertion takes (dana.F90:666-668)
+  if (lane < 3) s.gb_g[3 * o + lane] = gc_gauss(P, g, w, lane, call);
+  const double rx = u2 * P.box[0];
+  const double ry = gc_unif(P, u, w, 3, call) * P.box[1];
+  const double rz = gc_unif(P, u, w, 4, call) * (P.zmax - P.z0) + P.z0;
+  const int ci0 = (int)(rx / P.cell[0]) + 1, ci1 = (int)(ry / P.cell[1]) + 1;
+  const int ci2 = (int)(rz / P.cell[2]) + 1;
+  if (lane == 0) {
+    s.gb_pt[3 * o] = rx;
+    s.gb_pt[3 * o + 1] = ry;
+    s.gb_pt[3 * o + 2] = rz;
+  }
+  if (ci0 < 0 || ci1 < 0 || ci2 < 0 || ci0 > P.ncells[0] + 1 || ci1 > P.ncells[1] + 1 ||
+      ci2 > P.ncells[2] + 1) {
+    if (lane == 0) s.gb_kind[o] = 2;
+    return;
+  }
+  int cells[27];
+  const int ncell = near_cells(P, ci0, ci1, ci2, cells);
+  const double rc2 = P.rcut * P.rcut;
+  double e = 0.0;
+  int nb = 0, perm = 0;
+  for (int k = 0; k < ncell; k++) {
+    const int cell = cells[k];
+    int cnt = s.cell_cnt[P.coff + cell];
+    if (cnt > P.bcap) cnt = P.bcap;
+    const int64_t base = P.boff + (int64_t)cell * P.bcap;
+    for (int q0 = 0; q0 < cnt; q0 += WAVE) {
+      const int q = q0 + lane;
+      int hit = 0, vp = -1;
+      if (q < cnt) {
+        const int j = s.bucket[base + q];
+        double dx, dy, dz;
+        vdist_mic(P, rx, ry, rz, s.px[j], s.py[j], s.pz[j], &dx, &dy, &dz);
+        const double dr2 = dot3(dx, dy, dz);
+        const int m = s.species[j];
+        const double r0 = P.r0[m - 1], eps = P.eps[m - 1];  // pair (Li, m)
+        if (!(dr2 > r0 * r0)) {
+          const double b = pow6d(r0) / pow6d(sqrt(dr2));
+          e += eps * b * (b - 2) + eps;
+        }
+        if (s.flags[j] & DML_FL_GCMC) {
+          double hx, hy, hz;  // as the trial loop tests it: against the bucket copy
+          vdist_mic(P, rx, ry, rz, s.bx[base + q], s.by[base + q], s.bz[base + q], &hx, &hy, &hz);
+          if (dot3(hx, hy, hz) < rc2) {
+            hit = 1;
+            vp = gb_vpos_of(s, P, j);
+          }
+        }
+      }
+      perm |= __ballot(hit && vp < 0) != 0;
+      const unsigned long long hm = __ballot(hit);
+      if (hit) {
+        const int at = nb + __popcll(hm & ((1ull << lane) - 1ull));
+        if (at < GB_NB) s.gb_blk[(int64_t)o * GB_NB + at] = vp;
+      }
+      nb += __popcll(hm);
+    }
+  }
+  for (int off = WAVE / 2; off > 0; off >>= 1) e += __shfl_xor(e, off);
+  if (lane == 0) {
+    s.gb_kind[o] = 0;
+    s.gb_cell[o] = cell_lin(P, ci0, ci1, ci2);
+    s.gb_nblk[o] = perm ? -1 : (nb > GB_NB ? GB_NB + 1 : nb);
+    s.gb_du[o] = e;
+  }
+}
+
+// one wave per insertion trial: the earlier insertion points within rcut
+// (in the reference the earlier atom, once inserted, sits in a cell chain)
+__global__ void __launch_bounds__(DML_BLOCK) k_gb_conflicts(DevState s, int r) {
+  const int w = (blockIdx.x * DML_BLOCK + threadIdx.x) / WAVE, lane = threadIdx.x % WAVE;
+  const RepParams &P = s.rp[r];
+  if (!gb_on(P) || w >= P.gb_t) return;
+  const int o = P.gb_off + w;
+  if (s.gb_kind[o] != 0) {
+    if (lane == 0) s.gb_ncf[o] = 0;
+    return;
+  }
+  const double rx = s.gb_pt[3 * o], ry = s.gb_pt[3 * o + 1], rz = s.gb_pt[3 * o + 2];
+  const double rc2 = P.rcut * P.rcut;
+  int nc = 0;
+  for (int t0 = 0; t0 < w; t0 += WAVE) {
+    const int tp = t0 + lane;
+    int hit = 0;
+    if (tp < w && s.gb_kind[P.gb_off + tp] == 0) {
+      const int op = P.gb_off + tp;
+      double dx, dy, dz;
+      vdist_mic(P, rx, ry, rz, s.gb_pt[3 * op], s.gb_pt[3 * op + 1], s.gb_pt[3 * op + 2], &dx,
+                &dy, &dz);
+      hit = dot3(dx, dy, dz) < rc2;
+    }
+    const unsigned long long hm = __ballot(hit);
+    if (hit) {
+      const int at = nc + __popcll(hm & ((1ull << lane) - 1ull));
+      if (at < GB_NC) s.gb_cf[(int64_t)o * GB_NC + at] = tp;
+    }
+    nc += __popcll(hm);
+  }
+  if (lane == 0) s.gb_ncf[o] = nc > GB_NC ? GB_NC + 1 : nc;
+}
+
+// inclusive prefix sum over the wave by DPP moves: row_shr 1, 2, 4, 8
+// inside each row of 16 lanes, then row_bcast 15 and 31 carry the row
+// totals (rows 1, 3 and rows 2, 3); a lane with no source adds 0
+__device__ inline int gb_wscan(int v) {
+  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);
+  return v;
+}
+
+// the decisions, one wave walking the trials in order (dana.F90:621-710):
+// n, the Metropolis tests, the hard-core test against the neighbours still
+// present and the earlier points accepted and still present, and for a
+// deletion the m-th present entry of the in-volume list (list order, the
+// call's insertions appended).  The list is a bitmap in LDS with per-chunk
+// counts (1024 entries a chunk, 16 chunks a lane, lane totals in registers).
+// Nothing of the replica's state is written here.
+__global__ void __launch_bounds__(WAVE) k_gb_decide(DevState s, int r) {
+  __shared__ uint32_t vb[GB_VWORDS];
+  __shared__ int32_t ccnt[WAVE * 16];
+  __shared__ int32_t res[DML_GCMC_BATCH_MAX];
+  // the neighbour lists of 64 trials, double-buffered: [buffer][trial][entry]
+  __shared__ int32_t lblk[2][WAVE][GB_NB], lcf[2][WAVE][GB_NC];
+  RepParams *Pw = s.rp + r;
+  if (!gb_on(*Pw)) return;
+  const int lane = threadIdx.x;
+  const int T = Pw->gb_t, ilen = Pw->gb_ilen, off = Pw->gb_off;
+  const double act = Pw->gb_act;
+  const double vol = Pw->box[0] * Pw->box[1] * (Pw->zmax - Pw->z0);
+  const int nwords = (ilen + T + 31) / 32;
+  for (int q = lane; q < nwords; q += WAVE) {
+    const int b0 = q * 32;
+    vb[q] = b0 + 32 <= ilen ? 0xffffffffu : (b0 < ilen ? (1u << (ilen - b0)) - 1u : 0u);
+  }
+  int tot = 0;  // entries present in this lane's 16 chunks
+  for (int k = 0; k < 16; k++) {
+    const int c = 16 * lane + k;
+    const int v = min(max(ilen - 1024 * c, 0), 1024);
+    ccnt[c] = v;
+    tot += v;
+  }
+  // per-trial data of a 64-trial chunk and its lists, loaded into
+  // registers one chunk ahead and staged into LDS at the chunk's start, so
+  // that the walk reads LDS and never waits on memory (a load result kept
+  // in a register across the loop would make every iteration wait for the
+  // iteration's own stores too); the walk is one wave, whose LDS accesses
+  // execute in program order
+  __shared__ int32_t lkind[2][WAVE], lnblk[2][WAVE], lncf[2][WAVE];
+  __shared__ double lu1[2][WAVE], lu2[2][WAVE];
+  int p_kind = 0, p_nblk = 0, p_ncf = 0;
+  double p_u1 = 0.0, p_u2 = 0.0;
+  int pb[GB_NB], pc[GB_NC];
+  auto load_chunk = [&](int t0) {
+    const int q = t0 + lane;
+    if (q < T) {
+      p_kind = s.gb_kind[off + q];
+      p_nblk = s.gb_nblk[off + q];
+      p_ncf = s.gb_ncf[off + q];
+      p_u1 = s.gb_u[2 * (off + q)];
+      p_u2 = s.gb_u[2 * (off + q) + 1];
+    }
+    // the lists of the chunk's 64 trials, 16 entries each: coalesced
+#pragma unroll
+    for (int j = 0; j < GB_NB; j++) {
+      const int64_t e = (int64_t)(off + t0) * GB_NB + j * WAVE + lane;
+      pb[j] = t0 + (j * WAVE + lane) / GB_NB < T ? s.gb_blk[e] : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < GB_NC; j++) {
+      const int64_t e = (int64_t)(off + t0) * GB_NC + j * WAVE + lane;
+      pc[j] = t0 + (j * WAVE + lane) / GB_NC < T ? s.gb_cf[e] : 0;
+    }
+  };
+  auto stage_chunk = [&](int buf) {
+    lkind[buf][lane] = p_kind;
+    lnblk[buf][lane] = p_nblk;
+    lncf[buf][lane] = p_ncf;
+    lu1[buf][lane] = p_u1;
+    lu2[buf][lane] = p_u2;
+#pragma unroll
+    for (int j = 0; j < GB_NB; j++) {
+      const int e = j * WAVE + lane;
+      lblk[buf][e / GB_NB][e % GB_NB] = pb[j];
+    }
+#pragma unroll
+    for (int j = 0; j < GB_NC; j++) {
+      const int e = j * WAVE + lane;
+      lcf[buf][e / GB_NC][e % GB_NC] = pc[j];
+    }
+  };
+  load_chunk(0);
+  __syncthreads();
+  int n = ilen, nins = 0, ndel = 0;
+  int fail = 0;
+  for (int t = 0; t < T && !fail; t++) {
+    const int a = t & (WAVE - 1);
+    const int buf = (t / WAVE) & 1;
+    if (a == 0) {
+      stage_chunk(buf);
+      if (t + WAVE < T) load_chunk(t + WAVE);
+      __builtin_amdgcn_wave_barrier();
+    }
+    const int kind = lkind[buf][a];
+    const int nblk = lnblk[buf][a];
+    const int ncf = lncf[buf][a];
+    const double u1 = lu1[buf][a], u2 = lu2[buf][a];
+    int out = -1;
+    if (kind == 2) {
+      fail = 1;  // a point outside the cells: the sequential kernel's error path
+    } else if (kind == 0) {
+      if (!(act * vol / (double)(n + 1) < u1) && nblk >= 0) {
+        if (nblk > GB_NB || ncf > GB_NC) {
+          fail = 1;
+        } else {
+          int here = 0;
+          if (lane < nblk) {
+            const int vp = lblk[buf][a][lane];
+            here = (vb[vp >> 5] >> (vp & 31)) & 1;
+          } else if (lane >= GB_NB && lane < GB_NB + ncf) {
+            const int k = res[lcf[buf][a][lane - GB_NB]];  // an earlier insertion: accepted, still present?
+            if (k >= 0) {
+              const int vp = ilen + k;
+              here = (vb[vp >> 5] >> (vp & 31)) & 1;
+            }
+          }
+          if (!__ballot(here)) {
+            const int k = nins++;
+            const int vp = ilen + k;
+            if (lane == 0) {
+              vb[vp >> 5] |= 1u << (vp & 31);
+              ccnt[vp >> 10]++;
+              s.gi_trial[off + k] = t;
+              s.gi_del[off + k] = 0x7fffffff;
+            }
+            if (lane == (vp >> 14)) tot++;
+            n++;
+            out = k;
+          }
+        }
+      }
+    } else {
+      if (!((double)n / (vol * act) < u1)) {
+        int m = (int)floor(u2 * n) + 1;
+        if (m > n) m = n;
+        if (m < 1) {
+          fail = 1;  // nothing to delete: the sequential kernel reports it
+        } else {
+          // the (m-1)-th present entry: lane, chunk, word, bit (each level's
+          // ballot is non-empty while the counts agree with the bitmap)
+          int rem = m - 1;
+          const int incl = gb_wscan(tot);
+          const unsigned long long b1 = __ballot(incl > rem);
+          const int L = b1 ? __builtin_ctzll(b1) : 0;
+          rem -= __builtin_amdgcn_readlane(incl - tot, L);
+          const int c16 = lane < 16 ? ccnt[16 * L + lane] : 0;
+          const int incl2 = gb_wscan(c16);
+          const unsigned long long b2 = __ballot(lane < 16 && incl2 > rem);
+          const int j = b2 ? __builtin_ctzll(b2) : 0;
+          rem -= __builtin_amdgcn_readlane(incl2 - c16, j);
+          const int chunk = 16 * L + j;
+          const uint32_t wv = lane < 32 ? vb[32 * chunk + lane] : 0u;
+          const int pc1 = __popc(wv);
+          const int incl3 = gb_wscan(pc1);
+          const unsigned long long b3 = __ballot(lane < 32 && incl3 > rem);
+          const int k = b3 ? __builtin_ctzll(b3) : 0;
+          rem -= __builtin_amdgcn_readlane(incl3 - pc1, k);
+          const uint32_t word = (uint32_t)__builtin_amdgcn_readlane((int)wv, k);
+          const unsigned long long b4 = __ballot(
+              lane < 32 && ((word >> (lane & 31)) & 1u) &&
+              __popc(word & ((1u << (lane & 31)) - 1u)) == rem);
+          const int bit = b4 ? __builtin_ctzll(b4) : 0;
+          if (!b1 || !b2 || !b3 || !b4 || chunk >= GB_CHUNKS) {
+            fail = 1;  // cannot happen; leave the call to the sequential kernel
+            break;
+          }
+          const int p = chunk * 1024 + k * 32 + bit;
+          if (lane == 0) {
+            vb[32 * chunk + k] = word & ~(1u << bit);
+            ccnt[chunk]--;
+            const int d = ndel;
+            s.gd_trial[off + d] = t;
+            s.gd_vpos[off + d] = p;
+            if (p >= ilen) s.gi_del[off + (p - ilen)] = t;
+          }
+          if (lane == L) tot--;
+          ndel++;
+          n--;
+          out = p;
+        }
+      }
+    }
+    if (lane == 0) {
+      res[t] = out;
+      s.gb_res[off + t] = out;
+      s.gb_slot[off + t] = -1;
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  if (lane == 0) {
+    Pw->gb_nins = nins;
+    Pw->gb_ndel = ndel;
+    // slots appended beyond the free ones, and the gcmc and ref list
+    // entries appended, must fit the replica (the sequential kernel
+    // compacts those lists instead)
+    if (fail || Pw->gb_nslots + max(0, nins - Pw->gb_nF) > Pw->pcap ||
+        Pw->gb_glen + nins > Pw->pcap || Pw->gb_rlen + nins > Pw->pcap)
+      gb_fallback(Pw);
+  }
+}
+
+// accepted insertions: each cell's list of the call's insertions (heads
+// stamped with the call, the order of the list is not used)
+__global__ void __launch_bounds__(DML_BLOCK) k_gb_lists(DevState s, int r) {
+  const int k = blockIdx.x * DML_BLOCK + threadIdx.x;
+  const RepParams &P = s.rp[r];
+  if (!gb_on(P) || k >= P.gb_nins) return;
+  const int t = s.gi_trial[P.gb_off + k];
+  const int cell = s.gb_cell[P.gb_off + t];
+  const unsigned long long old =
+      atomicExch(&s.gb_chead[P.coff + cell], ((unsigned long long)P.gb_stamp << 32) | (uint32_t)k);
+  s.gi_next[P.gb_off + k] = (uint32_t)(old >> 32) == P.gb_stamp ? (int)(uint32_t)old : -1;
+}
+
+// capacity checks before anything is written, one wave per accepted
+// insertion, each an upper bound of what the sequential trials can reach:
+// the bucket of its cell (every insertion into it, no deletion counted),
+// its own row (every atom of the call's start and every accepted point in
+// its stencil within the list radius), and the rows it joins (each ref atom
+// within the list radius gets at most one entry per accepted point)
+__global__ void __launch_bounds__(DML_BLOCK) k_gb_check(DevState s, int r) {
+  const int w = (blockIdx.x * DML_BLOCK + threadIdx.x) / WAVE, lane = threadIdx.x % WAVE;
+  RepParams *Pw = s.rp + r;
+  const RepParams &P = *Pw;
+  if (!gb_on(P) || w >= P.gb_nins) return;
+  const int off = P.gb_off;
+  const int t = s.gi_trial[off + w];
+  const int cell = s.gb_cell[off + t];
+  const double rx = s.gb_pt[3 * (off + t)], ry = s.gb_pt[3 * (off + t) + 1];
+  const double rz = s.gb_pt[3 * (off + t) + 2];
+  const int maxnn = s.maxnn;
+  int bad = 0;
+  if (lane == 0) {
+    const int before = gb_stamped_inc(&s.gb_ccnt[P.coff + cell], P.gb_stamp * 2u);
+    const int cap = min(P.bcap, GB_CELLMAX);
+    if (min(s.cell_cnt[P.coff + cell], P.bcap) + before + 1 > cap) bad = 1;
+  }
+  const int ci0 = cell % (P.ncells[0] + 2);
+  const int ci1 = (cell / (P.ncells[0] + 2)) % (P.ncells[1] + 2);
+  const int ci2 = cell / ((P.ncells[0] + 2) * (P.ncells[1] + 2));
+  int rowcnt = 0;
+  for (int k = 0; k < 27; k++) {
+    const int n0 = cell_wrap(ci0 + kMap[k][0], P.ncells[0]);
+    const int n1 = cell_wrap(ci1 + kMap[k][1], P.ncells[1]);
+    const int n2 = cell_wrap(ci2 + kMap[k][2], P.ncells[2]);
+    const int cc = cell_lin(P, n0, n1, n2);
+    const int64_t base = P.boff + (int64_t)cc * P.bcap;
+    const int cnt = min(s.cell_cnt[P.coff + cc], P.bcap);
+    for (int q = lane; q < cnt; q += WAVE) {
+      double dx, dy, dz;
+      vdist_mic(P, s.bx[base + q], s.by[base + q], s.bz[base + q], rx, ry, rz, &dx, &dy, &dz);
+      rowcnt += !(dot3(dx, dy, dz) > P.rl2);
+    }
+    for (int kk = gb_list_head(s, P, cc); kk >= 0; kk = s.gi_next[off + kk]) {
+      const int tp = s.gi_trial[off + kk];
+      if (lane == 0 && tp != t) {
+        double dx, dy, dz;
+        vdist_mic(P, s.gb_pt[3 * (off + tp)], s.gb_pt[3 * (off + tp) + 1],
+                  s.gb_pt[3 * (off + tp) + 2], rx, ry, rz, &dx, &dy, &dz);
+        rowcnt += !(dot3(dx, dy, dz) > P.rl2);
+      }
+    }
+  }
+  for (int o2 = WAVE / 2; o2 > 0; o2 >>= 1) rowcnt += __shfl_xor(rowcnt, o2);
+  if (rowcnt > maxnn) bad = 1;
+  int cells[27];
+  const int ncell = near_cells(P, ci0, ci1, ci2, cells);
+  for (int k = 0; k < ncell; k++) {
+    const int cc = cells[k];
+    const int64_t base = P.boff + (int64_t)cc * P.bcap;
+    const int cnt = min(s.cell_cnt[P.coff + cc], P.bcap);
+    for (int q = lane; q < cnt; q += WAVE) {
+      const int aj = s.bucket[base + q];
+      if (!(s.flags[aj] & DML_FL_REF)) continue;
+      double dx, dy, dz;
+      vdist_mic(P, rx, ry, rz, s.bx[base + q], s.by[base + q], s.bz[base + q], &dx, &dy, &dz);
+      if (dot3(dx, dy, dz) > P.rl2) continue;
+      const int before = gb_stamped_inc(&s.gb_app[aj], P.gb_stamp * 2u);
+      if (s.nn[aj] + before + 1 > maxnn) bad = 1;
+    }
+  }
+  if (__ballot(bad)) {
+    if (lane == 0) gb_fallback(Pw);
+  }
+}
+
+// one wave per accepted deletion: the atom (an atom of the call's start, or
+// one inserted earlier in it), and for the former what its removal reads
+// (hs%b index, ref list position, flags, the cell holding it: Cells.F90:
+// 304-352 fails when it is not there) and its FP64 removal energy
+__global__ void __launch_bounds__(DML_BLOCK) k_gb_dels(DevState s, int r) {
+  const int w = (blockIdx.x * DML_BLOCK + threadIdx.x) / WAVE, lane = threadIdx.x % WAVE;
+  RepParams *Pw = s.rp + r;
+  const RepParams &P = *Pw;
+  if (!gb_on(P) || w >= P.gb_ndel) return;
+  const int off = P.gb_off;
+  const int t = s.gd_trial[off + w];
+  const int p = s.gd_vpos[off + w];
+  if (p >= P.gb_ilen) {
+    const int k = p - P.gb_ilen;
+    const int o = gb_ins_slot(s, P, k);
+    if (lane == 0) {
+      s.gd_slot[off + w] = o;
+      s.gb_slot[off + t] = o;
+      s.gd_bso[off + w] = -1;
+      s.gd_cell[off + w] = s.gb_cell[off + s.gi_trial[off + k]];
+      s.gb_du[off + t] = s.gb_du[off + s.gi_trial[off + k]];
+    }
+    return;
+  }
+  const int o = s.ref_tmp[P.poff + p];
+  const double x = s.px[o], y = s.py[o], z = s.pz[o];
+  const int c0 = (int)(x / P.cell[0]) + 1, c1 = (int)(y / P.cell[1]) + 1;
+  const int c2 = (int)(z / P.cell[2]) + 1;
+  if (c0 < 1 || c1 < 1 || c2 < 1 || c0 > P.ncells[0] || c1 > P.ncells[1] || c2 > P.ncells[2]) {
+    if (lane == 0) gb_fallback(Pw);
+    return;
+  }
+  const int cell = cell_lin(P, c0, c1, c2);
+  {
+    const int64_t base = P.boff + (int64_t)cell * P.bcap;
+    const int cnt = min(s.cell_cnt[P.coff + cell], P.bcap);
+    if (cnt > GB_CELLMAX) {  // a chain longer than k_gb_cells stages
+      if (lane == 0) gb_fallback(Pw);
+      return;
+    }
+    int found = 0;
+    for (int q = lane; q < cnt; q += WAVE) found |= s.bucket[base + q] == o;
+    if (!__ballot(found)) {
+      if (lane == 0) gb_fallback(Pw);
+      return;
+    }
+  }
+  // removal energy at the call's start (k_gcmc_energy's sum without the atom)
+  int cells[27];
+  const int ncell = near_cells(P, c0, c1, c2, cells);
+  double e = 0.0;
+  for (int k = 0; k < ncell; k++) {
+    const int cc = cells[k];
+    int cnt = s.cell_cnt[P.coff + cc];
+    if (cnt > P.bcap) cnt = P.bcap;
+    const int64_t base = P.boff + (int64_t)cc * P.bcap;
+    for (int q = lane; q < cnt; q += WAVE) {
+      const int j = s.bucket[base + q];
+      if (j == o) continue;
+      double dx, dy, dz;
+      vdist_mic(P, x, y, z, s.px[j], s.py[j], s.pz[j], &dx, &dy, &dz);
+      const double dr2 = dot3(dx, dy, dz);
+      const int m = s.species[j];
+      const double r0 = P.r0[m - 1], eps = P.eps[m - 1];
+      if (dr2 > r0 * r0) continue;
+      const double b = pow6d(r0) / pow6d(sqrt(dr2));
+      e += eps * b * (b - 2) + eps;
+    }
+  }
+  for (int o2 = WAVE / 2; o2 > 0; o2 >>= 1) e += __shfl_xor(e, o2);
+  if (lane == 0) {
+    s.gd_slot[off + w] = o;
+    s.gb_slot[off + t] = o;
+    s.gd_bso[off + w] = s.bkey[o];
+    s.gd_rpos[off + w] = s.ref_pos[o];
+    s.gd_oflags[off + w] = s.flags[o];
+    s.gd_cell[off + w] = cell;
+    s.gb_delt[o] = t;
+    s.gb_du[off + t] = e;
+  }
+}
+
+// the free hs%b indices an insertion of the call can take, in ascending
+// order: the first free ones of the call's start (at most one per trial)
+// and those of the atoms it deletes (rank = position in that order)
+__global__ void __launch_bounds__(DML_BLOCK) k_gb_brank(DevState s, int r) {
+  const int q = blockIdx.x * DML_BLOCK + threadIdx.x;
+  const RepParams &P = s.rp[r];
+  if (!gb_on(P)) return;
+  const int off = P.gb_off, nUF = min(P.gb_nFB, P.gb_t), nd = P.gb_ndel;
+  const int32_t *fb = s.gb_freeb + off;
+  if (q < nUF) {
+    const int x = fb[q];
+    int rank = q;
+    for (int d = 0; d < nd; d++) {
+      const int b = s.gd_bso[off + d];
+      rank += b >= 0 && b < x;
+    }
+    s.gb_frank[off + q] = rank;
+    s.gb_U[3 * off + rank] = x;
+  } else if (q < nUF + nd) {
+    const int d = q - nUF;
+    const int x = s.gd_bso[off + d];
+    if (x < 0) return;
+    int lo = 0, hi = nUF;  // free indices of the start below x
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (fb[mid] < x) lo = mid + 1; else hi = mid;
+    }
+    int rank = lo;
+    for (int e = 0; e < nd; e++) {
+      const int b = s.gd_bso[off + e];
+      rank += b >= 0 && b < x;
+    }
+    s.gd_brank[off + d] = rank;
+    s.gb_U[3 * off + rank] = x;
+  }
+}
+
+// the accepted moves in trial order, one wave: hs%b indices (the lowest
+// free one, indices freed earlier in the call included, Groups.F90:1094-
+// 1100), the template (first live gcmc atom) and list tail whose velocity
+// the insertion sets (dana.F90:620-668), and the ordered stores: flags,
+// hs%b maps, ref and gcmc list entries, limbo (Groups.F90:433-467).  The
+// universe of free hs%b indices is a bitmap in registers (lane l holds
+// words l, l + 64, ...), their values in LDS.
+__global__ void __launch_bounds__(WAVE) k_gb_commit(DevState s, int r, const double *g) {
+  // universe values (the free indices of the start and those freed by the
+  // call's deletions: at most two per trial counting the appended ones)
+  __shared__ int32_t U[2 * DML_GCMC_BATCH_MAX];
+  __shared__ uint32_t ubl[GB_UW * WAVE];
+  __shared__ int32_t ibr[DML_GCMC_BATCH_MAX];  // hs%b rank of each insertion
+  __shared__ int32_t ibv[DML_GCMC_BATCH_MAX];  // and the index itself
+  RepParams *Pw = s.rp + r;
+  if (!gb_on(*Pw)) return;
+  const RepParams &P = *Pw;
+  const int lane = threadIdx.x;
+  const GbView V = gb_view(P);
+  const int off = P.gb_off, T = P.gb_t, poff = P.poff, ilen = P.gb_ilen;
+  const int nins = P.gb_nins, ndel = P.gb_ndel, rlen = P.gb_rlen;
+  const int nUF = min(P.gb_nFB, T);
+  const double mass[3] = {P.mass[0], P.mass[1], P.mass[2]};
+  int nU = 0;
+  for (int d = lane; d < ndel; d += WAVE) nU += s.gd_bso[off + d] >= 0;
+  for (int o2 = WAVE / 2; o2 > 0; o2 >>= 1) nU += __shfl_xor(nU, o2);
+  nU += nUF;
+  for (int q = lane; q < GB_UW * WAVE; q += WAVE) ubl[q] = 0u;
+  __syncthreads();
+  for (int q = lane; q < nU; q += WAVE) U[q] = s.gb_U[3 * off + q];
+  for (int q = lane; q < nUF; q += WAVE) {
+    const int rk = s.gb_frank[off + q];
+    atomicOr(&ubl[rk >> 5], 1u << (rk & 31));
+  }
+  __syncthreads();
+  uint32_t ub[GB_UW];
+#pragma unroll
+  for (int k = 0; k < GB_UW; k++) ub[k] = ubl[k * WAVE + lane];
+  int b_amax = P.b_amax, b_nat = P.b_nat;
+  int nappend = 0;          // hs%b indices appended at b_amax (ranks nU + ...)
+  int bmin = P.bfree_lo;    // lower bound of the free hs%b indices after the call
+  // gcmc list: the staged list (glen entries) then the call's insertions
+  const int32_t *gc = s.wl[1] + poff;
+  const int glen = P.gb_glen;
+  int hd = 0;                 // first live entry (the template)
+  int tmpl = gc[0];
+  int e_tail = glen - 1;      // last live entry (whose velocity an insertion sets)
+  const double ui_ev = 1.0e2 * 1.0e2 * 1.6605402e-27 * (1.0 / 1.60219e-19);
+  const double kB_ui = 8.617385e-05 * (1.0 / ui_ev);  // Constants.F90:100-167
+  double beta = sqrt(kB_ui * 300.0 / P.mass[s.species[tmpl] - 1]);
+  double tv[3] = {s.vx[tmpl], s.vy[tmpl], s.vz[tmpl]};
+  int8_t tsp = s.species[tmpl];
+  const int call = P.gcmc_calls;
+  (void)g;
+  // live at trial t: an entry of the start not deleted before t, an
+  // insertion made before t and not deleted before it
+  auto entry_atom = [&](int e) { return e < glen ? gc[e] : gb_ins_slot_v(s, V, e - glen); };
+  auto entry_dead = [&](int e, int t) {
+    if (e < glen) return gb_del_time_v(s, V, gc[e]) <= t;
+    return s.gi_del[off + (e - glen)] <= t;
+  };
+  // three streams read 64 entries at a time, one per lane, staged into LDS
+  // when the walk reaches the chunk (loaded there and then: a load result
+  // carried in registers across iterations makes every iteration wait for
+  // the stores before it): the trials (decision, kind, normals), the
+  // deletions in order (atom, hs%b index and rank, ref position, flags) and
+  // the insertions' slots
+  __shared__ int32_t lres[WAVE], lkind[WAVE];
+  __shared__ double lg[3][WAVE];
+  __shared__ int32_t ld[5][WAVE];
+  __shared__ int32_t lis[WAVE];
+  int ki = 0, kd = 0;
+  int ta = gc[glen - 1];  // the tail atom
+  for (int t = 0; t < T; t++) {
+    // one wave in program order: the barrier only keeps the compiler (and a
+    // lane-by-lane CPU model of the wave) from moving LDS accesses across
+    // iterations
+    __builtin_amdgcn_wave_barrier();
+    const int tl = t & (WAVE - 1);
+    if (tl == 0) {
+      const int q = t + lane;
+      if (q < T) {
+        lres[lane] = s.gb_res[off + q];
+        lkind[lane] = s.gb_kind[off + q];
+        lg[0][lane] = s.gb_g[3 * (off + q)];
+        lg[1][lane] = s.gb_g[3 * (off + q) + 1];
+        lg[2][lane] = s.gb_g[3 * (off + q) + 2];
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+    const int res = lres[tl];
+    const int kind = lkind[tl];
+    if (res < 0) continue;
+    if (kind == 0) {
+      const int k = ki++;
+      if ((k & (WAVE - 1)) == 0) {
+        if (k + lane < nins) lis[lane] = gb_ins_slot_v(s, V, k + lane);
+        __builtin_amdgcn_wave_barrier();
+      }
+      const int i = lis[k & (WAVE - 1)];
+      // hs%b index: the lowest free one, or a new one at b_amax
+      int bs, rank;
+      if (b_amax >= b_nat + 1) {
+        int w = -1;
+#pragma unroll
+        for (int q = 0; q < GB_UW; q++) {
+          const unsigned long long m = __ballot(ub[q] != 0u);
+          if (w < 0 && m) w = q * WAVE + __builtin_ctzll(m);
+        }
+        const int wl = w & (WAVE - 1), wq = w / WAVE;
+        uint32_t word = 0;
+#pragma unroll
+        for (int q = 0; q < GB_UW; q++)
+          if (q == wq) word = (uint32_t)__builtin_amdgcn_readlane((int)ub[q], wl);
+        if (w < 0) {  // cannot happen: b_amax - b_nat counts the set bits
+          if (lane == 0) rep_fail(s, r, ERR_UNSUP);
+          return;
+        }
+        const int bit = __builtin_ctz(word);
+        rank = w * 32 + bit;
+#pragma unroll
+        for (int q = 0; q < GB_UW; q++)
+          if (q == wq && lane == wl) ub[q] &= ~(1u << bit);
+        bs = U[rank];
+      } else {
+        bs = b_amax++;
+        rank = nU + nappend++;
+        if (lane == 0) U[rank] = bs;
+      }
+      b_nat++;
+      // the velocity lands on the list's last live atom, the new atom copies
+      // the template (atom_asign, Groups.F90:484-502)
+      const double g0 = lg[0][tl], g1 = lg[1][tl], g2 = lg[2][tl];
+      if (lane == 0) {
+        s.vx[ta] = beta * g0;
+        s.vy[ta] = beta * g1;
+        s.vz[ta] = beta * g2;
+        s.vx[i] = tv[0];
+        s.vy[i] = tv[1];
+        s.vz[i] = tv[2];
+        s.species[i] = tsp;
+        s.flags[i] = DML_FL_ALIVE | DML_FL_REF | DML_FL_GCMC;
+        s.bkey[i] = bs;
+        s.b2slot[poff + bs] = i;
+        s.ref_order[poff + rlen + k] = i;
+        s.ref_pos[i] = rlen + k;
+        s.wl[1][poff + glen + k] = i;
+        s.gi_tmpl[off + k] = tmpl;
+        s.gb_slot[off + t] = i;
+        ibr[k] = rank;
+        ibv[k] = bs;
+      }
+      e_tail = glen + k;
+      ta = i;
+    } else {
+      const int d = kd++;
+      if ((d & (WAVE - 1)) == 0) {
+        const int q = d + lane;
+        if (q < ndel) {
+          ld[0][lane] = s.gd_slot[off + q];
+          ld[1][lane] = s.gd_bso[off + q];
+          ld[2][lane] = s.gd_brank[off + q];
+          ld[3][lane] = s.gd_rpos[off + q];
+          ld[4][lane] = s.gd_oflags[off + q];
+        }
+        __builtin_amdgcn_wave_barrier();
+      }
+      const int dl = d & (WAVE - 1);
+      const int o = ld[0][dl];
+
+      int bso, rank, rpos, ofl;
+      if (res >= ilen) {
+        const int k = res - ilen;
+        bso = ibv[k];
+        rank = ibr[k];
+        rpos = rlen + k;
+        ofl = 0;
+      } else {
+        bso = ld[1][dl];
+        rank = ld[2][dl];
+        rpos = ld[3][dl];
+        ofl = ld[4][dl];
+      }
+      // the freed hs%b index is free again for the later insertions
+      const int wq = (rank >> 5) / WAVE, wl = (rank >> 5) & (WAVE - 1);
+#pragma unroll
+      for (int q = 0; q < GB_UW; q++)
+        if (q == wq && lane == wl) ub[q] |= 1u << (rank & 31);
+      b_nat--;
+      bmin = min(bmin, bso);
+      if (lane == 0) {
+        s.flags[o] = (uint8_t)(DML_FL_LIMBO | (ofl & DML_FL_SKIP));
+        s.nn[o] = 0;
+        s.ref_order[poff + rpos] = -1;
+        s.ref_pos[o] = -1;
+        s.b2slot[poff + bso] = -1;
+      }
+      // the template and the tail step over dead entries (dana.F90:620,
+      // the gcmc list after gcmc%detach)
+      // (values loaded here are taken into scalar registers at once: a load
+      // still in flight at the merge below would make every later
+      // iteration wait for it)
+      if (o == tmpl) {
+        while (entry_dead(hd, t)) hd++;
+        tmpl = __builtin_amdgcn_readfirstlane(entry_atom(hd));
+        tsp = (int8_t)__builtin_amdgcn_readfirstlane((int)s.species[tmpl]);
+        beta = sqrt(kB_ui * 300.0 / mass[tsp - 1]);
+        tv[0] = rdlane_d(s.vx[tmpl], 0);
+        tv[1] = rdlane_d(s.vy[tmpl], 0);
+        tv[2] = rdlane_d(s.vz[tmpl], 0);
+      }
+      if (o == ta) {
+        while (e_tail > hd && entry_dead(e_tail, t)) e_tail--;
+        ta = __builtin_amdgcn_readfirstlane(entry_atom(e_tail));
+      }
+    }
+  }
+  if (lane == 0) {
+    const int nF = P.gb_nF;
+    const int nsl = P.gb_nslots + max(0, nins - nF);
+    Pw->nslots = nsl;
+    Pw->free_lo = nins < nF ? (nins < P.gb_cap ? s.gb_free[off + nins] : s.gb_free[off + P.gb_cap - 1] + 1)
+                            : nsl;
+    Pw->bfree_lo = bmin;
+    Pw->nat = P.nat + nins - ndel;
+    Pw->nlimbo = P.nlimbo + ndel;
+    Pw->b_amax = b_amax;
+    Pw->b_nat = b_nat;
+    Pw->pid_next = P.pid_next + (uint32_t)nins;
+    Pw->seq_next = P.seq_next + nins;
+    Pw->gc_len = glen + nins;
+    Pw->ref_len = P.gb_rlen + nins;
+    Pw->gc_pending = 2;
+    if (nins || ndel) Pw->ovn_stamp = 0;
+    Pw->gcmc_calls = call + 1;
+    if (nins) atomicAdd(&s.cnt[r].ins, (unsigned long long)nins);
+    if (ndel) atomicAdd(&s.cnt[r].del, (unsigned long long)ndel);
+  }
+}
+
+// accepted insertions: the fields atom_asign copies from the template and
+// those gcmc_run sets (dana.F90:659-668, Groups.F90:484-502)
+__global__ void __launch_bounds__(DML_BLOCK) k_gb_fields(DevState s, int r) {
+  const int k = blockIdx.x * DML_BLOCK + threadIdx.x;
+  const RepParams &P = s.rp[r];
+  if (!gb_on(P) || k >= P.gb_nins) return;
+  const int off = P.gb_off;
+  const int t = s.gi_trial[off + k];
+  const int i = gb_ins_slot(s, P, k);
+  const int tm = s.gi_tmpl[off + k];
+  const double rx = s.gb_pt[3 * (off + t)], ry = s.gb_pt[3 * (off + t) + 1];
+  const double rz = s.gb_pt[3 * (off + t) + 2];
+  s.fx[i] = s.fx[tm]; s.fy[i] = s.fy[tm]; s.fz[i] = s.fz[tm];
+  s.ax[i] = s.ax[tm]; s.ay[i] = s.ay[tm]; s.az[i] = s.az[tm];
+  s.epot[i] = s.epot[tm];
+  s.px[i] = rx; s.py[i] = ry; s.pz[i] = rz;
+  s.ox[i] = rx; s.oy[i] = ry; s.oz[i] = rz;
+  s.cx[i] = 1e8; s.cy[i] = 1e8; s.cz[i] = 1e8;
+  s.pid[i] = P.pid_next - (uint32_t)P.gb_nins + (uint32_t)k;
+  s.seq[i] = P.seq_next - P.gb_nins + k;
+  s.rowflag[i] = 0;
+  put_pf4(s, i, rx, ry, rz, s.species[i]);
+  // the row built at insertion (kept even when the atom leaves later in the
+  // call) marks a ghost-layer cell
+  const int cell = s.gb_cell[off + t];
+  const int ci0 = cell % (P.ncells[0] + 2);
+  const int ci1 = (cell / (P.ncells[0] + 2)) % (P.ncells[1] + 2);
+  const int ci2 = cell / ((P.ncells[0] + 2) * (P.ncells[1] + 2));
+  const int ghost = ci2 == 0 || ci2 == P.ncells[2] + 1 || ci0 == 0 || ci0 == P.ncells[0] + 1 ||
+                    ci1 == 0 || ci1 == P.ncells[1] + 1;
+  s.rowflag[i] = (uint8_t)ghost;
+  if (ghost) s.rp[r].any_ghost = 1;
+}
+
+// an inserted atom's own row (ngroup_sort_atom, Neighbor.F90:271-318) as
+// the sequential trials build it at insertion k: the 27 stencil cells in
+// order; in each, the call's earlier insertions into it that are still
+// present (newest first: push-front, Cells.F90:299-300), then the chain of
+// the call's start without the atoms deleted before; strict '<' on the
+// list radius.  One wave per insertion kept to the end of the call.
+__global__ void __launch_bounds__(DML_BLOCK) k_gb_ownrow(DevState s, int r) {
+  __shared__ int lbuf[DML_BLOCK / WAVE][GB_CELLMAX];
+  const int w = (blockIdx.x * DML_BLOCK + threadIdx.x) / WAVE, lane = threadIdx.x % WAVE;
+  const int wv = threadIdx.x / WAVE;
+  const RepParams &P = s.rp[r];
+  if (!gb_on(P) || w >= P.gb_nins) return;
+  const int off = P.gb_off, k = w;
+  if (s.gi_del[off + k] != 0x7fffffff) return;
+  const int t = s.gi_trial[off + k];
+  const int i = gb_ins_slot(s, P, k);
+  const double rx = s.gb_pt[3 * (off + t)], ry = s.gb_pt[3 * (off + t) + 1];
+  const double rz = s.gb_pt[3 * (off + t) + 2];
+  const int cell = s.gb_cell[off + t];
+  const int ci0 = cell % (P.ncells[0] + 2);
+  const int ci1 = (cell / (P.ncells[0] + 2)) % (P.ncells[1] + 2);
+  const int ci2 = cell / ((P.ncells[0] + 2) * (P.ncells[1] + 2));
+  const int maxnn = s.maxnn;
+  int32_t *row = s.nbr + (int64_t)i * maxnn;
+  int nn = 0;
+  for (int kq = 0; kq < 27; kq++) {
+    const int n0 = cell_wrap(ci0 + kMap[kq][0], P.ncells[0]);
+    const int n1 = cell_wrap(ci1 + kMap[kq][1], P.ncells[1]);
+    const int n2 = cell_wrap(ci2 + kMap[kq][2], P.ncells[2]);
+    const int cc = cell_lin(P, n0, n1, n2);
+    // the earlier insertions into cc still present at t, within the radius
+    int nl = 0;
+    for (int kk = gb_list_head(s, P, cc); kk >= 0; kk = s.gi_next[off + kk]) {
+      if (kk < k && s.gi_del[off + kk] > t) {
+        const int tp = s.gi_trial[off + kk];
+        double dx, dy, dz;
+        vdist_mic(P, s.gb_pt[3 * (off + tp)], s.gb_pt[3 * (off + tp) + 1],
+                  s.gb_pt[3 * (off + tp) + 2], rx, ry, rz, &dx, &dy, &dz);
+        if (dot3(dx, dy, dz) < P.rl2) {
+          if (lane == 0) lbuf[wv][nl] = kk;
+          nl++;
+        }
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    // newest first: entry q goes after the entries of higher rank
+    for (int q = lane; q < nl; q += WAVE) {
+      const int mine = lbuf[wv][q];
+      int pos = 0;
+      for (int q2 = 0; q2 < nl; q2++) pos += lbuf[wv][q2] > mine;
+      if (nn + pos < maxnn) row[nn + pos] = gb_ins_slot(s, P, mine);
+    }
+    nn += nl;
+    __builtin_amdgcn_wave_barrier();
+    // the chain of the start without the atoms deleted before t
+    const int64_t base = P.boff + (int64_t)cc * P.bcap;
+    const int cnt = min(s.cell_cnt[P.coff + cc], P.bcap);
+    for (int q0 = 0; q0 < cnt; q0 += WAVE) {
+      const int q = q0 + lane;
+      int hit = 0, j = -1;
+      if (q < cnt) {
+        j = s.bucket[base + q];
+        if (j != i && gb_del_time(s, P, j) > t) {
+          double dx, dy, dz;
+          vdist_mic(P, s.bx[base + q], s.by[base + q], s.bz[base + q], rx, ry, rz, &dx, &dy, &dz);
+          hit = dot3(dx, dy, dz) < P.rl2;
+        }
+      }
+      const unsigned long long m = __ballot(hit);
+      const int pr = __popcll(m & ((1ull << lane) - 1ull));
+      if (hit && nn + pr < maxnn) row[nn + pr] = j;
+      nn += __popcll(m);
+    }
+  }
+  if (lane == 0) s.nn[i] = min(nn, maxnn);
+}
+
+// the rows an insertion joins (gcmc_run's attach: every ref atom within the
+// list radius of the new atom gets it appended, in insertion order).  Pass
+// 1, one wave per accepted insertion: each such row present at insertion
+// time and kept to the call's end gets the entry at a position taken from
+// a per-row counter; the first entry of a row lists the row.
+__global__ void __launch_bounds__(DML_BLOCK) k_gb_append(DevState s, int r) {
+  const int w = (blockIdx.x * DML_BLOCK + threadIdx.x) / WAVE, lane = threadIdx.x % WAVE;
+  RepParams *Pw = s.rp + r;
+  const RepParams &P = *Pw;
+  if (!gb_on(P) || w >= P.gb_nins) return;
+  const int off = P.gb_off, k = w;
+  const int t = s.gi_trial[off + k];
+  const int i = gb_ins_slot(s, P, k);
+  const double rx = s.gb_pt[3 * (off + t)], ry = s.gb_pt[3 * (off + t) + 1];
+  const double rz = s.gb_pt[3 * (off + t) + 2];
+  const int cell = s.gb_cell[off + t];
+  const int ci0 = cell % (P.ncells[0] + 2);
+  const int ci1 = (cell / (P.ncells[0] + 2)) % (P.ncells[1] + 2);
+  const int ci2 = cell / ((P.ncells[0] + 2) * (P.ncells[1] + 2));
+  const int maxnn = s.maxnn;
+  const uint32_t stamp = P.gb_stamp * 2u + 1u;
+  int cells[27];
+  const int ncell = near_cells(P, ci0, ci1, ci2, cells);
+  for (int kc = 0; kc < ncell; kc++) {
+    const int cc = cells[kc];
+    // atoms of the start (present to the end, ref)
+    const int64_t base = P.boff + (int64_t)cc * P.bcap;
+    const int cnt = min(s.cell_cnt[P.coff + cc], P.bcap);
+    for (int q = lane; q < cnt; q += WAVE) {
+      const int aj = s.bucket[base + q];
+      if (!(s.flags[aj] & DML_FL_REF) || gb_del_time(s, P, aj) != 0x7fffffff) continue;
+      double dx, dy, dz;
+      vdist_mic(P, rx, ry, rz, s.bx[base + q], s.by[base + q], s.bz[base + q], &dx, &dy, &dz);
+      if (dot3(dx, dy, dz) > P.rl2) continue;
+      const int at = gb_stamped_inc(&s.gb_app[aj], stamp);
+      const int pos = s.nn[aj] + at;
+      if (pos < maxnn) s.nbr[(int64_t)aj * maxnn + pos] = i;
+      if (at == 0) s.gb_alist[P.poff + atomicAdd(&Pw->gb_nalist, 1)] = aj;
+    }
+    // the call's earlier insertions into cc kept to the end
+    for (int kk0 = gb_list_head(s, P, cc); kk0 >= 0;) {
+      int kk = kk0;  // the list one entry at a time, all lanes in step
+      kk0 = s.gi_next[off + kk];
+      if (lane != 0 || kk >= k || s.gi_del[off + kk] != 0x7fffffff) continue;
+      const int tp = s.gi_trial[off + kk];
+      double dx, dy, dz;
+      vdist_mic(P, rx, ry, rz, s.gb_pt[3 * (off + tp)], s.gb_pt[3 * (off + tp) + 1],
+                s.gb_pt[3 * (off + tp) + 2], &dx, &dy, &dz);
+      if (dot3(dx, dy, dz) > P.rl2) continue;
+      const int aj = gb_ins_slot(s, P, kk);
+      const int at = gb_stamped_inc(&s.gb_app[aj], stamp);
+      const int pos = s.nn[aj] + at;
+      if (pos < maxnn) s.nbr[(int64_t)aj * maxnn + pos] = i;
+      if (at == 0) s.gb_alist[P.poff + atomicAdd(&Pw->gb_nalist, 1)] = aj;
+    }
+  }
+}
+
+// pass 2, per listed row: the appended entries in insertion order, and the
+// row length
+__global__ void __launch_bounds__(DML_BLOCK) k_gb_append_fix(DevState s, int r) {
+  const RepParams &P = s.rp[r];
+  if (!gb_on(P)) return;
+  const int maxnn = s.maxnn;
+  const uint32_t stamp = P.gb_stamp * 2u + 1u;
+  for (int q = blockIdx.x * DML_BLOCK + threadIdx.x; q < P.gb_nalist;
+       q += gridDim.x * DML_BLOCK) {
+    const int aj = s.gb_alist[P.poff + q];
+    const int cnt = gb_stamped_get(&s.gb_app[aj], stamp);
+    const int base = s.nn[aj];
+    int32_t *row = s.nbr + (int64_t)aj * maxnn + base;
+    const int m = min(cnt, maxnn - base);
+    // insertion sort by insertion rank (a handful of entries)
+    for (int a = 1; a < m; a++) {
+      const int v = row[a];
+      const int kv = gb_ins_rank(s, P, v);
+      int b = a - 1;
+      while (b >= 0 && gb_ins_rank(s, P, row[b]) > kv) {
+        row[b + 1] = row[b];
+        b--;
+      }
+      row[b + 1] = v;
+    }
+    s.nn[aj] = min(base + cnt, maxnn);
+  }
+}
+
+// each touched cell's chain, rebuilt once (claimed with the call's stamp):
+// the call's insertions into it kept to the end, newest first, then the
+// chain of the start without the deleted atoms.  One wave per accepted
+// move (insertions, then deletions); the new chain is staged in LDS.
+__global__ void __launch_bounds__(WAVE) k_gb_cells(DevState s, int r) {
+  __shared__ int32_t lst[GB_CELLMAX];
+  __shared__ int32_t cj[GB_CELLMAX];
+  __shared__ double cxs[GB_CELLMAX], cys[GB_CELLMAX], czs[GB_CELLMAX];
+  __shared__ float4 cf[GB_CELLMAX];
+  const int e = blockIdx.x, lane = threadIdx.x;
+  const RepParams &P = s.rp[r];
+  if (!gb_on(P) || e >= P.gb_nins + P.gb_ndel) return;
+  const int off = P.gb_off;
+  const int cell = e < P.gb_nins ? s.gb_cell[off + s.gi_trial[off + e]]
+                                 : s.gd_cell[off + (e - P.gb_nins)];
+  int claimed = 0;
+  if (lane == 0)
+    claimed = atomicExch(&s.gb_claim[P.coff + cell], (int)P.gb_stamp) != (int)P.gb_stamp;
+  if (!__builtin_amdgcn_readfirstlane(claimed)) return;
+  const int64_t base = P.boff + (int64_t)cell * P.bcap;
+  const int cnt = min(s.cell_cnt[P.coff + cell], P.bcap);
+  // the call's insertions into the cell kept to the end
+  int nl = 0;
+  for (int kk = gb_list_head(s, P, cell); kk >= 0; kk = s.gi_next[off + kk]) {
+    if (s.gi_del[off + kk] == 0x7fffffff) {
+      if (lane == 0) lst[nl] = kk;
+      nl++;
+    }
+  }
+  __syncthreads();
+  // newest first: an insertion goes after those of higher rank
+  for (int q = lane; q < nl; q += WAVE) {
+    const int kk = lst[q];
+    int pos = 0;
+    for (int q2 = 0; q2 < nl; q2++) pos += lst[q2] > kk;
+    const int i = gb_ins_slot(s, P, kk);
+    const int tp = s.gi_trial[off + kk];
+    cj[pos] = i;
+    cxs[pos] = s.gb_pt[3 * (off + tp)];
+    cys[pos] = s.gb_pt[3 * (off + tp) + 1];
+    czs[pos] = s.gb_pt[3 * (off + tp) + 2];
+    cf[pos] = s.pf4[i];
+  }
+  // then the chain of the start without the deleted atoms, in order
+  int nk = nl;
+  for (int q0 = 0; q0 < cnt; q0 += WAVE) {
+    const int q = q0 + lane;
+    int j = -1, kp = 0;
+    if (q < cnt) {
+      j = s.bucket[base + q];
+      kp = gb_del_time(s, P, j) == 0x7fffffff;
+    }
+    const unsigned long long m = __ballot(kp);
+    if (kp) {
+      const int dst = nk + __popcll(m & ((1ull << lane) - 1ull));
+      cj[dst] = j;
+      cxs[dst] = s.bx[base + q];
+      cys[dst] = s.by[base + q];
+      czs[dst] = s.bz[base + q];
+      cf[dst] = s.bf4[base + q];
+    }
+    nk += __popcll(m);
+  }
+  __syncthreads();
+  for (int q = lane; q < nk; q += WAVE) {
+    s.bucket[base + q] = cj[q];
+    s.bx[base + q] = cxs[q];
+    s.by[base + q] = cys[q];
+    s.bz[base + q] = czs[q];
+    s.bf4[base + q] = cf[q];
+  }
+  if (lane == 0) s.cell_cnt[P.coff + cell] = nk;
 }
 
 // ------------------------------------------------------------------------

@@ -17,6 +17,8 @@ BROWNIAN, ERMAK = 0, 1
 PISTON, CHUNKS, GCMC, NO_RESERVOIR = 0, 1, 2, 3
 FP64, FP32_FORCE = 0, 1
 OVERLAP_EXACT, OVERLAP_ROUNDS = 0, 1
+GCMC_BATCHED, GCMC_ALTERNATE = 1, 2
+GCMC_BATCH_MAX = 8192  # DML_GCMC_BATCH_MAX
 FL_ALIVE, FL_LIMBO, FL_REF, FL_SKIP, FL_GCMC = 1, 2, 4, 8, 16
 
 STATUS = {0: "ok", -1: "bad argument", -2: "HIP runtime error", -3: "out of device memory",
@@ -33,7 +35,7 @@ EXPORTS = ["dml_create", "dml_create_ensemble", "dml_destroy", "dml_sync", "dml_
            "dml_overlap_moveback", "dml_overlap_rounds", "dml_f_to_cg", "dml_gcmc",
            "dml_gcmc_trial_energy", "dml_gcmc_trial_energy_noalloc", "dml_run", "dml_get_counters", "dml_list_stats",
            "dml_last_run_ms", "dml_time_kernel", "dml_set_tuning",
-           "dml_replica_status", "dml_get_gcmc_order", "dml_get_slots",
+           "dml_replica_status", "dml_get_gcmc_order", "dml_gcmc_outcomes", "dml_get_slots",
            "dml_calc_rho", "dml_observables", "dml_maxz", "dml_end_step",
            "dml_get_tuning", "dml_reconfigure", "dml_set_step_limit", "dml_set_active",
            "dml_progress", "dml_set_chunk", "dml_bloques"]
@@ -56,7 +58,8 @@ class Config(C.Structure):
                 ("integrator", C.c_int32), ("reservoir", C.c_int32), ("precision", C.c_int32),
                 ("overlap_mode", C.c_int32), ("capacity", C.c_int32),
                 ("max_neighbors", C.c_int32), ("seed", C.c_uint64), ("act", C.c_double),
-                ("nadj", C.c_int32), ("lanes_per_atom", C.c_int32)]
+                ("nadj", C.c_int32), ("lanes_per_atom", C.c_int32),
+                ("gcmc_mode", C.c_int32), ("reserved", C.c_int32)]
 
 
 class Counters(C.Structure):
@@ -103,6 +106,7 @@ def lib():
         L.dml_f_to_cg.argtypes = [vp]
         L.dml_gcmc.argtypes = [vp, C.c_int32, C.c_double, C.c_int32, vp, vp]
         L.dml_gcmc_trial_energy.argtypes = [vp, C.c_int32, C.c_int32, vp, vp, vp, vp]
+        L.dml_gcmc_outcomes.argtypes = [vp, C.c_int32, C.c_int32, vp, vp, vp, vp, vp]
         L.dml_gcmc_trial_energy_noalloc.argtypes = [vp, C.c_int32, C.c_int32, vp, vp, vp, vp]
         L.dml_run.argtypes = [vp, C.c_int32, C.c_int32]
         L.dml_get_counters.argtypes = [vp, C.c_int32, C.POINTER(Counters)]
@@ -140,7 +144,7 @@ def _chk(st, where):
 def dana_config(box, skin, capacity, h=1e-2, prob=1.0, z0=100.0, zmax=200.0, dif_sc=250.0,
                 dif_sei=250.0, integrator=ERMAK, reservoir=PISTON, precision=FP64,
                 overlap_mode=OVERLAP_EXACT, seed=104012, rcut=3.2, max_neighbors=0, act=0.0,
-                nadj=0, lanes_per_atom=0):
+                nadj=0, lanes_per_atom=0, gcmc_mode=0):
     """A Config with dana's constants (dana.F90:12-17, 86-100, 113)."""
     c = Config()
     c.box[:] = list(box)
@@ -161,6 +165,7 @@ def dana_config(box, skin, capacity, h=1e-2, prob=1.0, z0=100.0, zmax=200.0, dif
     c.max_neighbors = max_neighbors
     c.seed = seed
     c.act, c.nadj, c.lanes_per_atom = act, nadj, lanes_per_atom
+    c.gcmc_mode = gcmc_mode
     return c
 
 
@@ -390,6 +395,18 @@ class Context:
         fn = "dml_gcmc_trial_energy_noalloc" if noalloc else "dml_gcmc_trial_energy"
         _chk(getattr(self.L, fn)(self.h, rep, t, _p(p), _p(ex), _p(du), _p(hard)), fn)
         return du[:t], hard[:t]
+
+    def gcmc_outcomes(self, ntrial, rep=0):
+        """the trials of the last batched gcmc call: dict of kind, accepted,
+        slot, du, or None when that call ran trial by trial"""
+        kind = np.zeros(max(ntrial, 1), np.int32); acc = np.zeros_like(kind)
+        slot = np.zeros_like(kind); du = np.zeros(max(ntrial, 1))
+        b = C.c_int32()
+        _chk(self.L.dml_gcmc_outcomes(self.h, rep, ntrial, _p(kind), _p(acc), _p(slot), _p(du),
+                                      C.byref(b)), "dml_gcmc_outcomes")
+        if not b.value:
+            return None
+        return dict(kind=kind[:ntrial], accepted=acc[:ntrial], slot=slot[:ntrial], du=du[:ntrial])
 
     def run(self, nsteps, graph=True):
         _chk(self.L.dml_run(self.h, nsteps, int(graph)), "dml_run")

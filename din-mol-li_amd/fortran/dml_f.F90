@@ -40,6 +40,7 @@ module dml_f
     integer(c_int64_t) :: seed
     real(c_double) :: act
     integer(c_int32_t) :: nadj, lanes_per_atom
+    integer(c_int32_t) :: gcmc_mode, reserved
   end type dml_config
 
   ! struct dml_counters
@@ -238,6 +239,8 @@ contains
     cfg%act = 0._c_double
     cfg%nadj = 0
     cfg%lanes_per_atom = 0
+    cfg%gcmc_mode = 0
+    cfg%reserved = 0
   end subroutine dml_dana_config
 
   ! the status text, as a Fortran string

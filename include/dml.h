@@ -54,6 +54,16 @@ enum { DML_BROWNIAN = 0, DML_ERMAK = 1 };
 enum { DML_PISTON = 0, DML_CHUNKS = 1, DML_GCMC = 2, DML_NO_RESERVOIR = 3 };
 enum { DML_FP64 = 0, DML_FP32_FORCE = 1 };
 enum { DML_OVERLAP_EXACT = 0, DML_OVERLAP_ROUNDS = 1 };
+/* gcmc_mode bits.  BATCHED: the nadj trials of a call are evaluated in
+ * parallel (points, hard-core neighbours, FP64 trial energies) and then
+ * committed in trial order; the result is the sequential gcmc_run's, bit for
+ * bit (calls the batch cannot take fall back to the sequential kernel).
+ * ALTERNATE: trial t inserts when t is even and deletes when t is odd
+ * instead of the ran < 0.5 coin (dana.F90:626), so a call of nadj trials is
+ * nadj/2 insertions plus nadj/2 deletions. */
+enum { DML_GCMC_BATCHED = 1, DML_GCMC_ALTERNATE = 2 };
+/* largest nadj a batched call takes (larger calls run sequentially) */
+#define DML_GCMC_BATCH_MAX 8192
 
 /* flags bits per slot */
 #define DML_FL_ALIVE 1u
@@ -82,6 +92,8 @@ typedef struct dml_config {
   double act;           /* GCMC activity (movedor.ini line 3) */
   int32_t nadj;         /* GCMC trials per step */
   int32_t lanes_per_atom; /* force/overlap row lanes (0 = tuned default) */
+  int32_t gcmc_mode;    /* DML_GCMC_* bits (0: gcmc_run as in the reference) */
+  int32_t reserved;     /* 0 */
 } dml_config;
 
 typedef struct dml_counters {
@@ -208,6 +220,15 @@ int dml_gcmc(dml_ctx *ctx, int32_t rep, double act, int32_t nadj,
 /* the gcmc group in list order (slots of rep; the template is order[0], the
  * velocity of an insertion lands on order[ngc-1]: dana.F90:660-668) */
 int dml_get_gcmc_order(dml_ctx *ctx, int32_t rep, int32_t *ngc, int32_t *order);
+/* the trials of rep's last gcmc call when it ran batched (*batched = 1;
+ * 0 when it ran trial by trial, and nothing else is written): kind (0
+ * insertion, 1 deletion), accepted (0/1), slot inserted or deleted (-1 when
+ * rejected) and the FP64 trial energy (insertion: the WCA sum at the point
+ * over the atoms of the call's start, as dml_gcmc_trial_energy; deletion:
+ * the removed atom's, at the call's start).  Arrays may be NULL.  No
+ * counterpart in the reference (gcmc_run keeps no record). */
+int dml_gcmc_outcomes(dml_ctx *ctx, int32_t rep, int32_t ntrial, int32_t *kind,
+                      int32_t *accepted, int32_t *slot, double *du, int32_t *batched);
 /* per slot: hs%b index (cell-chain order key), particle id and creation
  * number (the order of sys%alist, which salida writes); *nslots is the
  * hs%amax of igroup_attach_atom (Groups.F90:1083-1093).  Arrays may be NULL. */

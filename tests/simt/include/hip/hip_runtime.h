@@ -100,6 +100,30 @@ inline T sim_shfl_(T v, int l, int site, int op, int width = 64) {
   return sim_from<T>(::sim::wave_op(op, site, sim_bits<T>(v), l, width));
 }
 
+// DPP moves (gfx9 controls row_shr:1-15, row_bcast:15, row_bcast:31): the
+// source lane's value where the lane is written and the source exists,
+// else 0 (bound_ctrl) or `old`
+template <class T>
+inline T sim_update_dpp_(T old, T src, int ctrl, int rm, int bm, bool bc, int site) {
+  const int l = (int)::sim::lane_id(), row = l >> 4, bank = (l >> 2) & 3;
+  int from = -1;
+  if (ctrl >= 0x111 && ctrl <= 0x11f) {
+    if ((l & 15) >= ctrl - 0x110) from = l - (ctrl - 0x110);
+  } else if (ctrl == 0x142) {
+    if (row >= 1) from = row * 16 - 1;
+  } else if (ctrl == 0x143) {
+    if (row >= 2) from = 31;
+  } else {
+    abort();
+  }
+  const T v = sim_shfl_(src, from < 0 ? l : from, site, ::sim::OP_SHFL);
+  if (!((rm >> row) & 1) || !((bm >> bank) & 1)) return old;
+  if (from < 0) return bc ? T(0) : old;
+  return v;
+}
+#define __builtin_amdgcn_update_dpp(old, src, ctrl, rm, bm, bc) \
+  sim_update_dpp_((old), (src), (ctrl), (rm), (bm), (bc), __LINE__)
+
 inline unsigned __builtin_amdgcn_mbcnt_lo(unsigned m, unsigned acc) {
   const unsigned l = ::sim::lane_id();
   const unsigned mask = l >= 32 ? 0xffffffffu : ((1u << l) - 1u);
@@ -112,15 +136,16 @@ inline unsigned __builtin_amdgcn_mbcnt_hi(unsigned m, unsigned acc) {
 }
 
 inline int __popcll(unsigned long long v) { return __builtin_popcountll(v); }
+inline int __popc(unsigned v) { return __builtin_popcount(v); }
 inline int __ffsll(long long v) { return __builtin_ffsll(v); }
 inline float __int_as_float(int v) { return sim_from<float>((uint32_t)v); }
 inline int __float_as_int(float v) { return (int)(uint32_t)sim_bits<float>(v); }
 inline long long __double_as_longlong(double v) { return (long long)sim_bits<double>(v); }
 inline double __longlong_as_double(long long v) { return sim_from<double>((uint64_t)v); }
-inline void sincospi(double x, double *s, double *c) {
-  *s = sin(M_PI * x);
-  *c = cos(M_PI * x);
-}
+// out of line, both results always computed: inlined, the host compiler
+// may call sincos() at one site and cos() alone at another, and the host
+// libm's variants can differ in the last bit
+void sincospi(double x, double *s, double *c);
 
 template <class A, class B>
 inline typename std::common_type<A, B>::type min(A a, B b) {
@@ -155,6 +180,12 @@ template <class T, class U>
 inline T atomicMin(T *p, U v) {
   T o = *p;
   if ((T)v < o) *p = (T)v;
+  return o;
+}
+template <class T, class U>
+inline T atomicCAS(T *p, U cmp, U v) {
+  T o = *p;
+  if (o == (T)cmp) *p = (T)v;
   return o;
 }
 template <class T, class U>

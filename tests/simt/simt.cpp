@@ -374,6 +374,10 @@ hipError_t hipStreamDestroy(hipStream_t st) {
 hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
 hipError_t hipDeviceSynchronize() { return hipSuccess; }
 hipError_t hipGetLastError() { return hipSuccess; }
+void sincospi(double x, double *s, double *c) {
+  *s = sin(M_PI * x);
+  *c = cos(M_PI * x);
+}
 hipError_t hipGetDevice(int *dev) {
   *dev = 0;
   return hipSuccess;

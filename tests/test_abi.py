@@ -31,5 +31,23 @@ def test_strerror_is_host_only():
 
 def test_config_struct_matches_header_layout():
     import dml
-    # dml_config: 3+1+1+9+9+3+3+3+3 doubles, 6 int32, u64, double, 2 int32
-    assert ctypes.sizeof(dml.Config) == 8 * 35 + 4 * 6 + 8 + 8 + 4 * 2
+    # dml_config: 3+1+1+9+9+3+3+3+3 doubles, 6 int32, u64, double, 4 int32
+    assert ctypes.sizeof(dml.Config) == 8 * 35 + 4 * 6 + 8 + 8 + 4 * 4
+
+
+def test_config_layout_equals_c_compiler(tmp_path):
+    """the ctypes mirror against gcc's own layout of include/dml.h"""
+    import dml
+    import subprocess
+    src = tmp_path / "lay.c"
+    src.write_text('#include <stddef.h>\n#include <stdio.h>\n#include "dml.h"\n'
+                   'int main(void) { printf("%zu %zu %zu %zu\\n", sizeof(dml_config), '
+                   'offsetof(dml_config, seed), offsetof(dml_config, nadj), '
+                   'offsetof(dml_config, gcmc_mode)); return 0; }\n')
+    exe = tmp_path / "lay"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True, timeout=60)
+    got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True,
+                                          check=True).stdout.split()]
+    C = dml.Config
+    assert got == [ctypes.sizeof(C), C.seed.offset, C.nadj.offset, C.gcmc_mode.offset]
