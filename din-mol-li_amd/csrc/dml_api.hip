@@ -86,8 +86,10 @@ struct dml_ctx {
   int lpa_ov = 8;  // lanes per atom of the overlap scan (1 = thread per atom)
   // list kernel: 0 cell tiles, 1 wave per atom, 2 per replica by density
   // (0 or 1), 3 wave per cell
-  int lb_mode = 3;
-  int any_lb[3] = {0, 0, 1};  // some replica uses list kernel 0 / 1 / 2 (wave per cell)
+  int lb_mode = 4;
+  // some replica uses list kernel 0 (cell tiles) / 1 (wave per atom) / 2
+  // (wave per cell) / 3 (wave per cell, candidates in registers)
+  int any_lb[4] = {0, 0, 1, 0};
   int rounds = 16;  // overlap rounds per step (the C3 ensemble resolves in < 16)
   int any_ermak = 0, any_brown = 0, any_rounds = 0, any_exact = 0, any_piston = 0, any_gcmc = 0;
   int any_chunks = 0;
@@ -382,7 +384,7 @@ void enq_listbuild(dml_ctx *c) {
   if (c->d.n_tm > 0)
     hipLaunchKernelGGL(k_listbuild_cells, dim3(clamp_grid(c->d.n_tm)), dim3(DML_BLOCK), 0, c->st,
                        c->d);
-  if (c->d.n_wm > 0)
+  if (c->any_lb[1] && c->d.n_wm > 0)
     hipLaunchKernelGGL(k_listbuild<4>, dim3((unsigned)(c->d.n_wm * (DML_BLOCK / 4))),
                        dim3(4 * 64), 0, c->st, c->d);
   if (c->any_lb[2]) {
@@ -391,13 +393,21 @@ void enq_listbuild(dml_ctx *c) {
     g = std::max(8u, g / 8 * 8);
     hipLaunchKernelGGL(k_listbuild_wc, dim3(g), dim3(DML_BLOCK), 0, c->st, c->d);
   }
+  if (c->any_lb[3]) {
+    unsigned g = clamp_grid((c->d.n_cm + 3) / 4);
+    g = std::max(8u, g / 8 * 8);
+    if (c->lb_mode == 5)
+      hipLaunchKernelGGL(k_listbuild_wr<6>, dim3(g), dim3(DML_BLOCK), 0, c->st, c->d);
+    else
+      hipLaunchKernelGGL(k_listbuild_wr<4>, dim3(g), dim3(DML_BLOCK), 0, c->st, c->d);
+  }
 }
 
 // the list kernel of each replica (host mirror; the caller uploads): cell
 // tiles pay off from a few atoms per cell, below that one wave per atom
 // wastes less on staging empty neighbour cells
 void assign_list_kernels(dml_ctx *c) {
-  c->any_lb[0] = c->any_lb[1] = c->any_lb[2] = 0;
+  c->any_lb[0] = c->any_lb[1] = c->any_lb[2] = c->any_lb[3] = 0;
   for (auto &h : c->reps) {
     RepParams &p = h.p;
     int kind = c->lb_mode;
@@ -406,6 +416,8 @@ void assign_list_kernels(dml_ctx *c) {
       kind = (p.use_cells && (double)p.nslots / cells >= 4.0) ? 0 : 1;
     } else if (kind == 3) {
       kind = 2;
+    } else if (kind >= 4) {
+      kind = 3;
     }
     p.lb_kind = kind;
     c->any_lb[kind] = 1;
@@ -1728,8 +1740,8 @@ int dml_run(dml_ctx *c, int32_t nsteps, int32_t use_graph) {
                         (int64_t)c->any_chunks << 31 |
                         (int64_t)c->rounds << 6 | (int64_t)c->lpa << 12 |
                         (int64_t)c->lpa_ov << 19 | (int64_t)c->lb_mode << 26 |
-                        (int64_t)c->any_lb[0] << 28 | (int64_t)c->any_lb[1] << 29 |
-                        (int64_t)c->any_lb[2] << 30;
+                        (int64_t)c->any_lb[0] << 32 | (int64_t)c->any_lb[1] << 33 |
+                        (int64_t)c->any_lb[2] << 34 | (int64_t)c->any_lb[3] << 35;
   // a captured step holds the device state by value: any reallocation or
   // resize (rows, cells, work lists, chunk pool) needs a new capture
   uint64_t dsig = 1469598103934665603ull;
@@ -1821,7 +1833,7 @@ int dml_set_tuning(dml_ctx *c, int32_t lanes_per_atom, int32_t overlap_rounds,
   // the overlap scan is instantiated for 1, 2, 4, 8 and 16 lanes
   auto ov_ok = [](int v) { return v == 1 || v == 2 || v == 4 || v == 8 || v == 16; };
   if ((lanes_per_atom && !pow2_le64(lanes_per_atom)) || (overlap_lanes && !ov_ok(overlap_lanes)) ||
-      overlap_rounds < 0 || overlap_rounds > kMaxRounds || list_kernel < 0 || list_kernel > 4)
+      overlap_rounds < 0 || overlap_rounds > kMaxRounds || list_kernel < 0 || list_kernel > 6)
     return DML_E_ARG;
   if (lanes_per_atom) c->lpa = lanes_per_atom;
   if (overlap_rounds) c->rounds = overlap_rounds;

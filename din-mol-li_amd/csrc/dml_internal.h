@@ -263,6 +263,12 @@ __constant__ static const int8_t kMap[27][3] = {
 
 // cell_pbc with mic (Cells.F90:387-391), C '%' == Fortran mod here
 __device__ inline int cell_wrap(int r, int n) { return ((r - 1 + n) % n) + 1; }
+// the same for r in [-1, n + 2] (a stencil step from any reserved cell,
+// ghost layer included) without an integer division
+__device__ inline int cell_wrap1(int r, int n) {
+  const int w = r + (r < 1 ? n : 0);
+  return n == 1 ? 1 : w - (w > n ? n : 0);
+}
 
 __device__ inline int cell_lin(const RepParams &p, int i, int j, int k) {
   return i + (p.ncells[0] + 2) * (j + (p.ncells[1] + 2) * k);

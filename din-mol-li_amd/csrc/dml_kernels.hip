@@ -569,6 +569,33 @@ __global__ void __launch_bounds__(DML_BLOCK) k_listbuild_cells(DevState s) {
 // within the list radius (a pair whose images disagree is more than 1.5
 // cells apart either way), so the pair test needs no image arithmetic.
 // ------------------------------------------------------------------------
+// a wave-uniform value read into scalar registers
+__device__ inline int rfl_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ inline float rfl_f(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ inline int64_t rfl_l(int64_t v) {
+  const int lo = __builtin_amdgcn_readfirstlane((int)(v & 0xffffffff));
+  const int hi = __builtin_amdgcn_readfirstlane((int)(v >> 32));
+  return (int64_t)(uint32_t)lo | ((int64_t)hi << 32);
+}
+__device__ inline double rfl_d(double v) { return __longlong_as_double(rfl_l(__double_as_longlong(v))); }
+
+// inclusive prefix sum over the wave by DPP moves: row_shr 1, 2, 4, 8
+// inside each row of 16 lanes, then row_bcast 15 and 31 carry the row
+// totals (rows 1, 3 and rows 2, 3); a lane with no source adds 0.  No lane
+// index registers (a shuffle's addresses, hoisted by the compiler, would
+// hold VGPRs through the kernel)
+__device__ inline int wave_scan_incl(int v) {
+  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);
+  return v;
+}
+
 __device__ inline float rdlane_f(float v, int a) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), a));
 }
@@ -840,6 +867,389 @@ k_listbuild_wc(DevState s) {
     atomicAdd(&s.cnt[cur_r].nn_rows, acc_rows);
   }
 }
+
+// ------------------------------------------------------------------------
+// Rows of one centre cell with its candidates held in registers: a wave
+// loads up to NCH chunks of 64 candidates of the 27-cell stream once,
+// then walks the cell's ref atoms one at a time; each atom's count, row
+// address and position stay in scalar registers across the chunks, and a
+// chunk's hits are one compare, one ballot and one store of the chunk's
+// lanes in stream order (stencil order, then chain order: ngroup_cells,
+// Neighbor.F90:509-535, strict '<'; ngroup_verlet, ':358-424', rejects on
+// '>').  FP32 prefilter with margin, FP64 in the band.  Entries within
+// rcut (FP32 with margin) are listed like k_listbuild_wc lists them.
+// ------------------------------------------------------------------------
+template <int NCH>
+__global__ void __launch_bounds__(DML_BLOCK) __attribute__((amdgpu_waves_per_eu(5)))
+k_listbuild_wr(DevState s) {
+  __shared__ int32_t lbr_pos[DML_BLOCK / WAVE][NCH * WAVE];
+  int32_t *wpos = lbr_pos[threadIdx.x / WAVE];
+  const int lane = threadIdx.x % WAVE;
+  const int maxnn = s.maxnn;
+  const int kofs = lane < 27 ? (kMap[lane][0] + 1) | (kMap[lane][1] + 1) << 4 |
+                                   (kMap[lane][2] + 1) << 8
+                             : 0x111;
+  // XCD-aware runs of cells, as in k_listbuild_wc
+  const int nb = gridDim.x;  // a multiple of 8
+  const int64_t vb = (int64_t)(blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8;
+  const int64_t wid =
+      vb * (DML_BLOCK / WAVE) + __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+  // (32-bit division: the cell list holds fewer than 2^31 cells)
+  const uint32_t nwv = (uint32_t)nb * (DML_BLOCK / WAVE);
+  const int64_t run = (int64_t)(((uint32_t)s.n_cm + nwv - 1u) / nwv);
+  const int64_t g_end = min(s.n_cm, (wid + 1) * run);
+  int cur_r = -1;
+  unsigned long long acc_nn = 0, acc_rows = 0;
+  // the current replica's parameters, read once per replica (the parameter
+  // block is written by this kernel's rare error paths, so its fields are
+  // vector loads; read in the cell loop they would be a chain of waits)
+  int64_t q_boff = 0;
+  int q_cells = 0, q_n0 = 0, q_n1 = 0, q_n2 = 0, q_coff = 0, q_bcap = 0, q_img = 0;
+  double q_rl2 = 0.0;
+  float q_lo = 0.f, q_hi = 0.f, q_rcf = 0.f, q_bx = 0.f, q_by = 0.f, q_ibx = 0.f, q_iby = 0.f;
+  float q_bandw = 0.f;
+  for (int64_t g0 = wid * run; g0 < g_end; g0 += WAVE) {
+    // 64 cells of the run, one per lane: count, coordinates (y, z packed,
+    // the ghost-layer flag in bit 31) and the image reference point, so
+    // the cell loop below needs no division and no parameter reads
+    int lr = 0, lc = 0, lcnt = 0, lx = 0, lyz = 0;
+    float lmx = 0.f, lmy = 0.f;
+    if (g0 + lane < g_end) {
+      lr = s.cm_rep[g0 + lane];
+      lc = s.cm_cell[g0 + lane];
+      const RepParams &Q = s.rp[lr];
+      if (!Q.halted && Q.rebuild && Q.lb_kind == 3 && lc < ncell_total(Q)) {
+        lcnt = min(s.cell_cnt[Q.coff + lc], Q.bcap);
+        if (Q.use_cells) {
+          const int w0 = Q.ncells[0] + 2, w1 = Q.ncells[1] + 2;
+          const int x = lc % w0, y = lc / w0 % w1, z = lc / (w0 * w1);
+          const int gh = x < 1 || y < 1 || z < 1 || x > Q.ncells[0] || y > Q.ncells[1] ||
+                         z > Q.ncells[2];
+          lx = x;
+          lyz = y | z << 16 | gh << 31;
+          lmx = (float)((x - 0.5) * Q.cell[0]);
+          lmy = (float)((y - 0.5) * Q.cell[1]);
+        }
+      }
+    }
+    unsigned long long work = __ballot(lcnt > 0);
+    while (work) {
+      const int q = __builtin_ctzll(work);
+      work &= work - 1;
+      const int r = __builtin_amdgcn_readlane(lr, q);
+      const int c = __builtin_amdgcn_readlane(lc, q);
+      const int ccnt = __builtin_amdgcn_readlane(lcnt, q);
+      const int cx0 = __builtin_amdgcn_readlane(lx, q);
+      const int cyz = __builtin_amdgcn_readlane(lyz, q);
+      const float mxf = rdlane_f(lmx, q), myf = rdlane_f(lmy, q);
+      const RepParams &P = s.rp[r];
+      if (r != cur_r) {
+        if (cur_r >= 0 && lane == 0 && acc_rows) {
+          atomicAdd(&s.cnt[cur_r].nn_sum, acc_nn);
+          atomicAdd(&s.cnt[cur_r].nn_rows, acc_rows);
+        }
+        cur_r = r;
+        acc_nn = acc_rows = 0;
+        q_boff = rfl_l(P.boff);
+        q_cells = rfl_i(P.use_cells);
+        q_n0 = rfl_i(P.ncells[0]);
+        q_n1 = rfl_i(P.ncells[1]);
+        q_n2 = rfl_i(P.ncells[2]);
+        q_coff = rfl_i(P.coff);
+        q_bcap = rfl_i(P.bcap);
+        q_img = q_cells && q_n0 >= 4 && q_n1 >= 4;
+        q_rl2 = rfl_d(P.rl2);
+        const float pfm = rfl_f(P.pf_margin);
+        q_lo = (float)q_rl2 - pfm;
+        q_hi = (float)q_rl2 + pfm;
+        q_rcf = (float)rfl_d(P.rcut2) + pfm;
+        q_bx = (float)rfl_d(P.box[0]);
+        q_by = (float)rfl_d(P.box[1]);
+        q_ibx = (float)rfl_d(P.one_box[0]);
+        q_iby = (float)rfl_d(P.one_box[1]);
+        // a pair is near the list radius when |d^2 - rl^2| <= bandw in
+        // FP32: a superset of [rl2lo, rl2hi] (the difference of two floats
+        // this close is exact; rl2lo, rl2hi are within half an ulp of
+        // rl^2 -+ margin)
+        q_bandw = pfm * 1.0001f + 1.0e-6f * (float)q_rl2;
+      }
+      const int64_t cbase = q_boff + (int64_t)c * q_bcap;
+      int ghost = 0, ci0 = 0, ci1 = 0, ci2 = 0;
+      if (q_cells) {
+        ci0 = cx0;
+        ci1 = cyz & 0xffff;
+        ci2 = cyz >> 16 & 0x7fff;
+        ghost = (uint32_t)cyz >> 31;
+      }
+      // the first 64 centre atoms' entries, loaded with the stencil counts
+      // (one wait for both)
+      int my_i = -1;
+      float my_x = 0.f, my_y = 0.f, my_z = 0.f;
+      if (lane < ccnt) {
+        my_i = s.bucket[cbase + lane];
+        const float4 f = s.bf4[cbase + lane];
+        my_x = f.x;
+        my_y = f.y;
+        my_z = f.z;
+      }
+      // the 27 stencil cells: lane k holds count and first entry of cell k
+      int kcnt = 0;
+      int64_t kbase = 0;
+      if (q_cells) {
+        if (lane < 27) {
+          const int n0 = cell_wrap1(ci0 + (kofs & 15) - 1, q_n0);
+          const int n1 = cell_wrap1(ci1 + (kofs >> 4 & 15) - 1, q_n1);
+          const int n2 = cell_wrap1(ci2 + (kofs >> 8 & 15) - 1, q_n2);
+          const int cell = n0 + (q_n0 + 2) * (n1 + (q_n1 + 2) * n2);
+          kcnt = min(s.cell_cnt[q_coff + cell], q_bcap);
+          kbase = q_boff + (int64_t)cell * q_bcap;
+        }
+      } else if (lane == 0) {
+        kcnt = ccnt;
+        kbase = q_boff;
+      }
+      const int incl = wave_scan_incl(kcnt);
+      const int kpre = incl - kcnt;
+      // (read from lane 26 into a scalar register: the compiler then keeps
+      // the chunk loops' bounds and masks uniform)
+      const int total = __builtin_amdgcn_readlane(incl, 26);
+      const double rl2 = q_rl2;
+      const int strict = q_cells;
+      const float rl2lo = q_lo, rl2hi = q_hi, rcf = q_rcf;
+      const float bxf = q_bx, byf = q_by, ibx = q_ibx, iby = q_iby;
+      const int img = q_img;
+      // this stencil lane's first bucket entry, relative to the replica's
+      const int kb32 = (int)(kbase - q_boff);
+      for (int a0 = 0; a0 < ccnt; a0 += WAVE) {
+        // centre atoms a0 .. a0+63 of this cell, one per lane
+        const int na = (ccnt - a0) < WAVE ? (ccnt - a0) : WAVE;
+        const int64_t e0 = cbase + a0;
+        if (a0 > 0) {
+          // the next 64 centre atoms
+          my_i = -1;
+          if (lane < na) {
+            my_i = s.bucket[e0 + lane];
+            const float4 f = s.bf4[e0 + lane];
+            my_x = f.x;
+            my_y = f.y;
+            my_z = f.z;
+          }
+        }
+        const int my_ref = lane < na && (s.flags[my_i] & DML_FL_REF) != 0;
+        const unsigned long long refm = __ballot(my_ref);
+        if (!refm) continue;
+        // row length (bits 0-23) and entries within rcut (24-31, saturating)
+        int my_nn = 0;
+        for (int s0 = 0; s0 < total; s0 += NCH * WAVE) {
+          // (the previous super-chunk's reads of the slice are done)
+          __builtin_amdgcn_wave_barrier();
+          const int nch = min(NCH, (total - s0 + WAVE - 1) / WAVE);
+          const int last = min(total - s0, NCH * WAVE) - 1;
+          // the bucket entries of stream positions s0 .. s0+last, in stream
+          // order: each stencil lane writes those of its cell (one wave's
+          // LDS slice, written and read by that wave only, in order)
+          {
+            const int lo = max(0, s0 - kpre), hi = min(kcnt, s0 + last + 1 - kpre);
+            for (int e = lo; e < hi; e++) wpos[kpre + e - s0] = kb32 + e;
+          }
+          __builtin_amdgcn_wave_barrier();
+          // (the slots' reads first, then the loads: no wait between chunks)
+          int slot[NCH];
+#pragma unroll
+          for (int h = 0; h < NCH; h++) slot[h] = wpos[min(h * WAVE + lane, last)];
+          int cj[NCH];
+          float cx[NCH], cy[NCH], cz[NCH];
+#pragma unroll
+          for (int h = 0; h < NCH; h++) {
+            if (h < nch) {
+              const int64_t pos = q_boff + slot[h];
+              cj[h] = s.bucket[pos];
+              const float4 f = s.bf4[pos];
+              cx[h] = f.x;
+              cy[h] = f.y;
+              cz[h] = f.z;
+            }
+          }
+          if (img) {
+            // each candidate at its image nearest the centre cell's midpoint
+#pragma unroll
+            for (int h = 0; h < NCH; h++) {
+              if (h < nch) {
+                cx[h] = cx[h] - bxf * rintf((cx[h] - mxf) * ibx);
+                cy[h] = cy[h] - byf * rintf((cy[h] - myf) * iby);
+              }
+            }
+          }
+          // lanes of the last chunk that hold a stream entry
+          const int nl = last + 1 - (nch - 1) * WAVE;
+          const unsigned long long vlast = nl >= WAVE ? ~0ull : ((1ull << nl) - 1ull);
+          // the candidates are in registers before the centre loop starts
+          // (its row stores would otherwise be waited for at each chunk)
+          __builtin_amdgcn_s_waitcnt(0);
+          const int my_nn0 = my_nn;
+          // centres with a pair near the list radius: their rows of this
+          // super-chunk are written again below, the band decided in FP64
+          unsigned long long redo = 0;
+          const float rl2f = (float)rl2, bandw = q_bandw;
+          auto d2f = [&](auto img_c, float x, float y, float z, float xa, float ya,
+                         float za) -> float {
+            if (decltype(img_c)::value) {
+              const float dx = x - xa, dy = y - ya, dz = z - za;
+              return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+            }
+            return d2_f32(P, make_float4(x, y, z, 0.f), make_float4(xa, ya, za, 0.f), bxf, byf,
+                          ibx, iby);
+          };
+          // the centre loop, one form per pair-distance arithmetic
+          // one centre atom's walk over the chunks: its row count, near
+          // count and row address in scalar registers
+          struct Ctr {
+            int a, i, nn, nc;
+            float xa, ya, za, bmin;
+            int32_t *row;
+          };
+          auto start = [&](Ctr &c, int a) {
+            c.a = a;
+            c.i = __builtin_amdgcn_readlane(my_i, a);
+            c.xa = rdlane_f(my_x, a);
+            c.ya = rdlane_f(my_y, a);
+            c.za = rdlane_f(my_z, a);
+            const int pk = __builtin_amdgcn_readlane(my_nn, a);
+            c.nn = pk & 0xffffff;
+            c.nc = (int)((uint32_t)pk >> 24);
+            c.row = s.nbr + (int64_t)c.i * maxnn;
+            // the least |d^2 - rl^2| this lane meets (lanes past the
+            // stream's end repeat its last entry)
+            c.bmin = 3.0e38f;
+          };
+          auto step = [&](auto img_c, Ctr &c, int h) {
+            const unsigned long long vm = h == nch - 1 ? vlast : ~0ull;
+            const float df = d2f(img_c, cx[h], cy[h], cz[h], c.xa, c.ya, c.za);
+            c.bmin = fminf(c.bmin, fabsf(df - rl2f));
+            const unsigned long long lt = __builtin_amdgcn_ballot_w64(df < rl2lo);
+            const unsigned long long hm = lt & vm & __builtin_amdgcn_ballot_w64(cj[h] != c.i);
+            const int nh = __popcll(hm);
+            const uint32_t pr = __builtin_amdgcn_mbcnt_hi(
+                (uint32_t)(hm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)hm, 0u));
+            const unsigned long long ok =
+                c.nn + nh <= maxnn ? hm : hm & __builtin_amdgcn_ballot_w64(c.nn + (int)pr < maxnn);
+            if (__builtin_amdgcn_inverse_ballot_w64(ok)) c.row[(uint32_t)c.nn + pr] = cj[h];
+            const unsigned long long nm = __builtin_amdgcn_ballot_w64(df <= rcf) & hm;
+            if (nm) {
+              const int qn = c.nc + (int)__builtin_amdgcn_mbcnt_hi(
+                                        (uint32_t)(nm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)nm, 0u));
+              if (__builtin_amdgcn_inverse_ballot_w64(nm) && qn < NEAR_MAX)
+                s.nearl[(int64_t)c.i * NEAR_MAX + qn] = c.nn + (int)pr;
+              const int nc2 = c.nc + __popcll(nm);
+              c.nc = nc2 < 255 ? nc2 : 255;
+            }
+            c.nn += nh;
+          };
+          auto finish = [&](Ctr &c) {
+            if (__builtin_amdgcn_ballot_w64(c.bmin <= bandw)) redo |= 1ull << c.a;
+            my_nn = lane == c.a ? (c.nn | c.nc << 24) : my_nn;
+          };
+          // the centre loop, one form per pair-distance arithmetic
+          auto centres = [&](auto img_c) {
+            unsigned long long m = refm;
+            while (m) {
+              Ctr A;
+              start(A, __builtin_ctzll(m));
+              m &= m - 1;
+#pragma unroll
+              for (int h = 0; h < NCH; h++)
+                if (h < nch) step(img_c, A, h);
+              finish(A);
+            }
+          };
+          if (img)
+            centres(std::true_type());
+          else
+            centres(std::false_type());
+          // the band, in FP64, for the centres that met it: their rows of
+          // this super-chunk from its start, candidates read again
+          while (redo) {
+            const int a = __builtin_ctzll(redo);
+            redo &= redo - 1;
+            const int i = __builtin_amdgcn_readlane(my_i, a);
+            const float xa = rdlane_f(my_x, a), ya = rdlane_f(my_y, a), za = rdlane_f(my_z, a);
+            const int pk = __builtin_amdgcn_readlane(my_nn0, a);
+            int nn = pk & 0xffffff, nc = (int)((uint32_t)pk >> 24);
+            int32_t *row = s.nbr + (int64_t)i * maxnn;
+            for (int h = 0; h < nch; h++) {
+              const int cidx = s0 + h * WAVE + lane;
+              const int64_t pos = q_boff + wpos[min(h * WAVE + lane, last)];
+              const int j = s.bucket[pos];
+              const float4 f = s.bf4[pos];
+              float fx = f.x, fy = f.y;
+              if (img) {
+                fx = f.x - bxf * rintf((f.x - mxf) * ibx);
+                fy = f.y - byf * rintf((f.y - myf) * iby);
+              }
+              const float df = img ? d2f(std::true_type(), fx, fy, f.z, xa, ya, za)
+                                   : d2f(std::false_type(), fx, fy, f.z, xa, ya, za);
+              const bool valid = cidx < total;
+              bool hit = valid && df < rl2lo;
+              if (valid && !hit && df <= rl2hi) {
+                double ex, ey, ez;
+                const uint32_t uj = (uint32_t)j;
+                vdist_mic(P, s.px[uj], s.py[uj], s.pz[uj], s.bx[e0 + a], s.by[e0 + a],
+                          s.bz[e0 + a], &ex, &ey, &ez);
+                const double rd = dot3(ex, ey, ez);
+                hit = strict ? (rd < rl2) : !(rd > rl2);
+              }
+              hit = hit && j != i;
+              const unsigned long long hm = __builtin_amdgcn_ballot_w64(hit);
+              if (hm) {
+                const int nh = __popcll(hm);
+                const uint32_t pr = __builtin_amdgcn_mbcnt_hi(
+                    (uint32_t)(hm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)hm, 0u));
+                if (hit && nn + (int)pr < maxnn) row[(uint32_t)nn + pr] = j;
+                const bool near = hit && df <= rcf;
+                const unsigned long long nm = __builtin_amdgcn_ballot_w64(near);
+                if (nm) {
+                  const int qn = nc + (int)__builtin_amdgcn_mbcnt_hi(
+                                          (uint32_t)(nm >> 32),
+                                          __builtin_amdgcn_mbcnt_lo((uint32_t)nm, 0u));
+                  if (near && qn < NEAR_MAX) s.nearl[(int64_t)i * NEAR_MAX + qn] = nn + (int)pr;
+                  const int nc2 = nc + __popcll(nm);
+                  nc = nc2 < 255 ? nc2 : 255;
+                }
+                nn += nh;
+              }
+            }
+            my_nn = lane == a ? (nn | nc << 24) : my_nn;
+          }
+        }
+        const int my_nc = (int)((uint32_t)my_nn >> 24);
+        my_nn &= 0xffffff;
+        if (my_ref) {
+          int nn = my_nn;
+          if (nn > maxnn) {
+            atomicOr(s.err, ERR_LIST);
+            atomicMax(&s.rp[r].max_nn_seen, nn);
+            rep_fail(s, r, ERR_LIST);
+            nn = maxnn;
+          }
+          s.nn[my_i] = nn;
+          s.rowflag[my_i] = (uint8_t)ghost;
+          s.ov_near[my_i] = (uint8_t)(img && my_nc <= NEAR_MAX ? my_nc : 255);
+          if (ghost) s.rp[r].any_ghost = 1;
+        }
+        const int v = wave_scan_incl(my_ref ? my_nn : 0);
+        acc_nn += (unsigned long long)(uint32_t)__builtin_amdgcn_readlane(v, WAVE - 1);
+        acc_rows += (unsigned long long)__popcll(refm);
+      }
+    }
+  }
+  if (cur_r >= 0 && lane == 0 && acc_rows) {
+    atomicAdd(&s.cnt[cur_r].nn_sum, acc_nn);
+    atomicAdd(&s.cnt[cur_r].nn_rows, acc_rows);
+  }
+}
+
+template __global__ void k_listbuild_wr<4>(DevState s);
+template __global__ void k_listbuild_wr<6>(DevState s);
 
 // bucket order by a rank sort inside one wave (cells of up to 256 atoms,
 // four per lane: electrode cells hold ~100; larger cells fall back to an
@@ -3263,18 +3673,6 @@ __global__ void __launch_bounds__(DML_BLOCK) k_gb_conflicts(DevState s, int r) {
   if (lane == 0) s.gb_ncf[o] = nc > GB_NC ? GB_NC + 1 : nc;
 }
 
-// inclusive prefix sum over the wave by DPP moves: row_shr 1, 2, 4, 8
-// inside each row of 16 lanes, then row_bcast 15 and 31 carry the row
-// totals (rows 1, 3 and rows 2, 3); a lane with no source adds 0
-__device__ inline int gb_wscan(int v) {
-  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);
-  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);
-  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);
-  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);
-  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);
-  v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);
-  return v;
-}
 
 // the decisions, one wave walking the trials in order (dana.F90:621-710):
 // n, the Metropolis tests, the hard-core test against the neighbours still
@@ -3416,19 +3814,19 @@ __global__ void __launch_bounds__(WAVE) k_gb_decide(DevState s, int r) {
           // the (m-1)-th present entry: lane, chunk, word, bit (each level's
           // ballot is non-empty while the counts agree with the bitmap)
           int rem = m - 1;
-          const int incl = gb_wscan(tot);
+          const int incl = wave_scan_incl(tot);
           const unsigned long long b1 = __ballot(incl > rem);
           const int L = b1 ? __builtin_ctzll(b1) : 0;
           rem -= __builtin_amdgcn_readlane(incl - tot, L);
           const int c16 = lane < 16 ? ccnt[16 * L + lane] : 0;
-          const int incl2 = gb_wscan(c16);
+          const int incl2 = wave_scan_incl(c16);
           const unsigned long long b2 = __ballot(lane < 16 && incl2 > rem);
           const int j = b2 ? __builtin_ctzll(b2) : 0;
           rem -= __builtin_amdgcn_readlane(incl2 - c16, j);
           const int chunk = 16 * L + j;
           const uint32_t wv = lane < 32 ? vb[32 * chunk + lane] : 0u;
           const int pc1 = __popc(wv);
-          const int incl3 = gb_wscan(pc1);
+          const int incl3 = wave_scan_incl(pc1);
           const unsigned long long b3 = __ballot(lane < 32 && incl3 > rem);
           const int k = b3 ? __builtin_ctzll(b3) : 0;
           rem -= __builtin_amdgcn_readlane(incl3 - pc1, k);

@@ -27,7 +27,7 @@ import numpy as np
 from . import K_FORCE, K_LISTBUILD, Context
 
 FORCE_LANES = (2, 4, 8, 16, 32)
-LIST_KERNELS = (1, 2, 3, 4)
+LIST_KERNELS = (1, 2, 3, 4, 5, 6)
 OVERLAP_LANES = (1, 2, 4, 8, 16)
 OVERLAP_ROUNDS = (8, 16, 32)
 

@@ -285,7 +285,9 @@ int dml_time_kernel(dml_ctx *ctx, int32_t which, int32_t reps, double *avg_ms);
  * and of the overlap scan's first round (1, 2, 4, 8 or 16; later rounds use at least 2),
  * overlap rounds per step of dml_run (default 16, at most 32),
  * list kernel (1 = cell tiles through LDS, 2 = one wave per atom, 3 = 1 or 2
- * per replica by atoms per cell, 4 = one wave per cell, the default); 0
+ * per replica by atoms per cell, 4 = one wave per cell, 5 = one wave per
+ * cell with its candidates in registers, 4 chunks of 64, the default, 6 = 5
+ * with 6 chunks); 0
  * keeps the current value.  Results do not depend on them beyond FP64 summation
  * order.  No counterpart in the reference: used by the autotuner. */
 int dml_set_tuning(dml_ctx *ctx, int32_t lanes_per_atom, int32_t overlap_rounds,

@@ -90,6 +90,7 @@ inline T sim_from(uint64_t u) {
 #define __shfl_xor(v, l, ...) sim_shfl_((v), (l), __LINE__, ::sim::OP_SHFL_XOR, ##__VA_ARGS__)
 #define __builtin_amdgcn_readlane(v, l) sim_shfl_((v), (l), __LINE__, ::sim::OP_READLANE)
 #define __builtin_amdgcn_readfirstlane(v) sim_shfl_((v), 0, __LINE__, ::sim::OP_READFIRST)
+#define __builtin_amdgcn_s_waitcnt(x) ((void)(x))
 #define __builtin_amdgcn_wave_barrier() ((void)::sim::wave_op(::sim::OP_WAVEBAR, __LINE__, 0, 0, 64))
 #define __builtin_amdgcn_fence(...) ((void)0)
 #define __syncthreads() ((void)::sim::block_barrier(__LINE__, 0))
@@ -135,6 +136,9 @@ inline unsigned __builtin_amdgcn_mbcnt_hi(unsigned m, unsigned acc) {
   return acc + (unsigned)__builtin_popcount(m & mask);
 }
 
+inline bool __builtin_amdgcn_inverse_ballot_w64(unsigned long long m) {
+  return (m >> ::sim::lane_id()) & 1ull;
+}
 inline int __popcll(unsigned long long v) { return __builtin_popcountll(v); }
 inline int __popc(unsigned v) { return __builtin_popcount(v); }
 inline int __ffsll(long long v) { return __builtin_ffsll(v); }

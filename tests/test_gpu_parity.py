@@ -109,11 +109,12 @@ def assert_lists_equal(c, s):
 NB = np.load(os.path.join(G, "nb_cases.npz"))
 
 
-@pytest.mark.parametrize("lb_mode", ["0", "1", "3"])
+@pytest.mark.parametrize("lb_mode", ["0", "1", "3", "4", "5"])
 @pytest.mark.parametrize("name", list(NB["names"]))
 def test_cells_and_lists_equal_reference_modules(dml, name, lb_mode, monkeypatch):
     """lb_mode 0: cell-tile list build through LDS; 1: one wave per atom;
-    3: one wave per cell"""
+    3: one wave per cell; 4, 5: one wave per cell, candidates in registers
+    (4 or 6 chunks of 64)"""
     monkeypatch.setenv("DML_LISTBUILD", lb_mode)
     d = {k.split("/", 1)[1]: NB[k] for k in NB.files if k.startswith(name + "/")}
     say("case", name, len(d["z"]), "lb_mode", lb_mode)
@@ -374,7 +375,8 @@ def test_ermak_steps_with_round_overlap_match_oracle(orc, dml):
 @pytest.mark.parametrize("lb_mode,wide,lpa", [("3", False, "8"), ("1", False, "8"),
                                               ("0", False, "8"), ("3", True, "8"),
                                               ("1", True, "8"), ("0", True, "8"),
-                                              ("3", True, "1"), ("0", True, "2")])
+                                              ("3", True, "1"), ("0", True, "2"),
+                                              ("4", False, "8"), ("4", True, "8")])
 def test_brownian_round_overlap_after_rebuilds_matches_oracle(orc, dml, lb_mode, wide, lpa,
                                                               monkeypatch):
     """Brownian steps (h 3e-3: some first skin tests rebuild, some do not)
