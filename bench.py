@@ -164,7 +164,7 @@ def c3_configs(dml, reps, precision=None):
     return cfgs
 
 
-def batched_leg(dml, nrep, steps, warmup, full_steps=0):
+def batched_leg(dml, nrep, steps, warmup, full_steps=0, list_kernel=0):
     t0 = time.perf_counter()
     reps = build_replicas(nrep)
     log("built %d replicas in %.1f s" % (nrep, time.perf_counter() - t0))
@@ -175,6 +175,9 @@ def batched_leg(dml, nrep, steps, warmup, full_steps=0):
                                    (r["chunk"], r["dist"], r["z1"]) if "chunk" in r else None)
                                   for r in reps], log=log)
     log("autotune %.1f s%s" % (time.perf_counter() - t0, " (cached)" if choice.get("cached") else ""))
+    if list_kernel:
+        # a fixed row-build kernel (profiles of one variant against another)
+        choice = dict(choice, list_kernel=list_kernel)
     ens = dml.Context(cfgs)
     autotune.apply(ens, choice)
     ntot = 0
@@ -425,7 +428,11 @@ def calibrate_copy(nbytes=1 << 30):
     log("copy calibration: %d bytes read, %d bytes written" % (nbytes, nbytes))
 
 
-def pmc_summary(fetch_csv, write_csv, kernel="k_listbuild_wc", last=10, copy_bytes=1 << 30):
+ROW_KERNELS = {1: "k_listbuild_cells", 2: "k_listbuild<4>", 4: "k_listbuild_wc",
+               5: "k_listbuild_wr<4>", 6: "k_listbuild_wr<6>"}
+
+
+def pmc_summary(fetch_csv, write_csv, kernel="k_listbuild_wr<4>", last=10, copy_bytes=1 << 30):
     """HBM bytes per launch of `kernel` from two rocprofv3 --pmc runs of
     `bench.py --legs batched --steps 1 --warmup 0 --no-cpu --calibrate-copy`
     (FETCH_SIZE, then WRITE_SIZE; counter_collection.csv of each): the mean
@@ -436,7 +443,8 @@ def pmc_summary(fetch_csv, write_csv, kernel="k_listbuild_wc", last=10, copy_byt
 
     def read(path, counter):
         rows = [r for r in csv.DictReader(open(path)) if r["Counter_Name"] == counter]
-        mine = [float(r["Counter_Value"]) for r in rows if r["Kernel_Name"].startswith(kernel + "(")]
+        mine = [float(r["Counter_Value"]) for r in rows
+                if r["Kernel_Name"].replace("void ", "", 1).startswith(kernel + "(")]
         copy = max(float(r["Counter_Value"]) for r in rows
                    if r["Kernel_Name"].startswith("__amd_rocclr_copyBuffer"))
         return sum(mine[-last:]) / len(mine[-last:]), copy, len(mine)
@@ -450,6 +458,41 @@ def pmc_summary(fetch_csv, write_csv, kernel="k_listbuild_wc", last=10, copy_byt
             "copy_calibration": {"bytes": copy_bytes, "fetch_size_kib": fc, "write_size_kib": wc},
             "fetch_scale": fs, "write_scale": ws, "read_bytes": rd, "write_bytes": wr,
             "hbm_bytes_per_launch": rd + wr}
+
+
+def flops_summary(csv_path, last=10):
+    """FP32 / FP64 FLOP rates per kernel from one rocprofv3 --pmc run of the
+    eight SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_{F32,F64} counters: 64 x (ADD +
+    MUL + TRANS + 2 FMA) per dispatch (every lane counted) over that
+    dispatch's own duration, averaged over each kernel's last `last`
+    dispatches; peaks FP32 157.3, FP64 78.6 TFLOP/s (vector)."""
+    import csv
+    per = {}
+    for r in csv.DictReader(open(csv_path)):
+        name = r["Kernel_Name"].replace("void ", "", 1).split("(")[0]
+        d = per.setdefault(name, {}).setdefault(r["Dispatch_Id"], {"ns": 0.0})
+        d["ns"] = float(r["End_Timestamp"]) - float(r["Start_Timestamp"])
+        d[r["Counter_Name"]] = d.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+    out = {}
+    for name, ds in per.items():
+        keys = sorted(ds, key=int)[-last:]
+        f32 = f64 = ns = 0.0
+        for k in keys:
+            d = ds[k]
+            g = lambda c: d.get(c, 0.0)
+            f32 += 64 * (g("SQ_INSTS_VALU_ADD_F32") + g("SQ_INSTS_VALU_MUL_F32") +
+                         g("SQ_INSTS_VALU_TRANS_F32") + 2 * g("SQ_INSTS_VALU_FMA_F32"))
+            f64 += 64 * (g("SQ_INSTS_VALU_ADD_F64") + g("SQ_INSTS_VALU_MUL_F64") +
+                         g("SQ_INSTS_VALU_TRANS_F64") + 2 * g("SQ_INSTS_VALU_FMA_F64"))
+            ns += d["ns"]
+        if ns <= 0:
+            continue
+        t32, t64 = f32 / ns * 1e-3, f64 / ns * 1e-3
+        out[name] = {"dispatches": len(keys), "avg_ms": round(ns / len(keys) * 1e-6, 4),
+                     "fp32_tflops": round(t32, 3), "fp64_tflops": round(t64, 3),
+                     "fp32_frac_of_157_3": round(t32 / 157.3, 4),
+                     "fp64_frac_of_78_6": round(t64 / 78.6, 4)}
+    return dict(sorted(out.items(), key=lambda kv: -kv[1]["avg_ms"] * kv[1]["dispatches"]))
 
 
 def main():
@@ -469,9 +512,15 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=30.0,
                     help="arrival window of the serving leg")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--list-kernel", type=int, default=0,
+                    help="row-build kernel of the batched leg (dml_set_tuning's list_kernel; "
+                         "0 = the autotuner's choice)")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="print the per-launch HBM traffic of the row build from two "
                          "rocprofv3 --pmc counter_collection.csv files and exit")
+    ap.add_argument("--flops-summary", metavar="CSV",
+                    help="print FP32/FP64 FLOP rates per kernel from a rocprofv3 --pmc run of "
+                         "the eight SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_{F32,F64} counters and exit")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 1 GiB device copy (scale of the rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -487,8 +536,12 @@ def main():
     except Exception:
         sync = lambda: None
     legs = args.legs.split(",")
+    if args.flops_summary:
+        print(json.dumps(flops_summary(args.flops_summary), indent=1))
+        return
     if args.pmc_summary:
-        print(json.dumps(pmc_summary(*args.pmc_summary), indent=1))
+        print(json.dumps(pmc_summary(*args.pmc_summary,
+                                     kernel=ROW_KERNELS[args.list_kernel or 5]), indent=1))
         return
     plain = batched = None
     if "plain" in legs:
@@ -500,7 +553,8 @@ def main():
     if "batched" in legs:
         log("batched leg: C3 %d replicas FP32 forces" % args.replicas)
         sync()
-        batched = batched_leg(dml, args.replicas, args.steps, args.warmup, args.c3_full_steps)
+        batched = batched_leg(dml, args.replicas, args.steps, args.warmup, args.c3_full_steps,
+                              args.list_kernel)
         sync()
         log("batched", json.dumps(batched))
     serving = None
@@ -560,14 +614,19 @@ def main():
         # HBM bytes per launch from the two rocprofv3 PMC passes (FETCH_SIZE,
         # WRITE_SIZE, scaled by a 1 GiB copy) of the same 256-replica launch
         measured = None
-        tf = os.path.join(ROOT, "profiles", "k_listbuild_wc_c3_traffic.json")
+        kname = ROW_KERNELS.get(batched["tuning"]["list_kernel"], "row build")
+        tf = os.path.join(ROOT, "profiles", "%s_c3_traffic.json" % kname.replace("<", "").replace(">", ""))
         if args.replicas == 256 and os.path.exists(tf):
             measured = json.load(open(tf))["hbm_bytes_per_launch"]
-        res["roofline"] = {"kernel": "k_listbuild_wc", "bound": "hbm", "achieved": ach,
+        # priced against HBM (its compulsory bytes); what holds it back is
+        # latency: its waves wait on memory about half their cycles and
+        # issue VALU a fifth of them (instruction counters under profiles/)
+        res["roofline"] = {"kernel": kname, "bound": "hbm", "achieved": ach,
                            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
                            "traffic": measured, "avg_launch_ms": batched["list_ms"],
-                           "algorithmic_bytes_per_launch": traffic}
-        res["kernel_ms"] = {"k_listbuild_wc": batched["list_ms"], "k_force": batched["force_ms"]}
+                           "algorithmic_bytes_per_launch": traffic,
+                           "limiter": "memory latency (waits ~50% of wave cycles, VALU issue ~20%)"}
+        res["kernel_ms"] = {kname: batched["list_ms"], "k_force": batched["force_ms"]}
         res["particles_stepped_mean"] = batched["particles_stepped"]
         res["c3_full_run"] = batched["full_run"]
         res["mean_list_length"] = batched["mean_nn"]
