@@ -381,7 +381,7 @@ int alloc_cells(dml_ctx *c) {
 unsigned clamp_grid(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>(n, 65535)); }
 
 void enq_listbuild(dml_ctx *c) {
-  if (c->d.n_tm > 0)
+  if (c->any_lb[0] && c->d.n_tm > 0)
     hipLaunchKernelGGL(k_listbuild_cells, dim3(clamp_grid(c->d.n_tm)), dim3(DML_BLOCK), 0, c->st,
                        c->d);
   if (c->any_lb[1] && c->d.n_wm > 0)

@@ -20,6 +20,14 @@ __device__ inline void put_pf4(const DevState &s, int i, double x, double y, dou
 __device__ inline void put_pf4_sp(const DevState &s, int i, int sp) {
   reinterpret_cast<float *>(&s.pf4[i])[3] = __int_as_float(sp);
 }
+// the bucket-ordered FP32 mirror of slot i: position, species (bits 0-7 of
+// w) and the slot's flags at the time of writing (bits 8-15), so the row
+// build reads whether a bucket entry is a ref atom with its position
+__device__ inline float4 bucket_f4(const DevState &s, int i) {
+  float4 f = s.pf4[i];
+  f.w = __int_as_float((__float_as_int(f.w) & 0xff) | (int)s.flags[i] << 8);
+  return f;
+}
 // squared minimum-image distance in FP32.  A pair is decided in FP32 only
 // outside the band cutoff^2 +- RepParams::pf_margin (set by the host from
 // the largest box and the list radius, see pf_margin_for); inside it the FP64
@@ -254,7 +262,7 @@ __global__ void k_cell_sort(DevState s, int gcs) {
       s.bx[base + q] = s.px[v];
       s.by[base + q] = s.py[v];
       s.bz[base + q] = s.pz[v];
-      s.bf4[base + q] = s.pf4[v];
+      s.bf4[base + q] = bucket_f4(s, v);
     }
   }
 }
@@ -983,7 +991,7 @@ k_listbuild_wr(DevState s) {
       }
       // the first 64 centre atoms' entries, loaded with the stencil counts
       // (one wait for both)
-      int my_i = -1;
+      int my_i = -1, my_ref = 0;
       float my_x = 0.f, my_y = 0.f, my_z = 0.f;
       if (lane < ccnt) {
         my_i = s.bucket[cbase + lane];
@@ -991,6 +999,7 @@ k_listbuild_wr(DevState s) {
         my_x = f.x;
         my_y = f.y;
         my_z = f.z;
+        my_ref = (__float_as_int(f.w) >> 8 & DML_FL_REF) != 0;
       }
       // the 27 stencil cells: lane k holds count and first entry of cell k
       int kcnt = 0;
@@ -1027,15 +1036,16 @@ k_listbuild_wr(DevState s) {
         if (a0 > 0) {
           // the next 64 centre atoms
           my_i = -1;
+          my_ref = 0;
           if (lane < na) {
             my_i = s.bucket[e0 + lane];
             const float4 f = s.bf4[e0 + lane];
             my_x = f.x;
             my_y = f.y;
             my_z = f.z;
+            my_ref = (__float_as_int(f.w) >> 8 & DML_FL_REF) != 0;
           }
         }
-        const int my_ref = lane < na && (s.flags[my_i] & DML_FL_REF) != 0;
         const unsigned long long refm = __ballot(my_ref);
         if (!refm) continue;
         // row length (bits 0-23) and entries within rcut (24-31, saturating)
@@ -1251,6 +1261,7 @@ k_listbuild_wr(DevState s) {
 template __global__ void k_listbuild_wr<4>(DevState s);
 template __global__ void k_listbuild_wr<6>(DevState s);
 
+
 // bucket order by a rank sort inside one wave (cells of up to 256 atoms,
 // four per lane: electrode cells hold ~100; larger cells fall back to an
 // insertion sort by lane 0)
@@ -1301,7 +1312,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_cell_sort_w(DevState s, int gcs) 
         x = s.px[v];
         y = s.py[v];
         z = s.pz[v];
-        f = s.pf4[v];
+        f = bucket_f4(s, v);
       }
       int rank = 0;
       const int cmax = cnt > cnt2 ? cnt : cnt2;
@@ -1338,7 +1349,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_cell_sort_w(DevState s, int gcs) 
         x = s.px[v];
         y = s.py[v];
         z = s.pz[v];
-        f = s.pf4[v];
+        f = bucket_f4(s, v);
       }
       int rank = 0;
       for (int q = 0; q < cnt; q++) {
@@ -1380,7 +1391,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_cell_sort_w(DevState s, int gcs) 
           s.bx[base + rj] = s.px[vj];
           s.by[base + rj] = s.py[vj];
           s.bz[base + rj] = s.pz[vj];
-          s.bf4[base + rj] = s.pf4[vj];
+          s.bf4[base + rj] = bucket_f4(s, vj);
         }
       }
     } else {
@@ -1404,7 +1415,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_cell_sort_w(DevState s, int gcs) 
         s.bx[base + q] = s.px[v];
         s.by[base + q] = s.py[v];
         s.bz[base + q] = s.pz[v];
-        s.bf4[base + q] = s.pf4[v];
+        s.bf4[base + q] = bucket_f4(s, v);
       }
     }
     }
@@ -3114,7 +3125,7 @@ __global__ void __launch_bounds__(GC_BLOCK) k_gcmc_run(DevState s, int rep_only,
           }
           s.bucket[base + at] = i;
           s.bx[base + at] = rx; s.by[base + at] = ry; s.bz[base + at] = rz;
-          s.bf4[base + at] = s.pf4[i];
+          s.bf4[base + at] = bucket_f4(s, i);
           s.cell_cnt[P.coff + cell] = cnt + 1;
         }
       }
@@ -4557,7 +4568,7 @@ __global__ void __launch_bounds__(WAVE) k_gb_cells(DevState s, int r) {
     cxs[pos] = s.gb_pt[3 * (off + tp)];
     cys[pos] = s.gb_pt[3 * (off + tp) + 1];
     czs[pos] = s.gb_pt[3 * (off + tp) + 2];
-    cf[pos] = s.pf4[i];
+    cf[pos] = bucket_f4(s, i);
   }
   // then the chain of the start without the deleted atoms, in order
   int nk = nl;
