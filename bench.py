@@ -429,10 +429,16 @@ def calibrate_copy(nbytes=1 << 30):
 
 
 ROW_KERNELS = {1: "k_listbuild_cells", 2: "k_listbuild<4>", 4: "k_listbuild_wc",
-               5: "k_listbuild_wr<4>", 6: "k_listbuild_wr<6>"}
+               5: "k_listbuild_wr<3, 7>", 6: "k_listbuild_wr<2, 8>"}
 
 
-def pmc_summary(fetch_csv, write_csv, kernel="k_listbuild_wr<4>", last=10, copy_bytes=1 << 30):
+def kernel_file_stem(name):
+    """a kernel's name as a file name: k_listbuild_wr<3, 7> -> k_listbuild_wr_3_7"""
+    import re
+    return re.sub(r"_+$", "", re.sub(r"[^A-Za-z0-9]+", "_", name))
+
+
+def pmc_summary(fetch_csv, write_csv, kernel="k_listbuild_wr<3, 7>", last=10, copy_bytes=1 << 30):
     """HBM bytes per launch of `kernel` from two rocprofv3 --pmc runs of
     `bench.py --legs batched --steps 1 --warmup 0 --no-cpu --calibrate-copy`
     (FETCH_SIZE, then WRITE_SIZE; counter_collection.csv of each): the mean
@@ -615,7 +621,7 @@ def main():
         # WRITE_SIZE, scaled by a 1 GiB copy) of the same 256-replica launch
         measured = None
         kname = ROW_KERNELS.get(batched["tuning"]["list_kernel"], "row build")
-        tf = os.path.join(ROOT, "profiles", "%s_c3_traffic.json" % kname.replace("<", "").replace(">", ""))
+        tf = os.path.join(ROOT, "profiles", "%s_c3_traffic.json" % kernel_file_stem(kname))
         if args.replicas == 256 and os.path.exists(tf):
             measured = json.load(open(tf))["hbm_bytes_per_launch"]
         # priced against HBM (its compulsory bytes); what holds it back is

@@ -84,6 +84,7 @@ struct dml_ctx {
   double last_ms = 0.0;
   int lpa = 8;
   int lpa_ov = 8;  // lanes per atom of the overlap scan (1 = thread per atom)
+  int lpa_tail = 0;  // lanes per atom from round 3 on (0: as rounds 1-2)
   // list kernel: 0 cell tiles, 1 wave per atom, 2 per replica by density
   // (0 or 1), 3 wave per cell
   int lb_mode = 4;
@@ -396,10 +397,12 @@ void enq_listbuild(dml_ctx *c) {
   if (c->any_lb[3]) {
     unsigned g = clamp_grid((c->d.n_cm + 3) / 4);
     g = std::max(8u, g / 8 * 8);
+    // 3 chunks of 64 a pass at 7 waves per SIMD (4 at 6 waves, 2 at 8 were
+    // slower on the 256-replica ensemble); 2 chunks at 8 waves a choice
     if (c->lb_mode == 5)
-      hipLaunchKernelGGL(k_listbuild_wr<6>, dim3(g), dim3(DML_BLOCK), 0, c->st, c->d);
+      hipLaunchKernelGGL((k_listbuild_wr<2, 8>), dim3(g), dim3(DML_BLOCK), 0, c->st, c->d);
     else
-      hipLaunchKernelGGL(k_listbuild_wr<4>, dim3(g), dim3(DML_BLOCK), 0, c->st, c->d);
+      hipLaunchKernelGGL((k_listbuild_wr<3, 7>), dim3(g), dim3(DML_BLOCK), 0, c->st, c->d);
   }
 }
 
@@ -473,7 +476,8 @@ void enq_overlap_rounds(dml_ctx *c, const double *u_round, int64_t stride, int r
   for (int q = 0; q < rounds; q++) {
     // round 0 mostly visits the few listed entries within rcut (the tuned
     // lanes, often 1); later rounds scan whole rows, with at least 2 lanes
-    const int lpa = q == 0 ? c->lpa_ov : std::max(c->lpa_ov, 2);
+    int lpa = q == 0 ? c->lpa_ov : std::max(c->lpa_ov, 2);
+    if (q >= 3 && c->lpa_tail) lpa = c->lpa_tail;
     // round 0 covers every ref position; later rounds walk their work lists
     // (grid-stride, a bounded grid)
     unsigned gs = (unsigned)(c->P * lpa / DML_BLOCK);
@@ -927,6 +931,10 @@ int dml_create_ensemble(int32_t nrep, const dml_config *cfgs, dml_ctx **out) {
   if (const char *ev = getenv("DML_OVERLAP_LPA")) {
     const int v = atoi(ev);
     if (v == 1 || v == 2 || v == 4 || v == 8 || v == 16) c->lpa_ov = v;
+  }
+  if (const char *ev = getenv("DML_OVERLAP_TAIL_LPA")) {
+    const int v = atoi(ev);
+    if (v == 0 || v == 2 || v == 4 || v == 8 || v == 16) c->lpa_tail = v;
   }
   refresh_modes(c);
   if (hipStreamSynchronize(c->st) != hipSuccess) {

@@ -50,7 +50,7 @@ __global__ void k_gc_scatter(DevState s, int rep_only, int stage);
 __global__ void k_gc_copy(DevState s, int rep_only);
 __global__ void k_gcmc_run(DevState s, int rep_only, const double *u, const double *g,
                            double act_in, int nadj_in);
-template <int NCH>
+template <int NCH, int WPE>
 __global__ void k_listbuild_wr(DevState s);
 __global__ void k_gb_begin(DevState s, int rep_only, double act_in, int nadj_in);
 __global__ void k_gb_count(DevState s, int r);

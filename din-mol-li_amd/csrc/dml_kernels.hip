@@ -878,7 +878,8 @@ k_listbuild_wc(DevState s) {
 
 // ------------------------------------------------------------------------
 // Rows of one centre cell with its candidates held in registers: a wave
-// loads up to NCH chunks of 64 candidates of the 27-cell stream once,
+// loads up to NCH chunks of 64 candidates of the 27-cell stream at a time
+// (WPE waves per SIMD: the walk is latency-bound, occupancy pays),
 // then walks the cell's ref atoms one at a time; each atom's count, row
 // address and position stay in scalar registers across the chunks, and a
 // chunk's hits are one compare, one ballot and one store of the chunk's
@@ -887,11 +888,25 @@ k_listbuild_wc(DevState s) {
 // '>').  FP32 prefilter with margin, FP64 in the band.  Entries within
 // rcut (FP32 with margin) are listed like k_listbuild_wc lists them.
 // ------------------------------------------------------------------------
-template <int NCH>
-__global__ void __launch_bounds__(DML_BLOCK) __attribute__((amdgpu_waves_per_eu(5)))
+template <int NCH, int WPE>
+__global__ void __launch_bounds__(DML_BLOCK) __attribute__((amdgpu_waves_per_eu(WPE)))
 k_listbuild_wr(DevState s) {
-  __shared__ int32_t lbr_pos[DML_BLOCK / WAVE][NCH * WAVE];
-  int32_t *wpos = lbr_pos[threadIdx.x / WAVE];
+  // one wave's LDS slice, at a wave-uniform base (one lane offset serves
+  // every array)
+  struct Slice {
+    int32_t pos[NCH * WAVE];  // bucket entries of a pass's stream positions
+    // the 64 cells of a batch (replica, cell, count, x, y|z|ghost, image
+    // reference x, y), kept here rather than in registers
+    int32_t bat[7][WAVE];
+    int32_t nn0[WAVE];  // a lane's row length at the start of a pass
+    int4 ctr[WAVE];     // the cell's current centre atoms: slot, FP32 position
+  };
+  __shared__ Slice lbr[DML_BLOCK / WAVE];
+  Slice &L = lbr[__builtin_amdgcn_readfirstlane(threadIdx.x / WAVE)];
+  int32_t *wpos = L.pos;
+  int32_t(*bat)[WAVE] = L.bat;
+  int32_t *nn0s = L.nn0;
+  int4 *ctr = L.ctr;
   const int lane = threadIdx.x % WAVE;
   const int maxnn = s.maxnn;
   const int kofs = lane < 27 ? (kMap[lane][0] + 1) | (kMap[lane][1] + 1) << 4 |
@@ -941,15 +956,24 @@ k_listbuild_wr(DevState s) {
       }
     }
     unsigned long long work = __ballot(lcnt > 0);
+    __builtin_amdgcn_wave_barrier();
+    bat[0][lane] = lr;
+    bat[1][lane] = lc;
+    bat[2][lane] = lcnt;
+    bat[3][lane] = lx;
+    bat[4][lane] = lyz;
+    bat[5][lane] = __float_as_int(lmx);
+    bat[6][lane] = __float_as_int(lmy);
+    __builtin_amdgcn_wave_barrier();
     while (work) {
       const int q = __builtin_ctzll(work);
       work &= work - 1;
-      const int r = __builtin_amdgcn_readlane(lr, q);
-      const int c = __builtin_amdgcn_readlane(lc, q);
-      const int ccnt = __builtin_amdgcn_readlane(lcnt, q);
-      const int cx0 = __builtin_amdgcn_readlane(lx, q);
-      const int cyz = __builtin_amdgcn_readlane(lyz, q);
-      const float mxf = rdlane_f(lmx, q), myf = rdlane_f(lmy, q);
+      const int r = rfl_i(bat[0][q]);
+      const int c = rfl_i(bat[1][q]);
+      const int ccnt = rfl_i(bat[2][q]);
+      const int cx0 = rfl_i(bat[3][q]);
+      const int cyz = rfl_i(bat[4][q]);
+      const float mxf = __int_as_float(rfl_i(bat[5][q])), myf = __int_as_float(rfl_i(bat[6][q]));
       const RepParams &P = s.rp[r];
       if (r != cur_r) {
         if (cur_r >= 0 && lane == 0 && acc_rows) {
@@ -991,31 +1015,35 @@ k_listbuild_wr(DevState s) {
       }
       // the first 64 centre atoms' entries, loaded with the stencil counts
       // (one wait for both)
-      int my_i = -1, my_ref = 0;
-      float my_x = 0.f, my_y = 0.f, my_z = 0.f;
+      int4 my_c = make_int4(-1, 0, 0, 0);
+      int my_ref = 0;
       if (lane < ccnt) {
-        my_i = s.bucket[cbase + lane];
+        my_c.x = s.bucket[cbase + lane];
         const float4 f = s.bf4[cbase + lane];
-        my_x = f.x;
-        my_y = f.y;
-        my_z = f.z;
+        my_c.y = __float_as_int(f.x);
+        my_c.z = __float_as_int(f.y);
+        my_c.w = __float_as_int(f.z);
         my_ref = (__float_as_int(f.w) >> 8 & DML_FL_REF) != 0;
       }
       // the 27 stencil cells: lane k holds count and first entry of cell k
-      int kcnt = 0;
-      int64_t kbase = 0;
+      // (the first entry relative to the replica's bucket region, which
+      // holds fewer than 2^31 entries)
+      int kcnt = 0, kb32 = 0;
+      // (the offsets are unpacked here, each cell: kept unpacked across the
+      // loop they would hold three registers)
+      int ko = kofs;
+      asm volatile("" : "+v"(ko));
       if (q_cells) {
         if (lane < 27) {
-          const int n0 = cell_wrap1(ci0 + (kofs & 15) - 1, q_n0);
-          const int n1 = cell_wrap1(ci1 + (kofs >> 4 & 15) - 1, q_n1);
-          const int n2 = cell_wrap1(ci2 + (kofs >> 8 & 15) - 1, q_n2);
+          const int n0 = cell_wrap1(ci0 + (ko & 15) - 1, q_n0);
+          const int n1 = cell_wrap1(ci1 + (ko >> 4 & 15) - 1, q_n1);
+          const int n2 = cell_wrap1(ci2 + (ko >> 8 & 15) - 1, q_n2);
           const int cell = n0 + (q_n0 + 2) * (n1 + (q_n1 + 2) * n2);
           kcnt = min(s.cell_cnt[q_coff + cell], q_bcap);
-          kbase = q_boff + (int64_t)cell * q_bcap;
+          kb32 = cell * q_bcap;
         }
       } else if (lane == 0) {
         kcnt = ccnt;
-        kbase = q_boff;
       }
       const int incl = wave_scan_incl(kcnt);
       const int kpre = incl - kcnt;
@@ -1027,27 +1055,29 @@ k_listbuild_wr(DevState s) {
       const float rl2lo = q_lo, rl2hi = q_hi, rcf = q_rcf;
       const float bxf = q_bx, byf = q_by, ibx = q_ibx, iby = q_iby;
       const int img = q_img;
-      // this stencil lane's first bucket entry, relative to the replica's
-      const int kb32 = (int)(kbase - q_boff);
       for (int a0 = 0; a0 < ccnt; a0 += WAVE) {
         // centre atoms a0 .. a0+63 of this cell, one per lane
         const int na = (ccnt - a0) < WAVE ? (ccnt - a0) : WAVE;
         const int64_t e0 = cbase + a0;
         if (a0 > 0) {
           // the next 64 centre atoms
-          my_i = -1;
+          my_c = make_int4(-1, 0, 0, 0);
           my_ref = 0;
           if (lane < na) {
-            my_i = s.bucket[e0 + lane];
+            my_c.x = s.bucket[e0 + lane];
             const float4 f = s.bf4[e0 + lane];
-            my_x = f.x;
-            my_y = f.y;
-            my_z = f.z;
+            my_c.y = __float_as_int(f.x);
+            my_c.z = __float_as_int(f.y);
+            my_c.w = __float_as_int(f.z);
             my_ref = (__float_as_int(f.w) >> 8 & DML_FL_REF) != 0;
           }
         }
         const unsigned long long refm = __ballot(my_ref);
         if (!refm) continue;
+        // (read back at wave-uniform addresses by the walks below)
+        __builtin_amdgcn_wave_barrier();
+        ctr[lane] = my_c;
+        __builtin_amdgcn_wave_barrier();
         // row length (bits 0-23) and entries within rcut (24-31, saturating)
         int my_nn = 0;
         for (int s0 = 0; s0 < total; s0 += NCH * WAVE) {
@@ -1096,7 +1126,7 @@ k_listbuild_wr(DevState s) {
           // the candidates are in registers before the centre loop starts
           // (its row stores would otherwise be waited for at each chunk)
           __builtin_amdgcn_s_waitcnt(0);
-          const int my_nn0 = my_nn;
+          nn0s[lane] = my_nn;
           // centres with a pair near the list radius: their rows of this
           // super-chunk are written again below, the band decided in FP64
           unsigned long long redo = 0;
@@ -1120,10 +1150,11 @@ k_listbuild_wr(DevState s) {
           };
           auto start = [&](Ctr &c, int a) {
             c.a = a;
-            c.i = __builtin_amdgcn_readlane(my_i, a);
-            c.xa = rdlane_f(my_x, a);
-            c.ya = rdlane_f(my_y, a);
-            c.za = rdlane_f(my_z, a);
+            const int4 v = ctr[a];
+            c.i = rfl_i(v.x);
+            c.xa = __int_as_float(rfl_i(v.y));
+            c.ya = __int_as_float(rfl_i(v.z));
+            c.za = __int_as_float(rfl_i(v.w));
             const int pk = __builtin_amdgcn_readlane(my_nn, a);
             c.nn = pk & 0xffffff;
             c.nc = (int)((uint32_t)pk >> 24);
@@ -1181,9 +1212,11 @@ k_listbuild_wr(DevState s) {
           while (redo) {
             const int a = __builtin_ctzll(redo);
             redo &= redo - 1;
-            const int i = __builtin_amdgcn_readlane(my_i, a);
-            const float xa = rdlane_f(my_x, a), ya = rdlane_f(my_y, a), za = rdlane_f(my_z, a);
-            const int pk = __builtin_amdgcn_readlane(my_nn0, a);
+            const int4 v = ctr[a];
+            const int i = rfl_i(v.x);
+            const float xa = __int_as_float(rfl_i(v.y)), ya = __int_as_float(rfl_i(v.z));
+            const float za = __int_as_float(rfl_i(v.w));
+            const int pk = rfl_i(nn0s[a]);
             int nn = pk & 0xffffff, nc = (int)((uint32_t)pk >> 24);
             int32_t *row = s.nbr + (int64_t)i * maxnn;
             for (int h = 0; h < nch; h++) {
@@ -1233,7 +1266,9 @@ k_listbuild_wr(DevState s) {
         }
         const int my_nc = (int)((uint32_t)my_nn >> 24);
         my_nn &= 0xffffff;
-        if (my_ref) {
+        const bool mine = (refm >> lane) & 1ull;
+        if (mine) {
+          const int my_i = ctr[lane].x;
           int nn = my_nn;
           if (nn > maxnn) {
             atomicOr(s.err, ERR_LIST);
@@ -1246,7 +1281,7 @@ k_listbuild_wr(DevState s) {
           s.ov_near[my_i] = (uint8_t)(img && my_nc <= NEAR_MAX ? my_nc : 255);
           if (ghost) s.rp[r].any_ghost = 1;
         }
-        const int v = wave_scan_incl(my_ref ? my_nn : 0);
+        const int v = wave_scan_incl(mine ? my_nn : 0);
         acc_nn += (unsigned long long)(uint32_t)__builtin_amdgcn_readlane(v, WAVE - 1);
         acc_rows += (unsigned long long)__popcll(refm);
       }
@@ -1258,8 +1293,8 @@ k_listbuild_wr(DevState s) {
   }
 }
 
-template __global__ void k_listbuild_wr<4>(DevState s);
-template __global__ void k_listbuild_wr<6>(DevState s);
+template __global__ void k_listbuild_wr<3, 7>(DevState s);
+template __global__ void k_listbuild_wr<2, 8>(DevState s);
 
 
 // bucket order by a rank sort inside one wave (cells of up to 256 atoms,

@@ -286,8 +286,8 @@ int dml_time_kernel(dml_ctx *ctx, int32_t which, int32_t reps, double *avg_ms);
  * overlap rounds per step of dml_run (default 16, at most 32),
  * list kernel (1 = cell tiles through LDS, 2 = one wave per atom, 3 = 1 or 2
  * per replica by atoms per cell, 4 = one wave per cell, 5 = one wave per
- * cell with its candidates in registers, 4 chunks of 64, the default, 6 = 5
- * with 6 chunks); 0
+ * cell with its candidates in registers, 3 chunks of 64 a pass, the
+ * default, 6 = 5 with 2 chunks a pass at more waves per SIMD); 0
  * keeps the current value.  Results do not depend on them beyond FP64 summation
  * order.  No counterpart in the reference: used by the autotuner. */
 int dml_set_tuning(dml_ctx *ctx, int32_t lanes_per_atom, int32_t overlap_rounds,

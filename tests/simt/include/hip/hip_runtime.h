@@ -45,6 +45,10 @@ struct uint4 {
   unsigned x, y, z, w;
 };
 inline float4 make_float4(float a, float b, float c, float d) { return float4{a, b, c, d}; }
+struct int4 {
+  int x, y, z, w;
+};
+inline int4 make_int4(int a, int b, int c, int d) { return int4{a, b, c, d}; }
 
 // ---- the SIMT runtime (tests/simt/simt.cpp) -------------------------------
 namespace sim {

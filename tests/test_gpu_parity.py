@@ -114,7 +114,7 @@ NB = np.load(os.path.join(G, "nb_cases.npz"))
 def test_cells_and_lists_equal_reference_modules(dml, name, lb_mode, monkeypatch):
     """lb_mode 0: cell-tile list build through LDS; 1: one wave per atom;
     3: one wave per cell; 4, 5: one wave per cell, candidates in registers
-    (4 or 6 chunks of 64)"""
+    (3 or 2 chunks of 64 a pass)"""
     monkeypatch.setenv("DML_LISTBUILD", lb_mode)
     d = {k.split("/", 1)[1]: NB[k] for k in NB.files if k.startswith(name + "/")}
     say("case", name, len(d["z"]), "lb_mode", lb_mode)
