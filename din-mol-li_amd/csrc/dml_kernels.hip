@@ -1933,6 +1933,16 @@ __global__ void k_ov_begin(DevState s) {
   if (q == 0) s.wl_cnt[2] = 0;  // round 0's list of reverted atoms outside ref
 }
 
+// v of the lane `off` away within groups of LPA lanes (off < LPA): DPP
+// quad permutes for 1 and 2, a shuffle beyond (a shuffle's lane addresses,
+// hoisted by the compiler, would hold registers through the kernel)
+template <int LPA>
+__device__ inline int lpa_xor(int v, int off) {
+  if (off == 1) return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xf, 0xf, false);  // [1,0,3,2]
+  if (off == 2) return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xf, 0xf, false);  // [2,3,0,1]
+  return __shfl_xor(v, off, LPA);
+}
+
 template <int LPA>
 __global__ void __launch_bounds__(DML_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) k_ov_scan_l(DevState s, const double *u_round,
                                                           int64_t u_stride, int round,
@@ -1954,8 +1964,7 @@ __global__ void __launch_bounds__(DML_BLOCK) __attribute__((amdgpu_waves_per_eu(
   const int64_t stride = (int64_t)gridDim.x * (DML_BLOCK / LPA);
   // one ref position: its verdict, and what it adds to its replica's
   // counters (k_r = the replica, -1 when the position is not scanned)
-  auto scan_one = [&](int64_t item, unsigned long long &k_try, unsigned long long &k_ch,
-                      unsigned long long &k_ch3, int &k_more, int &k_r) {
+  auto scan_one = [&](int64_t item, int &k_try, int &k_ch, int &k_ch3, int &k_more, int &k_r) {
     const int t = round == 0 ? (int)item : wl[item];
     if ((uint32_t)t >= (uint64_t)ptotal) return;
     const int r = s.blk2rep[t / DML_BLOCK];
@@ -2021,7 +2030,7 @@ __global__ void __launch_bounds__(DML_BLOCK) __attribute__((amdgpu_waves_per_eu(
     if (nlist <= NEAR_MAX) {
       int bad = 0;
       for (int k = sub; k < nlist; k += LPA) bad |= (unsigned)nl[k] >= (unsigned)nn;
-      for (int off = LPA / 2; off > 0; off >>= 1) bad |= __shfl_xor(bad, off, LPA);
+      for (int off = LPA / 2; off > 0; off >>= 1) bad |= lpa_xor<LPA>(bad, off);
       if (bad) nlist = 255;
     }
     if (nlist <= NEAR_MAX) {
@@ -2037,12 +2046,12 @@ __global__ void __launch_bounds__(DML_BLOCK) __attribute__((amdgpu_waves_per_eu(
       }
     }
     for (int off = LPA / 2; off > 0; off >>= 1) {
-      dom |= __shfl_xor(dom, off, LPA);
-      over |= __shfl_xor(over, off, LPA);
-      int o = __shfl_xor(firstcg, off, LPA);
+      dom |= lpa_xor<LPA>(dom, off);
+      over |= lpa_xor<LPA>(over, off);
+      const int o = lpa_xor<LPA>(firstcg, off);
       firstcg = o < firstcg ? o : firstcg;
     }
-    unsigned long long n_ch = 0, n_ch3 = 0;
+    int n_ch = 0, n_ch3 = 0;  // reverts of this atom: at most its row length
     const int stamp = ov_stamp(P, round);
     // a revert of neighbour j (or, piston, the "unsolvable" count when
     // both sit where they were, dana.F90:920-927); returns 1 for the latter.
@@ -2078,8 +2087,8 @@ __global__ void __launch_bounds__(DML_BLOCK) __attribute__((amdgpu_waves_per_eu(
       }
     }
     for (int off = LPA / 2; off > 0; off >>= 1) {
-      n_ch += __shfl_xor(n_ch, off, LPA);
-      n_ch3 += __shfl_xor(n_ch3, off, LPA);
+      n_ch += lpa_xor<LPA>(n_ch, off);
+      n_ch3 += lpa_xor<LPA>(n_ch3, off);
     }
     if (sub == 0) {
       uint8_t st = 1;
@@ -2114,32 +2123,27 @@ __global__ void __launch_bounds__(DML_BLOCK) __attribute__((amdgpu_waves_per_eu(
   // chip when every atom adds to them.
   for (int64_t it0 = (int64_t)blockIdx.x * (DML_BLOCK / LPA); it0 < nitems; it0 += stride) {
     const int64_t item = it0 + threadIdx.x / LPA;
-    unsigned long long k_try = 0, k_ch = 0, k_ch3 = 0;
-    int k_more = 0, k_r = -1;
+    int k_try = 0, k_ch = 0, k_ch3 = 0, k_more = 0, k_r = -1;
     if (item < nitems) scan_one(item, k_try, k_ch, k_ch3, k_more, k_r);
-    int r_hi = k_r, r_lo = k_r < 0 ? 0x7fffffff : k_r;
-    for (int off = WAVE / 2; off > 0; off >>= 1) {
-      r_hi = max(r_hi, __shfl_xor(r_hi, off));
-      r_lo = min(r_lo, __shfl_xor(r_lo, off));
-    }
-    if (r_hi < 0) continue;  // nothing scanned in this wave
-    if (r_lo == r_hi) {
-      for (int off = WAVE / 2; off > 0; off >>= 1) {
-        k_try += __shfl_xor(k_try, off);
-        k_ch += __shfl_xor(k_ch, off);
-        k_ch3 += __shfl_xor(k_ch3, off);
-        k_more |= __shfl_xor(k_more, off);
-      }
+    const unsigned long long valid = __ballot(k_r >= 0);
+    if (!valid) continue;  // nothing scanned in this wave
+    const int r0 = __builtin_amdgcn_readlane(k_r, __builtin_ctzll(valid));
+    if (!__ballot(k_r >= 0 && k_r != r0)) {
+      // (per-atom counts are small: 32-bit sums by DPP)
+      const int t_try = __builtin_amdgcn_readlane(wave_scan_incl(k_try), WAVE - 1);
+      const int t_ch = __builtin_amdgcn_readlane(wave_scan_incl(k_ch), WAVE - 1);
+      const int t_ch3 = __builtin_amdgcn_readlane(wave_scan_incl(k_ch3), WAVE - 1);
+      const bool more = __ballot(k_more) != 0;
       if (threadIdx.x % WAVE == 0) {
-        if (k_try) atomicAdd(&s.cnt[r_hi].try_, k_try);
-        if (k_ch) atomicAdd(&s.cnt[r_hi].choques, k_ch);
-        if (k_ch3) atomicAdd(&s.cnt[r_hi].choques3, k_ch3);
-        if (k_more) s.ov_more[cur * s.nrep + r_hi] = 1;
+        if (t_try) atomicAdd(&s.cnt[r0].try_, (unsigned long long)(uint32_t)t_try);
+        if (t_ch) atomicAdd(&s.cnt[r0].choques, (unsigned long long)(uint32_t)t_ch);
+        if (t_ch3) atomicAdd(&s.cnt[r0].choques3, (unsigned long long)(uint32_t)t_ch3);
+        if (more) s.ov_more[cur * s.nrep + r0] = 1;
       }
     } else if (k_r >= 0) {
-      if (k_try) atomicAdd(&s.cnt[k_r].try_, k_try);
-      if (k_ch) atomicAdd(&s.cnt[k_r].choques, k_ch);
-      if (k_ch3) atomicAdd(&s.cnt[k_r].choques3, k_ch3);
+      if (k_try) atomicAdd(&s.cnt[k_r].try_, (unsigned long long)(uint32_t)k_try);
+      if (k_ch) atomicAdd(&s.cnt[k_r].choques, (unsigned long long)(uint32_t)k_ch);
+      if (k_ch3) atomicAdd(&s.cnt[k_r].choques3, (unsigned long long)(uint32_t)k_ch3);
       if (k_more) s.ov_more[cur * s.nrep + k_r] = 1;
     }
   }

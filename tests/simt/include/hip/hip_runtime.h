@@ -112,7 +112,9 @@ template <class T>
 inline T sim_update_dpp_(T old, T src, int ctrl, int rm, int bm, bool bc, int site) {
   const int l = (int)::sim::lane_id(), row = l >> 4, bank = (l >> 2) & 3;
   int from = -1;
-  if (ctrl >= 0x111 && ctrl <= 0x11f) {
+  if (ctrl >= 0 && ctrl <= 0xff) {  // quad_perm
+    from = (l & ~3) | ((ctrl >> (2 * (l & 3))) & 3);
+  } else if (ctrl >= 0x111 && ctrl <= 0x11f) {
     if ((l & 15) >= ctrl - 0x110) from = l - (ctrl - 0x110);
   } else if (ctrl == 0x142) {
     if (row >= 1) from = row * 16 - 1;
