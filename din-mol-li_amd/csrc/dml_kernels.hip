@@ -1493,6 +1493,22 @@ __device__ inline void ermak_b_one(const DevState &s, const RepParams &P, int i,
   s.az[i] = fz / m;
 }
 
+// v of the lane `off` away within groups of LPA lanes (off < LPA): DPP
+// quad permutes for 1 and 2, a shuffle beyond (a shuffle's lane addresses,
+// hoisted by the compiler, would hold registers through the kernel)
+template <int LPA>
+__device__ inline int lpa_xor(int v, int off) {
+  if (off == 1) return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xf, 0xf, false);  // [1,0,3,2]
+  if (off == 2) return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xf, 0xf, false);  // [2,3,0,1]
+  return __shfl_xor(v, off, LPA);
+}
+template <int LPA>
+__device__ inline double lpa_xor_d(double v, int off) {
+  const int64_t b = __double_as_longlong(v);
+  const int lo = lpa_xor<LPA>((int)(b & 0xffffffff), off), hi = lpa_xor<LPA>((int)(b >> 32), off);
+  return __longlong_as_double((int64_t)(uint32_t)lo | ((int64_t)hi << 32));
+}
+
 template <int LPA, int FUSEB>
 __global__ void __launch_bounds__(DML_BLOCK) k_force(DevState s, int only_ermak) {
   int gid = blockIdx.x * DML_BLOCK + threadIdx.x;
@@ -1590,12 +1606,12 @@ __global__ void __launch_bounds__(DML_BLOCK) k_force(DevState s, int only_ermak)
   const unsigned long long one = __ballot(ncnt == 1);
   int tot = ncnt, ovf = ncnt > 1;
   for (int off = LPA / 2; off > 0; off >>= 1) {
-    fx += __shfl_xor(fx, off);
-    fy += __shfl_xor(fy, off);
-    fz += __shfl_xor(fz, off);
-    ea += __shfl_xor(ea, off);
-    tot += __shfl_xor(tot, off);
-    ovf |= __shfl_xor(ovf, off);
+    fx += lpa_xor_d<LPA>(fx, off);
+    fy += lpa_xor_d<LPA>(fy, off);
+    fz += lpa_xor_d<LPA>(fz, off);
+    ea += lpa_xor_d<LPA>(ea, off);
+    tot += lpa_xor<LPA>(tot, off);
+    ovf |= lpa_xor<LPA>(ovf, off);
   }
   const int listed = !ovf && tot <= NEAR_MAX;
   if (active && listed && ncnt == 1) {
@@ -1931,16 +1947,6 @@ __global__ void k_ov_begin(DevState s) {
   int q = blockIdx.x * blockDim.x + threadIdx.x;
   for (int k = q; k < 3 * s.nrep; k += gridDim.x * blockDim.x) s.ov_more[k] = 0;
   if (q == 0) s.wl_cnt[2] = 0;  // round 0's list of reverted atoms outside ref
-}
-
-// v of the lane `off` away within groups of LPA lanes (off < LPA): DPP
-// quad permutes for 1 and 2, a shuffle beyond (a shuffle's lane addresses,
-// hoisted by the compiler, would hold registers through the kernel)
-template <int LPA>
-__device__ inline int lpa_xor(int v, int off) {
-  if (off == 1) return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xf, 0xf, false);  // [1,0,3,2]
-  if (off == 2) return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xf, 0xf, false);  // [2,3,0,1]
-  return __shfl_xor(v, off, LPA);
 }
 
 template <int LPA>
