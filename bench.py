@@ -363,7 +363,8 @@ def c5_leg(dml, steps, warmup, trials=4096):
     return dict(n=n, steps=steps, wall=wall, rate=ps / wall if wall > 0 else None,
                 trials_per_step=nadj, trials_per_s=nadj * steps / wall,
                 moves_committed_per_s=moves / wall, batched=out is not None, last_call=last,
-                status=st)
+                status=st, tuning={k: choice[k] for k in ("lanes_per_atom", "list_kernel",
+                                                          "overlap_lanes", "overlap_rounds")})
 
 
 def cpu_baseline():

@@ -603,6 +603,19 @@ __device__ inline int wave_scan_incl(int v) {
   v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);
   return v;
 }
+// the same within each row of 16 lanes, and within each half of 32
+__device__ inline int row_scan16(int v) {
+  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);
+  return v;
+}
+__device__ inline int half_scan32(int v) {
+  v = row_scan16(v);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);
+  return v;
+}
 
 __device__ inline float rdlane_f(float v, int a) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), a));
@@ -3428,6 +3441,7 @@ __global__ void __launch_bounds__(GC_BLOCK) k_gcmc_run(DevState s, int rep_only,
 #define GB_CHUNKS 768                // its 1024-entry chunks
 #define GB_UW 12                     // hs%b universe bitmap words per lane (24,576 entries)
 #define GB_CELLMAX 512               // chain length a rebuilt cell may reach
+#define GB_NMAX (GB_VWORDS * 32)     // bound of the count n a call can reach (the bitmap's)
 
 __device__ inline bool gb_on(const RepParams &P) { return P.gb_state == GB_ON && !P.halted; }
 __device__ inline void gb_fallback(RepParams *Pw) { Pw->gb_state = GB_FALLBACK; }
@@ -3618,7 +3632,18 @@ __global__ void __launch_bounds__(DML_BLOCK) k_gb_trials(DevState s, int r, cons
                                                       : !(gc_unif(P, u, w, 0, call) < 0.5);
   const double u1 = gc_unif(P, u, w, 1, call), u2 = gc_unif(P, u, w, 2, call);
   if (lane == 0) {
-    s.gb_u[2 * o] = u1;
+    // the Metropolis test of the ordered walk as a threshold on n, found
+    // by bisection over the same FP64 expressions (each is monotone in n):
+    // an insertion passes while n <= thr (dana.F90:632, !(act V/(n+1) <
+    // ran)), a deletion once n >= thr (:683, !(n/(V act) < ran))
+    const double act = P.gb_act, vol = P.box[0] * P.box[1] * (P.zmax - P.z0);
+    int lo = -1, hi = GB_NMAX + 1;
+    while (hi - lo > 1) {
+      const int mid = lo + (hi - lo) / 2;
+      const bool pass = kind ? !((double)mid / (vol * act) < u1) : !(act * vol / (double)(mid + 1) < u1);
+      if (pass == !kind) lo = mid; else hi = mid;
+    }
+    s.gb_u[2 * o] = (double)(kind ? hi : lo);
     s.gb_u[2 * o + 1] = u2;
     s.gb_nblk[o] = 0;
     s.gb_du[o] = 0.0;
@@ -3747,8 +3772,6 @@ __global__ void __launch_bounds__(WAVE) k_gb_decide(DevState s, int r) {
   if (!gb_on(*Pw)) return;
   const int lane = threadIdx.x;
   const int T = Pw->gb_t, ilen = Pw->gb_ilen, off = Pw->gb_off;
-  const double act = Pw->gb_act;
-  const double vol = Pw->box[0] * Pw->box[1] * (Pw->zmax - Pw->z0);
   const int nwords = (ilen + T + 31) / 32;
   for (int q = lane; q < nwords; q += WAVE) {
     const int b0 = q * 32;
@@ -3761,6 +3784,8 @@ __global__ void __launch_bounds__(WAVE) k_gb_decide(DevState s, int r) {
     ccnt[c] = v;
     tot += v;
   }
+  // and in the chunks of lanes 0 .. lane (kept up to date, not rescanned)
+  int pre = wave_scan_incl(tot);
   // per-trial data of a 64-trial chunk and its lists, loaded into
   // registers one chunk ahead and staged into LDS at the chunk's start, so
   // that the walk reads LDS and never waits on memory (a load result kept
@@ -3825,12 +3850,13 @@ __global__ void __launch_bounds__(WAVE) k_gb_decide(DevState s, int r) {
     const int kind = lkind[buf][a];
     const int nblk = lnblk[buf][a];
     const int ncf = lncf[buf][a];
-    const double u1 = lu1[buf][a], u2 = lu2[buf][a];
+    const int thr = (int)lu1[buf][a];  // the Metropolis threshold on n
+    const double u2 = lu2[buf][a];
     int out = -1;
     if (kind == 2) {
       fail = 1;  // a point outside the cells: the sequential kernel's error path
     } else if (kind == 0) {
-      if (!(act * vol / (double)(n + 1) < u1) && nblk >= 0) {
+      if (n <= thr && nblk >= 0) {
         if (nblk > GB_NB || ncf > GB_NC) {
           fail = 1;
         } else {
@@ -3855,13 +3881,14 @@ __global__ void __launch_bounds__(WAVE) k_gb_decide(DevState s, int r) {
               s.gi_del[off + k] = 0x7fffffff;
             }
             if (lane == (vp >> 14)) tot++;
+            if (lane >= (vp >> 14)) pre++;
             n++;
             out = k;
           }
         }
       }
     } else {
-      if (!((double)n / (vol * act) < u1)) {
+      if (n >= thr) {
         int m = (int)floor(u2 * n) + 1;
         if (m > n) m = n;
         if (m < 1) {
@@ -3870,19 +3897,18 @@ __global__ void __launch_bounds__(WAVE) k_gb_decide(DevState s, int r) {
           // the (m-1)-th present entry: lane, chunk, word, bit (each level's
           // ballot is non-empty while the counts agree with the bitmap)
           int rem = m - 1;
-          const int incl = wave_scan_incl(tot);
-          const unsigned long long b1 = __ballot(incl > rem);
+          const unsigned long long b1 = __ballot(pre > rem);
           const int L = b1 ? __builtin_ctzll(b1) : 0;
-          rem -= __builtin_amdgcn_readlane(incl - tot, L);
+          rem -= __builtin_amdgcn_readlane(pre - tot, L);
           const int c16 = lane < 16 ? ccnt[16 * L + lane] : 0;
-          const int incl2 = wave_scan_incl(c16);
+          const int incl2 = row_scan16(c16);
           const unsigned long long b2 = __ballot(lane < 16 && incl2 > rem);
           const int j = b2 ? __builtin_ctzll(b2) : 0;
           rem -= __builtin_amdgcn_readlane(incl2 - c16, j);
           const int chunk = 16 * L + j;
           const uint32_t wv = lane < 32 ? vb[32 * chunk + lane] : 0u;
           const int pc1 = __popc(wv);
-          const int incl3 = wave_scan_incl(pc1);
+          const int incl3 = half_scan32(pc1);
           const unsigned long long b3 = __ballot(lane < 32 && incl3 > rem);
           const int k = b3 ? __builtin_ctzll(b3) : 0;
           rem -= __builtin_amdgcn_readlane(incl3 - pc1, k);
@@ -3905,6 +3931,7 @@ __global__ void __launch_bounds__(WAVE) k_gb_decide(DevState s, int r) {
             if (p >= ilen) s.gi_del[off + (p - ilen)] = t;
           }
           if (lane == L) tot--;
+          if (lane >= L) pre--;
           ndel++;
           n--;
           out = p;
