@@ -4196,9 +4196,9 @@ __global__ void __launch_bounds__(WAVE) k_gb_commit(DevState s, int r, const dou
     atomicOr(&ubl[rk >> 5], 1u << (rk & 31));
   }
   __syncthreads();
-  uint32_t ub[GB_UW];
-#pragma unroll
-  for (int k = 0; k < GB_UW; k++) ub[k] = ubl[k * WAVE + lane];
+  // the free ranks stay in the LDS bitmap; every rank below ulo is taken
+  // (an insertion takes the lowest free rank, a deletion can only lower it)
+  int ulo = 0;
   int b_amax = P.b_amax, b_nat = P.b_nat;
   int nappend = 0;          // hs%b indices appended at b_amax (ranks nU + ...)
   int bmin = P.bfree_lo;    // lower bound of the free hs%b indices after the call
@@ -4222,16 +4222,15 @@ __global__ void __launch_bounds__(WAVE) k_gb_commit(DevState s, int r, const dou
     if (e < glen) return gb_del_time_v(s, V, gc[e]) <= t;
     return s.gi_del[off + (e - glen)] <= t;
   };
-  // three streams read 64 entries at a time, one per lane, staged into LDS
-  // when the walk reaches the chunk (loaded there and then: a load result
-  // carried in registers across iterations makes every iteration wait for
-  // the stores before it): the trials (decision, kind, normals), the
-  // deletions in order (atom, hs%b index and rank, ref position, flags) and
-  // the insertions' slots
-  __shared__ int32_t lres[WAVE], lkind[WAVE];
-  __shared__ double lg[3][WAVE];
-  __shared__ int32_t ld[5][WAVE];
-  __shared__ int32_t lis[WAVE];
+  // three streams read 64 entries at a time, one per lane, into registers
+  // when the walk reaches the chunk, and taken lane by lane with readlane:
+  // the trials (decision, kind, normals), the deletions in order (atom,
+  // hs%b index and rank, ref position, flags) and the insertions' slots.
+  // Each chunk load is waited for where it is made (every memory operation
+  // of the wave before it included, stores too): a load still counted as in
+  // flight where the paths merge would make every later iteration wait.
+  int c_res = -1, c_kind = 0, c_is = 0, c_d[5] = {0, 0, 0, 0, 0};
+  double c_g[3] = {0.0, 0.0, 0.0};
   int ki = 0, kd = 0;
   int ta = gc[glen - 1];  // the tail atom
   for (int t = 0; t < T; t++) {
@@ -4243,47 +4242,45 @@ __global__ void __launch_bounds__(WAVE) k_gb_commit(DevState s, int r, const dou
     if (tl == 0) {
       const int q = t + lane;
       if (q < T) {
-        lres[lane] = s.gb_res[off + q];
-        lkind[lane] = s.gb_kind[off + q];
-        lg[0][lane] = s.gb_g[3 * (off + q)];
-        lg[1][lane] = s.gb_g[3 * (off + q) + 1];
-        lg[2][lane] = s.gb_g[3 * (off + q) + 2];
+        c_res = s.gb_res[off + q];
+        c_kind = s.gb_kind[off + q];
+        c_g[0] = s.gb_g[3 * (off + q)];
+        c_g[1] = s.gb_g[3 * (off + q) + 1];
+        c_g[2] = s.gb_g[3 * (off + q) + 2];
       }
+      __builtin_amdgcn_s_waitcnt(0);
       __builtin_amdgcn_wave_barrier();
     }
-    const int res = lres[tl];
-    const int kind = lkind[tl];
+    const int res = __builtin_amdgcn_readlane(c_res, tl);
+    const int kind = __builtin_amdgcn_readlane(c_kind, tl);
     if (res < 0) continue;
     if (kind == 0) {
       const int k = ki++;
       if ((k & (WAVE - 1)) == 0) {
-        if (k + lane < nins) lis[lane] = gb_ins_slot_v(s, V, k + lane);
+        if (k + lane < nins) c_is = gb_ins_slot_v(s, V, k + lane);
+        __builtin_amdgcn_s_waitcnt(0);
         __builtin_amdgcn_wave_barrier();
       }
-      const int i = lis[k & (WAVE - 1)];
+      const int i = __builtin_amdgcn_readlane(c_is, k & (WAVE - 1));
       // hs%b index: the lowest free one, or a new one at b_amax
       int bs, rank;
       if (b_amax >= b_nat + 1) {
-        int w = -1;
-#pragma unroll
-        for (int q = 0; q < GB_UW; q++) {
-          const unsigned long long m = __ballot(ub[q] != 0u);
-          if (w < 0 && m) w = q * WAVE + __builtin_ctzll(m);
+        int w = ulo >> 5;
+        uint32_t word = (uint32_t)rfl_i((int)ubl[w]);
+        uint32_t live = word & (0xffffffffu << (ulo & 31));
+        while (live == 0u && w + 1 < GB_UW * WAVE) {
+          w++;
+          word = (uint32_t)rfl_i((int)ubl[w]);
+          live = word;
         }
-        const int wl = w & (WAVE - 1), wq = w / WAVE;
-        uint32_t word = 0;
-#pragma unroll
-        for (int q = 0; q < GB_UW; q++)
-          if (q == wq) word = (uint32_t)__builtin_amdgcn_readlane((int)ub[q], wl);
-        if (w < 0) {  // cannot happen: b_amax - b_nat counts the set bits
+        if (live == 0u) {  // cannot happen: b_amax - b_nat counts the set bits
           if (lane == 0) rep_fail(s, r, ERR_UNSUP);
           return;
         }
-        const int bit = __builtin_ctz(word);
+        const int bit = __builtin_ctz(live);
         rank = w * 32 + bit;
-#pragma unroll
-        for (int q = 0; q < GB_UW; q++)
-          if (q == wq && lane == wl) ub[q] &= ~(1u << bit);
+        if (lane == 0) ubl[w] = word & ~(1u << bit);
+        ulo = rank + 1;
         bs = U[rank];
       } else {
         bs = b_amax++;
@@ -4293,7 +4290,7 @@ __global__ void __launch_bounds__(WAVE) k_gb_commit(DevState s, int r, const dou
       b_nat++;
       // the velocity lands on the list's last live atom, the new atom copies
       // the template (atom_asign, Groups.F90:484-502)
-      const double g0 = lg[0][tl], g1 = lg[1][tl], g2 = lg[2][tl];
+      const double g0 = rdlane_d(c_g[0], tl), g1 = rdlane_d(c_g[1], tl), g2 = rdlane_d(c_g[2], tl);
       if (lane == 0) {
         s.vx[ta] = beta * g0;
         s.vy[ta] = beta * g1;
@@ -4320,16 +4317,17 @@ __global__ void __launch_bounds__(WAVE) k_gb_commit(DevState s, int r, const dou
       if ((d & (WAVE - 1)) == 0) {
         const int q = d + lane;
         if (q < ndel) {
-          ld[0][lane] = s.gd_slot[off + q];
-          ld[1][lane] = s.gd_bso[off + q];
-          ld[2][lane] = s.gd_brank[off + q];
-          ld[3][lane] = s.gd_rpos[off + q];
-          ld[4][lane] = s.gd_oflags[off + q];
+          c_d[0] = s.gd_slot[off + q];
+          c_d[1] = s.gd_bso[off + q];
+          c_d[2] = s.gd_brank[off + q];
+          c_d[3] = s.gd_rpos[off + q];
+          c_d[4] = s.gd_oflags[off + q];
         }
+        __builtin_amdgcn_s_waitcnt(0);
         __builtin_amdgcn_wave_barrier();
       }
       const int dl = d & (WAVE - 1);
-      const int o = ld[0][dl];
+      const int o = __builtin_amdgcn_readlane(c_d[0], dl);
 
       int bso, rank, rpos, ofl;
       if (res >= ilen) {
@@ -4339,16 +4337,14 @@ __global__ void __launch_bounds__(WAVE) k_gb_commit(DevState s, int r, const dou
         rpos = rlen + k;
         ofl = 0;
       } else {
-        bso = ld[1][dl];
-        rank = ld[2][dl];
-        rpos = ld[3][dl];
-        ofl = ld[4][dl];
+        bso = __builtin_amdgcn_readlane(c_d[1], dl);
+        rank = __builtin_amdgcn_readlane(c_d[2], dl);
+        rpos = __builtin_amdgcn_readlane(c_d[3], dl);
+        ofl = __builtin_amdgcn_readlane(c_d[4], dl);
       }
       // the freed hs%b index is free again for the later insertions
-      const int wq = (rank >> 5) / WAVE, wl = (rank >> 5) & (WAVE - 1);
-#pragma unroll
-      for (int q = 0; q < GB_UW; q++)
-        if (q == wq && lane == wl) ub[q] |= 1u << (rank & 31);
+      if (lane == 0) atomicOr(&ubl[rank >> 5], 1u << (rank & 31));
+      ulo = min(ulo, rank);
       b_nat--;
       bmin = min(bmin, bso);
       if (lane == 0) {
