@@ -3786,6 +3786,10 @@ __global__ void __launch_bounds__(WAVE) k_gb_decide(DevState s, int r) {
   }
   // and in the chunks of lanes 0 .. lane (kept up to date, not rescanned)
   int pre = wave_scan_incl(tot);
+  // each trial's result (insertion rank, in-volume position, -1): the walk
+  // writes it to LDS only; the per-trial and per-move records in memory are
+  // written from it after the walk, 64 trials at a time
+  for (int q = lane; q < T; q += WAVE) res[q] = -1;
   // per-trial data of a 64-trial chunk and its lists, loaded into
   // registers one chunk ahead and staged into LDS at the chunk's start, so
   // that the walk reads LDS and never waits on memory (a load result kept
@@ -3877,8 +3881,6 @@ __global__ void __launch_bounds__(WAVE) k_gb_decide(DevState s, int r) {
             if (lane == 0) {
               vb[vp >> 5] |= 1u << (vp & 31);
               ccnt[vp >> 10]++;
-              s.gi_trial[off + k] = t;
-              s.gi_del[off + k] = 0x7fffffff;
             }
             if (lane == (vp >> 14)) tot++;
             if (lane >= (vp >> 14)) pre++;
@@ -3925,10 +3927,6 @@ __global__ void __launch_bounds__(WAVE) k_gb_decide(DevState s, int r) {
           if (lane == 0) {
             vb[32 * chunk + k] = word & ~(1u << bit);
             ccnt[chunk]--;
-            const int d = ndel;
-            s.gd_trial[off + d] = t;
-            s.gd_vpos[off + d] = p;
-            if (p >= ilen) s.gi_del[off + (p - ilen)] = t;
           }
           if (lane == L) tot--;
           if (lane >= L) pre--;
@@ -3938,12 +3936,39 @@ __global__ void __launch_bounds__(WAVE) k_gb_decide(DevState s, int r) {
         }
       }
     }
-    if (lane == 0) {
-      res[t] = out;
-      s.gb_res[off + t] = out;
+    if (lane == 0) res[t] = out;
+    __builtin_amdgcn_wave_barrier();
+  }
+  // the records of the walk, in trial order: every trial's result, each
+  // insertion's trial (its rank is its result), each deletion's trial and
+  // position in deletion order; then, once those are written, the trial
+  // that deletes an insertion of this call
+  int kd = 0;
+  for (int t0 = 0; t0 < T; t0 += WAVE) {
+    const int t = t0 + lane;
+    const int o = t < T ? res[t] : -1;
+    const int kind = t < T ? s.gb_kind[off + t] : 2;
+    if (t < T) {
+      s.gb_res[off + t] = o;
       s.gb_slot[off + t] = -1;
     }
-    __builtin_amdgcn_wave_barrier();
+    if (o >= 0 && kind == 0) {
+      s.gi_trial[off + o] = t;
+      s.gi_del[off + o] = 0x7fffffff;
+    }
+    const bool del = o >= 0 && kind == 1;
+    const unsigned long long dm = __ballot(del);
+    if (del) {
+      const int d = kd + __popcll(dm & ((1ull << lane) - 1ull));
+      s.gd_trial[off + d] = t;
+      s.gd_vpos[off + d] = o;
+    }
+    kd += __popcll(dm);
+  }
+  __syncthreads();
+  for (int t = lane; t < T; t += WAVE) {
+    const int o = res[t];
+    if (o >= ilen && s.gb_kind[off + t] == 1) s.gi_del[off + (o - ilen)] = t;
   }
   if (lane == 0) {
     Pw->gb_nins = nins;
