@@ -253,7 +253,10 @@ def _request_system(args):
     return W.system_replica(int(n), int(seed), reservoir_band=res != "NO_RESERVOIR")
 
 
-def serving_leg(dml, duration, rate=20.0, step_scale=0.01, seed=104012 + 4):
+SERVE_POOL = "8192:48,32768:16,131072:4,212992:2"
+
+
+def serving_leg(dml, duration, rate=20.0, step_scale=0.01, seed=104012 + 4, pool_spec=SERVE_POOL):
     """C4 on one GPU: Poisson arrivals at `rate` per second for `duration`
     seconds; sizes lognormal (median 10k, sigma 1) clipped to 1k..200k; step
     counts uniform in 1e3..1e5 times `step_scale`.  Requests are admitted into
@@ -292,7 +295,8 @@ def serving_leg(dml, duration, rate=20.0, step_scale=0.01, seed=104012 + 4):
         reqs.append(serve.Request(k, arr[k], cfg, sysd["pos"], sysd["species"], st, need=need))
     # slot classes: the HBM reserved for serving (state + rows)
     classes = []
-    for cap, count in ((8192, 48), (32768, 16), (131072, 4), (204800 + 8192, 2)):
+    for item in pool_spec.split(","):
+        cap, count = (int(v) for v in item.split(":"))
         classes.append((cap, count, W.system_replica_box(cap)))
     pool = serve.SlotPool(classes)
     log("serving pool: %d slots, %.1f GB reserved" % (sum(c[1] for c in classes),
@@ -518,6 +522,8 @@ def main():
                          "0 = only the timed steps)")
     ap.add_argument("--serve-seconds", type=float, default=30.0,
                     help="arrival window of the serving leg")
+    ap.add_argument("--serve-pool", default=SERVE_POOL,
+                    help="serving slot classes, capacity:count,... (the HBM reserved for serving)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--list-kernel", type=int, default=0,
                     help="row-build kernel of the batched leg (dml_set_tuning's list_kernel; "
@@ -568,7 +574,7 @@ def main():
     if "serving" in legs:
         log("serving leg: Poisson 20/s for %.0f s" % args.serve_seconds)
         sync()
-        serving = serving_leg(dml, args.serve_seconds)
+        serving = serving_leg(dml, args.serve_seconds, pool_spec=args.serve_pool)
         sync()
         log("serving", json.dumps(serving))
     c5 = None
