@@ -282,7 +282,8 @@ int dml_last_run_ms(dml_ctx *ctx, double *ms);
 enum { DML_K_LISTBUILD = 0, DML_K_FORCE = 1, DML_K_CELLS = 2, DML_K_DISPMAX = 3 };
 int dml_time_kernel(dml_ctx *ctx, int32_t which, int32_t reps, double *avg_ms);
 /* launch shapes: lanes per ref atom of the force kernel (power of two <= 64)
- * and of the overlap scan's first round (1, 2, 4, 8 or 16; later rounds use at least 2),
+ * and of the overlap scan's first round (1, 2, 4, 8 or 16; the second and third
+ * rounds use at least 2, later ones at least 4),
  * overlap rounds per step of dml_run (default 16, at most 32),
  * list kernel (1 = cell tiles through LDS, 2 = one wave per atom, 3 = 1 or 2
  * per replica by atoms per cell, 4 = one wave per cell, 5 = one wave per
