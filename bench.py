@@ -118,6 +118,20 @@ def plain_leg(dml, W, steps, warmup, drift_steps=10000):
     mean_nn, reb, st = ctx.list_stats()
     snap = ctx.state()
     ctx.close()
+    # the configuration's own integrator, Brownian (cbrownian_hs computes no
+    # forces, so the FP32-force comparison below needs the Ermak run)
+    mkb = lambda: dml.dana_config(c2["box"], c2["skin"], n, h=c2["h"], z0=c2["z0"],
+                                  zmax=c2["zmax"], integrator=dml.BROWNIAN,
+                                  reservoir=dml.NO_RESERVOIR, precision=dml.FP64,
+                                  overlap_mode=dml.OVERLAP_ROUNDS, seed=104012)
+    choice_b = autotune.tune([mkb()], [(c2["pos"], c2["species"])], log=log)
+    cb = dml.Context(mkb())
+    autotune.apply(cb, choice_b)
+    cb.set_state(c2["pos"], species=c2["species"])
+    cb.run(warmup, graph=True)
+    cb.sync()
+    ms_b = cb.run(steps, graph=True)
+    cb.close()
     # FP32 vs FP64 forces where fuerza runs in the step: right after ermak_a
     # (dana.F90:178-184), before the overlap resolution removes the pairs
     # inside r0.  Both contexts draw the same Philox noise.
@@ -137,7 +151,7 @@ def plain_leg(dml, W, steps, warmup, drift_steps=10000):
     drift = energy_drift(dml, mk, snap, drift_steps) if drift_steps else None
     return dict(n=n, ms=ms, wall=wall, rate=n * steps / (ms * 1e-3), mean_nn=mean_nn,
                 rebuilds=reb, steps_done=st, force_err=err, interacting=int(hot.sum()),
-                energy_drift=drift)
+                energy_drift=drift, brownian_rate=n * steps / (ms_b * 1e-3))
 
 
 def c3_configs(dml, reps, precision=None):
@@ -603,15 +617,18 @@ def main():
         "dtype": "fp64 (fp32 pair forces)" if batched else "fp64",
         "data": "synthetic (seeded bcc electrode + random-sequential Li+ at 0.01/A^3)",
         "config": {"workload": "C3 batched ensemble, %d replicas (2k-50k log-uniform), FP32 forces, "
-                               "hipGraph; plain = C2 single 100k system FP64" % args.replicas,
+                               "hipGraph; plain = C2 single 100k system FP64, Ermak (Brownian "
+                               "in plain_brownian_value)" % args.replicas,
                    "particles": head["n"], "parallelism": "replicas on one GPU"},
     }
     if plain:
         res["plain_value"] = plain["rate"]
+        res["plain_brownian_value"] = plain["brownian_rate"]
         res["force_rel_err_fp32_vs_fp64"] = plain["force_err"]
         res["energy_drift_fp32_vs_fp64"] = plain["energy_drift"]
     if plain and batched:
         res["batched_over_plain"] = batched["rate"] / plain["rate"]
+        res["batched_over_plain_brownian"] = batched["rate"] / plain["brownian_rate"]
     res["p99_run_latency_s"] = serving["p99_latency_s"] if serving else None
     if c5:
         res["c5"] = c5
