@@ -84,6 +84,7 @@ struct dml_ctx {
   double last_ms = 0.0;
   int lpa = 8;
   int lpa_ov = 8;  // lanes per atom of the overlap scan (1 = thread per atom)
+  int ov_tail = 10;  // overlap rounds from this one on run in one workgroup (0: none)
   // list kernel: 0 cell tiles, 1 wave per atom, 2 per replica by density
   // (0 or 1), 3 wave per cell
   int lb_mode = 4;
@@ -472,7 +473,11 @@ void enq_forces(dml_ctx *c, int fuse_b, int only_ermak) {
 void enq_overlap_rounds(dml_ctx *c, const double *u_round, int64_t stride, int rounds,
                         int err_if_pending) {
   hipLaunchKernelGGL(k_ov_begin, dim3(1), dim3(64), 0, c->st, c->d);
-  for (int q = 0; q < rounds; q++) {
+  // rounds from ov_tail on run in one workgroup, one launch (their lists
+  // are short; most of them are empty, and an empty round costs three
+  // launches otherwise)
+  const int grid_rounds = c->ov_tail > 0 && rounds > c->ov_tail ? c->ov_tail : rounds;
+  for (int q = 0; q < grid_rounds; q++) {
     // round 0 mostly visits the few listed entries within rcut (the tuned
     // lanes, often 1); later rounds scan whole rows, with at least 2 lanes,
     // and from round 3 on, whose few atoms wait on their rows' loads, at
@@ -514,6 +519,18 @@ void enq_overlap_rounds(dml_ctx *c, const double *u_round, int64_t stride, int r
     hipLaunchKernelGGL(k_ov_apply, dim3(ga), dim3(DML_BLOCK), 0, c->st, c->d, q, c->P);
     hipLaunchKernelGGL(k_ov_apply_r, dim3(q == 0 ? std::min(gr, 64u) : gr), dim3(DML_BLOCK), 0,
                        c->st, c->d, q);
+  }
+  if (grid_rounds < rounds) {
+    const int lpa = std::max(c->lpa_ov, 4);
+    if (lpa >= 16)
+      hipLaunchKernelGGL(k_ov_tail<16>, dim3(1), dim3(DML_BLOCK), 0, c->st, c->d, u_round, stride,
+                         grid_rounds, rounds, c->P);
+    else if (lpa == 8)
+      hipLaunchKernelGGL(k_ov_tail<8>, dim3(1), dim3(DML_BLOCK), 0, c->st, c->d, u_round, stride,
+                         grid_rounds, rounds, c->P);
+    else
+      hipLaunchKernelGGL(k_ov_tail<4>, dim3(1), dim3(DML_BLOCK), 0, c->st, c->d, u_round, stride,
+                         grid_rounds, rounds, c->P);
   }
   hipLaunchKernelGGL(k_ov_end, dim3((c->nrep + 63) / 64), dim3(64), 0, c->st, c->d, rounds,
                      err_if_pending);
@@ -928,6 +945,7 @@ int dml_create_ensemble(int32_t nrep, const dml_config *cfgs, dml_ctx **out) {
   hipEventCreate(&c->e0);
   hipEventCreate(&c->e1);
   if (const char *ev = getenv("DML_LISTBUILD")) c->lb_mode = atoi(ev);
+  if (const char *ev = getenv("DML_OV_TAIL")) c->ov_tail = std::max(0, atoi(ev));
   if (const char *ev = getenv("DML_OVERLAP_LPA")) {
     const int v = atoi(ev);
     if (v == 1 || v == 2 || v == 4 || v == 8 || v == 16) c->lpa_ov = v;

@@ -26,6 +26,9 @@ __global__ void k_ov_exact(DevState s, const double *u, int64_t n_u, int max_pas
 __global__ void k_ov_begin(DevState s);
 __global__ void k_ov_apply_r(DevState s, int round);
 __global__ void k_ov_apply(DevState s, int round, int64_t ptotal);
+template <int LPA>
+__global__ void k_ov_tail(DevState s, const double *u_round, int64_t u_stride, int r0, int r1,
+                          int64_t ptotal);
 __global__ void k_ov_end(DevState s, int rounds, int err_if_pending);
 __global__ void k_ov_close(DevState s, int rounds);
 __global__ void k_zero_i32(int32_t *a, int64_t n);

@@ -1962,14 +1962,16 @@ __global__ void k_ov_begin(DevState s) {
   if (q == 0) s.wl_cnt[2] = 0;  // round 0's list of reverted atoms outside ref
 }
 
+// one round's scan as workgroup bid of nbk (k_ov_scan_l: the whole grid;
+// k_ov_tail: one workgroup)
 template <int LPA>
-__global__ void __launch_bounds__(DML_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) k_ov_scan_l(DevState s, const double *u_round,
-                                                          int64_t u_stride, int round,
-                                                          int64_t ptotal) {
+__device__ __forceinline__ void ov_scan_body(const DevState &s, const double *u_round,
+                                             int64_t u_stride, int round, int64_t ptotal,
+                                             int bid, int nbk) {
   // progress flags rotate over three buffers: this round writes round % 3,
   // is gated by (round - 1) % 3, and clears (round + 1) % 3 for the next one
   const int cur = round % 3, prv = (round + 2) % 3, nxt = (round + 1) % 3;
-  if (blockIdx.x == 0) {
+  if (bid == 0) {
     for (int q = threadIdx.x; q < s.nrep; q += DML_BLOCK) s.ov_more[nxt * s.nrep + q] = 0;
     if (threadIdx.x == 0) s.wl_cnt[(round + 1) % 2] = 0;  // the next round's list
   }
@@ -1978,9 +1980,9 @@ __global__ void __launch_bounds__(DML_BLOCK) __attribute__((amdgpu_waves_per_eu(
   const int64_t nitems =
       round == 0 ? ptotal : min((int64_t)s.wl_cnt[round % 2], ptotal);
   const int32_t *wl = s.ovw[round % 2];
-  const int gtid = blockIdx.x * DML_BLOCK + threadIdx.x;
+  const int gtid = bid * DML_BLOCK + threadIdx.x;
   const int sub = gtid % LPA;
-  const int64_t stride = (int64_t)gridDim.x * (DML_BLOCK / LPA);
+  const int64_t stride = (int64_t)nbk * (DML_BLOCK / LPA);
   // one ref position: its verdict, and what it adds to its replica's
   // counters (k_r = the replica, -1 when the position is not scanned)
   auto scan_one = [&](int64_t item, int &k_try, int &k_ch, int &k_ch3, int &k_more, int &k_r) {
@@ -2140,7 +2142,7 @@ __global__ void __launch_bounds__(DML_BLOCK) __attribute__((amdgpu_waves_per_eu(
   // lists being appended block by block), one per atom otherwise.  The
   // counters of a replica are single addresses, which serialise across the
   // chip when every atom adds to them.
-  for (int64_t it0 = (int64_t)blockIdx.x * (DML_BLOCK / LPA); it0 < nitems; it0 += stride) {
+  for (int64_t it0 = (int64_t)bid * (DML_BLOCK / LPA); it0 < nitems; it0 += stride) {
     const int64_t item = it0 + threadIdx.x / LPA;
     int k_try = 0, k_ch = 0, k_ch3 = 0, k_more = 0, k_r = -1;
     if (item < nitems) scan_one(item, k_try, k_ch, k_ch3, k_more, k_r);
@@ -2166,6 +2168,12 @@ __global__ void __launch_bounds__(DML_BLOCK) __attribute__((amdgpu_waves_per_eu(
       if (k_more) s.ov_more[cur * s.nrep + k_r] = 1;
     }
   }
+}
+
+template <int LPA>
+__global__ void __launch_bounds__(DML_BLOCK) __attribute__((amdgpu_waves_per_eu(5)))
+k_ov_scan_l(DevState s, const double *u_round, int64_t u_stride, int round, int64_t ptotal) {
+  ov_scan_body<LPA>(s, u_round, u_stride, round, ptotal, blockIdx.x, gridDim.x);
 }
 template __global__ void k_ov_scan_l<1>(DevState, const double *, int64_t, int, int64_t);
 template __global__ void k_ov_scan_l<2>(DevState, const double *, int64_t, int, int64_t);
@@ -2216,20 +2224,21 @@ __device__ inline int ov_apply_one(const DevState &s, int r, const RepParams &P,
 // the unresolved ones to the next round's list with one atomic (a per-item
 // or per-wave atomic on the single list counter serialises across the chip);
 // the list order is free, every round's outcome depends on its start state only
-__global__ void __launch_bounds__(DML_BLOCK) k_ov_apply(DevState s, int round, int64_t ptotal) {
+__device__ __forceinline__ void ov_apply_body(const DevState &s, int round, int64_t ptotal,
+                                              int bid, int nbk) {
   __shared__ int32_t buf[DML_BLOCK * OVA_PER];
   __shared__ int wcnt[DML_BLOCK / WAVE];
   __shared__ int64_t base_sh;
   const int prv = (round + 2) % 3;
   const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
-  if (blockIdx.x == 0 && threadIdx.x == 0) s.wl_cnt[2 + (round + 1) % 2] = 0;  // next reverts
+  if (bid == 0 && threadIdx.x == 0) s.wl_cnt[2 + (round + 1) % 2] = 0;  // next reverts
   const int64_t nitems =
       round == 0 ? ptotal : min((int64_t)s.wl_cnt[round % 2], ptotal);
   const int32_t *wl = s.ovw[round % 2];
   int32_t *next = s.ovw[(round + 1) % 2];
   int32_t *wb = buf + w * WAVE * OVA_PER;
   const int64_t chunk = (int64_t)DML_BLOCK * OVA_PER;
-  for (int64_t b0 = blockIdx.x * chunk; b0 < nitems; b0 += (int64_t)gridDim.x * chunk) {
+  for (int64_t b0 = bid * chunk; b0 < nitems; b0 += (int64_t)nbk * chunk) {
     int n = 0;  // this wave's appends so far (wave-uniform)
     for (int k = 0; k < OVA_PER; k++) {
       const int64_t item = b0 + (int64_t)k * DML_BLOCK + threadIdx.x;
@@ -2265,17 +2274,20 @@ __global__ void __launch_bounds__(DML_BLOCK) k_ov_apply(DevState s, int round, i
     __syncthreads();  // buf, wcnt and base_sh are reused
   }
 }
+__global__ void __launch_bounds__(DML_BLOCK) k_ov_apply(DevState s, int round, int64_t ptotal) {
+  ov_apply_body(s, round, ptotal, blockIdx.x, gridDim.x);
+}
 
 // the atoms reverted in a round that k_ov_apply does not reach: in round 0
 // those outside the ref list (it walks every ref position), later the ones
 // the round did not scan (those that were are applied, and their stamp
 // cleared, by k_ov_apply)
-__global__ void __launch_bounds__(DML_BLOCK) k_ov_apply_r(DevState s, int round) {
+__device__ __forceinline__ void ov_apply_r_body(const DevState &s, int round, int bid, int nbk) {
   const int64_t cap = (int64_t)s.nblocks * DML_BLOCK;
   const int64_t nitems = min((int64_t)s.wl_cnt[2 + round % 2], cap);
   const int32_t *rl = s.ovr[round % 2];
-  const int64_t stride = (int64_t)gridDim.x * DML_BLOCK;
-  for (int64_t item = blockIdx.x * DML_BLOCK + threadIdx.x; item < nitems; item += stride) {
+  const int64_t stride = (int64_t)nbk * DML_BLOCK;
+  for (int64_t item = (int64_t)bid * DML_BLOCK + threadIdx.x; item < nitems; item += stride) {
     const int j = rl[item];
     if ((uint32_t)j >= (uint64_t)cap) continue;
     const int r = s.blk2rep[j / DML_BLOCK];
@@ -2296,6 +2308,29 @@ __global__ void __launch_bounds__(DML_BLOCK) k_ov_apply_r(DevState s, int round)
     }
   }
 }
+__global__ void __launch_bounds__(DML_BLOCK) k_ov_apply_r(DevState s, int round) {
+  ov_apply_r_body(s, round, blockIdx.x, gridDim.x);
+}
+
+// the last rounds, whose work lists are short, in one workgroup: the three
+// phases of each round (scan, apply, apply of the other reverted atoms)
+// separated by the workgroup's barriers instead of kernel boundaries, one
+// launch for all of them instead of three per round.  The same code per
+// phase as the grid-wide kernels, so the same result.
+template <int LPA>
+__global__ void __launch_bounds__(DML_BLOCK) k_ov_tail(DevState s, const double *u_round, int64_t u_stride, int r0, int r1, int64_t ptotal) {
+  for (int q = r0; q < r1; q++) {
+    ov_scan_body<LPA>(s, u_round, u_stride, q, ptotal, 0, 1);
+    __syncthreads();
+    ov_apply_body(s, q, ptotal, 0, 1);
+    __syncthreads();
+    ov_apply_r_body(s, q, 0, 1);
+    __syncthreads();
+  }
+}
+template __global__ void k_ov_tail<4>(DevState, const double *, int64_t, int, int, int64_t);
+template __global__ void k_ov_tail<8>(DevState, const double *, int64_t, int, int, int64_t);
+template __global__ void k_ov_tail<16>(DevState, const double *, int64_t, int, int, int64_t);
 
 // after the rounds: a replica whose last round still made progress would
 // need more rounds

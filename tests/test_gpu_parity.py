@@ -294,10 +294,11 @@ def _step_pair(orc, dml, s, c, rng, mode, nsteps, brown=True, gcmc=False, r1s=No
     return so, sd
 
 
-@pytest.mark.parametrize("mode", ["exact", "rounds", "rounds_lpa1"])
+@pytest.mark.parametrize("mode", ["exact", "rounds", "rounds_lpa1", "rounds_tail"])
 def test_brownian_steps_with_overlap_match_oracle(orc, dml, mode, monkeypatch):
     """several whole steps: the exact resolver follows the reference order
-    bit for bit; the round schedule follows its own oracle."""
+    bit for bit; the round schedule follows its own oracle, also with every
+    round after the first in the one-workgroup tail kernel (rounds_tail)."""
     # h and the skin are chosen like the reference's own runs (skin 10 with
     # steps of ~1 A): moved-back atoms then stay inside the list radius
     rng = np.random.default_rng(21)
@@ -309,6 +310,8 @@ def test_brownian_steps_with_overlap_match_oracle(orc, dml, mode, monkeypatch):
     say("steps", mode, len(pos))
     if mode == "rounds_lpa1":
         monkeypatch.setenv("DML_OVERLAP_LPA", "1")
+    if mode == "rounds_tail":
+        monkeypatch.setenv("DML_OV_TAIL", "1")
     s, c = pair(orc, dml, box, 10.0, pos, z, ermak=False, zmax=60.0, z0=30.0, h=1e-3,
                 overlap_mode=dml.OVERLAP_EXACT if mode == "exact" else dml.OVERLAP_ROUNDS)
     s.test_update(); c.test_update()
