@@ -2799,6 +2799,44 @@ __device__ inline int blk_min(int v, int *sh) {
   return t;
 }
 // exclusive prefix of a 0/1 flag over the block, and the block total
+// move the bucket entries of one cell (slot, FP64 and FP32 positions) one
+// place up (up = 1: entries at .. cnt-1, to open entry at) or one place down
+// (up = 0: entries at+1 .. cnt-1, closing entry at), by the block's threads:
+// chunks of blockDim entries, from the top when moving up and from the
+// bottom when moving down, each read before a barrier and written after it.
+// Starts with a barrier: every thread has read the cell's count (at, cnt)
+// before anyone, the caller's thread 0 after the return included, writes
+__device__ inline void blk_bucket_shift(const DevState &s, int64_t base, int at, int cnt, int up) {
+  __syncthreads();
+  const int nb = (int)blockDim.x;
+  const int first = up ? at : at + 1;
+  const int nchunks = cnt > first ? (cnt - first + nb - 1) / nb : 0;
+  for (int k = 0; k < nchunks; k++) {
+    const int lo = up ? max(first, cnt - (k + 1) * nb) : first + k * nb;
+    const int hi = up ? cnt - k * nb : min(cnt, first + (k + 1) * nb);
+    const int q = lo + (int)threadIdx.x;
+    int32_t b = 0;
+    double x = 0.0, y = 0.0, z = 0.0;
+    float4 f = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (q < hi) {
+      b = s.bucket[base + q];
+      x = s.bx[base + q];
+      y = s.by[base + q];
+      z = s.bz[base + q];
+      f = s.bf4[base + q];
+    }
+    __syncthreads();
+    if (q < hi) {
+      const int64_t d = base + q + (up ? 1 : -1);
+      s.bucket[d] = b;
+      s.bx[d] = x;
+      s.by[d] = y;
+      s.bz[d] = z;
+      s.bf4[d] = f;
+    }
+    __syncthreads();
+  }
+}
 __device__ inline int blk_scan01(int f, int *sh, int *total) {
   __syncthreads();
   unsigned long long m = __ballot(f);
@@ -3003,7 +3041,7 @@ __global__ void __launch_bounds__(GC_BLOCK) k_gcmc_run(DevState s, int rep_only,
   __shared__ int sh[GC_BLOCK / WAVE];
   __shared__ int sh_cells[27];
   __shared__ int sh_ncell;
-  __shared__ int sh_hit, sh_full, sh_flag;
+  __shared__ int sh_hit, sh_full;
   __shared__ int sh_freed_b[GC_TRACK];  // hs%b indices freed in this call (ascending)
   __shared__ int sh_tomb[GC_TRACK];     // deleted positions of invl (ascending)
   const int r = rep_only >= 0 ? rep_only : (int)blockIdx.x;
@@ -3197,25 +3235,23 @@ __global__ void __launch_bounds__(GC_BLOCK) k_gcmc_run(DevState s, int rep_only,
         rord[rlen] = i;
         s.ref_pos[i] = rlen;
         put_pf4(s, i, rx, ry, rz, s.species[tmpl]);
+      }
+      {
         // cgroup_sort_atom: push-front into its cell chain (Cells.F90:299-300);
-        // the Verlet bucket keeps ascending hs%b order
+        // the Verlet bucket keeps ascending hs%b order (the insertion place
+        // is the number of smaller keys); the block moves the entries above
         const int cell = P.use_cells ? cell_lin(P, ci0, ci1, ci2) : 0;
         const int64_t base = P.boff + (int64_t)cell * P.bcap;
         const int cnt = s.cell_cnt[P.coff + cell];
-        if (cnt >= P.bcap) {
-          sh_flag = ERR_BUCKET;
-        } else {
-          sh_flag = 0;
-          int at = 0;
-          if (!P.use_cells)
-            while (at < cnt && s.bkey[s.bucket[base + at]] < bs) at++;
-          for (int q = cnt; q > at; q--) {
-            s.bucket[base + q] = s.bucket[base + q - 1];
-            s.bx[base + q] = s.bx[base + q - 1];
-            s.by[base + q] = s.by[base + q - 1];
-            s.bz[base + q] = s.bz[base + q - 1];
-            s.bf4[base + q] = s.bf4[base + q - 1];
-          }
+        if (cnt >= P.bcap) { fail = ERR_BUCKET; break; }
+        int at = 0;
+        if (!P.use_cells) {
+          int below = 0;
+          for (int q = tid; q < cnt; q += GC_BLOCK) below += s.bkey[s.bucket[base + q]] < bs;
+          at = blk_sum(below, sh);
+        }
+        blk_bucket_shift(s, base, at, cnt, 1);
+        if (tid == 0) {
           s.bucket[base + at] = i;
           s.bx[base + at] = rx; s.by[base + at] = ry; s.bz[base + at] = rz;
           s.bf4[base + at] = bucket_f4(s, i);
@@ -3223,7 +3259,6 @@ __global__ void __launch_bounds__(GC_BLOCK) k_gcmc_run(DevState s, int rep_only,
         }
       }
       __syncthreads();
-      if (sh_flag) { fail = sh_flag; break; }
       pid_next++;
       seq_next++;
       nat++;
@@ -3369,22 +3404,6 @@ __global__ void __launch_bounds__(GC_BLOCK) k_gcmc_run(DevState s, int rep_only,
         }
         rord[s.ref_pos[o]] = -1;
         s.ref_pos[o] = -1;
-        const int64_t base = P.boff + (int64_t)cell * P.bcap;
-        int cnt = min(s.cell_cnt[P.coff + cell], P.bcap);
-        int at = -1;
-        for (int q = 0; q < cnt; q++)
-          if (s.bucket[base + q] == o) { at = q; break; }
-        if (at >= 0) {
-          for (int q = at; q < cnt - 1; q++) {
-            s.bucket[base + q] = s.bucket[base + q + 1];
-            s.bx[base + q] = s.bx[base + q + 1];
-            s.by[base + q] = s.by[base + q + 1];
-            s.bz[base + q] = s.bz[base + q + 1];
-            s.bf4[base + q] = s.bf4[base + q + 1];
-          }
-          s.cell_cnt[P.coff + cell] = cnt - 1;
-        }
-        sh_flag = at < 0;  // 'Unsort cgroup particle fail' (Cells.F90:350)
         s.b2slot[poff + bso] = -1;
         if (listed) s.nn[o] = 0;
         // a destroyed atom keeps its skip value (atom_destroy, Groups.F90:433-467)
@@ -3399,8 +3418,20 @@ __global__ void __launch_bounds__(GC_BLOCK) k_gcmc_run(DevState s, int rep_only,
           sh_freed_b[q] = bso;
         }
       }
+      {
+        // its entry leaves the cell's chain: found and closed by the block
+        const int64_t base = P.boff + (int64_t)cell * P.bcap;
+        const int cnt = min(s.cell_cnt[P.coff + cell], P.bcap);
+        int cand = 0x7fffffff;
+        for (int q = tid; q < cnt; q += GC_BLOCK)
+          if (s.bucket[base + q] == o) cand = min(cand, q);
+        const int at = blk_min(cand, sh);
+        // 'Unsort cgroup particle fail' (Cells.F90:350)
+        if (at == 0x7fffffff) { fail = ERR_UNSUP; break; }
+        blk_bucket_shift(s, base, at, cnt, 0);
+        if (tid == 0) s.cell_cnt[P.coff + cell] = cnt - 1;
+      }
       __syncthreads();
-      if (sh_flag) { fail = ERR_UNSUP; break; }
       if (nfreed_b < GC_TRACK) {
         nfreed_b++;
       } else {  // list full: scan the whole index range again
