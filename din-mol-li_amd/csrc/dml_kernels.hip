@@ -3274,35 +3274,52 @@ __global__ void __launch_bounds__(GC_BLOCK) k_gcmc_run(DevState s, int rep_only,
           // Verlet scan ('>' rejects)
           int32_t *row = s.nbr + (int64_t)i * maxnn;
           int nn = 0;
-          const int nst = P.use_cells ? 27 : 1;
-          for (int k = 0; k < nst; k++) {
-            int cc = 0;
-            if (P.use_cells) {
-              int n0 = cell_wrap(ci0 + kMap[k][0], P.ncells[0]);
-              int n1 = cell_wrap(ci1 + kMap[k][1], P.ncells[1]);
-              int n2 = cell_wrap(ci2 + kMap[k][2], P.ncells[2]);
-              cc = cell_lin(P, n0, n1, n2);
+          // the stencil's cells as one stream (stencil order, then chain
+          // order): lane k holds the count and bucket start of cell k, all
+          // loaded at once; 64 stream entries a pass, each lane finding its
+          // cell among the stencil's prefix sums
+          int kcnt = 0;
+          int64_t kbase = 0;
+          if (P.use_cells) {
+            if (lane < 27) {
+              const int n0 = cell_wrap(ci0 + kMap[lane][0], P.ncells[0]);
+              const int n1 = cell_wrap(ci1 + kMap[lane][1], P.ncells[1]);
+              const int n2 = cell_wrap(ci2 + kMap[lane][2], P.ncells[2]);
+              const int cc = cell_lin(P, n0, n1, n2);
+              kbase = P.boff + (int64_t)cc * P.bcap;
+              kcnt = min(s.cell_cnt[P.coff + cc], P.bcap);
             }
-            const int64_t base = P.boff + (int64_t)cc * P.bcap;
-            const int cnt = min(s.cell_cnt[P.coff + cc], P.bcap);
-            for (int q0 = 0; q0 < cnt; q0 += WAVE) {
-              const int q = q0 + lane;
-              int hitq = 0, j = -1;
-              if (q < cnt) {
-                j = s.bucket[base + q];
-                if (j != i) {
-                  double dx, dy, dz;
-                  vdist_mic(P, s.bx[base + q], s.by[base + q], s.bz[base + q], rx, ry, rz, &dx,
-                            &dy, &dz);
-                  const double rd = dot3(dx, dy, dz);
-                  hitq = P.use_cells ? (rd < rl2) : !(rd > rl2);
-                }
+          } else if (lane == 0) {
+            kbase = P.boff;
+            kcnt = min(s.cell_cnt[P.coff], P.bcap);
+          }
+          const int kincl = wave_scan_incl(kcnt);
+          const int total = __builtin_amdgcn_readlane(kincl, WAVE - 1);
+          for (int s0 = 0; s0 < total; s0 += WAVE) {
+            const int sp = s0 + lane;
+            // the stencil cell of stream position sp: the number of cells
+            // whose stream ends at or before it
+            int k = 0;
+            for (int c = 0; c < 27; c++) k += __builtin_amdgcn_readlane(kincl, c) <= sp;
+            const int kk = min(k, WAVE - 1);
+            const int64_t base = ((int64_t)__shfl((int)(kbase >> 32), kk) << 32) |
+                                 (uint32_t)__shfl((int)(kbase & 0xffffffff), kk);
+            const int q = sp - (__shfl(kincl, kk) - __shfl(kcnt, kk));
+            int hitq = 0, j = -1;
+            if (sp < total) {
+              j = s.bucket[base + q];
+              if (j != i) {
+                double dx, dy, dz;
+                vdist_mic(P, s.bx[base + q], s.by[base + q], s.bz[base + q], rx, ry, rz, &dx,
+                          &dy, &dz);
+                const double rd = dot3(dx, dy, dz);
+                hitq = P.use_cells ? (rd < rl2) : !(rd > rl2);
               }
-              const unsigned long long m = __ballot(hitq);
-              const int pr = __popcll(m & ((1ull << lane) - 1ull));
-              if (hitq && nn + pr < maxnn) row[nn + pr] = j;
-              nn += __popcll(m);
             }
+            const unsigned long long m = __ballot(hitq);
+            const int pr = __popcll(m & ((1ull << lane) - 1ull));
+            if (hitq && nn + pr < maxnn) row[nn + pr] = j;
+            nn += __popcll(m);
           }
           if (lane == 0) {
             if (nn > maxnn) sh_full = 1;
@@ -3312,26 +3329,40 @@ __global__ void __launch_bounds__(GC_BLOCK) k_gcmc_run(DevState s, int rep_only,
           }
         } else {
           // and i joins the row of every ref atom within the list radius
-          // (all of them lie in a neighbouring cell)
+          // (all of them lie in a neighbouring cell; each such row gets one
+          // entry, so the order of the visits is free): the near cells as
+          // one stream, 64 entries a pass, the passes dealt to waves 1-3
           const int nw3 = GC_BLOCK / WAVE - 1;
-          for (int k = wv - 1; k < ncell; k += nw3) {
-            const int cc = sh_cells[k];
-            const int64_t base = P.boff + (int64_t)cc * P.bcap;
-            const int cnt = min(s.cell_cnt[P.coff + cc], P.bcap);
-            for (int q = lane; q < cnt; q += WAVE) {
-              const int aj = s.bucket[base + q];
-              if (aj == i || !(s.flags[aj] & DML_FL_REF)) continue;
-              double dx, dy, dz;
-              vdist_mic(P, rx, ry, rz, s.bx[base + q], s.by[base + q], s.bz[base + q], &dx, &dy,
-                        &dz);
-              if (dot3(dx, dy, dz) > rl2) continue;
-              const int m = s.nn[aj];
-              if (m < maxnn) {
-                s.nbr[(int64_t)aj * maxnn + m] = i;
-                s.nn[aj] = m + 1;
-              } else {
-                sh_full = 1;
-              }
+          int kcnt = 0;
+          int64_t kbase = 0;
+          if (lane < ncell) {
+            const int cc = sh_cells[lane];
+            kbase = P.boff + (int64_t)cc * P.bcap;
+            kcnt = min(s.cell_cnt[P.coff + cc], P.bcap);
+          }
+          const int kincl = wave_scan_incl(kcnt);
+          const int total = __builtin_amdgcn_readlane(kincl, WAVE - 1);
+          for (int s0 = (wv - 1) * WAVE; s0 < total; s0 += nw3 * WAVE) {
+            const int sp = s0 + lane;
+            int k = 0;
+            for (int c = 0; c < ncell; c++) k += __builtin_amdgcn_readlane(kincl, c) <= sp;
+            const int kk = min(k, WAVE - 1);
+            const int64_t base = ((int64_t)__shfl((int)(kbase >> 32), kk) << 32) |
+                                 (uint32_t)__shfl((int)(kbase & 0xffffffff), kk);
+            const int q = sp - (__shfl(kincl, kk) - __shfl(kcnt, kk));
+            if (sp >= total) continue;
+            const int aj = s.bucket[base + q];
+            if (aj == i || !(s.flags[aj] & DML_FL_REF)) continue;
+            double dx, dy, dz;
+            vdist_mic(P, rx, ry, rz, s.bx[base + q], s.by[base + q], s.bz[base + q], &dx, &dy,
+                      &dz);
+            if (dot3(dx, dy, dz) > rl2) continue;
+            const int m = s.nn[aj];
+            if (m < maxnn) {
+              s.nbr[(int64_t)aj * maxnn + m] = i;
+              s.nn[aj] = m + 1;
+            } else {
+              sh_full = 1;
             }
           }
         }
