@@ -3114,12 +3114,28 @@ __global__ void __launch_bounds__(GC_BLOCK) k_gcmc_run(DevState s, int rep_only,
       __syncthreads();
       const int ncell = sh_ncell;
       // hard core against the gcmc group (dana.F90:638-653): every gcmc atom
-      // closer than rc lies in a neighbouring cell
-      for (int k = wv; k < ncell; k += GC_BLOCK / WAVE) {
-        const int cc = sh_cells[k];
-        const int64_t base = P.boff + (int64_t)cc * P.bcap;
-        const int cnt = min(s.cell_cnt[P.coff + cc], P.bcap);
-        for (int q = lane; q < cnt; q += WAVE) {
+      // closer than rc lies in a neighbouring cell.  The near cells as one
+      // stream (counts in one load, a wave prefix sum), 64 entries a pass,
+      // the passes dealt to the workgroup's waves
+      {
+        int kcnt = 0;
+        int64_t kbase = 0;
+        if (lane < ncell) {
+          const int cc = sh_cells[lane];
+          kbase = P.boff + (int64_t)cc * P.bcap;
+          kcnt = min(s.cell_cnt[P.coff + cc], P.bcap);
+        }
+        const int kincl = wave_scan_incl(kcnt);
+        const int total = __builtin_amdgcn_readlane(kincl, WAVE - 1);
+        for (int s0 = wv * WAVE; s0 < total; s0 += GC_BLOCK) {
+          const int sp = s0 + lane;
+          int k = 0;
+          for (int c = 0; c < ncell; c++) k += __builtin_amdgcn_readlane(kincl, c) <= sp;
+          const int kk = min(k, WAVE - 1);
+          const int64_t base = ((int64_t)__shfl((int)(kbase >> 32), kk) << 32) |
+                               (uint32_t)__shfl((int)(kbase & 0xffffffff), kk);
+          const int q = sp - (__shfl(kincl, kk) - __shfl(kcnt, kk));
+          if (sp >= total) continue;
           const int o = s.bucket[base + q];
           if (!(s.flags[o] & DML_FL_GCMC)) continue;
           double dx, dy, dz;  // distance(o%pos, r): r - o%pos
