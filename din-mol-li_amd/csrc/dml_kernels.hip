@@ -2695,6 +2695,34 @@ __device__ inline int near_cells(const RepParams &P, int c0, int c1, int c2, int
   return n;
 }
 
+// the k-th cell of near_cells' list (same order) and, in *n, their number:
+// one lane per cell without a per-lane array
+__device__ inline int near_cell_at(const RepParams &P, int c0, int c1, int c2, int k, int *n) {
+  int xs0 = 0, xs1 = 0, xs2 = 0, ys0 = 0, ys1 = 0, ys2 = 0, zs0 = 0, zs1 = 0, zs2 = 0;
+  int nx = 0, ny = 0, nz = 0;
+  for (int d = -1; d <= 1; d++) {
+    const int x = cell_wrap(c0 + d, P.ncells[0]);
+    if (!(nx > 0 && xs0 == x) && !(nx > 1 && xs1 == x)) {
+      if (nx == 0) xs0 = x; else if (nx == 1) xs1 = x; else xs2 = x;
+      nx++;
+    }
+    const int y = cell_wrap(c1 + d, P.ncells[1]);
+    if (!(ny > 0 && ys0 == y) && !(ny > 1 && ys1 == y)) {
+      if (ny == 0) ys0 = y; else if (ny == 1) ys1 = y; else ys2 = y;
+      ny++;
+    }
+    const int z = c2 + d;
+    if (z >= 0 && z <= P.ncells[2] + 1) {
+      if (nz == 0) zs0 = z; else if (nz == 1) zs1 = z; else zs2 = z;
+      nz++;
+    }
+  }
+  *n = nx * ny * nz;
+  const int i = k % nx, j = (k / nx) % ny, kz = k / (nx * ny);
+  return cell_lin(P, i == 0 ? xs0 : i == 1 ? xs1 : xs2, j == 0 ? ys0 : j == 1 ? ys1 : ys2,
+                  kz == 0 ? zs0 : kz == 1 ? zs1 : zs2);
+}
+
 // batched trial energies (insertion at a point, or removal of an atom: the
 // same sum without the atom itself): one wave per trial over the cells
 // around the point.  The cells are those of the last sort; every atom has
@@ -4667,29 +4695,44 @@ __global__ void __launch_bounds__(DML_BLOCK) k_gb_append(DevState s, int r) {
   const int ci2 = cell / ((P.ncells[0] + 2) * (P.ncells[1] + 2));
   const int maxnn = s.maxnn;
   const uint32_t stamp = P.gb_stamp * 2u + 1u;
-  int cells[27];
-  const int ncell = near_cells(P, ci0, ci1, ci2, cells);
-  for (int kc = 0; kc < ncell; kc++) {
-    const int cc = cells[kc];
-    // atoms of the start (present to the end, ref)
-    const int64_t base = P.boff + (int64_t)cc * P.bcap;
-    const int cnt = min(s.cell_cnt[P.coff + cc], P.bcap);
-    for (int q = lane; q < cnt; q += WAVE) {
-      const int aj = s.bucket[base + q];
-      if (!(s.flags[aj] & DML_FL_REF) || gb_del_time(s, P, aj) != 0x7fffffff) continue;
-      double dx, dy, dz;
-      vdist_mic(P, rx, ry, rz, s.bx[base + q], s.by[base + q], s.bz[base + q], &dx, &dy, &dz);
-      if (dot3(dx, dy, dz) > P.rl2) continue;
-      const int at = gb_stamped_inc(&s.gb_app[aj], stamp);
-      const int pos = s.nn[aj] + at;
-      if (pos < maxnn) s.nbr[(int64_t)aj * maxnn + pos] = i;
-      if (at == 0) s.gb_alist[P.poff + atomicAdd(&Pw->gb_nalist, 1)] = aj;
-    }
-    // the call's earlier insertions into cc kept to the end
-    for (int kk0 = gb_list_head(s, P, cc); kk0 >= 0;) {
-      int kk = kk0;  // the list one entry at a time, all lanes in step
-      kk0 = s.gi_next[off + kk];
-      if (lane != 0 || kk >= k || s.gi_del[off + kk] != 0x7fffffff) continue;
+  // the near cells, one per lane (each row gets one entry per insertion,
+  // put in insertion order by the second pass: the visiting order is free)
+  int ncell = 0;
+  const int mycell = near_cell_at(P, ci0, ci1, ci2, lane < 27 ? lane : 0, &ncell);
+  int kcnt = 0;
+  int64_t kbase = 0;
+  if (lane < ncell) {
+    kbase = P.boff + (int64_t)mycell * P.bcap;
+    kcnt = min(s.cell_cnt[P.coff + mycell], P.bcap);
+  }
+  // atoms of the start (present to the end, ref): the cells as one stream
+  // (a wave prefix sum of the counts), 64 entries a pass
+  const int kincl = wave_scan_incl(kcnt);
+  const int total = __builtin_amdgcn_readlane(kincl, WAVE - 1);
+  for (int s0 = 0; s0 < total; s0 += WAVE) {
+    const int sp = s0 + lane;
+    int kc = 0;
+    for (int c = 0; c < ncell; c++) kc += __builtin_amdgcn_readlane(kincl, c) <= sp;
+    const int kk = min(kc, WAVE - 1);
+    const int64_t base = ((int64_t)__shfl((int)(kbase >> 32), kk) << 32) |
+                         (uint32_t)__shfl((int)(kbase & 0xffffffff), kk);
+    const int q = sp - (__shfl(kincl, kk) - __shfl(kcnt, kk));
+    if (sp >= total) continue;
+    const int aj = s.bucket[base + q];
+    if (!(s.flags[aj] & DML_FL_REF) || gb_del_time(s, P, aj) != 0x7fffffff) continue;
+    double dx, dy, dz;
+    vdist_mic(P, rx, ry, rz, s.bx[base + q], s.by[base + q], s.bz[base + q], &dx, &dy, &dz);
+    if (dot3(dx, dy, dz) > P.rl2) continue;
+    const int at = gb_stamped_inc(&s.gb_app[aj], stamp);
+    const int pos = s.nn[aj] + at;
+    if (pos < maxnn) s.nbr[(int64_t)aj * maxnn + pos] = i;
+    if (at == 0) s.gb_alist[P.poff + atomicAdd(&Pw->gb_nalist, 1)] = aj;
+  }
+  // the call's earlier insertions into each cell kept to the end: lane c
+  // walks the list of cell c
+  if (lane < ncell) {
+    for (int kk = gb_list_head(s, P, mycell); kk >= 0; kk = s.gi_next[off + kk]) {
+      if (kk >= k || s.gi_del[off + kk] != 0x7fffffff) continue;
       const int tp = s.gi_trial[off + kk];
       double dx, dy, dz;
       vdist_mic(P, rx, ry, rz, s.gb_pt[3 * (off + tp)], s.gb_pt[3 * (off + tp) + 1],
