@@ -3810,20 +3810,32 @@ __global__ void __launch_bounds__(DML_BLOCK) k_gb_trials(DevState s, int r, cons
     if (lane == 0) s.gb_kind[o] = 2;
     return;
   }
-  int cells[27];
-  const int ncell = near_cells(P, ci0, ci1, ci2, cells);
+  // the near cells as one stream (lane k: count and bucket start of cell
+  // k, loaded at once; a wave prefix sum), 64 entries a pass, in cell order
+  int ncell = 0;
+  const int mycell = near_cell_at(P, ci0, ci1, ci2, lane < 27 ? lane : 0, &ncell);
+  int kcnt = 0;
+  int64_t kbase = 0;
+  if (lane < ncell) {
+    kbase = P.boff + (int64_t)mycell * P.bcap;
+    kcnt = min(s.cell_cnt[P.coff + mycell], P.bcap);
+  }
+  const int kincl = wave_scan_incl(kcnt);
+  const int total = __builtin_amdgcn_readlane(kincl, WAVE - 1);
   const double rc2 = P.rcut * P.rcut;
   double e = 0.0;
   int nb = 0, perm = 0;
-  for (int k = 0; k < ncell; k++) {
-    const int cell = cells[k];
-    int cnt = s.cell_cnt[P.coff + cell];
-    if (cnt > P.bcap) cnt = P.bcap;
-    const int64_t base = P.boff + (int64_t)cell * P.bcap;
-    for (int q0 = 0; q0 < cnt; q0 += WAVE) {
-      const int q = q0 + lane;
+  {
+    for (int s0 = 0; s0 < total; s0 += WAVE) {
+      const int sp = s0 + lane;
+      int kc = 0;
+      for (int c = 0; c < ncell; c++) kc += __builtin_amdgcn_readlane(kincl, c) <= sp;
+      const int kk = min(kc, WAVE - 1);
+      const int64_t base = ((int64_t)__shfl((int)(kbase >> 32), kk) << 32) |
+                           (uint32_t)__shfl((int)(kbase & 0xffffffff), kk);
+      const int q = sp - (__shfl(kincl, kk) - __shfl(kcnt, kk));
       int hit = 0, vp = -1;
-      if (q < cnt) {
+      if (sp < total) {
         const int j = s.bucket[base + q];
         double dx, dy, dz;
         vdist_mic(P, rx, ry, rz, s.px[j], s.py[j], s.pz[j], &dx, &dy, &dz);
@@ -4162,38 +4174,72 @@ __global__ void __launch_bounds__(DML_BLOCK) k_gb_check(DevState s, int r) {
   const int ci0 = cell % (P.ncells[0] + 2);
   const int ci1 = (cell / (P.ncells[0] + 2)) % (P.ncells[1] + 2);
   const int ci2 = cell / ((P.ncells[0] + 2) * (P.ncells[1] + 2));
+  // its own row: the 27 stencil cells as one stream (lane k: count and
+  // bucket start of stencil cell k, loaded at once; a wave prefix sum), and
+  // each lane walks the list of the call's insertions of its stencil cell
   int rowcnt = 0;
-  for (int k = 0; k < 27; k++) {
-    const int n0 = cell_wrap(ci0 + kMap[k][0], P.ncells[0]);
-    const int n1 = cell_wrap(ci1 + kMap[k][1], P.ncells[1]);
-    const int n2 = cell_wrap(ci2 + kMap[k][2], P.ncells[2]);
-    const int cc = cell_lin(P, n0, n1, n2);
-    const int64_t base = P.boff + (int64_t)cc * P.bcap;
-    const int cnt = min(s.cell_cnt[P.coff + cc], P.bcap);
-    for (int q = lane; q < cnt; q += WAVE) {
-      double dx, dy, dz;
-      vdist_mic(P, s.bx[base + q], s.by[base + q], s.bz[base + q], rx, ry, rz, &dx, &dy, &dz);
-      rowcnt += !(dot3(dx, dy, dz) > P.rl2);
+  {
+    int kcnt = 0, mycc = 0;
+    int64_t kbase = 0;
+    if (lane < 27) {
+      const int n0 = cell_wrap(ci0 + kMap[lane][0], P.ncells[0]);
+      const int n1 = cell_wrap(ci1 + kMap[lane][1], P.ncells[1]);
+      const int n2 = cell_wrap(ci2 + kMap[lane][2], P.ncells[2]);
+      mycc = cell_lin(P, n0, n1, n2);
+      kbase = P.boff + (int64_t)mycc * P.bcap;
+      kcnt = min(s.cell_cnt[P.coff + mycc], P.bcap);
     }
-    for (int kk = gb_list_head(s, P, cc); kk >= 0; kk = s.gi_next[off + kk]) {
-      const int tp = s.gi_trial[off + kk];
-      if (lane == 0 && tp != t) {
+    const int kincl = wave_scan_incl(kcnt);
+    const int total = __builtin_amdgcn_readlane(kincl, WAVE - 1);
+    for (int s0 = 0; s0 < total; s0 += WAVE) {
+      const int sp = s0 + lane;
+      int kc = 0;
+      for (int c = 0; c < 27; c++) kc += __builtin_amdgcn_readlane(kincl, c) <= sp;
+      const int kk = min(kc, WAVE - 1);
+      const int64_t base = ((int64_t)__shfl((int)(kbase >> 32), kk) << 32) |
+                           (uint32_t)__shfl((int)(kbase & 0xffffffff), kk);
+      const int q = sp - (__shfl(kincl, kk) - __shfl(kcnt, kk));
+      if (sp < total) {
         double dx, dy, dz;
-        vdist_mic(P, s.gb_pt[3 * (off + tp)], s.gb_pt[3 * (off + tp) + 1],
-                  s.gb_pt[3 * (off + tp) + 2], rx, ry, rz, &dx, &dy, &dz);
+        vdist_mic(P, s.bx[base + q], s.by[base + q], s.bz[base + q], rx, ry, rz, &dx, &dy, &dz);
         rowcnt += !(dot3(dx, dy, dz) > P.rl2);
+      }
+    }
+    if (lane < 27) {
+      for (int kk = gb_list_head(s, P, mycc); kk >= 0; kk = s.gi_next[off + kk]) {
+        const int tp = s.gi_trial[off + kk];
+        if (tp != t) {
+          double dx, dy, dz;
+          vdist_mic(P, s.gb_pt[3 * (off + tp)], s.gb_pt[3 * (off + tp) + 1],
+                    s.gb_pt[3 * (off + tp) + 2], rx, ry, rz, &dx, &dy, &dz);
+          rowcnt += !(dot3(dx, dy, dz) > P.rl2);
+        }
       }
     }
   }
   for (int o2 = WAVE / 2; o2 > 0; o2 >>= 1) rowcnt += __shfl_xor(rowcnt, o2);
   if (rowcnt > maxnn) bad = 1;
-  int cells[27];
-  const int ncell = near_cells(P, ci0, ci1, ci2, cells);
-  for (int k = 0; k < ncell; k++) {
-    const int cc = cells[k];
-    const int64_t base = P.boff + (int64_t)cc * P.bcap;
-    const int cnt = min(s.cell_cnt[P.coff + cc], P.bcap);
-    for (int q = lane; q < cnt; q += WAVE) {
+  // the rows it joins: the near cells as one stream
+  {
+    int ncell = 0;
+    const int mycell = near_cell_at(P, ci0, ci1, ci2, lane < 27 ? lane : 0, &ncell);
+    int kcnt = 0;
+    int64_t kbase = 0;
+    if (lane < ncell) {
+      kbase = P.boff + (int64_t)mycell * P.bcap;
+      kcnt = min(s.cell_cnt[P.coff + mycell], P.bcap);
+    }
+    const int kincl = wave_scan_incl(kcnt);
+    const int total = __builtin_amdgcn_readlane(kincl, WAVE - 1);
+    for (int s0 = 0; s0 < total; s0 += WAVE) {
+      const int sp = s0 + lane;
+      int kc = 0;
+      for (int c = 0; c < ncell; c++) kc += __builtin_amdgcn_readlane(kincl, c) <= sp;
+      const int kk = min(kc, WAVE - 1);
+      const int64_t base = ((int64_t)__shfl((int)(kbase >> 32), kk) << 32) |
+                           (uint32_t)__shfl((int)(kbase & 0xffffffff), kk);
+      const int q = sp - (__shfl(kincl, kk) - __shfl(kcnt, kk));
+      if (sp >= total) continue;
       const int aj = s.bucket[base + q];
       if (!(s.flags[aj] & DML_FL_REF)) continue;
       double dx, dy, dz;
