@@ -4340,16 +4340,21 @@ __global__ void __launch_bounds__(DML_BLOCK) k_gb_dels(DevState s, int r) {
 // order: the first free ones of the call's start (at most one per trial)
 // and those of the atoms it deletes (rank = position in that order)
 __global__ void __launch_bounds__(DML_BLOCK) k_gb_brank(DevState s, int r) {
+  // the deletions' hs%b indices, staged once per workgroup (every thread
+  // counts over all of them)
+  __shared__ int32_t lb[DML_GCMC_BATCH_MAX];
   const int q = blockIdx.x * DML_BLOCK + threadIdx.x;
   const RepParams &P = s.rp[r];
   if (!gb_on(P)) return;
-  const int off = P.gb_off, nUF = min(P.gb_nFB, P.gb_t), nd = P.gb_ndel;
+  const int off = P.gb_off, nUF = min(P.gb_nFB, P.gb_t), nd = min(P.gb_ndel, DML_GCMC_BATCH_MAX);
   const int32_t *fb = s.gb_freeb + off;
+  for (int d = threadIdx.x; d < nd; d += DML_BLOCK) lb[d] = s.gd_bso[off + d];
+  __syncthreads();
   if (q < nUF) {
     const int x = fb[q];
     int rank = q;
     for (int d = 0; d < nd; d++) {
-      const int b = s.gd_bso[off + d];
+      const int b = lb[d];
       rank += b >= 0 && b < x;
     }
     s.gb_frank[off + q] = rank;
@@ -4365,7 +4370,7 @@ __global__ void __launch_bounds__(DML_BLOCK) k_gb_brank(DevState s, int r) {
     }
     int rank = lo;
     for (int e = 0; e < nd; e++) {
-      const int b = s.gd_bso[off + e];
+      const int b = lb[e];
       rank += b >= 0 && b < x;
     }
     s.gd_brank[off + d] = rank;
@@ -4378,8 +4383,8 @@ __global__ void __launch_bounds__(DML_BLOCK) k_gb_brank(DevState s, int r) {
 // 1100), the template (first live gcmc atom) and list tail whose velocity
 // the insertion sets (dana.F90:620-668), and the ordered stores: flags,
 // hs%b maps, ref and gcmc list entries, limbo (Groups.F90:433-467).  The
-// universe of free hs%b indices is a bitmap in registers (lane l holds
-// words l, l + 64, ...), their values in LDS.
+// universe of free hs%b indices is a bitmap in LDS searched from a lower
+// bound of the lowest free rank, their values in LDS too.
 __global__ void __launch_bounds__(WAVE) k_gb_commit(DevState s, int r, const double *g) {
   // universe values (the free indices of the start and those freed by the
   // call's deletions: at most two per trial counting the appended ones)
