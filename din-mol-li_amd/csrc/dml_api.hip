@@ -84,7 +84,7 @@ struct dml_ctx {
   double last_ms = 0.0;
   int lpa = 8;
   int lpa_ov = 8;  // lanes per atom of the overlap scan (1 = thread per atom)
-  int ov_tail = 10;  // overlap rounds from this one on run in one workgroup (0: none)
+  int ov_tail = 8;  // overlap rounds from this one on run in one workgroup (0: none)
   // list kernel: 0 cell tiles, 1 wave per atom, 2 per replica by density
   // (0 or 1), 3 wave per cell
   int lb_mode = 4;
